@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of the batched b2World.Step hot path on one MI355X.
+
+Workload: 4096 independent worlds (lognormal body counts, median 64, sigma 0.8,
+clipped to [8, 1024]) of convex polygons and circles dropped into 40 m x 80 m
+U containers, stepped together at dt = 1/60 with 8 velocity and 3 position
+iterations, warm starting and sleeping on, continuous physics off.
+
+Prints one JSON line.  value = dynamic bodies (awake or asleep) x steps / wall
+seconds over the timed steps, with the worlds already resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from dbox_amd import native, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 60.0
+VEL, POS = 8, 3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def shape_bytes(fixtures: np.ndarray) -> int:
+    """Shape bytes read per proxy: 12 + 16 per polygon vertex, 20 per circle, 36 per edge."""
+    kind = fixtures["shape"]
+    nv = np.where(kind == scenes.SHAPE_BOX, 4, fixtures["count"])
+    b = np.where(kind == scenes.SHAPE_CIRCLE, 20, np.where(kind == scenes.SHAPE_EDGE, 36, 12 + 16 * nv))
+    return int(b.sum())
+
+
+def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int, sbytes: int, npairs: int) -> float:
+    """Compulsory HBM bytes of one launch of each step kernel (per step)."""
+    c, touch = t["contacts"], t["touching"]
+    if kernel == "solve":
+        # bodies: read c,a,v,w,F,tau,invMass,invI,localCenter,gravityScale,dampings,flags/sleep,fat AABB (92 B),
+        # write c,a,v,w,xf,c0,a0,flags/sleep,fat AABB (76 B); touching contacts: read manifold (84 B), write impulses (16 B)
+        return ndyn * (92 + 76) + touch * (84 + 16) + sbytes
+    if kernel == "collide":
+        # every contact read and written once (84 B each way), shapes and body transforms read once
+        return c * (84 + 84) + sbytes + nbodies * 16
+    if kernel == "broadphase":
+        # fat AABBs and moved flags read, unique pairs written
+        return nprox * 20 + npairs * 8
+    # islands: contact keys and flags, body flags
+    return c * 12 + nbodies * 8
+
+
+def run_cpu_baseline(worlds, warmup, steps, threads, sample_every):
+    from oracle import pyoracle as po
+    sample = worlds[::sample_every]
+    ows = [po.OracleWorld(s) for s in sample]
+    dyn = sum(w.dynamic_count for w in ows)
+
+    def run(ws, n):
+        for _ in range(n):
+            for w in ws:
+                w.step(DT, VEL, POS)
+
+    def timed(nthreads):
+        parts = [ows[i::nthreads] for i in range(nthreads)]
+        ts = [threading.Thread(target=run, args=(p, warmup)) for p in parts]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        t0 = time.perf_counter()
+        ts = [threading.Thread(target=run, args=(p, steps)) for p in parts]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return time.perf_counter() - t0
+
+    # single thread on a fresh copy, then all threads on the sample itself
+    single = [po.OracleWorld(s) for s in sample]
+    run(single, warmup)
+    t0 = time.perf_counter()
+    run(single, steps)
+    t1 = time.perf_counter() - t0
+    tn = timed(threads)
+    return dyn, t1, tn, len(sample)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--ordered", action="store_true", help="solve in the reference's exact order")
+    ap.add_argument("--cpu-sample-every", type=int, default=32)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (worlds shard across GPUs with no "
+              "exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t_gen = time.perf_counter()
+    worlds = scenes.batched_worlds(args.worlds)
+    log(f"generated {len(worlds)} worlds in {time.perf_counter() - t_gen:.1f}s")
+    h = native.StepWorlds()
+    for s in worlds:
+        h.add_scene(s)
+    h.finalize()
+    ndyn = sum(s.dynamic_count for s in worlds)
+    nbodies = sum(len(s.bodies) for s in worlds)
+    nprox = sum(len(s.fixtures) for s in worlds)
+    sbytes = sum(shape_bytes(s.fixtures) for s in worlds)
+    flags = native.WARM | native.SLEEP | (native.ORDERED if args.ordered else 0)
+    log(f"bodies {nbodies} dynamic {ndyn} proxies {nprox}")
+
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:  # torch is only used for an extra device barrier
+        have_torch = False
+
+    h.step(DT, VEL, POS, flags, n=args.warmup)
+    h.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    h.profile_totals(reset=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        h.step(DT, VEL, POS, flags)
+        if (i + 1) % 100 == 0:
+            log(f"step {i + 1}/{args.steps}")
+    h.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof, nsteps = h.profile_totals(reset=True)
+    tot = h.totals()
+    value = ndyn * args.steps / elapsed
+
+    kern_ms = {"collide": prof["collide"], "islands": prof["solveInit"], "solve": prof["solveVelocity"],
+               "broadphase": prof["broadphase"]}
+    dom = max(kern_ms, key=kern_ms.get)
+    avg_ms = kern_ms[dom] / max(nsteps, 1)
+    npairs = sum(h.pairs(w).shape[0] for w in range(len(worlds)))
+    abytes = algorithmic_bytes(dom, tot, nbodies, ndyn, nprox, sbytes, npairs)
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    log("per-kernel ms/step: " + ", ".join(f"{k} {v / max(nsteps, 1):.3f}" for k, v in kern_ms.items()))
+
+    result = {
+        "metric": "body-steps/s at 4096 worlds",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(worlds)} independent 40x80 m container worlds, lognormal body counts (median 64, "
+                        "sigma 0.8, clip 8-1024), 60% convex polygons 3-8 vertices / 40% circles",
+            "worlds": len(worlds),
+            "bodies": nbodies,
+            "dynamic_bodies": ndyn,
+            "dt": DT,
+            "velocity_iterations": VEL,
+            "position_iterations": POS,
+            "warm_starting": True,
+            "sleep": True,
+            "continuous": False,
+            "solver": "ordered" if args.ordered else "coloured",
+            "contacts_end": tot["contacts"],
+            "touching_end": tot["touching"],
+            "awake_end": tot["awake"],
+        },
+        "roofline": {
+            "kernel": f"k_{dom}",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": abytes,
+            "avg_kernel_ms": avg_ms,
+        },
+        "phase_ms_per_step": {k: v / max(nsteps, 1) for k, v in kern_ms.items()},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.warmup, args.steps, threads, args.cpu_sample_every)
+        result["cpu_baseline"] = {
+            "value": dyn * args.steps / tn,
+            "unit": "body-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"every {args.cpu_sample_every}th world ({nsample} worlds, {dyn} dynamic bodies), same "
+                      f"{args.warmup} warm-up + {args.steps} timed steps, C++ restatement of dbox -O2 "
+                      "-ffp-contract=off, one world per task on a thread pool",
+            "single_thread_value": dyn * args.steps / t1,
+        }
+        result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+// dbx_device.h -- device-side data layout shared by the kernels and the host runtime.
+//
+// All worlds of a handle live in flat arrays; world w owns the slices
+//   bodies   [bodyBase,    bodyBase + bodyCount)
+//   fixtures [fixBase,     fixBase + fixCount)        (one proxy per fixture)
+//   contacts [contactBase, contactBase + contactCap)  (capacity; live count in WorldState)
+//   pairs    [pairBase,    pairBase + pairCap)
+// Indices stored inside a world (fixture -> body, contact -> fixture, adjacency)
+// are world-local, so a world's results never depend on which other worlds share
+// the launch.  Contacts are kept in creation order (oldest first); the reference's
+// contact list (b2ContactManager.m_contactList, newest first) is that order reversed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dbx_geom.h"
+
+namespace dbx {
+
+// b2Body.m_flags (b2body.d:1118-1127) in the low 16 bits, body type in bits 16-17.
+enum : uint32_t {
+    kBodyIsland = 0x1,
+    kBodyAwake = 0x2,
+    kBodyAutoSleep = 0x4,
+    kBodyBullet = 0x8,
+    kBodyFixedRotation = 0x10,
+    kBodyActive = 0x20,
+};
+enum : int { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+__device__ __host__ __forceinline__ int body_type(uint32_t f) { return (int)((f >> 16) & 3u); }
+
+// b2Contact.m_flags (b2contact.d:242-261) plus transient bits of one step.
+enum : uint32_t {
+    kContactIsland = 0x1,
+    kContactTouching = 0x2,
+    kContactEnabled = 0x4,
+    kContactDestroyed = 0x100, // fat AABBs stopped overlapping this step
+    kContactDormant = 0x200,   // both bodies asleep or static when Collide began
+    kContactBegan = 0x400,     // became touching this step (BeginContact)
+    kContactEnded = 0x800,     // stopped touching this step (EndContact)
+    kContactNewPt0 = 0x1000,   // manifold point 0 has no match in the previous manifold
+    kContactNewPt1 = 0x2000,
+};
+
+struct WorldDesc {
+    int bodyBase, bodyCount;
+    int fixBase, fixCount;
+    int contactBase, contactCap;
+    int pairBase, pairCap;
+    float gx, gy;
+    int islandBase, pad;
+};
+
+struct WorldState {
+    int contactCount;
+    int pairCount;    // unique pairs of the last broadphase pass
+    int islandCount;
+    int batchCount;
+    int error;        // bit0 contact capacity, bit1 pair capacity
+    int touching;
+    int iterations;   // velocity iterations used by the last solve
+    int constraints;  // constraints solved by the last step
+};
+
+// A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).
+struct alignas(16) Contact {
+    int fa, fb;            // world-local fixture indices, registry order
+    uint32_t flags;
+    int meta;              // manifold type | pointCount << 4
+    float friction, restitution;
+    uint32_t id0, id1;     // b2ContactID keys
+    float4 man;            // localNormal.xy, localPoint.xy
+    float4 pt0, pt1;       // point localPoint.xy, normalImpulse, tangentImpulse
+};
+static_assert(sizeof(Contact) == 80, "contact layout");
+
+// b2ContactVelocityConstraint (b2contactsolver.d:43-58), 160 bytes.
+struct alignas(16) VelCon {
+    float2 rA0, rB0;
+    float2 rA1, rB1;
+    float nImp0, tImp0, nImp1, tImp1;
+    float nMass0, tMass0, bias0, nMass1;
+    float tMass1, bias1, friction, tangentSpeed;
+    float2 normal;
+    float invMassA, invMassB;
+    float invIA, invIB;
+    int indexA, indexB;     // world-local bodies
+    float4 K;               // ex.x, ex.y, ey.x, ey.y
+    float4 normalMass;      // ex.x, ex.y, ey.x, ey.y
+    int pointCount, contact, island, dynMask; // dynMask bit0/1: body A/B is dynamic
+};
+static_assert(sizeof(VelCon) == 160, "velocity constraint layout");
+
+// b2ContactPositionConstraint (b2contactsolver.d:801-814) plus body indices, 96 bytes.
+struct alignas(16) PosCon {
+    float2 lp0, lp1;
+    float2 localNormal, localPoint;
+    float2 lcA, lcB;
+    float invMassA, invMassB, invIA, invIB;
+    float radiusA, radiusB;
+    int type, pointCount;
+    int indexA, indexB, dynMask, island;
+};
+static_assert(sizeof(PosCon) == 96, "position constraint layout");
+
+struct StepParams {
+    float dt, inv_dt, dtRatio;
+    int velocityIterations, positionIterations;
+    uint32_t flags;
+    float residualTol;
+    int maxIterations;
+};
+
+// Flat device arrays of all worlds.
+struct Dev {
+    int nworlds;
+    const int *order;       // launch order of worlds (largest first)
+    const WorldDesc *W;
+    WorldState *S;
+    // bodies
+    uint32_t *bflags;
+    float4 *bxf;            // p.x, p.y, q.s, q.c
+    float4 *bpos;           // sweep c.x, c.y, a, sleepTime
+    float4 *bpos0;          // sweep c0.x, c0.y, a0, 0
+    float4 *bvel;           // v.x, v.y, w, 0
+    float4 *bforce;         // force.x, force.y, torque, 0
+    const float4 *bmass;    // invMass, invI, localCenter.x, localCenter.y
+    const float4 *bparam;   // linearDamping, angularDamping, gravityScale, mass
+    // fixtures / proxies
+    const int *fbody;
+    const Shape *fshape;
+    const float4 *fmat;     // friction, restitution, density, shape radius
+    const int2 *ffilter;    // category | mask << 16, group
+    float4 *ffat;           // fat AABB
+    int *fmoved;
+    int *fcount;            // per fixture scratch (pair counts)
+    // contacts, double buffered (collide compacts one into the other)
+    Contact *cont[2];
+    // per body scratch
+    int *adjOff, *adjCnt;   // contact adjacency of each body (CSR)
+    int *bisl;              // island of a body this step, -1 when not simulated
+    int *bscratch;          // DFS stack / schedule scratch
+    int *wake;              // list position of the first wake in Collide
+    unsigned long long *bmask; // colour masks
+    // per contact-slot scratch
+    int *adjList;           // 2 entries per contact slot: contact * 2 + side
+    int2 *pairs;
+    int *ordC;              // island-ordered contact indices
+    int *cbatch;            // batch of each ordered constraint
+    int *cisl;              // island of each ordered constraint
+    int *sched;             // ordered constraints grouped by batch
+    int *batchOff;          // per world: contactCap + 65 batch offsets at contactBase + 65 * world
+    VelCon *vc;
+    PosCon *pc;
+    // per island scratch (island index < body count)
+    int *islMin;            // order-preserving int image of a float minimum
+    int *islSolved;
+    // learned warm start (optional)
+    float *feat;            // 14 features per new manifold point
+    int *featSlot;          // constraint slot * 2 + point for each feature row
+    float *pred;            // 2 predictions per feature row
+    int *featCount;         // single counter
+    float *residual;        // per world residual of the last iteration
+};
+
+// Order-preserving int image of a float (for atomicMin on floats).
+__device__ __forceinline__ int f2ord(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff)); }
+
+} // namespace dbx

@@ -1,0 +1,22 @@
+// dbx_kernels.h -- host-callable launchers of the step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dbx_device.h"
+
+namespace dbx {
+
+constexpr int kMaxBlock = 1024;
+
+hipError_t launch_collide(const Dev &d, int cur, int block, hipStream_t s);
+hipError_t launch_islands(const Dev &d, int cur, int ordered, int block, hipStream_t s);
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int block, hipStream_t s);
+hipError_t launch_broadphase(const Dev &d, int cur, int block, hipStream_t s);
+hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, int *cnt, int2 *out, int cap, int *total,
+                            hipStream_t s);
+hipError_t launch_collide_pairs(const Shape *sa, const float *xa, const Shape *sb, const float *xb, int n, float *out,
+                                hipStream_t s);
+hipError_t launch_max_penetration(const Dev &d, int cur, float *out, hipStream_t s);
+
+} // namespace dbx

@@ -1,0 +1,1234 @@
+// dbx_kernels.hip -- step kernels for gfx950 (MI355X).
+//
+// One workgroup owns one world for the length of a phase (worlds are
+// independent, so there is no grid-wide synchronisation).  Workgroups are
+// launched largest world first.  Inside a world, per-contact and per-body work
+// is spread over the workgroup's threads; Gauss-Seidel order is kept by solving
+// the constraints in batches (dependency levels of the reference order, or
+// greedy colours) with a workgroup barrier between batches.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include "dbx_device.h"
+#include "dbx_kernels.h"
+
+namespace dbx {
+
+// ---------------------------------------------------------------------------
+// workgroup helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+// Exclusive scan over the workgroup; every thread must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int &total, int *tmp) {
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) tmp[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < nw; ++i) {
+            int t = tmp[i];
+            tmp[i] = s;
+            s += t;
+        }
+        tmp[nw] = s;
+    }
+    __syncthreads();
+    int r = inc - v + tmp[wid];
+    total = tmp[nw];
+    __syncthreads();
+    return r;
+}
+// In-place exclusive scan of a[0..n) by the workgroup; returns the total.
+__device__ int block_scan_array(int *a, int n, int *tmp) {
+    int base = 0;
+    for (int c = 0; c < n; c += blockDim.x) {
+        int i = c + threadIdx.x;
+        int v = i < n ? a[i] : 0;
+        int tot;
+        int off = block_excl_scan(v, tot, tmp);
+        if (i < n) a[i] = base + off;
+        base += tot;
+    }
+    __syncthreads();
+    return base;
+}
+
+__device__ __forceinline__ Xf load_xf(const float4 *bxf, int b) {
+    float4 t = bxf[b];
+    Xf x;
+    x.p = v2(t.x, t.y);
+    x.q.s = t.z;
+    x.q.c = t.w;
+    return x;
+}
+__device__ __forceinline__ int man_type(int meta) { return meta & 0xf; }
+__device__ __forceinline__ int man_count(int meta) { return (meta >> 4) & 0xf; }
+
+// b2Body.SetAwake(true) (b2body.d:827-846).
+__device__ __forceinline__ void wake_body(uint32_t *bfl, float4 *bpos, int b) {
+    uint32_t f = bfl[b];
+    if ((f & kBodyAwake) == 0) {
+        bfl[b] = f | kBodyAwake;
+        bpos[b].w = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Collide: b2ContactManager.Collide (b2contactmanager.d:251-317) for one contact.
+// pos is the contact's position in the reference's list (head = 0).
+// ---------------------------------------------------------------------------
+__device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbody, const Shape *__restrict__ shp,
+                                const float4 *__restrict__ fat, const float4 *__restrict__ bxf, int *wake) {
+    int ba = fbody[c.fa], bb = fbody[c.fb];
+    int count = man_count(c.meta);
+    if (!overlap(fat[c.fa], fat[c.fb])) {
+        // b2ContactManager.Destroy -> b2Contact.Destroy wakes both bodies of a touching manifold
+        c.flags |= kContactDestroyed;
+        if (c.flags & kContactTouching) c.flags |= kContactEnded;
+        if (count > 0) {
+            atomicMin(&wake[ba], pos);
+            atomicMin(&wake[bb], pos);
+        }
+        return;
+    }
+    // b2Contact.Update (b2contact.d:270-356)
+    c.flags |= kContactEnabled;
+    bool wasTouching = (c.flags & kContactTouching) != 0;
+    Xf xfA = load_xf(bxf, ba), xfB = load_xf(bxf, bb);
+    Manifold m;
+    m.type = man_type(c.meta);
+    m.count = count;
+    m.ln = v2(c.man.x, c.man.y);
+    m.lp = v2(c.man.z, c.man.w);
+    m.p[0] = v2(c.pt0.x, c.pt0.y);
+    m.p[1] = v2(c.pt1.x, c.pt1.y);
+    m.id[0] = c.id0;
+    m.id[1] = c.id1;
+    uint32_t oid0 = c.id0, oid1 = c.id1;
+    float4 o0 = c.pt0, o1 = c.pt1;
+    evaluate(m, shp + c.fa, xfA, shp + c.fb, xfB);
+    bool touching = m.count > 0;
+    float ni[2] = {0.0f, 0.0f}, ti[2] = {0.0f, 0.0f};
+    uint32_t newPts = 0;
+    for (int i = 0; i < m.count; ++i) {
+        uint32_t id = m.id[i];
+        if (count > 0 && oid0 == id) {
+            ni[i] = o0.z;
+            ti[i] = o0.w;
+        } else if (count > 1 && oid1 == id) {
+            ni[i] = o1.z;
+            ti[i] = o1.w;
+        } else {
+            newPts |= i == 0 ? kContactNewPt0 : kContactNewPt1;
+        }
+    }
+    if (touching != wasTouching) {
+        atomicMin(&wake[ba], pos);
+        atomicMin(&wake[bb], pos);
+    }
+    uint32_t f = c.flags & ~(kContactTouching | kContactNewPt0 | kContactNewPt1);
+    if (touching) f |= kContactTouching;
+    if (!wasTouching && touching) f |= kContactBegan;
+    if (wasTouching && !touching) f |= kContactEnded;
+    c.flags = f | newPts;
+    c.meta = m.type | (m.count << 4);
+    c.man = make_float4(m.ln.x, m.ln.y, m.lp.x, m.lp.y);
+    c.pt0 = make_float4(m.p[0].x, m.p[0].y, ni[0], ti[0]);
+    c.pt1 = make_float4(m.p[1].x, m.p[1].y, ni[1], ti[1]);
+    c.id0 = m.id[0];
+    c.id1 = m.id[1];
+}
+
+__global__ void k_collide(Dev d, int cur) {
+    __shared__ int s_tmp[kMaxBlock / 64 + 1];
+    const int w = d.order[blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount;
+    Contact *src = d.cont[cur] + wd.contactBase;
+    Contact *dst = d.cont[cur ^ 1] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    const float4 *fat = d.ffat + wd.fixBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    const float4 *bxf = d.bxf + wd.bodyBase;
+    int *wake = d.wake + wd.bodyBase;
+    const int nb = wd.bodyCount;
+    const int tid = threadIdx.x, BS = blockDim.x;
+
+    for (int b = tid; b < nb; b += BS) wake[b] = INT_MAX;
+    __syncthreads();
+
+    const uint32_t transient = kContactDestroyed | kContactDormant | kContactBegan | kContactEnded | kContactIsland |
+                               kContactNewPt0 | kContactNewPt1;
+    int dormant = 0;
+    for (int i = tid; i < n; i += BS) {
+        Contact c = src[i];
+        c.flags &= ~transient;
+        uint32_t fa = bfl[fbody[c.fa]], fb = bfl[fbody[c.fb]];
+        bool activeA = (fa & kBodyAwake) && body_type(fa) != kStatic;
+        bool activeB = (fb & kBodyAwake) && body_type(fb) != kStatic;
+        if (!activeA && !activeB) {
+            c.flags |= kContactDormant;
+            src[i].flags = c.flags;
+            dormant = 1;
+            continue;
+        }
+        process_contact(c, n - 1 - i, fbody, shp, fat, bxf, wake);
+        src[i] = c;
+    }
+    // A contact whose bodies were both asleep is still processed when an
+    // earlier contact in the list woke one of its non-static bodies: iterate to
+    // the fixed point of that rule.
+    int more = __syncthreads_or(dormant);
+    while (more) {
+        int changed = 0;
+        for (int i = tid; i < n; i += BS) {
+            if ((src[i].flags & kContactDormant) == 0) continue;
+            Contact c = src[i];
+            int pos = n - 1 - i;
+            int ba = fbody[c.fa], bb = fbody[c.fb];
+            bool wa = body_type(bfl[ba]) != kStatic && wake[ba] < pos;
+            bool wb = body_type(bfl[bb]) != kStatic && wake[bb] < pos;
+            if (!wa && !wb) continue;
+            c.flags &= ~kContactDormant;
+            process_contact(c, pos, fbody, shp, fat, bxf, wake);
+            src[i] = c;
+            changed = 1;
+        }
+        more = __syncthreads_or(changed);
+    }
+    for (int b = tid; b < nb; b += BS)
+        if (wake[b] != INT_MAX) wake_body(bfl, bpos, b);
+    __syncthreads();
+
+    // Stable compaction of the surviving contacts into the other buffer.
+    int m = 0, touching = 0;
+    for (int c0 = 0; c0 < n; c0 += BS) {
+        int i = c0 + tid;
+        int keep = 0;
+        Contact c;
+        if (i < n) {
+            c = src[i];
+            keep = (c.flags & kContactDestroyed) == 0;
+        }
+        int tot;
+        int off = block_excl_scan(keep, tot, s_tmp);
+        if (keep) {
+            c.flags &= ~kContactDormant;
+            dst[m + off] = c;
+        }
+        m += tot;
+        int t2;
+        block_excl_scan(keep && (c.flags & kContactTouching) ? 1 : 0, t2, s_tmp);
+        touching += t2;
+    }
+
+    // Contact adjacency of each body in list order (newest contact first),
+    // the order of b2Body.m_contactList.
+    int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
+    int *adj = d.adjList + 2 * wd.contactBase;
+    for (int b = tid; b < nb; b += BS) adjCnt[b] = 0;
+    __syncthreads();
+    for (int k = tid; k < m; k += BS) {
+        atomicAdd(&adjCnt[fbody[dst[k].fa]], 1);
+        atomicAdd(&adjCnt[fbody[dst[k].fb]], 1);
+    }
+    __syncthreads();
+    for (int b = tid; b < nb; b += BS) adjOff[b] = adjCnt[b];
+    __syncthreads();
+    block_scan_array(adjOff, nb, s_tmp);
+    for (int b = tid; b < nb; b += BS) adjCnt[b] = 0;
+    __syncthreads();
+    for (int k = tid; k < m; k += BS) {
+        int ba = fbody[dst[k].fa], bb = fbody[dst[k].fb];
+        adj[adjOff[ba] + atomicAdd(&adjCnt[ba], 1)] = 2 * k;
+        adj[adjOff[bb] + atomicAdd(&adjCnt[bb], 1)] = 2 * k + 1;
+    }
+    __syncthreads();
+    for (int b = tid; b < nb; b += BS) {
+        int o = adjOff[b], cnt = adjCnt[b];
+        for (int i = 1; i < cnt; ++i) {
+            int v = adj[o + i];
+            int j = i - 1;
+            while (j >= 0 && adj[o + j] < v) {
+                adj[o + j + 1] = adj[o + j];
+                --j;
+            }
+            adj[o + j + 1] = v;
+        }
+    }
+    if (tid == 0) {
+        d.S[w].contactCount = m;
+        d.S[w].touching = touching;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Islands: the depth-first search of b2World.Solve (b2world.d:963-1076) in the
+// reference's body-list and contact-list order, then the solve schedule.
+// ordered != 0: batch = dependency level of the reference's sequential order,
+//               which reproduces its Gauss-Seidel results exactly.
+// ordered == 0: batch = greedy colour in the same order.
+// ---------------------------------------------------------------------------
+__global__ void k_islands(Dev d, int cur, int ordered) {
+    const int w = d.order[blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int nb = wd.bodyCount;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    int *bisl = d.bisl + wd.bodyBase;
+    int *stack = d.bscratch + wd.bodyBase;
+    int *last = d.wake + wd.bodyBase;
+    unsigned long long *mask = d.bmask + wd.bodyBase;
+    const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
+    const int *adj = d.adjList + 2 * wd.contactBase;
+    int *ordC = d.ordC + wd.contactBase;
+    int *cbatch = d.cbatch + wd.contactBase;
+    int *cisl = d.cisl + wd.contactBase;
+    int *sched = d.sched + wd.contactBase;
+    int *boff = d.batchOff + wd.contactBase + 65 * w;
+    const int tid = threadIdx.x, BS = blockDim.x;
+
+    for (int b = tid; b < nb; b += BS) {
+        bisl[b] = -1;
+        last[b] = -1;
+        mask[b] = 0ull;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+
+    int nc = 0, nisl = 0;
+    for (int seed = nb - 1; seed >= 0; --seed) {
+        if (bisl[seed] >= 0) continue;
+        uint32_t sf = bfl[seed];
+        if ((sf & kBodyAwake) == 0 || (sf & kBodyActive) == 0) continue;
+        if (body_type(sf) == kStatic) continue;
+        int sp = 0;
+        stack[sp++] = seed;
+        bisl[seed] = nisl;
+        while (sp > 0) {
+            int b = stack[--sp];
+            wake_body(bfl, bpos, b);
+            int o = adjOff[b], cnt = adjCnt[b];
+            for (int e = 0; e < cnt; ++e) {
+                int ent = adj[o + e];
+                int k = ent >> 1;
+                uint32_t cf = cont[k].flags;
+                if (cf & kContactIsland) continue;
+                if ((cf & kContactEnabled) == 0 || (cf & kContactTouching) == 0) continue;
+                cont[k].flags = cf | kContactIsland;
+                ordC[nc] = k;
+                cisl[nc] = nisl;
+                ++nc;
+                int other = (ent & 1) ? fbody[cont[k].fa] : fbody[cont[k].fb];
+                if (bisl[other] >= 0) continue;
+                // Static bodies join an island without propagating it and are read-only
+                // in the solver, so they are not pushed.
+                if (body_type(bfl[other]) == kStatic) continue;
+                bisl[other] = nisl;
+                stack[sp++] = other;
+            }
+        }
+        ++nisl;
+    }
+
+    // Batches: constraints of one batch share no dynamic body.
+    int nbatch = 0, overflow = 0;
+    for (int k = 0; k < nc; ++k) {
+        const Contact &c = cont[ordC[k]];
+        int a = fbody[c.fa], b = fbody[c.fb];
+        bool da = body_type(bfl[a]) == kDynamic, db = body_type(bfl[b]) == kDynamic;
+        int batch;
+        if (ordered) {
+            int la = da ? last[a] : -1, lb = db ? last[b] : -1;
+            batch = (la > lb ? la : lb) + 1;
+            if (da) last[a] = batch;
+            if (db) last[b] = batch;
+        } else {
+            unsigned long long used = (da ? mask[a] : 0ull) | (db ? mask[b] : 0ull);
+            if (~used) {
+                batch = __ffsll((long long)~used) - 1;
+                if (da) mask[a] |= 1ull << batch;
+                if (db) mask[b] |= 1ull << batch;
+            } else {
+                batch = 64 + overflow++;
+            }
+        }
+        cbatch[k] = batch;
+        if (batch + 1 > nbatch) nbatch = batch + 1;
+    }
+    for (int i = 0; i <= nbatch; ++i) boff[i] = 0;
+    for (int k = 0; k < nc; ++k) boff[cbatch[k] + 1]++;
+    for (int i = 0; i < nbatch; ++i) boff[i + 1] += boff[i];
+    for (int k = 0; k < nc; ++k) sched[boff[cbatch[k]]++] = k;
+    for (int i = nbatch; i > 0; --i) boff[i] = boff[i - 1];
+    boff[0] = 0;
+    d.S[w].islandCount = nisl;
+    d.S[w].batchCount = nbatch;
+    d.S[w].constraints = nc;
+}
+
+// ---------------------------------------------------------------------------
+// Solver: b2Island.Solve (b2island.d:75-280) for every island of the world.
+// ---------------------------------------------------------------------------
+__device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, int island, const int *__restrict__ fbody,
+                                const float4 *__restrict__ fmat, const uint32_t *__restrict__ bfl,
+                                const float4 *__restrict__ bmass, const float4 *__restrict__ bpos,
+                                const float4 *__restrict__ bvel, const StepParams &sp) {
+    int ba = fbody[c.fa], bb = fbody[c.fb];
+    float4 ma = bmass[ba], mb = bmass[bb];
+    float radiusA = fmat[c.fa].w, radiusB = fmat[c.fb].w;
+    int count = man_count(c.meta), type = man_type(c.meta);
+    // b2ContactSolver constructor (b2contactsolver.d:244-329)
+    bool warm = (sp.flags & 1u) != 0;
+    v.nImp0 = warm ? sp.dtRatio * c.pt0.z : 0.0f;
+    v.tImp0 = warm ? sp.dtRatio * c.pt0.w : 0.0f;
+    v.nImp1 = warm ? sp.dtRatio * c.pt1.z : 0.0f;
+    v.tImp1 = warm ? sp.dtRatio * c.pt1.w : 0.0f;
+    v.friction = c.friction;
+    v.tangentSpeed = 0.0f;
+    v.indexA = ba;
+    v.indexB = bb;
+    v.invMassA = ma.x;
+    v.invMassB = mb.x;
+    v.invIA = ma.y;
+    v.invIB = mb.y;
+    v.contact = ci;
+    v.island = island;
+    v.dynMask = (body_type(bfl[ba]) == kDynamic ? 1 : 0) | (body_type(bfl[bb]) == kDynamic ? 2 : 0);
+    p.lp0 = v2(c.pt0.x, c.pt0.y);
+    p.lp1 = v2(c.pt1.x, c.pt1.y);
+    p.localNormal = v2(c.man.x, c.man.y);
+    p.localPoint = v2(c.man.z, c.man.w);
+    p.lcA = v2(ma.z, ma.w);
+    p.lcB = v2(mb.z, mb.w);
+    p.invMassA = ma.x;
+    p.invMassB = mb.x;
+    p.invIA = ma.y;
+    p.invIB = mb.y;
+    p.radiusA = radiusA;
+    p.radiusB = radiusB;
+    p.type = type;
+    p.pointCount = count;
+    p.indexA = ba;
+    p.indexB = bb;
+    p.dynMask = v.dynMask;
+    p.island = island;
+
+    // InitializeVelocityConstraints (b2contactsolver.d:338-450)
+    float mA = ma.x, mB = mb.x, iA = ma.y, iB = mb.y;
+    float4 pa = bpos[ba], pb = bpos[bb], va = bvel[ba], vb = bvel[bb];
+    float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
+    float2 vA = v2(va.x, va.y), vB = v2(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+    Xf xfA = xf_from(cA, pa.z, p.lcA);
+    Xf xfB = xf_from(cB, pb.z, p.lcB);
+    WorldManifold wm;
+    float2 pts[2] = {p.lp0, p.lp1};
+    world_manifold(wm, type, count, p.localNormal, p.localPoint, pts, xfA, radiusA, xfB, radiusB);
+    v.normal = wm.normal;
+    float2 tangent = crs(wm.normal, 1.0f);
+    float restitution = c.restitution;
+    float nm[2], tm[2], bias[2];
+    float2 rAs[2], rBs[2];
+    for (int j = 0; j < count; ++j) {
+        float2 rA = sub(wm.points[j], cA);
+        float2 rB = sub(wm.points[j], cB);
+        float rnA = crs(rA, wm.normal);
+        float rnB = crs(rB, wm.normal);
+        float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        nm[j] = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+        float rtA = crs(rA, tangent);
+        float rtB = crs(rB, tangent);
+        float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+        tm[j] = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+        bias[j] = 0.0f;
+        float vRel = dot(wm.normal, sub(sub(add(vB, crs(wB, rB)), vA), crs(wA, rA)));
+        if (vRel < -kVelocityThreshold) bias[j] = -restitution * vRel;
+        rAs[j] = rA;
+        rBs[j] = rB;
+    }
+    v.rA0 = rAs[0];
+    v.rB0 = rBs[0];
+    v.nMass0 = nm[0];
+    v.tMass0 = tm[0];
+    v.bias0 = bias[0];
+    v.pointCount = count;
+    v.K = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    v.normalMass = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (count == 2) {
+        v.rA1 = rAs[1];
+        v.rB1 = rBs[1];
+        v.nMass1 = nm[1];
+        v.tMass1 = tm[1];
+        v.bias1 = bias[1];
+        float rn1A = crs(rAs[0], wm.normal);
+        float rn1B = crs(rBs[0], wm.normal);
+        float rn2A = crs(rAs[1], wm.normal);
+        float rn2B = crs(rBs[1], wm.normal);
+        float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
+        float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
+        float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
+        const float k_maxConditionNumber = 1000.0f;
+        if (k11 * k11 < k_maxConditionNumber * (k11 * k22 - k12 * k12)) {
+            v.K = make_float4(k11, k12, k12, k22);
+            // b2Mat22.GetInverse (b2math.d:322-337)
+            float a = k11, b = k12, cc = k12, dd = k22;
+            float det = a * dd - b * cc;
+            if (det != 0.0f) det = 1.0f / det;
+            v.normalMass = make_float4(det * dd, -det * cc, -det * b, det * a);
+        } else {
+            v.pointCount = 1;
+        }
+    } else {
+        v.rA1 = v2(0.0f, 0.0f);
+        v.rB1 = v2(0.0f, 0.0f);
+        v.nMass1 = 0.0f;
+        v.tMass1 = 0.0f;
+        v.bias1 = 0.0f;
+    }
+}
+
+// b2ContactSolver.WarmStart (b2contactsolver.d:452-490) for one constraint.
+__device__ __forceinline__ void warm_start(const VelCon &v, float4 *bvel) {
+    float4 va = bvel[v.indexA], vb = bvel[v.indexB];
+    float2 vA = v2(va.x, va.y), vB = v2(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+    float2 normal = v.normal;
+    float2 tangent = crs(normal, 1.0f);
+    float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
+    for (int j = 0; j < v.pointCount; ++j) {
+        float nI = j == 0 ? v.nImp0 : v.nImp1;
+        float tI = j == 0 ? v.tImp0 : v.tImp1;
+        float2 rA = j == 0 ? v.rA0 : v.rA1;
+        float2 rB = j == 0 ? v.rB0 : v.rB1;
+        float2 P = add(scl(nI, normal), scl(tI, tangent));
+        wA -= iA * crs(rA, P);
+        vA = sub(vA, scl(mA, P));
+        wB += iB * crs(rB, P);
+        vB = add(vB, scl(mB, P));
+    }
+    if (v.dynMask & 1) bvel[v.indexA] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (v.dynMask & 2) bvel[v.indexB] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+// b2ContactSolver.SolveVelocityConstraints (b2contactsolver.d:492-772) for one
+// constraint; returns the constraint's velocity residual.
+__device__ float solve_velocity(VelCon &v, float4 *bvel, bool wantResidual) {
+    float4 va = bvel[v.indexA], vb = bvel[v.indexB];
+    float2 vA = v2(va.x, va.y), vB = v2(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+    float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
+    float2 normal = v.normal;
+    float2 tangent = crs(normal, 1.0f);
+    float friction = v.friction;
+    int pc = v.pointCount;
+    float res = 0.0f;
+    // friction
+    for (int j = 0; j < pc; ++j) {
+        float2 rA = j == 0 ? v.rA0 : v.rA1;
+        float2 rB = j == 0 ? v.rB0 : v.rB1;
+        float tMass = j == 0 ? v.tMass0 : v.tMass1;
+        float nI = j == 0 ? v.nImp0 : v.nImp1;
+        float tI = j == 0 ? v.tImp0 : v.tImp1;
+        float2 dv = sub(sub(add(vB, crs(wB, rB)), vA), crs(wA, rA));
+        float vt = dot(dv, tangent) - v.tangentSpeed;
+        float lambda = tMass * (-vt);
+        float maxFriction = friction * nI;
+        float newImpulse = clampr(tI + lambda, -maxFriction, maxFriction);
+        if (wantResidual && fabsr(tI) < maxFriction) res = fmaxr(res, fabsr(vt));
+        lambda = newImpulse - tI;
+        if (j == 0) v.tImp0 = newImpulse;
+        else v.tImp1 = newImpulse;
+        float2 P = scl(lambda, tangent);
+        vA = sub(vA, scl(mA, P));
+        wA -= iA * crs(rA, P);
+        vB = add(vB, scl(mB, P));
+        wB += iB * crs(rB, P);
+    }
+    if (pc == 1) {
+        float2 dv = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
+        float vn = dot(dv, normal);
+        float lambda = -v.nMass0 * (vn - v.bias0);
+        float newImpulse = fmaxr(v.nImp0 + lambda, 0.0f);
+        if (wantResidual) {
+            float e = vn - v.bias0;
+            res = fmaxr(res, v.nImp0 > 0.0f ? fabsr(e) : fmaxr(0.0f, -e));
+        }
+        lambda = newImpulse - v.nImp0;
+        v.nImp0 = newImpulse;
+        float2 P = scl(lambda, normal);
+        vA = sub(vA, scl(mA, P));
+        wA -= iA * crs(v.rA0, P);
+        vB = add(vB, scl(mB, P));
+        wB += iB * crs(v.rB0, P);
+    } else {
+        // 2-point block solver by total enumeration
+        float2 a = v2(v.nImp0, v.nImp1);
+        float2 dv1 = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
+        float2 dv2 = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
+        float vn1 = dot(dv1, normal);
+        float vn2 = dot(dv2, normal);
+        float2 b;
+        b.x = vn1 - v.bias0;
+        b.y = vn2 - v.bias1;
+        if (wantResidual) {
+            res = fmaxr(res, v.nImp0 > 0.0f ? fabsr(b.x) : fmaxr(0.0f, -b.x));
+            res = fmaxr(res, v.nImp1 > 0.0f ? fabsr(b.y) : fmaxr(0.0f, -b.y));
+        }
+        // b -= K * a
+        float Kax = v.K.x * a.x + v.K.z * a.y;
+        float Kay = v.K.y * a.x + v.K.w * a.y;
+        b.x -= Kax;
+        b.y -= Kay;
+        float2 x;
+        bool done = false;
+        // case 1: both points active
+        x = v2(-(v.normalMass.x * b.x + v.normalMass.z * b.y), -(v.normalMass.y * b.x + v.normalMass.w * b.y));
+        if (x.x >= 0.0f && x.y >= 0.0f) done = true;
+        if (!done) { // case 2: point 1 only
+            x.x = -v.nMass0 * b.x;
+            x.y = 0.0f;
+            vn2 = v.K.y * x.x + b.y;
+            if (x.x >= 0.0f && vn2 >= 0.0f) done = true;
+        }
+        if (!done) { // case 3: point 2 only
+            x.x = 0.0f;
+            x.y = -v.nMass1 * b.y;
+            vn1 = v.K.z * x.y + b.x;
+            if (x.y >= 0.0f && vn1 >= 0.0f) done = true;
+        }
+        if (!done) { // case 4: neither
+            x.x = 0.0f;
+            x.y = 0.0f;
+            vn1 = b.x;
+            vn2 = b.y;
+            if (vn1 >= 0.0f && vn2 >= 0.0f) done = true;
+        }
+        if (done) {
+            float2 dd = sub(x, a);
+            float2 P1 = scl(dd.x, normal);
+            float2 P2 = scl(dd.y, normal);
+            vA = sub(vA, scl(mA, add(P1, P2)));
+            wA -= iA * (crs(v.rA0, P1) + crs(v.rA1, P2));
+            vB = add(vB, scl(mB, add(P1, P2)));
+            wB += iB * (crs(v.rB0, P1) + crs(v.rB1, P2));
+            v.nImp0 = x.x;
+            v.nImp1 = x.y;
+        }
+    }
+    if (v.dynMask & 1) bvel[v.indexA] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (v.dynMask & 2) bvel[v.indexB] = make_float4(vB.x, vB.y, wB, 0.0f);
+    return res;
+}
+
+// b2ContactSolver.SolvePositionConstraints (b2contactsolver.d:73-149) for one
+// constraint; returns the minimum separation seen.
+__device__ float solve_position(const PosCon &p, int indexA, int indexB, int dynMask, float4 *bpos) {
+    float4 pa = bpos[indexA], pb = bpos[indexB];
+    float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
+    float aA = pa.z, aB = pb.z;
+    float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
+    float minSep = 0.0f;
+    for (int j = 0; j < p.pointCount; ++j) {
+        Xf xfA = xf_from(cA, aA, p.lcA);
+        Xf xfB = xf_from(cB, aB, p.lcB);
+        // b2PositionSolverManifold.Initialize (b2contactsolver.d:818-863)
+        float2 normal, point;
+        float separation;
+        if (p.type == kManCircles) {
+            float2 pointA = mul(xfA, p.localPoint);
+            float2 pointB = mul(xfB, p.lp0);
+            normal = sub(pointB, pointA);
+            normalize(normal);
+            point = scl(0.5f, add(pointA, pointB));
+            separation = dot(sub(pointB, pointA), normal) - p.radiusA - p.radiusB;
+        } else if (p.type == kManFaceA) {
+            normal = mul(xfA.q, p.localNormal);
+            float2 planePoint = mul(xfA, p.localPoint);
+            float2 clipPoint = mul(xfB, j == 0 ? p.lp0 : p.lp1);
+            separation = dot(sub(clipPoint, planePoint), normal) - p.radiusA - p.radiusB;
+            point = clipPoint;
+        } else {
+            normal = mul(xfB.q, p.localNormal);
+            float2 planePoint = mul(xfB, p.localPoint);
+            float2 clipPoint = mul(xfA, j == 0 ? p.lp0 : p.lp1);
+            separation = dot(sub(clipPoint, planePoint), normal) - p.radiusA - p.radiusB;
+            point = clipPoint;
+            normal = neg(normal);
+        }
+        float2 rA = sub(point, cA);
+        float2 rB = sub(point, cB);
+        minSep = fminr(minSep, separation);
+        float C = clampr(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+        float rnA = crs(rA, normal);
+        float rnB = crs(rB, normal);
+        float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        float impulse = K > 0.0f ? -C / K : 0.0f;
+        float2 P = scl(impulse, normal);
+        cA = sub(cA, scl(mA, P));
+        aA -= iA * crs(rA, P);
+        cB = add(cB, scl(mB, P));
+        aB += iB * crs(rB, P);
+    }
+    if (dynMask & 1) bpos[indexA] = make_float4(cA.x, cA.y, aA, pa.w);
+    if (dynMask & 2) bpos[indexB] = make_float4(cB.x, cB.y, aB, pb.w);
+    return minSep;
+}
+
+__global__ void k_solve(Dev d, int cur, StepParams sp) {
+    __shared__ float s_res;
+    const int w = d.order[blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int nb = wd.bodyCount, nf = wd.fixCount;
+    const int nc = d.S[w].constraints;
+    const int nbatch = d.S[w].batchCount;
+    const int nisl = d.S[w].islandCount;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bxf = d.bxf + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    float4 *bpos0 = d.bpos0 + wd.bodyBase;
+    float4 *bvel = d.bvel + wd.bodyBase;
+    float4 *bforce = d.bforce + wd.bodyBase;
+    const float4 *bmass = d.bmass + wd.bodyBase;
+    const float4 *bparam = d.bparam + wd.bodyBase;
+    const int *bisl = d.bisl + wd.bodyBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    float4 *fat = d.ffat + wd.fixBase;
+    int *moved = d.fmoved + wd.fixBase;
+    const int *ordC = d.ordC + wd.contactBase;
+    const int *cisl = d.cisl + wd.contactBase;
+    const int *sched = d.sched + wd.contactBase;
+    const int *boff = d.batchOff + wd.contactBase + 65 * w;
+    VelCon *vc = d.vc + wd.contactBase;
+    PosCon *pc = d.pc + wd.contactBase;
+    int *islMin = d.islMin + wd.islandBase;
+    int *islSolved = d.islSolved + wd.islandBase;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const float h = sp.dt;
+
+    // Integrate velocities, damping (b2island.d:82-116).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b];
+        bpos0[b] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+        uint32_t f = bfl[b];
+        if (body_type(f) != kDynamic) continue;
+        float4 vel = bvel[b], frc = bforce[b], ms = bmass[b], pr = bparam[b];
+        float2 v = v2(vel.x, vel.y);
+        float wv = vel.z;
+        float2 acc = add(scl(pr.z, v2(wd.gx, wd.gy)), scl(ms.x, v2(frc.x, frc.y)));
+        v = add(v, scl(h, acc));
+        wv += h * ms.y * frc.z;
+        float ld = 1.0f / (1.0f + h * pr.x);
+        float ad = 1.0f / (1.0f + h * pr.y);
+        v = v2(v.x * ld, v.y * ld);
+        wv *= ad;
+        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+    }
+    __syncthreads();
+
+    // Constraint set-up.
+    for (int k = tid; k < nc; k += BS) {
+        int ci = ordC[k];
+        VelCon v;
+        PosCon p;
+        init_constraint(v, p, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp);
+        vc[k] = v;
+        pc[k] = p;
+    }
+    __syncthreads();
+
+    if (sp.flags & 1u) {
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) warm_start(vc[sched[j]], bvel);
+            __syncthreads();
+        }
+    }
+
+    // Velocity iterations (b2island.d:153-161); with the residual flag the
+    // loop runs until the velocity residual falls below the tolerance.
+    const bool residualMode = (sp.flags & (1u << 5)) != 0;
+    int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
+    int used = 0;
+    for (int it = 0; it < iters; ++it) {
+        if (tid == 0) s_res = 0.0f;
+        __syncthreads();
+        float lres = 0.0f;
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) {
+                int k = sched[j];
+                VelCon v = vc[k];
+                float r = solve_velocity(v, bvel, residualMode);
+                lres = fmaxr(lres, r);
+                *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+            }
+            __syncthreads();
+        }
+        ++used;
+        if (residualMode) {
+            atomicMax((int *)&s_res, __float_as_int(lres));
+            __syncthreads();
+            float r = s_res;
+            __syncthreads();
+            if (r <= sp.residualTol) break;
+        }
+    }
+    if (tid == 0) d.S[w].iterations = used;
+
+    // StoreImpulses (b2contactsolver.d:774-787)
+    for (int k = tid; k < nc; k += BS) {
+        const VelCon &v = vc[k];
+        Contact *c = cont + v.contact;
+        c->pt0.z = v.nImp0;
+        c->pt0.w = v.tImp0;
+        if (v.pointCount > 1) {
+            c->pt1.z = v.nImp1;
+            c->pt1.w = v.tImp1;
+        }
+    }
+    // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b], vel = bvel[b];
+        float2 v = v2(vel.x, vel.y);
+        float wv = vel.z;
+        float2 tr = scl(h, v);
+        if (dot(tr, tr) > kMaxTranslationSquared) {
+            float ratio = kMaxTranslation / sqrtf(tr.x * tr.x + tr.y * tr.y);
+            v = v2(v.x * ratio, v.y * ratio);
+        }
+        float rotation = h * wv;
+        if (rotation * rotation > kMaxRotationSquared) {
+            float ratio = kMaxRotation / fabsr(rotation);
+            wv *= ratio;
+        }
+        bpos[b] = make_float4(pos.x + h * v.x, pos.y + h * v.y, pos.z + h * wv, pos.w);
+        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+    }
+    for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
+    __syncthreads();
+
+    // Position iterations with the per-island early exit (b2island.d:206-224).
+    for (int it = 0; it < sp.positionIterations; ++it) {
+        for (int i = tid; i < nisl; i += BS) islMin[i] = f2ord(0.0f);
+        __syncthreads();
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) {
+                int k = sched[j];
+                const PosCon p = pc[k];
+                int isl = p.island;
+                if (islSolved[isl]) continue;
+                float ms = solve_position(p, p.indexA, p.indexB, p.dynMask, bpos);
+                atomicMin(&islMin[isl], f2ord(ms));
+            }
+            __syncthreads();
+        }
+        int open = 0;
+        for (int i = tid; i < nisl; i += BS) {
+            if (islSolved[i]) continue;
+            if (ord2f(islMin[i]) >= -3.0f * kLinearSlop) islSolved[i] = 1;
+            else open = 1;
+        }
+        if (!__syncthreads_or(open)) break;
+    }
+    __syncthreads();
+
+    // Copy back and SynchronizeTransform (b2island.d:227-235, b2body.d:1143-1147).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b];
+        float4 ms = bmass[b];
+        Xf x = xf_from(v2(pos.x, pos.y), pos.z, v2(ms.z, ms.w));
+        bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+    }
+    // Sleep (b2island.d:241-279).
+    if (sp.flags & 2u) {
+        for (int i = tid; i < nisl; i += BS) islMin[i] = __float_as_int(kMaxFloat);
+        __syncthreads();
+        const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
+        const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
+        for (int b = tid; b < nb; b += BS) {
+            int isl = bisl[b];
+            if (isl < 0) continue;
+            uint32_t f = bfl[b];
+            if (body_type(f) == kStatic) continue;
+            float4 vel = bvel[b];
+            float4 pos = bpos[b];
+            if ((f & kBodyAutoSleep) == 0 || vel.z * vel.z > angTolSqr || (vel.x * vel.x + vel.y * vel.y) > linTolSqr) {
+                pos.w = 0.0f;
+                atomicMin(&islMin[isl], 0);
+            } else {
+                pos.w += h;
+                atomicMin(&islMin[isl], __float_as_int(pos.w));
+            }
+            bpos[b] = pos;
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += BS) {
+            int isl = bisl[b];
+            if (isl < 0) continue;
+            if (__int_as_float(islMin[isl]) >= kTimeToSleep && islSolved[isl]) {
+                // SetAwake(false) (b2body.d:836-845)
+                bfl[b] &= ~kBodyAwake;
+                bpos[b].w = 0.0f;
+                bvel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+
+    // SynchronizeFixtures + fat-AABB rule of b2DynamicTree.MoveProxy
+    // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
+    for (int f = tid; f < nf; f += BS) {
+        int b = fbody[f];
+        if (bisl[b] < 0 || body_type(bfl[b]) == kStatic) continue;
+        float4 p0 = bpos0[b];
+        float4 ms = bmass[b];
+        Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
+        Xf x2 = load_xf(bxf, b);
+        float4 a1 = shape_aabb(shp + f, x1);
+        float4 a2 = shape_aabb(shp + f, x2);
+        float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+        float2 disp = sub(x2.p, x1.p);
+        float4 fa = fat[f];
+        bool contains = fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w;
+        if (contains) continue;
+        float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                 aabb.w + kAabbExtension);
+        float2 dd = scl(kAabbMultiplier, disp);
+        if (dd.x < 0.0f) nbx.x += dd.x;
+        else nbx.z += dd.x;
+        if (dd.y < 0.0f) nbx.y += dd.y;
+        else nbx.w += dd.y;
+        fat[f] = nbx;
+        moved[f] = 1;
+    }
+    // b2World.ClearForces (b2world.d:443-450)
+    for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase: b2BroadPhase.UpdatePairs (b2broadphase.d:139-195) followed by
+// b2ContactManager.AddPair (b2contactmanager.d:52-176) for each unique pair.
+// The pair set is every (i < j) whose fat AABBs overlap with i or j moved,
+// which is exactly what the tree queries plus sort and de-duplication give.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int count_pairs_of(int i, float4 fi, int mi, const float4 *fat, const int *moved, int n,
+                                              const float4 *tileFat, const int *tileMoved, int tileBase, int tileLen) {
+    int c = 0;
+    for (int t = 0; t < tileLen; ++t) {
+        int j = tileBase + t;
+        if (j <= i) continue;
+        if ((mi || tileMoved[t]) && overlap(fi, tileFat[t])) ++c;
+    }
+    return c;
+}
+
+// Pair counting / emission over LDS tiles of the world's fat AABBs.
+// mode 0: count pairs per proxy into cnt[]; mode 1: write pairs at offsets cnt[].
+__device__ void pair_pass(int mode, const float4 *fat, const int *moved, int n, int *cnt, int2 *out, float4 *tileFat,
+                          int *tileMoved) {
+    const int BS = blockDim.x, tid = threadIdx.x;
+    for (int c0 = 0; c0 < n; c0 += BS) {
+        int i = c0 + tid;
+        float4 fi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int mi = 0;
+        if (i < n) {
+            fi = fat[i];
+            mi = moved[i];
+        }
+        int count = 0;
+        int wpos = (mode == 1 && i < n) ? cnt[i] : 0;
+        for (int t0 = c0; t0 < n; t0 += BS) {
+            __syncthreads();
+            int j = t0 + tid;
+            if (j < n) {
+                tileFat[tid] = fat[j];
+                tileMoved[tid] = moved[j];
+            }
+            __syncthreads();
+            if (i < n) {
+                int len = n - t0 < BS ? n - t0 : BS;
+                for (int t = 0; t < len; ++t) {
+                    int jj = t0 + t;
+                    if (jj <= i) continue;
+                    if ((mi || tileMoved[t]) && overlap(fi, tileFat[t])) {
+                        if (mode == 1) out[wpos++] = make_int2(i, jj);
+                        ++count;
+                    }
+                }
+            }
+        }
+        if (mode == 0 && i < n) cnt[i] = count;
+    }
+    __syncthreads();
+}
+
+__global__ void k_broadphase(Dev d, int cur) {
+    __shared__ int s_tmp[kMaxBlock / 64 + 1];
+    __shared__ float4 s_fat[kMaxBlock];
+    __shared__ int s_moved[kMaxBlock];
+    const int w = d.order[blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int nf = wd.fixCount;
+    const float4 *fat = d.ffat + wd.fixBase;
+    int *moved = d.fmoved + wd.fixBase;
+    int *cnt = d.fcount + wd.fixBase;
+    int2 *pairs = d.pairs + wd.pairBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    const int2 *ffilter = d.ffilter + wd.fixBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
+    const int *adj = d.adjList + 2 * wd.contactBase;
+    const int tid = threadIdx.x, BS = blockDim.x;
+
+    pair_pass(0, fat, moved, nf, cnt, nullptr, s_fat, s_moved);
+    int total = block_scan_array(cnt, nf, s_tmp);
+    if (total > wd.pairCap) {
+        if (tid == 0) {
+            d.S[w].error |= 2;
+            d.S[w].pairCount = 0;
+        }
+        for (int f = tid; f < nf; f += BS) moved[f] = 0;
+        return;
+    }
+    pair_pass(1, fat, moved, nf, cnt, pairs, s_fat, s_moved);
+
+    // AddPair filters; accepted pairs become contacts in pair order.
+    const int base = d.S[w].contactCount;
+    int created = 0;
+    for (int c0 = 0; c0 < total; c0 += BS) {
+        int k = c0 + tid;
+        int accept = 0;
+        int fa = 0, fb = 0;
+        if (k < total) {
+            int2 pr = pairs[k];
+            fa = pr.x;
+            fb = pr.y;
+            int ba = fbody[fa], bb = fbody[fb];
+            accept = ba != bb;
+            if (accept) { // does a contact already exist? (scan of bodyB's contact list)
+                int o = adjOff[bb], n = adjCnt[bb];
+                for (int e = 0; e < n; ++e) {
+                    const Contact &c = cont[adj[o + e] >> 1];
+                    if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) {
+                        accept = 0;
+                        break;
+                    }
+                }
+            }
+            if (accept) { // b2Body.ShouldCollide: at least one body dynamic
+                accept = body_type(bfl[ba]) == kDynamic || body_type(bfl[bb]) == kDynamic;
+            }
+            if (accept) { // b2ContactFilter.ShouldCollide (b2worldcallbacks.d:52-65)
+                int2 A = ffilter[fa], B = ffilter[fb];
+                int ga = A.y, gb = B.y;
+                if (ga == gb && ga != 0) accept = ga > 0;
+                else accept = ((A.x >> 16) & (B.x & 0xffff)) != 0 && ((A.x & 0xffff) & (B.x >> 16)) != 0;
+            }
+            if (accept) accept = pair_has_type(shp[fa].type, shp[fb].type);
+        }
+        int tot;
+        int off = block_excl_scan(accept, tot, s_tmp);
+        if (accept) {
+            int slot = base + created + off;
+            if (slot < wd.contactCap) {
+                if (!pair_is_primary(shp[fa].type, shp[fb].type)) {
+                    int t = fa;
+                    fa = fb;
+                    fb = t;
+                }
+                Contact c;
+                c.fa = fa;
+                c.fb = fb;
+                c.flags = kContactEnabled;
+                c.meta = 0;
+                c.friction = sqrtf(fmat[fa].x * fmat[fb].x);        // b2MixFriction
+                float ra = fmat[fa].y, rb = fmat[fb].y;
+                c.restitution = ra > rb ? ra : rb;                    // b2MixRestitution
+                c.id0 = 0;
+                c.id1 = 0;
+                c.man = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                cont[slot] = c;
+                wake_body(bfl, bpos, fbody[fa]);
+                wake_body(bfl, bpos, fbody[fb]);
+            }
+        }
+        created += tot;
+    }
+    __syncthreads();
+    for (int f = tid; f < nf; f += BS) moved[f] = 0;
+    if (tid == 0) {
+        int nc = base + created;
+        if (nc > wd.contactCap) {
+            d.S[w].error |= 1;
+            nc = wd.contactCap;
+        }
+        d.S[w].contactCount = nc;
+        d.S[w].pairCount = total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone kernels used by the tests.
+// ---------------------------------------------------------------------------
+__global__ void k_bp_single(const float4 *fat, const int *moved, int n, int *cnt, int2 *out, int cap, int *total) {
+    __shared__ int s_tmp[kMaxBlock / 64 + 1];
+    __shared__ float4 s_fat[kMaxBlock];
+    __shared__ int s_moved[kMaxBlock];
+    pair_pass(0, fat, moved, n, cnt, nullptr, s_fat, s_moved);
+    int t = block_scan_array(cnt, n, s_tmp);
+    if (threadIdx.x == 0) *total = t;
+    if (t > cap) return;
+    pair_pass(1, fat, moved, n, cnt, out, s_fat, s_moved);
+}
+
+__global__ void k_collide_pairs(const Shape *sa, const float *xa, const Shape *sb, const float *xb, int n, float *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float *o = out + 32 * i;
+    for (int k = 0; k < 32; ++k) o[k] = 0.0f;
+    const Shape *A = sa + i, *B = sb + i;
+    Xf xfA, xfB;
+    xfA.p = v2(xa[3 * i], xa[3 * i + 1]);
+    xfA.q = rot(xa[3 * i + 2]);
+    xfB.p = v2(xb[3 * i], xb[3 * i + 1]);
+    xfB.q = rot(xb[3 * i + 2]);
+    if (!pair_has_type(A->type, B->type)) {
+        o[0] = -1.0f;
+        return;
+    }
+    int swapped = 0;
+    if (!pair_is_primary(A->type, B->type)) {
+        const Shape *t = A;
+        A = B;
+        B = t;
+        Xf tx = xfA;
+        xfA = xfB;
+        xfB = tx;
+        swapped = 1;
+    }
+    Manifold m;
+    m.type = 0;
+    m.count = 0;
+    m.ln = m.lp = m.p[0] = m.p[1] = v2(0.0f, 0.0f);
+    m.id[0] = m.id[1] = 0;
+    evaluate(m, A, xfA, B, xfB);
+    o[0] = (float)m.type;
+    o[1] = (float)m.count;
+    o[2] = (float)swapped;
+    o[4] = m.ln.x;
+    o[5] = m.ln.y;
+    o[6] = m.lp.x;
+    o[7] = m.lp.y;
+    o[8] = m.p[0].x;
+    o[9] = m.p[0].y;
+    o[10] = m.p[1].x;
+    o[11] = m.p[1].y;
+    o[12] = __uint_as_float(m.id[0]);
+    o[13] = __uint_as_float(m.id[1]);
+    if (m.count > 0) {
+        WorldManifold wm;
+        world_manifold(wm, m.type, m.count, m.ln, m.lp, m.p, xfA, A->radius, xfB, B->radius);
+        o[14] = wm.normal.x;
+        o[15] = wm.normal.y;
+        o[16] = wm.points[0].x;
+        o[17] = wm.points[0].y;
+        o[18] = m.count > 1 ? wm.points[1].x : 0.0f;
+        o[19] = m.count > 1 ? wm.points[1].y : 0.0f;
+        o[20] = wm.sep[0];
+        o[21] = m.count > 1 ? wm.sep[1] : 0.0f;
+    }
+}
+
+// Maximum penetration per world over touching contacts (-min world-manifold separation).
+__global__ void k_max_penetration(Dev d, int cur, float *out) {
+    __shared__ int s_min;
+    const int w = blockIdx.x;
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount;
+    const Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    const float4 *bxf = d.bxf + wd.bodyBase;
+    if (threadIdx.x == 0) s_min = f2ord(0.0f);
+    __syncthreads();
+    float lmin = 0.0f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        Contact c = cont[i];
+        if ((c.flags & kContactTouching) == 0) continue;
+        Xf xfA = load_xf(bxf, fbody[c.fa]), xfB = load_xf(bxf, fbody[c.fb]);
+        WorldManifold wm;
+        float2 pts[2] = {v2(c.pt0.x, c.pt0.y), v2(c.pt1.x, c.pt1.y)};
+        int cnt = man_count(c.meta);
+        world_manifold(wm, man_type(c.meta), cnt, v2(c.man.x, c.man.y), v2(c.man.z, c.man.w), pts, xfA, fmat[c.fa].w,
+                       xfB, fmat[c.fb].w);
+        for (int j = 0; j < cnt; ++j) lmin = fminr(lmin, wm.sep[j]);
+    }
+    atomicMin(&s_min, f2ord(lmin));
+    __syncthreads();
+    if (threadIdx.x == 0) out[w] = -ord2f(s_min);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_collide(const Dev &d, int cur, int block, hipStream_t s) {
+    hipLaunchKernelGGL(k_collide, dim3(d.nworlds), dim3(block), 0, s, d, cur);
+    return hipGetLastError();
+}
+hipError_t launch_islands(const Dev &d, int cur, int ordered, int block, hipStream_t s) {
+    hipLaunchKernelGGL(k_islands, dim3(d.nworlds), dim3(block), 0, s, d, cur, ordered);
+    return hipGetLastError();
+}
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int block, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve, dim3(d.nworlds), dim3(block), 0, s, d, cur, sp);
+    return hipGetLastError();
+}
+hipError_t launch_broadphase(const Dev &d, int cur, int block, hipStream_t s) {
+    hipLaunchKernelGGL(k_broadphase, dim3(d.nworlds), dim3(block), 0, s, d, cur);
+    return hipGetLastError();
+}
+hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, int *cnt, int2 *out, int cap, int *total,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_bp_single, dim3(1), dim3(256), 0, s, fat, moved, n, cnt, out, cap, total);
+    return hipGetLastError();
+}
+hipError_t launch_collide_pairs(const Shape *sa, const float *xa, const Shape *sb, const float *xb, int n, float *out,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_collide_pairs, dim3((n + 127) / 128), dim3(128), 0, s, sa, xa, sb, xb, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_max_penetration(const Dev &d, int cur, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_penetration, dim3(d.nworlds), dim3(256), 0, s, d, cur, out);
+    return hipGetLastError();
+}
+
+} // namespace dbx

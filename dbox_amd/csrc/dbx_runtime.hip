@@ -1,0 +1,989 @@
+// dbx_runtime.hip -- host runtime behind include/dbox_hip.h.
+//
+// World construction mirrors what b2World.CreateBody and b2Body.CreateFixture do
+// in dbox (hull, normals, centroid, mass data, initial fat AABBs), so that a
+// world added here starts from the same state as the reference world built from
+// the same definitions.  Those host computations keep the reference's float
+// operation order and are compiled without FMA contraction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dbox_hip.h"
+#include "dbx_device.h"
+#include "dbx_kernels.h"
+#include "dbx_mlp.h"
+
+using namespace dbx;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// host geometry (creation time only)
+// ---------------------------------------------------------------------------
+struct HV {
+    float x, y;
+};
+inline HV hv(float x, float y) { return HV{x, y}; }
+inline HV operator+(HV a, HV b) { return hv(a.x + b.x, a.y + b.y); }
+inline HV operator-(HV a, HV b) { return hv(a.x - b.x, a.y - b.y); }
+inline HV operator*(float s, HV a) { return hv(s * a.x, s * a.y); }
+inline float hdot(HV a, HV b) { return a.x * b.x + a.y * b.y; }
+inline float hcross(HV a, HV b) { return a.x * b.y - a.y * b.x; }
+inline HV hmin(HV a, HV b) { return hv(a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y); }
+inline HV hmax(HV a, HV b) { return hv(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y); }
+inline float hnormalize(HV &v) {
+    float len = sqrtf(v.x * v.x + v.y * v.y);
+    if (len < kEps) return 0.0f;
+    float inv = 1.0f / len;
+    v.x *= inv;
+    v.y *= inv;
+    return len;
+}
+struct HXf {
+    HV p;
+    float s, c;
+};
+inline HXf hxf(HV p, float angle) { return HXf{p, sinf(angle), cosf(angle)}; }
+inline HV hrot(const HXf &x, HV v) { return hv(x.c * v.x - x.s * v.y, x.s * v.x + x.c * v.y); }
+inline HV hmul(const HXf &x, HV v) { return hv((x.c * v.x - x.s * v.y) + x.p.x, (x.s * v.x + x.c * v.y) + x.p.y); }
+
+void set_box(Shape &s, float hx, float hy) { // b2PolygonShape.SetAsBox (b2polygonshape.d:220-232)
+    s.type = kPolygon;
+    s.radius = kPolygonRadius;
+    s.count = 4;
+    const float vx[4] = {-hx, hx, hx, -hx}, vy[4] = {-hy, -hy, hy, hy};
+    const float nx[4] = {0.0f, 1.0f, 0.0f, -1.0f}, ny[4] = {-1.0f, 0.0f, 1.0f, 0.0f};
+    for (int i = 0; i < 4; ++i) {
+        s.v[i] = make_float2(vx[i], vy[i]);
+        s.n[i] = make_float2(nx[i], ny[i]);
+    }
+    s.c = make_float2(0.0f, 0.0f);
+}
+
+HV centroid_of(const HV *vs, int n) { // b2PolygonShape.ComputeCentroid (b2polygonshape.d:505-549)
+    HV c = hv(0.0f, 0.0f);
+    float area = 0.0f;
+    const float inv3 = 1.0f / 3.0f;
+    const HV o = hv(0.0f, 0.0f);
+    for (int i = 0; i < n; ++i) {
+        HV p2 = vs[i], p3 = i + 1 < n ? vs[i + 1] : vs[0];
+        HV e1 = p2 - o, e2 = p3 - o;
+        float tri = 0.5f * hcross(e1, e2);
+        area += tri;
+        c = c + (tri * inv3) * (o + p2 + p3);
+    }
+    float inv = 1.0f / area;
+    return hv(c.x * inv, c.y * inv);
+}
+
+// b2PolygonShape.Set: weld, gift-wrap hull, normals, centroid (b2polygonshape.d:74-209).
+bool set_polygon(Shape &s, const float *pts, int count) {
+    s.type = kPolygon;
+    s.radius = kPolygonRadius;
+    if (count < 3 || count > kMaxPolygonVertices) return false;
+    HV ps[kMaxPolygonVertices];
+    int n = 0;
+    for (int i = 0; i < count; ++i) {
+        HV v = hv(pts[2 * i], pts[2 * i + 1]);
+        bool unique = true;
+        for (int j = 0; j < n && unique; ++j) {
+            HV d = v - ps[j];
+            if (hdot(d, d) < 0.5f * kLinearSlop) unique = false;
+        }
+        if (unique) ps[n++] = v;
+    }
+    if (n < 3) return false;
+    int i0 = 0;
+    float x0 = ps[0].x;
+    for (int i = 1; i < n; ++i) {
+        float x = ps[i].x;
+        if (x > x0 || (x == x0 && ps[i].y < ps[i0].y)) {
+            i0 = i;
+            x0 = x;
+        }
+    }
+    int hull[kMaxPolygonVertices];
+    int m = 0, ih = i0;
+    while (true) {
+        hull[m] = ih;
+        int ie = 0;
+        for (int j = 1; j < n; ++j) {
+            if (ie == ih) {
+                ie = j;
+                continue;
+            }
+            HV r = ps[ie] - ps[hull[m]];
+            HV v = ps[j] - ps[hull[m]];
+            float c = hcross(r, v);
+            if (c < 0.0f) ie = j;
+            if (c == 0.0f && hdot(v, v) > hdot(r, r)) ie = j;
+        }
+        ++m;
+        ih = ie;
+        if (ie == i0 || m >= kMaxPolygonVertices) break;
+    }
+    if (m < 3) return false;
+    HV vs[kMaxPolygonVertices];
+    for (int i = 0; i < m; ++i) vs[i] = ps[hull[i]];
+    s.count = m;
+    for (int i = 0; i < m; ++i) {
+        HV e = vs[i + 1 < m ? i + 1 : 0] - vs[i];
+        HV nrm = hv(1.0f * e.y, -1.0f * e.x);
+        hnormalize(nrm);
+        s.v[i] = make_float2(vs[i].x, vs[i].y);
+        s.n[i] = make_float2(nrm.x, nrm.y);
+    }
+    HV c = centroid_of(vs, m);
+    s.c = make_float2(c.x, c.y);
+    return true;
+}
+
+struct HMass {
+    float mass;
+    HV center;
+    float I;
+};
+// b2PolygonShape.ComputeMass / b2CircleShape.ComputeMass / b2EdgeShape.ComputeMass.
+HMass shape_mass(const Shape &s, float density) {
+    HMass md{0.0f, hv(0.0f, 0.0f), 0.0f};
+    if (s.type == kCircle) {
+        HV p = hv(s.c.x, s.c.y);
+        md.mass = density * kPi * s.radius * s.radius;
+        md.center = p;
+        md.I = md.mass * (0.5f * s.radius * s.radius + hdot(p, p));
+        return md;
+    }
+    if (s.type == kEdge) {
+        md.center = 0.5f * (hv(s.v[0].x, s.v[0].y) + hv(s.v[1].x, s.v[1].y));
+        return md;
+    }
+    int n = s.count;
+    HV ref = hv(0.0f, 0.0f);
+    for (int i = 0; i < n; ++i) ref = ref + hv(s.v[i].x, s.v[i].y);
+    float invn = 1.0f / (float)n;
+    ref = hv(ref.x * invn, ref.y * invn);
+    HV center = hv(0.0f, 0.0f);
+    float area = 0.0f, I = 0.0f;
+    const float k_inv3 = 1.0f / 3.0f;
+    for (int i = 0; i < n; ++i) {
+        int j = i + 1 < n ? i + 1 : 0;
+        HV e1 = hv(s.v[i].x, s.v[i].y) - ref;
+        HV e2 = hv(s.v[j].x, s.v[j].y) - ref;
+        float D = hcross(e1, e2);
+        float tri = 0.5f * D;
+        area += tri;
+        center = center + (tri * k_inv3) * (e1 + e2);
+        float intx2 = e1.x * e1.x + e2.x * e1.x + e2.x * e2.x;
+        float inty2 = e1.y * e1.y + e2.y * e1.y + e2.y * e2.y;
+        I += (0.25f * k_inv3 * D) * (intx2 + inty2);
+    }
+    md.mass = density * area;
+    float inv = 1.0f / area;
+    center = hv(center.x * inv, center.y * inv);
+    md.center = center + ref;
+    md.I = density * I;
+    md.I += md.mass * (hdot(md.center, md.center) - hdot(center, center));
+    return md;
+}
+
+float4 shape_aabb_host(const Shape &s, const HXf &xf) { // ComputeAABB at creation
+    if (s.type == kPolygon) {
+        HV lo = hmul(xf, hv(s.v[0].x, s.v[0].y)), hi = lo;
+        for (int i = 1; i < s.count; ++i) {
+            HV w = hmul(xf, hv(s.v[i].x, s.v[i].y));
+            lo = hmin(lo, w);
+            hi = hmax(hi, w);
+        }
+        return make_float4(lo.x - s.radius, lo.y - s.radius, hi.x + s.radius, hi.y + s.radius);
+    }
+    if (s.type == kCircle) {
+        HV p = xf.p + hrot(xf, hv(s.c.x, s.c.y));
+        return make_float4(p.x - s.radius, p.y - s.radius, p.x + s.radius, p.y + s.radius);
+    }
+    HV a = hmul(xf, hv(s.v[0].x, s.v[0].y)), b = hmul(xf, hv(s.v[1].x, s.v[1].y));
+    HV lo = hmin(a, b), hi = hmax(a, b);
+    return make_float4(lo.x - s.radius, lo.y - s.radius, hi.x + s.radius, hi.y + s.radius);
+}
+
+bool build_shape(const dbx_fixture_def &d, Shape &s, std::string &err) {
+    memset(&s, 0, sizeof(s));
+    switch (d.shape) {
+    case 0:
+        s.type = kCircle;
+        s.radius = d.radius;
+        s.c = make_float2(d.p[0], d.p[1]);
+        if (!(d.radius > 0.0f)) { err = "circle radius must be positive"; return false; }
+        return true;
+    case 1:
+        s.type = kEdge;
+        s.radius = kPolygonRadius;
+        s.v[0] = make_float2(d.p[0], d.p[1]);
+        s.v[1] = make_float2(d.p[2], d.p[3]);
+        s.flags = ((d.flags & 2) ? 1 : 0) | ((d.flags & 4) ? 2 : 0);
+        s.v[2] = (d.flags & 2) ? make_float2(d.p[4], d.p[5]) : make_float2(0.0f, 0.0f);
+        s.v[3] = (d.flags & 4) ? make_float2(d.p[6], d.p[7]) : make_float2(0.0f, 0.0f);
+        return true;
+    case 2:
+        if (!set_polygon(s, d.p, d.count)) { err = "degenerate polygon"; return false; }
+        return true;
+    case 3:
+        set_box(s, d.p[0], d.p[1]);
+        if (d.flags & 8) { // SetAsBox(hx, hy, center, angle) (b2polygonshape.d:235-262)
+            HXf x = hxf(hv(d.p[2], d.p[3]), d.p[4]);
+            for (int i = 0; i < 4; ++i) {
+                HV v = hmul(x, hv(s.v[i].x, s.v[i].y));
+                HV n = hrot(x, hv(s.n[i].x, s.n[i].y));
+                s.v[i] = make_float2(v.x, v.y);
+                s.n[i] = make_float2(n.x, n.y);
+            }
+            s.c = make_float2(d.p[2], d.p[3]);
+        }
+        return true;
+    default:
+        err = "unknown shape kind";
+        return false;
+    }
+}
+
+bool finite(float x) { return std::isfinite(x); }
+
+struct WorldHost {
+    std::vector<uint32_t> bflags;
+    std::vector<float4> bxf, bpos, bpos0, bvel, bforce, bmass, bparam;
+    std::vector<int> fbody;
+    std::vector<Shape> fshape;
+    std::vector<float4> fmat, ffat;
+    std::vector<int2> ffilter;
+    int dynamicCount = 0;
+    float gx = 0.0f, gy = -10.0f;
+};
+
+// b2World.CreateBody + b2Body.CreateFixture for a whole definition list.
+bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, int nf, WorldHost &W, std::string &err) {
+    W.bflags.resize(nb);
+    W.bxf.resize(nb);
+    W.bpos.resize(nb);
+    W.bpos0.resize(nb);
+    W.bvel.resize(nb);
+    W.bforce.assign(nb, make_float4(0, 0, 0, 0));
+    W.bmass.resize(nb);
+    W.bparam.resize(nb);
+    std::vector<HXf> xf(nb);
+    std::vector<HV> lc(nb, hv(0.0f, 0.0f)), c(nb);
+    std::vector<float> mass(nb), invMass(nb), I(nb), invI(nb);
+    std::vector<std::vector<int>> fixturesOf(nb);
+    for (int i = 0; i < nb; ++i) {
+        const dbx_body_def &d = bodies[i];
+        if (d.type < 0 || d.type > 2) { err = "bad body type"; return false; }
+        if (!finite(d.px) || !finite(d.py) || !finite(d.angle) || !finite(d.vx) || !finite(d.vy) || !finite(d.w) ||
+            !finite(d.linear_damping) || d.linear_damping < 0.0f || !finite(d.angular_damping) || d.angular_damping < 0.0f ||
+            !finite(d.gravity_scale)) {
+            err = "body definition is not finite";
+            return false;
+        }
+        uint32_t f = kBodyActive;
+        if (d.bullet) f |= kBodyBullet;
+        if (d.fixed_rotation) f |= kBodyFixedRotation;
+        if (d.allow_sleep) f |= kBodyAutoSleep;
+        if (d.awake) f |= kBodyAwake;
+        f |= (uint32_t)d.type << 16;
+        W.bflags[i] = f;
+        xf[i] = hxf(hv(d.px, d.py), d.angle);
+        c[i] = xf[i].p;
+        mass[i] = invMass[i] = d.type == kDynamic ? 1.0f : 0.0f;
+        I[i] = invI[i] = 0.0f;
+        W.bvel[i] = make_float4(d.vx, d.vy, d.w, 0.0f);
+        W.bparam[i] = make_float4(d.linear_damping, d.angular_damping, d.gravity_scale, 0.0f);
+        if (d.type == kDynamic) W.dynamicCount++;
+    }
+    W.fbody.resize(nf);
+    W.fshape.resize(nf);
+    W.fmat.resize(nf);
+    W.ffat.resize(nf);
+    W.ffilter.resize(nf);
+    for (int i = 0; i < nf; ++i) {
+        const dbx_fixture_def &d = fx[i];
+        if (d.body < 0 || d.body >= nb) { err = "fixture body out of range"; return false; }
+        if (d.flags & 1) { err = "sensors are not supported"; return false; }
+        if (!finite(d.density) || d.density < 0.0f || !finite(d.friction) || !finite(d.restitution)) {
+            err = "fixture definition is not finite";
+            return false;
+        }
+        Shape s;
+        if (!build_shape(d, s, err)) return false;
+        W.fbody[i] = d.body;
+        W.fshape[i] = s;
+        W.fmat[i] = make_float4(d.friction, d.restitution, d.density, s.radius);
+        W.ffilter[i] = make_int2((int)d.category | ((int)d.mask << 16), (int)d.group);
+        // b2Fixture.CreateProxies -> b2DynamicTree.CreateProxy: fatten by the extension.
+        float4 a = shape_aabb_host(s, xf[d.body]);
+        W.ffat[i] = make_float4(a.x - kAabbExtension, a.y - kAabbExtension, a.z + kAabbExtension, a.w + kAabbExtension);
+        fixturesOf[d.body].insert(fixturesOf[d.body].begin(), i); // m_fixtureList: newest first
+        // b2Body.ResetMassData (b2body.d:555-625) after each fixture with density
+        if (d.density > 0.0f) {
+            int b = d.body;
+            int type = bodies[b].type;
+            mass[b] = invMass[b] = I[b] = invI[b] = 0.0f;
+            lc[b] = hv(0.0f, 0.0f);
+            if (type != kDynamic) {
+                c[b] = xf[b].p;
+                continue;
+            }
+            HV center = hv(0.0f, 0.0f);
+            for (int fi : fixturesOf[b]) {
+                if (fx[fi].density == 0.0f) continue;
+                HMass md = shape_mass(W.fshape[fi], fx[fi].density);
+                mass[b] += md.mass;
+                center = center + md.mass * md.center;
+                I[b] += md.I;
+            }
+            if (mass[b] > 0.0f) {
+                invMass[b] = 1.0f / mass[b];
+                center = hv(center.x * invMass[b], center.y * invMass[b]);
+            } else {
+                mass[b] = 1.0f;
+                invMass[b] = 1.0f;
+            }
+            if (I[b] > 0.0f && (W.bflags[b] & kBodyFixedRotation) == 0) {
+                I[b] -= mass[b] * hdot(center, center);
+                invI[b] = 1.0f / I[b];
+            } else {
+                I[b] = 0.0f;
+                invI[b] = 0.0f;
+            }
+            HV old = c[b];
+            lc[b] = center;
+            c[b] = hmul(xf[b], lc[b]);
+            float w = W.bvel[b].z;
+            HV dv = hv(-w * (c[b].y - old.y), w * (c[b].x - old.x));
+            W.bvel[b].x += dv.x;
+            W.bvel[b].y += dv.y;
+        }
+    }
+    for (int i = 0; i < nb; ++i) {
+        W.bxf[i] = make_float4(xf[i].p.x, xf[i].p.y, xf[i].s, xf[i].c);
+        W.bpos[i] = make_float4(c[i].x, c[i].y, bodies[i].angle, 0.0f);
+        W.bpos0[i] = W.bpos[i];
+        W.bpos0[i].w = 0.0f;
+        W.bmass[i] = make_float4(invMass[i], invI[i], lc[i].x, lc[i].y);
+        W.bparam[i].w = mass[i];
+    }
+    return true;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct dbx_handle {
+    dbx_config cfg{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<WorldHost> pending;
+    bool finalized = false;
+    int nworlds = 0;
+    std::vector<WorldDesc> W;
+    std::vector<int> dynamicCount;
+    Dev d{};
+    int cur = 0;
+    float inv_dt0 = 0.0f;
+    bool newFixture = true;
+    int block = 256;
+    std::vector<void *> allocs;
+    hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> ring;   // 5 events per step since the last sync
+    int ringUsed = 0;
+    dbx_profile prof{};             // last step
+    dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
+    int sumSteps = 0;
+    long long totalBodies = 0, totalFix = 0, totalContacts = 0, totalPairs = 0;
+    float *dPen = nullptr;
+    MlpState mlp{};
+};
+
+namespace {
+
+dbx_status fail(dbx_handle *h, dbx_status s, const std::string &msg) {
+    if (h) h->err = msg;
+    return s;
+}
+dbx_status hip_fail(dbx_handle *h, hipError_t e, const char *what) {
+    return fail(h, DBX_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define DBX_HIP(h, expr)                                                                                              \
+    do {                                                                                                              \
+        hipError_t _e = (expr);                                                                                       \
+        if (_e != hipSuccess) return hip_fail(h, _e, #expr);                                                          \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(dbx_handle *h, T **p, size_t n) {
+    size_t bytes = std::max<size_t>(1, n) * sizeof(T);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return e;
+    h->allocs.push_back(*p);
+    return hipMemsetAsync(*p, 0, bytes, h->stream);
+}
+template <class T>
+hipError_t upload(dbx_handle *h, T *dst, const std::vector<T> &src) {
+    if (src.empty()) return hipSuccess;
+    return hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, h->stream);
+}
+
+} // namespace
+
+extern "C" {
+
+dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
+    if (!out) return DBX_E_INVALID_ARG;
+    *out = nullptr;
+    dbx_handle *h = new dbx_handle();
+    if (config) h->cfg = *config;
+    if (h->cfg.contacts_per_fixture <= 0) h->cfg.contacts_per_fixture = 8;
+    if (h->cfg.block_size > 0) h->block = h->cfg.block_size;
+    if (h->block < 64 || h->block > kMaxBlock || (h->block & 63)) {
+        delete h;
+        return DBX_E_INVALID_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || h->cfg.device < 0 || h->cfg.device >= ndev) {
+        delete h;
+        return DBX_E_HIP;
+    }
+    if (hipSetDevice(h->cfg.device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+        delete h;
+        return DBX_E_HIP;
+    }
+    for (auto &e : h->ev) (void)hipEventCreate(&e);
+    *out = h;
+    return DBX_OK;
+}
+
+void dbx_destroy(dbx_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->cfg.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    mlp_release(h->mlp);
+    for (void *p : h->allocs) (void)hipFree(p);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : h->ring) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *dbx_last_error(const dbx_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+dbx_status dbx_add_world(dbx_handle *h, const dbx_body_def *bodies, int32_t nb, const dbx_fixture_def *fixtures, int32_t nf,
+                         float gx, float gy, int32_t *world_id) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (h->finalized) return fail(h, DBX_E_LOCKED, "worlds cannot be added after dbx_finalize");
+    if (nb <= 0 || nf < 0 || !bodies || (nf > 0 && !fixtures)) return fail(h, DBX_E_INVALID_ARG, "empty world");
+    if (!std::isfinite(gx) || !std::isfinite(gy)) return fail(h, DBX_E_INVALID_ARG, "gravity is not finite");
+    WorldHost W;
+    std::string err;
+    if (!build_world(bodies, nb, fixtures, nf, W, err)) {
+        bool unsupported = err.find("not supported") != std::string::npos;
+        return fail(h, unsupported ? DBX_E_UNSUPPORTED : DBX_E_INVALID_ARG, err);
+    }
+    W.gx = gx;
+    W.gy = gy;
+    h->pending.push_back(std::move(W));
+    if (world_id) *world_id = (int32_t)h->pending.size() - 1;
+    return DBX_OK;
+}
+
+dbx_status dbx_finalize(dbx_handle *h) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (h->finalized) return fail(h, DBX_E_STATE, "already finalized");
+    if (h->pending.empty()) return fail(h, DBX_E_STATE, "no worlds");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    const int nw = (int)h->pending.size();
+    h->nworlds = nw;
+    h->W.resize(nw);
+    long long nb = 0, nf = 0, nc = 0, np = 0;
+    int cpf = h->cfg.contacts_per_fixture;
+    for (int w = 0; w < nw; ++w) {
+        const WorldHost &H = h->pending[w];
+        WorldDesc &D = h->W[w];
+        D.bodyBase = (int)nb;
+        D.bodyCount = (int)H.bflags.size();
+        D.fixBase = (int)nf;
+        D.fixCount = (int)H.fbody.size();
+        D.contactBase = (int)nc;
+        D.contactCap = std::max(64, cpf * D.fixCount);
+        D.pairBase = (int)np;
+        D.pairCap = std::max(64, cpf * D.fixCount);
+        D.gx = H.gx;
+        D.gy = H.gy;
+        D.islandBase = D.bodyBase;
+        D.pad = 0;
+        nb += D.bodyCount;
+        nf += D.fixCount;
+        nc += D.contactCap;
+        np += D.pairCap;
+        h->dynamicCount.push_back(H.dynamicCount);
+    }
+    if (nc + nw > INT32_MAX / 2 || nb > INT32_MAX / 2) return fail(h, DBX_E_CAPACITY, "too many contacts for int32 indexing");
+    h->totalBodies = nb;
+    h->totalFix = nf;
+    h->totalContacts = nc;
+    h->totalPairs = np;
+    // launch order: largest world first
+    std::vector<int> order(nw);
+    for (int w = 0; w < nw; ++w) order[w] = w;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->W[a].fixCount > h->W[b].fixCount; });
+
+    Dev &d = h->d;
+    d.nworlds = nw;
+    int *dOrder;
+    WorldDesc *dW;
+    DBX_HIP(h, dalloc(h, &dOrder, nw));
+    DBX_HIP(h, dalloc(h, &dW, nw));
+    DBX_HIP(h, dalloc(h, &d.S, nw));
+    DBX_HIP(h, hipMemcpyAsync(dOrder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    DBX_HIP(h, hipMemcpyAsync(dW, h->W.data(), nw * sizeof(WorldDesc), hipMemcpyHostToDevice, h->stream));
+    d.order = dOrder;
+    d.W = dW;
+    // bodies
+    std::vector<uint32_t> bflags;
+    std::vector<float4> bxf, bpos, bpos0, bvel, bforce, bmass, bparam, fmat, ffat;
+    std::vector<int> fbody;
+    std::vector<Shape> fshape;
+    std::vector<int2> ffilter;
+    bflags.reserve(nb);
+    for (const WorldHost &H : h->pending) {
+        bflags.insert(bflags.end(), H.bflags.begin(), H.bflags.end());
+        bxf.insert(bxf.end(), H.bxf.begin(), H.bxf.end());
+        bpos.insert(bpos.end(), H.bpos.begin(), H.bpos.end());
+        bpos0.insert(bpos0.end(), H.bpos0.begin(), H.bpos0.end());
+        bvel.insert(bvel.end(), H.bvel.begin(), H.bvel.end());
+        bforce.insert(bforce.end(), H.bforce.begin(), H.bforce.end());
+        bmass.insert(bmass.end(), H.bmass.begin(), H.bmass.end());
+        bparam.insert(bparam.end(), H.bparam.begin(), H.bparam.end());
+        fbody.insert(fbody.end(), H.fbody.begin(), H.fbody.end());
+        fshape.insert(fshape.end(), H.fshape.begin(), H.fshape.end());
+        fmat.insert(fmat.end(), H.fmat.begin(), H.fmat.end());
+        ffat.insert(ffat.end(), H.ffat.begin(), H.ffat.end());
+        ffilter.insert(ffilter.end(), H.ffilter.begin(), H.ffilter.end());
+    }
+    float4 *p_bmass, *p_bparam, *p_fmat;
+    int *p_fbody;
+    Shape *p_fshape;
+    int2 *p_ffilter;
+    DBX_HIP(h, dalloc(h, &d.bflags, nb));
+    DBX_HIP(h, dalloc(h, &d.bxf, nb));
+    DBX_HIP(h, dalloc(h, &d.bpos, nb));
+    DBX_HIP(h, dalloc(h, &d.bpos0, nb));
+    DBX_HIP(h, dalloc(h, &d.bvel, nb));
+    DBX_HIP(h, dalloc(h, &d.bforce, nb));
+    DBX_HIP(h, dalloc(h, &p_bmass, nb));
+    DBX_HIP(h, dalloc(h, &p_bparam, nb));
+    DBX_HIP(h, dalloc(h, &p_fbody, nf));
+    DBX_HIP(h, dalloc(h, &p_fshape, nf));
+    DBX_HIP(h, dalloc(h, &p_fmat, nf));
+    DBX_HIP(h, dalloc(h, &p_ffilter, nf));
+    DBX_HIP(h, dalloc(h, &d.ffat, nf));
+    DBX_HIP(h, dalloc(h, &d.fmoved, nf));
+    DBX_HIP(h, dalloc(h, &d.fcount, nf));
+    DBX_HIP(h, upload(h, d.bflags, bflags));
+    DBX_HIP(h, upload(h, d.bxf, bxf));
+    DBX_HIP(h, upload(h, d.bpos, bpos));
+    DBX_HIP(h, upload(h, d.bpos0, bpos0));
+    DBX_HIP(h, upload(h, d.bvel, bvel));
+    DBX_HIP(h, upload(h, d.bforce, bforce));
+    DBX_HIP(h, upload(h, p_bmass, bmass));
+    DBX_HIP(h, upload(h, p_bparam, bparam));
+    DBX_HIP(h, upload(h, p_fbody, fbody));
+    DBX_HIP(h, upload(h, p_fshape, fshape));
+    DBX_HIP(h, upload(h, p_fmat, fmat));
+    DBX_HIP(h, upload(h, p_ffilter, ffilter));
+    DBX_HIP(h, upload(h, d.ffat, ffat));
+    std::vector<int> ones(nf, 1); // every new proxy is in the move buffer (b2BroadPhase.CreateProxy)
+    DBX_HIP(h, upload(h, d.fmoved, ones));
+    d.bmass = p_bmass;
+    d.bparam = p_bparam;
+    d.fbody = p_fbody;
+    d.fshape = p_fshape;
+    d.fmat = p_fmat;
+    d.ffilter = p_ffilter;
+    // contacts and scratch
+    DBX_HIP(h, dalloc(h, &d.cont[0], nc));
+    DBX_HIP(h, dalloc(h, &d.cont[1], nc));
+    DBX_HIP(h, dalloc(h, &d.adjOff, nb));
+    DBX_HIP(h, dalloc(h, &d.adjCnt, nb));
+    DBX_HIP(h, dalloc(h, &d.bisl, nb));
+    DBX_HIP(h, dalloc(h, &d.bscratch, nb));
+    DBX_HIP(h, dalloc(h, &d.wake, nb));
+    DBX_HIP(h, dalloc(h, &d.bmask, nb));
+    DBX_HIP(h, dalloc(h, &d.adjList, 2 * nc));
+    DBX_HIP(h, dalloc(h, &d.pairs, np));
+    DBX_HIP(h, dalloc(h, &d.ordC, nc));
+    DBX_HIP(h, dalloc(h, &d.cbatch, nc));
+    DBX_HIP(h, dalloc(h, &d.cisl, nc));
+    DBX_HIP(h, dalloc(h, &d.sched, nc));
+    DBX_HIP(h, dalloc(h, &d.batchOff, nc + 65LL * nw));
+    DBX_HIP(h, dalloc(h, &d.vc, nc));
+    DBX_HIP(h, dalloc(h, &d.pc, nc));
+    DBX_HIP(h, dalloc(h, &d.islMin, nb));
+    DBX_HIP(h, dalloc(h, &d.islSolved, nb));
+    DBX_HIP(h, dalloc(h, &h->dPen, nw));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    h->pending.clear();
+    h->pending.shrink_to_fit();
+    h->finalized = true;
+    return DBX_OK;
+}
+
+dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t flags) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (flags & DBX_STEP_CCD) return fail(h, DBX_E_UNSUPPORTED, "continuous physics (TOI) is not implemented");
+    if (!std::isfinite(dt) || vel < 0 || pos < 0) return fail(h, DBX_E_INVALID_ARG, "bad step parameters");
+    if ((flags & DBX_STEP_LEARNED) && !h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    StepParams sp{};
+    sp.dt = dt;
+    sp.inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    sp.dtRatio = h->inv_dt0 * dt;
+    sp.velocityIterations = vel;
+    sp.positionIterations = pos;
+    sp.flags = ((flags & DBX_STEP_WARM) ? 1u : 0u) | ((flags & DBX_STEP_SLEEP) ? 2u : 0u) |
+               ((flags & DBX_STEP_RESIDUAL) ? (1u << 5) : 0u) | ((flags & DBX_STEP_LEARNED) ? (1u << 4) : 0u);
+    sp.residualTol = 1e-3f;
+    sp.maxIterations = 100;
+    hipStream_t s = h->stream;
+    if ((int)h->ring.size() < 5 * (h->ringUsed + 1)) {
+        for (int i = 0; i < 5; ++i) {
+            hipEvent_t e;
+            DBX_HIP(h, hipEventCreate(&e));
+            h->ring.push_back(e);
+        }
+    }
+    hipEvent_t *ev = &h->ring[5 * h->ringUsed];
+    h->ringUsed++;
+    (void)hipEventRecord(ev[0], s);
+    if (h->newFixture) { // b2World.Step: FindNewContacts when fixtures were added (b2world.d:372-376)
+        DBX_HIP(h, launch_broadphase(h->d, h->cur, h->block, s));
+        h->newFixture = false;
+    }
+    DBX_HIP(h, launch_collide(h->d, h->cur, h->block, s));
+    h->cur ^= 1;
+    (void)hipEventRecord(ev[1], s);
+    if (dt > 0.0f) {
+        DBX_HIP(h, launch_islands(h->d, h->cur, (flags & DBX_STEP_ORDERED) ? 1 : 0, h->block, s));
+        (void)hipEventRecord(ev[2], s);
+        if (flags & DBX_STEP_LEARNED) {
+            dbx_status st = mlp_step_hook(h->mlp, h->d, h->cur, sp, h->block, s);
+            if (st != DBX_OK) return fail(h, st, "learned warm start failed");
+        }
+        DBX_HIP(h, launch_solve(h->d, h->cur, sp, h->block, s));
+        (void)hipEventRecord(ev[3], s);
+        DBX_HIP(h, launch_broadphase(h->d, h->cur, h->block, s));
+        (void)hipEventRecord(ev[4], s);
+        h->inv_dt0 = sp.inv_dt;
+    } else {
+        (void)hipEventRecord(ev[2], s);
+        (void)hipEventRecord(ev[3], s);
+        (void)hipEventRecord(ev[4], s);
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_sync(dbx_handle *h) {
+    if (!h) return DBX_E_INVALID_ARG;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    if (!h->finalized) return DBX_OK;
+    std::vector<WorldState> S(h->nworlds);
+    DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
+    for (int w = 0; w < h->nworlds; ++w) {
+        if (S[w].error) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "world %d exceeded its %s capacity", w, (S[w].error & 1) ? "contact" : "pair");
+            return fail(h, DBX_E_CAPACITY, buf);
+        }
+    }
+    for (int k = 0; k < h->ringUsed; ++k) {
+        hipEvent_t *ev = &h->ring[5 * k];
+        float ms[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        dbx_profile p{};
+        p.collide = ms[0];
+        p.solveInit = ms[1];
+        p.solveVelocity = ms[2];
+        p.solvePosition = 0.0f;
+        p.solve = ms[1] + ms[2] + ms[3];
+        p.broadphase = ms[3];
+        p.solveTOI = 0.0f;
+        p.step = ms[0] + ms[1] + ms[2] + ms[3];
+        h->prof = p;
+        h->sum.step += p.step;
+        h->sum.collide += p.collide;
+        h->sum.solve += p.solve;
+        h->sum.solveInit += p.solveInit;
+        h->sum.solveVelocity += p.solveVelocity;
+        h->sum.broadphase += p.broadphase;
+        h->sumSteps++;
+    }
+    h->ringUsed = 0;
+    return DBX_OK;
+}
+
+dbx_status dbx_world_count(const dbx_handle *h, int32_t *count) {
+    if (!h || !count) return DBX_E_INVALID_ARG;
+    *count = h->finalized ? h->nworlds : (int32_t)h->pending.size();
+    return DBX_OK;
+}
+
+static dbx_status check_world(dbx_handle *h, int32_t world) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (world < 0 || world >= h->nworlds) return fail(h, DBX_E_INVALID_ARG, "world index out of range");
+    return DBX_OK;
+}
+
+dbx_status dbx_world_info(dbx_handle *h, int32_t world, int32_t out[4]) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    out[0] = h->W[world].bodyCount;
+    out[1] = h->W[world].fixCount;
+    out[2] = S.contactCount;
+    out[3] = S.touching;
+    return DBX_OK;
+}
+
+dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, int32_t n) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    if (!out || n < D.bodyCount) return fail(h, DBX_E_INVALID_ARG, "output too small");
+    int nb = D.bodyCount, b0 = D.bodyBase;
+    std::vector<uint32_t> fl(nb);
+    std::vector<float4> xf(nb), pos(nb), pos0(nb), vel(nb), ms(nb);
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(fl.data(), h->d.bflags + b0, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(xf.data(), h->d.bxf + b0, nb * sizeof(float4), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(pos.data(), h->d.bpos + b0, nb * sizeof(float4), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(pos0.data(), h->d.bpos0 + b0, nb * sizeof(float4), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(vel.data(), h->d.bvel + b0, nb * sizeof(float4), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(ms.data(), h->d.bmass + b0, nb * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; ++i) {
+        dbx_body_state &s = out[i];
+        s.px = xf[i].x;
+        s.py = xf[i].y;
+        s.qs = xf[i].z;
+        s.qc = xf[i].w;
+        s.cx = pos[i].x;
+        s.cy = pos[i].y;
+        s.a = pos[i].z;
+        s.sleep_time = pos[i].w;
+        s.vx = vel[i].x;
+        s.vy = vel[i].y;
+        s.w = vel[i].z;
+        s.flags = (int32_t)(fl[i] & 0xffffu);
+        s.c0x = pos0[i].x;
+        s.c0y = pos0[i].y;
+        s.a0 = pos0[i].z;
+        s.inv_mass = ms[i].x;
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    int n = S.contactCount;
+    if (count) *count = n;
+    if (!out || cap <= 0) return DBX_OK;
+    std::vector<Contact> c(n);
+    if (n) DBX_HIP(h, hipMemcpy(c.data(), h->d.cont[h->cur] + D.contactBase, n * sizeof(Contact), hipMemcpyDeviceToHost));
+    int m = std::min(n, cap);
+    for (int k = 0; k < m; ++k) {
+        const Contact &s = c[n - 1 - k]; // list order: newest first
+        dbx_contact &o = out[k];
+        o.fixture_a = s.fa;
+        o.fixture_b = s.fb;
+        o.flags = (int32_t)(s.flags & (kContactTouching | kContactEnabled));
+        o.type = s.meta & 0xf;
+        o.point_count = (s.meta >> 4) & 0xf;
+        o.local_normal[0] = s.man.x;
+        o.local_normal[1] = s.man.y;
+        o.local_point[0] = s.man.z;
+        o.local_point[1] = s.man.w;
+        o.point_local[0] = s.pt0.x;
+        o.point_local[1] = s.pt0.y;
+        o.point_local[2] = s.pt1.x;
+        o.point_local[3] = s.pt1.y;
+        o.normal_impulse[0] = s.pt0.z;
+        o.normal_impulse[1] = s.pt1.z;
+        o.tangent_impulse[0] = s.pt0.w;
+        o.tangent_impulse[1] = s.pt1.w;
+        o.id[0] = s.id0;
+        o.id[1] = s.id1;
+        o.friction = s.friction;
+        o.restitution = s.restitution;
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_fat_aabbs(dbx_handle *h, int32_t world, float *out, int32_t n) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    if (!out || n < D.fixCount) return fail(h, DBX_E_INVALID_ARG, "output too small");
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(out, h->d.ffat + D.fixBase, D.fixCount * sizeof(float4), hipMemcpyDeviceToHost));
+    return DBX_OK;
+}
+
+dbx_status dbx_read_pairs(dbx_handle *h, int32_t world, int32_t *out, int32_t cap, int32_t *count) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    if (count) *count = S.pairCount;
+    int m = std::min(S.pairCount, cap);
+    if (out && m > 0) DBX_HIP(h, hipMemcpy(out, h->d.pairs + D.pairBase, m * sizeof(int2), hipMemcpyDeviceToHost));
+    return DBX_OK;
+}
+
+dbx_status dbx_profile_read(dbx_handle *h, dbx_profile *out) {
+    if (!h || !out) return DBX_E_INVALID_ARG;
+    *out = h->prof;
+    return DBX_OK;
+}
+
+dbx_status dbx_profile_totals(dbx_handle *h, dbx_profile *sum, int32_t *steps, int32_t reset) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (sum) *sum = h->sum;
+    if (steps) *steps = h->sumSteps;
+    if (reset) {
+        h->sum = dbx_profile{};
+        h->sumSteps = 0;
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_totals(dbx_handle *h, int64_t out[4]) {
+    if (!h || !out) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    std::vector<WorldState> S(h->nworlds);
+    DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> fl(h->totalBodies);
+    DBX_HIP(h, hipMemcpy(fl.data(), h->d.bflags, fl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int64_t dyn = 0, nc = 0, touch = 0, awake = 0;
+    for (int w = 0; w < h->nworlds; ++w) {
+        dyn += h->dynamicCount[w];
+        nc += S[w].contactCount;
+        touch += S[w].touching;
+    }
+    for (uint32_t f : fl) awake += (body_type(f) == kDynamic) && (f & kBodyAwake);
+    out[0] = dyn;
+    out[1] = nc;
+    out[2] = touch;
+    out[3] = awake;
+    return DBX_OK;
+}
+
+dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (nworlds < h->nworlds) return fail(h, DBX_E_INVALID_ARG, "output too small");
+    std::vector<WorldState> S(h->nworlds);
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
+    if (iterations)
+        for (int w = 0; w < h->nworlds; ++w) iterations[w] = S[w].iterations;
+    if (max_penetration) {
+        DBX_HIP(h, launch_max_penetration(h->d, h->cur, h->dPen, h->stream));
+        DBX_HIP(h, hipMemcpyAsync(max_penetration, h->dPen, h->nworlds * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        DBX_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uint8_t *moved, int32_t n, int32_t *out,
+                                int32_t cap, int32_t *count) {
+    if (!h || n < 0 || (n > 0 && (!fat_aabbs || !moved)) || cap < 0) return DBX_E_INVALID_ARG;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    std::vector<int> mv(n);
+    for (int i = 0; i < n; ++i) mv[i] = moved[i] ? 1 : 0;
+    float4 *dfat = nullptr;
+    int *dmv = nullptr, *dcnt = nullptr, *dtot = nullptr;
+    int2 *dout = nullptr;
+    size_t m = (size_t)std::max(1, n);
+    hipError_t e = hipMalloc(&dfat, m * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&dmv, m * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dcnt, m * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dtot, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dout, (size_t)std::max(1, cap) * sizeof(int2));
+    if (e == hipSuccess && n) e = hipMemcpy(dfat, fat_aabbs, n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(dmv, mv.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_bp_single(dfat, dmv, n, dcnt, dout, cap, dtot, h->stream);
+    int total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, dtot, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && total <= cap && total > 0 && out)
+        e = hipMemcpy(out, dout, total * sizeof(int2), hipMemcpyDeviceToHost);
+    (void)hipFree(dfat);
+    (void)hipFree(dmv);
+    (void)hipFree(dcnt);
+    (void)hipFree(dtot);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return hip_fail(h, e, "broadphase");
+    if (count) *count = total;
+    return total > cap ? fail(h, DBX_E_CAPACITY, "pair output too small") : DBX_OK;
+}
+
+dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const float *xa, const dbx_fixture_def *b,
+                             const float *xb, int32_t n, float *out) {
+    if (!h || n < 0 || (n > 0 && (!a || !b || !xa || !xb || !out))) return DBX_E_INVALID_ARG;
+    if (n == 0) return DBX_OK;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    std::vector<Shape> sa(n), sb(n);
+    std::string err;
+    for (int i = 0; i < n; ++i) {
+        if (!build_shape(a[i], sa[i], err) || !build_shape(b[i], sb[i], err)) return fail(h, DBX_E_INVALID_ARG, err);
+    }
+    Shape *dsa = nullptr, *dsb = nullptr;
+    float *dxa = nullptr, *dxb = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&dsa, n * sizeof(Shape));
+    if (e == hipSuccess) e = hipMalloc(&dsb, n * sizeof(Shape));
+    if (e == hipSuccess) e = hipMalloc(&dxa, n * 3 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dxb, n * 3 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dout, n * 32 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(dsa, sa.data(), n * sizeof(Shape), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dsb, sb.data(), n * sizeof(Shape), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dxa, xa, n * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dxb, xb, n * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_collide_pairs(dsa, dxa, dsb, dxb, n, dout, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, n * 32 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dsa);
+    (void)hipFree(dsb);
+    (void)hipFree(dxa);
+    (void)hipFree(dxb);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return hip_fail(h, e, "collide");
+    return DBX_OK;
+}
+
+} // extern "C"

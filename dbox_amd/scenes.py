@@ -1,0 +1,307 @@
+"""Seeded scene generators for the step workloads.
+
+A scene is a pair of numpy structured arrays in creation order: body
+definitions (mirroring b2BodyDef, src/dbox/dynamics/b2body.d:51-104) and
+fixture definitions (b2FixtureDef, src/dbox/dynamics/b2fixture.d:49-73, plus
+the shape constructor).  The byte layout is the C struct ``dbx_body_def`` /
+``dbx_fixture_def`` of include/dbox_hip.h, so one array feeds both the GPU
+library and any other consumer of the same definitions.
+
+Randomness is splitmix64 seeded per workload (``0xD80C + config``) and per
+world (``seed ^ world``); every draw is a 24-bit uniform in [0, 1), so the
+scenes are identical on every machine.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BODY_DTYPE = np.dtype([
+    ("type", "<i4"), ("px", "<f4"), ("py", "<f4"), ("angle", "<f4"),
+    ("vx", "<f4"), ("vy", "<f4"), ("w", "<f4"),
+    ("linear_damping", "<f4"), ("angular_damping", "<f4"), ("gravity_scale", "<f4"),
+    ("allow_sleep", "<i4"), ("awake", "<i4"), ("fixed_rotation", "<i4"), ("bullet", "<i4"),
+])
+FIXTURE_DTYPE = np.dtype([
+    ("body", "<i4"), ("shape", "<i4"), ("count", "<i4"), ("flags", "<i4"),
+    ("density", "<f4"), ("friction", "<f4"), ("restitution", "<f4"),
+    ("category", "<u2"), ("mask", "<u2"), ("group", "<i2"), ("pad", "<i2"),
+    ("radius", "<f4"), ("p", "<f4", (20,)),
+])
+assert BODY_DTYPE.itemsize == 56 and FIXTURE_DTYPE.itemsize == 120
+
+STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
+SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX = 0, 1, 2, 3
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class Stream:
+    """splitmix64 stream; ``uniform(n)`` returns n float64 draws in [0, 1)."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def next_u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            idx = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + idx * _GOLDEN
+            self.state = self.state + np.uint64(n) * _GOLDEN
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.next_u64(n) >> np.uint64(40)).astype(np.float64) * (1.0 / 16777216.0)
+        return lo + (hi - lo) * u
+
+    def normal(self, n: int) -> np.ndarray:
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
+
+    def integers(self, n: int, lo: int, hi_inclusive: int) -> np.ndarray:
+        span = hi_inclusive - lo + 1
+        return lo + np.minimum((self.uniform(n) * span).astype(np.int64), span - 1)
+
+
+def config_seed(config: int) -> int:
+    return 0xD80C + config
+
+
+@dataclass
+class Scene:
+    bodies: np.ndarray
+    fixtures: np.ndarray
+    gravity: tuple = (0.0, -10.0)
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def dynamic_count(self) -> int:
+        return int(np.count_nonzero(self.bodies["type"] == DYNAMIC))
+
+
+def body_defs(n: int) -> np.ndarray:
+    b = np.zeros(n, BODY_DTYPE)
+    b["gravity_scale"] = 1.0
+    b["allow_sleep"] = 1
+    b["awake"] = 1
+    return b
+
+
+def fixture_defs(n: int) -> np.ndarray:
+    f = np.zeros(n, FIXTURE_DTYPE)
+    f["friction"] = 0.2          # b2FixtureDef.friction default (b2fixture.d:59)
+    f["category"] = 1
+    f["mask"] = 0xFFFF
+    return f
+
+
+def _edge(f, body, x1, y1, x2, y2, friction):
+    f["body"] = body
+    f["shape"] = SHAPE_EDGE
+    f["p"][:4] = (x1, y1, x2, y2)
+    f["friction"] = friction
+
+
+# --------------------------------------------------------------------------
+# hello_world (examples/hello_world/hello_world.d:31-102)
+# --------------------------------------------------------------------------
+def hello_world() -> Scene:
+    b = body_defs(2)
+    b[0]["type"] = STATIC
+    b[0]["py"] = -10.0
+    b[1]["type"] = DYNAMIC
+    b[1]["py"] = 4.0
+    f = fixture_defs(2)
+    f[0]["body"] = 0
+    f[0]["shape"] = SHAPE_BOX
+    f[0]["p"][:2] = (50.0, 10.0)
+    f[1]["body"] = 1
+    f[1]["shape"] = SHAPE_BOX
+    f[1]["p"][:2] = (1.0, 1.0)
+    f[1]["density"] = 1.0
+    f[1]["friction"] = 0.3
+    return Scene(b, f, name="hello_world")
+
+
+# --------------------------------------------------------------------------
+# Pyramid: examples/demo/tests/pyramid.d:39-78, with the workload's friction
+# and a seeded +-0.01 m jitter of every box position.
+# --------------------------------------------------------------------------
+def pyramid(rows: int = 20, seed: int | None = None, jitter: float = 0.01, friction: float = 0.6,
+            density: float = 5.0, max_boxes: int | None = None, ground_half_width: float = 40.0) -> Scene:
+    seed = config_seed(1) if seed is None else seed
+    rng = Stream(seed)
+    xs, ys = [], []
+    x = np.array([-7.0, 0.75], np.float32)
+    dx = np.array([0.5625, 1.25], np.float32)
+    dy = np.array([1.125, 0.0], np.float32)
+    for i in range(rows):
+        y = x.copy()
+        for _ in range(i, rows):
+            xs.append(y[0])
+            ys.append(y[1])
+            y = (y + dy).astype(np.float32)
+        x = (x + dx).astype(np.float32)
+    n = len(xs) if max_boxes is None else min(len(xs), max_boxes)
+    px = np.array(xs[:n], np.float32)
+    py = np.array(ys[:n], np.float32)
+    if jitter:
+        px = (px + rng.uniform(n, -jitter, jitter).astype(np.float32)).astype(np.float32)
+        py = (py + rng.uniform(n, -jitter, jitter).astype(np.float32)).astype(np.float32)
+    b = body_defs(n + 1)
+    b[0]["type"] = STATIC
+    b[1:]["type"] = DYNAMIC
+    b[1:]["px"] = px
+    b[1:]["py"] = py
+    f = fixture_defs(n + 1)
+    _edge(f[0], 0, -ground_half_width, 0.0, ground_half_width, 0.0, friction)
+    f[1:]["body"] = np.arange(1, n + 1)
+    f[1:]["shape"] = SHAPE_BOX
+    f[1:]["p"][:, 0] = 0.5
+    f[1:]["p"][:, 1] = 0.5
+    f[1:]["density"] = density
+    f[1:]["friction"] = friction
+    return Scene(b, f, name=f"pyramid{rows}", meta={"rows": rows, "boxes": n})
+
+
+# --------------------------------------------------------------------------
+# Mixed polygons and circles in a static U container (configs 2 and 3).
+# --------------------------------------------------------------------------
+def mixed_container(n: int, width: float, height: float, seed: int) -> Scene:
+    rng = Stream(seed)
+    is_poly = rng.uniform(n) < 0.6
+    nverts = rng.integers(n, 3, 8)
+    prad = rng.uniform(n, 0.25, 1.0)
+    crad = rng.uniform(n, 0.2, 0.8)
+    jit = rng.uniform(n * 8, -0.3, 0.3).reshape(n, 8)
+    density = rng.uniform(n, 0.5, 5.0)
+    friction = rng.uniform(n, 0.2, 0.8)
+    ux = rng.uniform(n)
+    uy = rng.uniform(n)
+    angle = rng.uniform(n, 0.0, 2.0 * math.pi)
+    extent = np.where(is_poly, prad, crad) + 0.02
+    half = 0.5 * width
+    px = -half + extent + ux * (width - 2 * extent)
+    py = extent + uy * (height - 2 * extent)
+
+    b = body_defs(n + 1)
+    b[0]["type"] = STATIC
+    b[1:]["type"] = DYNAMIC
+    b[1:]["px"] = px
+    b[1:]["py"] = py
+    b[1:]["angle"] = angle
+    f = fixture_defs(n + 3)
+    _edge(f[0], 0, -half, 0.0, half, 0.0, 0.6)
+    _edge(f[1], 0, -half, 0.0, -half, height, 0.6)
+    _edge(f[2], 0, half, 0.0, half, height, 0.6)
+    d = f[3:]
+    d["body"] = np.arange(1, n + 1)
+    d["density"] = density
+    d["friction"] = friction
+    d["shape"] = np.where(is_poly, SHAPE_POLYGON, SHAPE_CIRCLE)
+    d["radius"] = np.where(is_poly, 0.0, crad)
+    d["count"] = np.where(is_poly, nverts, 0)
+    k = np.arange(8)[None, :]
+    ang = 2.0 * math.pi * (k + jit) / nverts[:, None]
+    vx = prad[:, None] * np.cos(ang)
+    vy = prad[:, None] * np.sin(ang)
+    pts = np.zeros((n, 20), np.float32)
+    pts[:, 0:16:2] = vx
+    pts[:, 1:16:2] = vy
+    valid = k < nverts[:, None]
+    pts[:, 0:16:2] *= valid
+    pts[:, 1:16:2] *= valid
+    pts[~is_poly] = 0.0
+    d["p"] = pts
+    return Scene(b, f, name=f"mixed{n}", meta={"width": width, "height": height})
+
+
+def big_mixed_world(n: int = 20000, seed: int | None = None) -> Scene:
+    """20,000 bodies dropped into a 200 m x 400 m U container."""
+    return mixed_container(n, 200.0, 400.0, config_seed(2) if seed is None else seed)
+
+
+def world_sizes(worlds: int, seed: int, median: float = 64.0, sigma: float = 0.8,
+                lo: int = 8, hi: int = 1024) -> np.ndarray:
+    z = Stream(seed).normal(worlds)
+    return np.clip(np.rint(np.exp(math.log(median) + sigma * z)), lo, hi).astype(np.int64)
+
+
+def batched_worlds(worlds: int = 4096, seed: int | None = None, sizes: np.ndarray | None = None) -> list[Scene]:
+    """Independent 40 m x 80 m container worlds of lognormal body counts."""
+    seed = config_seed(3) if seed is None else seed
+    sizes = world_sizes(worlds, seed) if sizes is None else sizes
+    return [mixed_container(int(sizes[w]), 40.0, 80.0, seed ^ (w + 1) * 0x632BE5AB) for w in range(worlds)]
+
+
+def big_pyramid(boxes: int = 50000, seed: int | None = None) -> Scene:
+    rows = int(math.ceil((math.sqrt(8 * boxes + 1) - 1) / 2))
+    return pyramid(rows=rows, seed=config_seed(4) if seed is None else seed, max_boxes=boxes,
+                   ground_half_width=400.0)
+
+
+def stacks_world(seed: int) -> Scene:
+    """1-4 stacks of 20-60 boxes with half-extents in [0.25, 0.75] on a ground edge."""
+    rng = Stream(seed)
+    nstacks = int(rng.integers(1, 1, 4)[0])
+    heights = rng.integers(nstacks, 20, 60)
+    total = int(heights.sum())
+    hx = rng.uniform(total, 0.25, 0.75).astype(np.float32)
+    hy = rng.uniform(total, 0.25, 0.75).astype(np.float32)
+    fr = rng.uniform(total, 0.3, 0.9)
+    b = body_defs(total + 1)
+    b[0]["type"] = STATIC
+    f = fixture_defs(total + 1)
+    _edge(f[0], 0, -60.0, 0.0, 60.0, 0.0, 0.6)
+    i = 0
+    for s in range(nstacks):
+        x0 = -30.0 + 20.0 * s
+        y = 0.0
+        for _ in range(int(heights[s])):
+            y += float(hy[i]) + 0.005
+            bb = b[i + 1]
+            bb["type"] = DYNAMIC
+            bb["px"] = x0
+            bb["py"] = y
+            y += float(hy[i]) + 0.005
+            ff = f[i + 1]
+            ff["body"] = i + 1
+            ff["shape"] = SHAPE_BOX
+            ff["p"][:2] = (hx[i], hy[i])
+            ff["density"] = 1.0
+            ff["friction"] = fr[i]
+            i += 1
+    return Scene(b, f, name="stacks", meta={"stacks": nstacks})
+
+
+def stack_worlds(worlds: int = 16384, seed: int | None = None) -> list[Scene]:
+    seed = config_seed(5) if seed is None else seed
+    return [stacks_world(seed ^ (w + 1) * 0x632BE5AB) for w in range(worlds)]
+
+
+def single_box_on_ground(y: float = 0.5, hx: float = 0.5, density: float = 1.0, friction: float = 0.6) -> Scene:
+    """One box resting on a static ground box (closed-form solver checks)."""
+    b = body_defs(2)
+    b[0]["type"] = STATIC
+    b[1]["type"] = DYNAMIC
+    b[1]["py"] = y
+    f = fixture_defs(2)
+    f[0]["body"] = 0
+    f[0]["shape"] = SHAPE_BOX
+    f[0]["p"][:5] = (20.0, 0.5, 0.0, -0.5, 0.0)
+    f[0]["flags"] = 8
+    f[0]["friction"] = friction
+    f[1]["body"] = 1
+    f[1]["shape"] = SHAPE_BOX
+    f[1]["p"][:2] = (hx, hx)
+    f[1]["density"] = density
+    f[1]["friction"] = friction
+    return Scene(b, f, name="box_on_ground")

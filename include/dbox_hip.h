@@ -1,0 +1,175 @@
+/*
+ * dbox_hip.h -- C ABI of the MI355X b2World.Step hot path.
+ *
+ * The reference (dbox, the D port of Box2D 2.3.1) has no FFI: its step is the
+ * D member function b2World.Step (src/dbox/dynamics/b2world.d:367-434).  This
+ * library replaces the body of that call for every world bound to a handle:
+ *   broadphase   b2BroadPhase.UpdatePairs + b2ContactManager.AddPair
+ *                (src/dbox/collision/b2broadphase.d:139-195, dynamics/b2contactmanager.d:52-176)
+ *   narrowphase  b2ContactManager.Collide -> b2Contact.Update -> b2Collide*
+ *                (dynamics/b2contactmanager.d:251-317, dynamics/contacts/b2contact.d:270-356)
+ *   solver       b2World.Solve -> b2Island.Solve -> b2ContactSolver
+ *                (dynamics/b2world.d:930-1124, dynamics/b2island.d:75-280,
+ *                 dynamics/contacts/b2contactsolver.d:73-787)
+ * The D side keeps b2Body, b2Fixture, b2Shape and b2ContactListener; a hook at
+ * the top of b2World.Step calls dbx_step and copies results back.
+ *
+ * All entry points are plain C: pointers and sizes, no C++ or torch types.
+ * Every pointer argument is caller-owned and is read (or written) before the
+ * call returns.  Status codes are negative on error; dbx_last_error gives the
+ * detail.  A handle is used by one host thread at a time.
+ */
+#ifndef DBOX_HIP_H
+#define DBOX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t dbx_status;
+enum {
+    DBX_OK = 0,
+    DBX_E_INVALID_ARG = -1,
+    DBX_E_CAPACITY = -2,
+    DBX_E_UNSUPPORTED = -3, /* joints, chains, sensors, continuous physics */
+    DBX_E_HIP = -4,
+    DBX_E_LOCKED = -5,      /* world mutation after dbx_finalize */
+    DBX_E_STATE = -6
+};
+
+/* dbx_step flags (b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics,
+ * b2world.d:616-753, plus the solver schedule). */
+enum {
+    DBX_STEP_WARM = 1u << 0,    /* warm starting (default on in dbox) */
+    DBX_STEP_SLEEP = 1u << 1,   /* allow sleeping (default on in dbox) */
+    DBX_STEP_CCD = 1u << 2,     /* continuous physics: not implemented -> DBX_E_UNSUPPORTED */
+    DBX_STEP_ORDERED = 1u << 3, /* solve in the reference's exact island order (dependency levels) */
+    DBX_STEP_LEARNED = 1u << 4, /* MLP warm start for new contact points */
+    DBX_STEP_RESIDUAL = 1u << 5 /* iterate velocities to a residual instead of a fixed count */
+};
+
+/* Body definition, same fields as b2BodyDef (src/dbox/dynamics/b2body.d:51-104). */
+typedef struct {
+    int32_t type; /* 0 static, 1 kinematic, 2 dynamic */
+    float px, py, angle;
+    float vx, vy, w;
+    float linear_damping, angular_damping, gravity_scale;
+    int32_t allow_sleep, awake, fixed_rotation, bullet;
+} dbx_body_def;
+
+/* Fixture definition: b2FixtureDef (src/dbox/dynamics/b2fixture.d:49-73) plus the
+ * shape constructor it was made with.
+ *   shape 0 circle : radius; p[0..1] = local centre            (b2CircleShape)
+ *   shape 1 edge   : p[0..3] = v1, v2; p[4..7] = v0, v3;
+ *                    flags bit1 / bit2 = has v0 / has v3        (b2EdgeShape.Set)
+ *   shape 2 polygon: count points in p[0 .. 2*count-1]          (b2PolygonShape.Set, hulled)
+ *   shape 3 box    : p[0..1] = hx, hy; flags bit3 = centre p[2..3], angle p[4]
+ *                                                               (b2PolygonShape.SetAsBox)
+ * flags bit0 = sensor (unsupported). */
+typedef struct {
+    int32_t body;
+    int32_t shape;
+    int32_t count;
+    int32_t flags;
+    float density, friction, restitution;
+    uint16_t category, mask;
+    int16_t group, pad;
+    float radius;
+    float p[20];
+} dbx_fixture_def;
+
+/* Body state read back after a step: b2Body.m_xf, m_sweep, velocities, m_flags. */
+typedef struct {
+    float px, py, qs, qc;
+    float cx, cy, a, sleep_time;
+    float vx, vy, w;
+    int32_t flags; /* b2Body flags: awake 0x2, autoSleep 0x4, ... */
+    float c0x, c0y, a0, inv_mass;
+} dbx_body_state;
+
+/* A contact in b2World.GetContactList order (most recently created first). */
+typedef struct {
+    int32_t fixture_a, fixture_b; /* world-local fixture creation indices after the type-order swap */
+    int32_t flags;                /* touching 0x2, enabled 0x4 */
+    int32_t type, point_count;    /* b2Manifold.Type, pointCount */
+    float local_normal[2], local_point[2];
+    float point_local[4];
+    float normal_impulse[2], tangent_impulse[2];
+    uint32_t id[2];               /* b2ContactID.key */
+    float friction, restitution;
+} dbx_contact;
+
+/* b2Profile (src/dbox/dynamics/b2timestep.d:37-46), milliseconds of the last step,
+ * measured with HIP events around each phase. */
+typedef struct {
+    float step, collide, solve, solveInit, solveVelocity, solvePosition, broadphase, solveTOI;
+} dbx_profile;
+
+typedef struct {
+    int32_t device;                 /* HIP device ordinal */
+    int32_t contacts_per_fixture;   /* contact capacity per fixture (0 = 8) */
+    int32_t block_size;             /* threads per world workgroup (0 = autotuned default) */
+    int32_t use_graph;              /* capture the step in a hipGraph */
+} dbx_config;
+
+typedef struct dbx_handle dbx_handle;
+
+/* Create a handle bound to one GPU.  Replaces constructing b2World (b2world.d:865-895). */
+dbx_status dbx_create(const dbx_config *config, dbx_handle **out);
+void dbx_destroy(dbx_handle *h);
+const char *dbx_last_error(const dbx_handle *h);
+
+/* Add one world, built from body and fixture definitions in creation order, as
+ * b2World.CreateBody (b2world.d:75-99) and b2Body.CreateFixture (b2body.d:116-155)
+ * would build it: hulls, mass data and proxies are computed here on the host. */
+dbx_status dbx_add_world(dbx_handle *h, const dbx_body_def *bodies, int32_t body_count,
+                         const dbx_fixture_def *fixtures, int32_t fixture_count, float gravity_x,
+                         float gravity_y, int32_t *world_id);
+/* Upload every added world to the device.  After this the world set is fixed. */
+dbx_status dbx_finalize(dbx_handle *h);
+
+/* b2World.Step(dt, velocityIterations, positionIterations) (b2world.d:367-434) for
+ * every world of the handle, asynchronously on the handle's stream. */
+dbx_status dbx_step(dbx_handle *h, float dt, int32_t velocity_iterations, int32_t position_iterations,
+                    uint32_t flags);
+dbx_status dbx_sync(dbx_handle *h);
+
+dbx_status dbx_world_count(const dbx_handle *h, int32_t *count);
+/* body count, fixture count, contact count, touching contacts */
+dbx_status dbx_world_info(dbx_handle *h, int32_t world, int32_t out[4]);
+dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, int32_t n);
+/* Contacts of one world in b2World contact-list order. */
+dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count);
+/* Fat AABBs of each fixture's proxy (lower x, lower y, upper x, upper y). */
+dbx_status dbx_read_fat_aabbs(dbx_handle *h, int32_t world, float *out, int32_t n);
+/* Pairs the last broadphase pass handed to AddPair, as (fixture, fixture) in the
+ * order b2PairLessThan sorts them (b2broadphase.d:312-325). */
+dbx_status dbx_read_pairs(dbx_handle *h, int32_t world, int32_t *out, int32_t cap, int32_t *count);
+dbx_status dbx_profile_read(dbx_handle *h, dbx_profile *out);
+/* Sums of the per-phase times of every step resolved by dbx_sync since the last
+ * reset, and the number of steps summed (the b2Profile accumulation the demo's
+ * Test.Step does, examples/demo/framework/test.d:558-578). */
+dbx_status dbx_profile_totals(dbx_handle *h, dbx_profile *sum, int32_t *steps, int32_t reset);
+/* Totals over all worlds: dynamic bodies, contacts, touching contacts, awake bodies. */
+dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
+/* Per-world velocity iterations used by the last step (RESIDUAL mode) and max penetration. */
+dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds);
+
+/* Stand-alone broadphase (b2BroadPhase.UpdatePairs semantics) on given fat
+ * AABBs and moved flags of n proxies; pairs are written sorted as (i, j), i < j. */
+dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uint8_t *moved, int32_t n,
+                                int32_t *out, int32_t cap, int32_t *count);
+/* Stand-alone narrowphase: for each of n pairs, shape a[i] at transform xa[i]
+ * (x, y, angle) against shape b[i] at xb[i]; the pair is put in b2Contact.Create's
+ * type order first.  out holds 32 floats per pair: type, pointCount, swapped, 0,
+ * localNormal(2), localPoint(2), points' local (4), ids (2, as uint32 bits),
+ * world normal (2), world points (4), separations (2), 8 unused. */
+dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const float *xa, const dbx_fixture_def *b,
+                             const float *xb, int32_t n, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+/*
+ * b2oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a scalar C++ restatement of dbox's
+ * (the D port of Box2D 2.3.1) per-step hot path, b2World.Step, written to keep
+ * the reference's floating-point operation order so that its results can be
+ * compared with the GPU path.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library never links
+ * it and never falls back to it.
+ *
+ * Every struct below is a plain C layout shared byte-for-byte with the scene
+ * descriptions that the tests hand to the GPU library (dbx_body_def /
+ * dbx_fixture_def in include/dbox_hip.h), so one numpy structured array feeds
+ * both sides.
+ */
+#ifndef DBOX_ORACLE_H
+#define DBOX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Body definition: mirrors b2BodyDef (src/dbox/dynamics/b2body.d:51-104). */
+typedef struct {
+    int32_t type; /* 0 static, 1 kinematic, 2 dynamic (b2BodyType) */
+    float px, py, angle;
+    float vx, vy, w;
+    float linear_damping, angular_damping, gravity_scale;
+    int32_t allow_sleep, awake, fixed_rotation, bullet;
+} orc_body_def; /* 56 bytes */
+
+/* Fixture definition: mirrors b2FixtureDef (src/dbox/dynamics/b2fixture.d:49-73)
+ * plus the shape constructor that the scene used.
+ *   shape 0 circle : radius, p[0..1] = local centre
+ *   shape 1 edge   : p[0..3] = v1, v2; p[4..7] = v0, v3; flags bit1/bit2 = has v0 / has v3
+ *   shape 2 polygon: count points in p[0..2*count-1], hulled by b2PolygonShape.Set
+ *   shape 3 box    : p[0..1] = hx, hy (b2PolygonShape.SetAsBox);
+ *                    flags bit3 = oriented box with centre p[2..3] and angle p[4]
+ */
+typedef struct {
+    int32_t body;
+    int32_t shape;
+    int32_t count;
+    int32_t flags; /* bit0 sensor */
+    float density, friction, restitution;
+    uint16_t category, mask;
+    int16_t group, pad;
+    float radius;
+    float p[20];
+} orc_fixture_def; /* 120 bytes */
+
+/* Body state read back after a step (indexed by body creation order). */
+typedef struct {
+    float px, py, qs, qc;      /* m_xf */
+    float cx, cy, a, sleep_time; /* m_sweep.c, m_sweep.a */
+    float vx, vy, w;
+    int32_t flags;             /* b2Body.m_flags (awake 0x2, ...) */
+    float c0x, c0y, a0, inv_mass;
+} orc_body_state; /* 64 bytes */
+
+/* A contact read back in world contact-list order (m_contactList head first). */
+typedef struct {
+    int32_t fixture_a, fixture_b; /* fixture creation indices after the type-order swap */
+    int32_t flags;                /* b2Contact.m_flags: touching 0x2, enabled 0x4 */
+    int32_t type, point_count;    /* b2Manifold.Type, pointCount */
+    float local_normal[2], local_point[2];
+    float point_local[4];         /* points[0].localPoint, points[1].localPoint */
+    float normal_impulse[2], tangent_impulse[2];
+    uint32_t id[2];               /* b2ContactID.key */
+    float friction, restitution;
+} orc_contact; /* 84 bytes */
+
+/* Manifold produced by one collide call (b2Manifold, b2collision.d:96-114)
+ * together with its b2WorldManifold (b2collision.d:123-191). */
+typedef struct {
+    int32_t type, point_count, swapped, pad;
+    float local_normal[2], local_point[2];
+    float point_local[4];
+    uint32_t id[2];
+    float normal[2], points[4], separations[2];
+} orc_manifold;
+
+void *orc_world_create(float gx, float gy);
+void orc_world_destroy(void *w);
+/* b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics.  The
+ * oracle restates no TOI solver, so continuous must be 0. */
+int orc_world_set_flags(void *w, int warm_starting, int allow_sleep, int continuous);
+int orc_create_body(void *w, const orc_body_def *def);
+int orc_create_fixture(void *w, const orc_fixture_def *def);
+/* Builds a whole scene: bodies in order, then fixtures in order. */
+int orc_build(void *w, const orc_body_def *bodies, int nb, const orc_fixture_def *fixtures, int nf);
+void orc_step(void *w, float dt, int velocity_iterations, int position_iterations);
+int orc_body_count(void *w);
+int orc_fixture_count(void *w);
+int orc_contact_count(void *w);
+void orc_get_bodies(void *w, orc_body_state *out);
+/* Contacts in m_contactList order; returns the number written (<= cap). */
+int orc_get_contacts(void *w, orc_contact *out, int cap);
+/* World manifold of contact-list entry i at the bodies' current transforms. */
+int orc_contact_world_manifold(void *w, int list_index, orc_manifold *out);
+/* Fat AABB of each fixture's proxy, 4 floats each (lower.x, lower.y, upper.x, upper.y). */
+void orc_get_fat_aabbs(void *w, float *out);
+/* The unique pairs that the most recent UpdatePairs handed to AddPair, as
+ * (fixture index, fixture index) with the smaller proxy id first, in the
+ * sorted order of b2PairLessThan. Returns the count (<= cap pairs written). */
+int orc_last_pairs(void *w, int32_t *out, int cap);
+/* b2Profile of the last step: step, collide, solve, solveInit, solveVelocity,
+ * solvePosition, broadphase, solveTOI (ms). */
+void orc_profile(void *w, float *out8);
+/* Number of dynamic bodies (for body-step counting). */
+int orc_dynamic_body_count(void *w);
+/* Maximum penetration over touching contacts: -min world-manifold separation. */
+float orc_max_penetration(void *w);
+
+/* Stand-alone collide call on two shapes built from fixture defs (bodies at
+ * (px, py, angle) transforms).  The pair is put in the order b2Contact.Create
+ * uses; out->swapped says whether A and B were exchanged.  Returns 0, or -1
+ * when the shape pair has no contact type (edge-edge). */
+int orc_collide(const orc_fixture_def *a, const float *xfa3, const orc_fixture_def *b,
+                const float *xfb3, orc_manifold *out);
+
+/* Shape geometry exactly as the reference builds it (hull, normals, centroid),
+ * for comparing the GPU library's own builder. out: count, radius, centroid(2),
+ * vertices(16), normals(16) = 36 floats; mass_out: mass, cx, cy, I. */
+int orc_shape_geometry(const orc_fixture_def *def, float *out36, float *mass_out4);
+
+/* b2BroadPhase.UpdatePairs on a b2DynamicTree holding the given fat AABBs,
+ * querying the proxies whose moved flag is set.  Returns the pair count; pairs
+ * are written as (proxy, proxy) with the smaller index first, sorted. */
+int orc_tree_pairs(const float *fat_aabbs, const uint8_t *moved, int n, int32_t *out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

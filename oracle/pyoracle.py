@@ -1,0 +1,184 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The product path (dbox_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+BODY_STATE_DTYPE = np.dtype([
+    ("px", "<f4"), ("py", "<f4"), ("qs", "<f4"), ("qc", "<f4"),
+    ("cx", "<f4"), ("cy", "<f4"), ("a", "<f4"), ("sleep_time", "<f4"),
+    ("vx", "<f4"), ("vy", "<f4"), ("w", "<f4"), ("flags", "<i4"),
+    ("c0x", "<f4"), ("c0y", "<f4"), ("a0", "<f4"), ("inv_mass", "<f4"),
+])
+CONTACT_DTYPE = np.dtype([
+    ("fixture_a", "<i4"), ("fixture_b", "<i4"), ("flags", "<i4"), ("type", "<i4"), ("point_count", "<i4"),
+    ("local_normal", "<f4", (2,)), ("local_point", "<f4", (2,)), ("point_local", "<f4", (4,)),
+    ("normal_impulse", "<f4", (2,)), ("tangent_impulse", "<f4", (2,)), ("id", "<u4", (2,)),
+    ("friction", "<f4"), ("restitution", "<f4"),
+])
+MANIFOLD_DTYPE = np.dtype([
+    ("type", "<i4"), ("point_count", "<i4"), ("swapped", "<i4"), ("pad", "<i4"),
+    ("local_normal", "<f4", (2,)), ("local_point", "<f4", (2,)), ("point_local", "<f4", (4,)),
+    ("id", "<u4", (2,)), ("normal", "<f4", (2,)), ("points", "<f4", (4,)), ("separations", "<f4", (2,)),
+])
+assert BODY_STATE_DTYPE.itemsize == 64 and CONTACT_DTYPE.itemsize == 84
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, f32 = C.c_void_p, C.c_int, C.c_float
+        L.orc_world_create.restype = vp
+        L.orc_world_create.argtypes = [f32, f32]
+        L.orc_world_destroy.argtypes = [vp]
+        L.orc_world_set_flags.argtypes = [vp, i32, i32, i32]
+        L.orc_build.argtypes = [vp, vp, i32, vp, i32]
+        L.orc_step.argtypes = [vp, f32, i32, i32]
+        for name in ("orc_body_count", "orc_fixture_count", "orc_contact_count", "orc_dynamic_body_count"):
+            getattr(L, name).argtypes = [vp]
+        L.orc_get_bodies.argtypes = [vp, vp]
+        L.orc_get_contacts.argtypes = [vp, vp, i32]
+        L.orc_contact_world_manifold.argtypes = [vp, i32, vp]
+        L.orc_get_fat_aabbs.argtypes = [vp, vp]
+        L.orc_last_pairs.argtypes = [vp, vp, i32]
+        L.orc_profile.argtypes = [vp, vp]
+        L.orc_max_penetration.argtypes = [vp]
+        L.orc_max_penetration.restype = f32
+        L.orc_collide.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_shape_geometry.argtypes = [vp, vp, vp]
+        L.orc_tree_pairs.argtypes = [vp, vp, i32, vp, i32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleWorld:
+    """b2World restated on the CPU, built from a scene's definitions."""
+
+    def __init__(self, scene, warm_starting=True, allow_sleep=True):
+        L = lib()
+        self.L = L
+        self.h = L.orc_world_create(float(scene.gravity[0]), float(scene.gravity[1]))
+        if L.orc_world_set_flags(self.h, int(warm_starting), int(allow_sleep), 0) != 0:
+            raise RuntimeError("oracle flag rejected")
+        self.bodies_def = np.ascontiguousarray(scene.bodies)
+        self.fixtures_def = np.ascontiguousarray(scene.fixtures)
+        if L.orc_build(self.h, _p(self.bodies_def), len(self.bodies_def), _p(self.fixtures_def), len(self.fixtures_def)) != 0:
+            raise ValueError("oracle rejected the scene")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_world_destroy(self.h)
+            self.h = None
+
+    def step(self, dt=1.0 / 60.0, vel=8, pos=3, n=1):
+        for _ in range(n):
+            self.L.orc_step(self.h, dt, vel, pos)
+
+    def bodies(self) -> np.ndarray:
+        out = np.zeros(self.L.orc_body_count(self.h), BODY_STATE_DTYPE)
+        self.L.orc_get_bodies(self.h, _p(out))
+        return out
+
+    def contacts(self) -> np.ndarray:
+        n = self.L.orc_contact_count(self.h)
+        out = np.zeros(n, CONTACT_DTYPE)
+        got = self.L.orc_get_contacts(self.h, _p(out), n)
+        return out[:got]
+
+    def world_manifold(self, list_index: int) -> np.ndarray:
+        out = np.zeros(1, MANIFOLD_DTYPE)
+        self.L.orc_contact_world_manifold(self.h, list_index, _p(out))
+        return out[0]
+
+    def fat_aabbs(self) -> np.ndarray:
+        out = np.zeros((self.L.orc_fixture_count(self.h), 4), np.float32)
+        self.L.orc_get_fat_aabbs(self.h, _p(out))
+        return out
+
+    def last_pairs(self) -> np.ndarray:
+        n = self.L.orc_last_pairs(self.h, None, 0)
+        out = np.zeros((max(n, 1), 2), np.int32)
+        self.L.orc_last_pairs(self.h, _p(out), n)
+        return out[:n]
+
+    def profile(self) -> np.ndarray:
+        out = np.zeros(8, np.float32)
+        self.L.orc_profile(self.h, _p(out))
+        return out
+
+    def max_penetration(self) -> float:
+        return float(self.L.orc_max_penetration(self.h))
+
+    @property
+    def dynamic_count(self) -> int:
+        return self.L.orc_dynamic_body_count(self.h)
+
+
+def collide(fix_a, xfa, fix_b, xfb) -> np.ndarray | None:
+    """One b2Collide* call on two shapes built from fixture defs."""
+    L = lib()
+    a = np.ascontiguousarray(np.array(fix_a).reshape(1))
+    b = np.ascontiguousarray(np.array(fix_b).reshape(1))
+    xa = np.asarray(xfa, np.float32)
+    xb = np.asarray(xfb, np.float32)
+    out = np.zeros(1, MANIFOLD_DTYPE)
+    if L.orc_collide(_p(a), _p(xa), _p(b), _p(xb), _p(out)) != 0:
+        return None
+    return out[0]
+
+
+def shape_geometry(fix):
+    L = lib()
+    a = np.ascontiguousarray(np.array(fix).reshape(1))
+    g = np.zeros(36, np.float32)
+    m = np.zeros(4, np.float32)
+    L.orc_shape_geometry(_p(a), _p(g), _p(m))
+    return g, m
+
+
+def tree_pairs(fat: np.ndarray, moved: np.ndarray) -> np.ndarray:
+    L = lib()
+    fat = np.ascontiguousarray(fat, np.float32)
+    moved = np.ascontiguousarray(moved, np.uint8)
+    n = len(fat)
+    cap = max(16, 64 * n)
+    out = np.zeros((cap, 2), np.int32)
+    cnt = L.orc_tree_pairs(_p(fat), _p(moved), n, _p(out), cap)
+    assert cnt <= cap
+    return out[:cnt]
+
+
+def brute_force_pairs(fat: np.ndarray, moved: np.ndarray) -> np.ndarray:
+    """Sorted unique (i < j) pairs with touching-inclusive fat overlap and a moved member."""
+    fat = np.asarray(fat, np.float32)
+    lo_x, lo_y, hi_x, hi_y = fat[:, 0], fat[:, 1], fat[:, 2], fat[:, 3]
+    ov = ~((lo_x[None, :] - hi_x[:, None] > 0) | (lo_y[None, :] - hi_y[:, None] > 0) |
+           (lo_x[:, None] - hi_x[None, :] > 0) | (lo_y[:, None] - hi_y[None, :] > 0))
+    m = np.asarray(moved, bool)
+    ov &= (m[:, None] | m[None, :])
+    i, j = np.nonzero(np.triu(ov, 1))
+    return np.stack([i, j], 1).astype(np.int32)

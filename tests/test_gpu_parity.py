@@ -1,0 +1,279 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Tiers:
+  * broadphase pairs bit-exact with b2BroadPhase.UpdatePairs on the same fat AABBs;
+  * manifold points, normals, separations within 1e-5 m;
+  * after 60 steps bodies within 1e-3 m / 1e-3 rad of the sequential-impulse run,
+    maximum penetration over 600 steps no worse than the reference by more than b2_linearSlop;
+  * batched worlds bit-identical to the same worlds run alone.
+"""
+import numpy as np
+import pytest
+
+from dbox_amd import native, scenes
+from oracle import pyoracle as po
+from tests.helpers import box, circle, edge, random_shape
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+SLOP = 0.005
+
+
+def gpu_world(scene_list, **kw):
+    h = native.StepWorlds(**kw)
+    for s in scene_list:
+        h.add_scene(s)
+    h.finalize()
+    return h
+
+
+def angle_diff(a, b):
+    d = (np.asarray(a, np.float64) - np.asarray(b, np.float64) + np.pi) % (2 * np.pi) - np.pi
+    return np.abs(d)
+
+
+# ---------------------------------------------------------------- narrowphase
+def test_narrowphase_random_pairs_match_oracle():
+    print("case: narrowphase random pairs, seed 11, n 3000", flush=True)
+    rng = np.random.default_rng(11)
+    n = 3000
+    fa = np.array([random_shape(rng) for _ in range(n)], scenes.FIXTURE_DTYPE)
+    fb = np.array([random_shape(rng) for _ in range(n)], scenes.FIXTURE_DTYPE)
+    xa = np.stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-3.2, 3.2, n)], 1).astype(np.float32)
+    xb = np.stack([xa[:, 0] + rng.uniform(-1.6, 1.6, n), xa[:, 1] + rng.uniform(-1.6, 1.6, n),
+                   rng.uniform(-3.2, 3.2, n)], 1).astype(np.float32)
+    h = native.StepWorlds()
+    out = native.collide_pairs(h, fa, xa, fb, xb)
+    touching = exact = 0
+    for i in range(n):
+        ref = po.collide(fa[i], xa[i], fb[i], xb[i])
+        g = out[i]
+        if ref is None:
+            assert g[0] == -1.0
+            continue
+        cnt = int(ref["point_count"])
+        assert int(g[2]) == ref["swapped"]
+        if int(g[1]) != cnt:
+            # only a near-tangent pair may flip between 0 and 1 points on a 1-ulp transform difference
+            assert cnt == 0 or int(g[1]) == 0
+            seps = ref["separations"][:cnt] if cnt else g[20:22][: int(g[1])]
+            assert np.abs(seps).min() < 0.02
+            continue
+        if cnt == 0:
+            continue
+        touching += 1
+        assert int(g[0]) == ref["type"]
+        ids = g[12:14].view(np.uint32)[:cnt]
+        assert np.array_equal(ids, ref["id"][:cnt])
+        np.testing.assert_allclose(g[4:6], ref["local_normal"], atol=1e-5)
+        np.testing.assert_allclose(g[6:8], ref["local_point"], atol=1e-5)
+        np.testing.assert_allclose(g[8:8 + 2 * cnt], ref["point_local"][: 2 * cnt], atol=1e-5)
+        np.testing.assert_allclose(g[14:16], ref["normal"], atol=1e-5)
+        np.testing.assert_allclose(g[16:16 + 2 * cnt], ref["points"][: 2 * cnt], atol=1e-5)
+        np.testing.assert_allclose(g[20:20 + cnt], ref["separations"][:cnt], atol=1e-5)
+        exact += int(np.array_equal(g[20:20 + cnt], ref["separations"][:cnt]))
+    assert touching > 500
+    assert exact > 0.95 * touching  # identical except where a libm sin/cos rounds differently
+
+
+# ---------------------------------------------------------------- broadphase
+def _fat(rng, n, snap):
+    lo = rng.uniform(-20, 20, (n, 2)).astype(np.float32)
+    sz = rng.uniform(0.05, 2.0, (n, 2)).astype(np.float32)
+    if snap:
+        lo = np.round(lo).astype(np.float32)
+        sz = (np.round(sz * 2) / 2 + 0.5).astype(np.float32)
+    return np.concatenate([lo, lo + sz], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed,n,snap", [(888, 128, False), (5, 2000, False), (6, 700, True), (7, 1, False), (8, 0, False),
+                                         (9, 257, True)])
+def test_broadphase_pairs_bit_exact(seed, n, snap):
+    print(f"case: broadphase seed {seed} n {n} snap {snap}", flush=True)
+    rng = np.random.default_rng(seed)
+    fat = _fat(rng, n, snap)
+    h = native.StepWorlds()
+    for frac in (1.0, 0.25, 0.0):
+        moved = (rng.random(n) < frac).astype(np.uint8)
+        got = native.broadphase_pairs(h, fat, moved)
+        want = po.tree_pairs(fat, moved)
+        assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------- full step
+def test_hello_world_ordered_matches_oracle():
+    print("case: hello_world ordered 60 steps", flush=True)
+    s = scenes.hello_world()
+    o = po.OracleWorld(s)
+    g = gpu_world([s])
+    for i in range(60):
+        o.step(DT, 6, 2)
+        g.step(DT, 6, 2, native.WARM | native.SLEEP | native.ORDERED)
+    bo, bg = o.bodies(), g.bodies()
+    np.testing.assert_allclose(bg["px"], bo["px"], atol=1e-5)
+    np.testing.assert_allclose(bg["py"], bo["py"], atol=1e-5)
+    assert angle_diff(bg["a"], bo["a"]).max() < 1e-5
+    assert abs(bg["py"][1] - 1.01) < 0.01
+
+
+def test_pyramid_first_step_pairs_contacts_and_manifolds():
+    print("case: pyramid step 1", flush=True)
+    s = scenes.pyramid()
+    o = po.OracleWorld(s)
+    g = gpu_world([s])
+    o.step()
+    g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
+    # pairs of the end-of-step UpdatePairs
+    assert np.array_equal(g.pairs(), o.last_pairs())
+    co, cg = o.contacts(), g.contacts()
+    assert len(co) == len(cg)
+    assert np.array_equal(cg["fixture_a"], co["fixture_a"]) and np.array_equal(cg["fixture_b"], co["fixture_b"])
+    assert np.array_equal(cg["flags"] & 6, co["flags"] & 6)
+    np.testing.assert_allclose(g.fat_aabbs(), o.fat_aabbs(), atol=1e-5)
+
+
+def _lockstep(scene, steps, flags, check_every=1):
+    o = po.OracleWorld(scene)
+    g = gpu_world([scene])
+    for i in range(steps):
+        o.step()
+        g.step(flags=flags)
+    return o, g
+
+
+def test_pyramid_manifolds_within_1e5_after_steps():
+    print("case: pyramid manifolds after 30 ordered steps", flush=True)
+    o, g = _lockstep(scenes.pyramid(), 30, native.WARM | native.SLEEP | native.ORDERED)
+    co, cg = o.contacts(), g.contacts()
+    assert len(co) == len(cg)
+    assert np.array_equal(cg["fixture_a"], co["fixture_a"]) and np.array_equal(cg["fixture_b"], co["fixture_b"])
+    t = (co["flags"] & 2) != 0
+    assert np.array_equal((cg["flags"] & 2) != 0, t)
+    for k in np.nonzero(t)[0][:400]:
+        n = co["point_count"][k]
+        assert cg["point_count"][k] == n and cg["type"][k] == co["type"][k]
+        np.testing.assert_allclose(cg["point_local"][k][: 2 * n], co["point_local"][k][: 2 * n], atol=1e-5)
+        np.testing.assert_allclose(cg["local_normal"][k], co["local_normal"][k], atol=1e-5)
+        assert np.array_equal(cg["id"][k][:n], co["id"][k][:n])
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+def test_pyramid_60_steps_within_tolerance(ordered):
+    print(f"case: pyramid 60 steps ordered={ordered}", flush=True)
+    flags = native.WARM | native.SLEEP | (native.ORDERED if ordered else 0)
+    o, g = _lockstep(scenes.pyramid(), 60, flags)
+    bo, bg = o.bodies(), g.bodies()
+    dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"]).max()
+    da = angle_diff(bg["a"], bo["a"]).max()
+    print(f"pyramid 60 steps ordered={ordered}: max |dx| {dp:.3e} m, max |da| {da:.3e} rad")
+    assert dp < 1e-3 and da < 1e-3
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+def test_pyramid_600_steps_penetration(ordered):
+    print(f"case: pyramid 600 steps penetration ordered={ordered}", flush=True)
+    flags = native.WARM | native.SLEEP | (native.ORDERED if ordered else 0)
+    s = scenes.pyramid()
+    o = po.OracleWorld(s)
+    g = gpu_world([s])
+    worst_o = worst_g = 0.0
+    for i in range(600):
+        o.step()
+        g.step(flags=flags)
+        if i % 20 == 19:
+            worst_o = max(worst_o, o.max_penetration())
+            worst_g = max(worst_g, float(g.solver_stats()[1][0]))
+    print(f"max penetration oracle {worst_o:.4f} gpu {worst_g:.4f}")
+    assert worst_g <= worst_o + SLOP
+
+
+def test_broadphase_on_oracle_inputs_every_step():
+    # Feed the oracle's own fat AABBs after each step to the GPU pair kernel: same inputs,
+    # bit-identical pair sets (b2broadphase.d:139-195).
+    print("case: pyramid lockstep broadphase 120 steps", flush=True)
+    s = scenes.pyramid()
+    o = po.OracleWorld(s)
+    h = native.StepWorlds()
+    prev = o.fat_aabbs()
+    for i in range(120):
+        o.step()
+        fat = o.fat_aabbs()
+        moved = np.any(fat != prev, axis=1).astype(np.uint8)
+        prev = fat
+        got = native.broadphase_pairs(h, fat, moved)
+        want = po.brute_force_pairs(fat, moved)
+        assert np.array_equal(got, want)
+        assert np.array_equal(o.last_pairs(), want) or moved.sum() == 0
+
+
+def test_batched_worlds_bit_identical_to_single():
+    print("case: 12 batched worlds vs single, 90 steps", flush=True)
+    worlds = scenes.batched_worlds(12, seed=99)
+    flags = native.DEFAULT_FLAGS
+    g = gpu_world(worlds)
+    g.step(flags=flags, n=90)
+    for k in (0, 5, 11):
+        single = gpu_world([worlds[k]])
+        single.step(flags=flags, n=90)
+        a, b = g.bodies(k), single.bodies(0)
+        assert a.tobytes() == b.tobytes()
+        assert g.contacts(k).tobytes() == single.contacts(0).tobytes()
+
+
+def test_repeated_runs_are_identical():
+    print("case: determinism 2 runs of 8 worlds", flush=True)
+    worlds = scenes.batched_worlds(8, seed=5)
+    outs = []
+    for _ in range(2):
+        g = gpu_world(worlds)
+        g.step(n=60)
+        outs.append(b"".join(g.bodies(k).tobytes() for k in range(8)))
+    assert outs[0] == outs[1]
+
+
+def test_mixed_world_tracks_oracle_ordered():
+    print("case: mixed 300-body container world, 60 ordered steps", flush=True)
+    s = scenes.mixed_container(300, 40.0, 80.0, seed=1234)
+    o, g = _lockstep(s, 60, native.WARM | native.SLEEP | native.ORDERED)
+    bo, bg = o.bodies(), g.bodies()
+    dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"])
+    print(f"mixed world: median |dx| {np.median(dp):.2e}, max {dp.max():.2e}")
+    assert np.median(dp) < 1e-4
+    co, cg = o.contacts(), g.contacts()
+    assert abs(len(co) - len(cg)) <= max(2, len(co) // 50)
+
+
+def test_capacity_error_is_reported_and_handle_survives():
+    print("case: capacity error", flush=True)
+    s = scenes.pyramid()
+    g = native.StepWorlds(contacts_per_fixture=1)
+    g.add_scene(s)
+    g.finalize()
+    g.step(n=3)
+    with pytest.raises(native.DbxError) as e:
+        g.sync()
+    assert e.value.status == native.DBX_E_CAPACITY
+    g.close()
+
+
+def test_invalid_inputs_are_rejected():
+    g = native.StepWorlds()
+    s = scenes.pyramid()
+    bad = scenes.Scene(s.bodies.copy(), s.fixtures.copy())
+    bad.bodies["px"][3] = np.nan
+    with pytest.raises(native.DbxError) as e:
+        g.add_scene(bad)
+    assert e.value.status == native.DBX_E_INVALID_ARG
+    sensor = scenes.Scene(s.bodies.copy(), s.fixtures.copy())
+    sensor.fixtures["flags"][5] = 1
+    with pytest.raises(native.DbxError) as e:
+        g.add_scene(sensor)
+    assert e.value.status == native.DBX_E_UNSUPPORTED
+    g.add_scene(s)
+    g.finalize()
+    with pytest.raises(native.DbxError) as e:
+        g.step(flags=native.CCD)
+    assert e.value.status == native.DBX_E_UNSUPPORTED
+    with pytest.raises(native.DbxError) as e:
+        g.add_scene(s)
+    assert e.value.status == native.DBX_E_LOCKED
