@@ -1368,8 +1368,11 @@ struct World {
     bool warmStarting = true, continuousPhysics = false, allowSleep = true;
     bool newFixture = false;
     float inv_dt0 = 0.0f;
+    int solveOrder = 0;
+    int islandCount = 0; /* test hook: 1 = solve island contacts in greedy-colour order */
     float profile[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<Pair> lastPairs; /* fixture-index pairs handed to AddPair */
+    std::vector<int> solveLog;   /* test hook: per solved contact (island, bodyA, bodyB, dynamic mask) */
     std::vector<int> proxyToFixture;
 
     /* island scratch */
@@ -2105,9 +2108,40 @@ struct World {
         (void)t0;
     }
 
+    /* Test hook, not reference behaviour: reorder the island's contacts by greedy
+     * colour (lowest colour not used by either dynamic body, in island order),
+     * stable within a colour. */
+    void colourOrder() {
+        std::vector<unsigned long long> mask(bodies.size(), 0ull);
+        std::vector<std::pair<int, int>> keyed;
+        for (size_t k = 0; k < islandContacts.size(); ++k) {
+            const Contact &c = contacts[islandContacts[k]];
+            int a = fixtures[c.fixtureA].body, b = fixtures[c.fixtureB].body;
+            bool da = bodies[a].type == b_dynamic, db = bodies[b].type == b_dynamic;
+            unsigned long long used = (da ? mask[a] : 0ull) | (db ? mask[b] : 0ull);
+            int col = 0;
+            while (col < 64 && (used >> col & 1ull)) ++col;
+            if (col < 64) {
+                if (da) mask[a] |= 1ull << col;
+                if (db) mask[b] |= 1ull << col;
+            } else {
+                col = 64 + (int)k;
+            }
+            keyed.push_back({col, (int)k});
+        }
+        std::stable_sort(keyed.begin(), keyed.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+            return x.first < y.first;
+        });
+        std::vector<int> reordered;
+        for (auto &kv : keyed) reordered.push_back(islandContacts[kv.second]);
+        islandContacts.swap(reordered);
+    }
+
     /* b2World.Solve b2world.d:930-1124 */
     void solve(const TimeStep &step) {
         profile[3] = profile[4] = profile[5] = 0.0f;
+        solveLog.clear();
+        islandCount = 0;
         for (int b = bodyList; b != kNull; b = bodies[b].next) bodies[b].flags &= ~bf_island;
         for (int c = contactList; c != kNull; c = contacts[c].next) contacts[c].flags &= ~cf_island;
         stack.resize(bodies.size());
@@ -2142,6 +2176,16 @@ struct World {
                     bodies[other].flags |= bf_island;
                 }
             }
+            if (solveOrder == 1) colourOrder();
+            for (int ci : islandContacts) {
+                const Contact &c = contacts[ci];
+                int a = fixtures[c.fixtureA].body, b = fixtures[c.fixtureB].body;
+                solveLog.push_back(islandCount);
+                solveLog.push_back(a);
+                solveLog.push_back(b);
+                solveLog.push_back((bodies[a].type == b_dynamic ? 1 : 0) | (bodies[b].type == b_dynamic ? 2 : 0));
+            }
+            ++islandCount;
             solveIsland(step);
             for (int bi : islandBodies) {
                 if (bodies[bi].type == b_static) bodies[bi].flags &= ~bf_island;
@@ -2230,6 +2274,16 @@ int orc_world_set_flags(void *w, int warm, int sleep, int continuous) {
     W->allowSleep = sleep != 0;
     if (continuous) return -1;
     W->continuousPhysics = false;
+    return 0;
+}
+int orc_solve_log(void *w, int32_t *out, int cap) {
+    World *W = (World *)w;
+    int n = (int)W->solveLog.size() / 4;
+    for (int i = 0; i < 4 * n && i < 4 * cap; ++i) out[i] = W->solveLog[i];
+    return n;
+}
+int orc_world_set_solve_order(void *w, int mode) {
+    ((World *)w)->solveOrder = mode;
     return 0;
 }
 int orc_create_body(void *w, const orc_body_def *def) { return ((World *)w)->createBody(*def); }

@@ -87,6 +87,12 @@ void orc_world_destroy(void *w);
 /* b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics.  The
  * oracle restates no TOI solver, so continuous must be 0. */
 int orc_world_set_flags(void *w, int warm_starting, int allow_sleep, int continuous);
+/* Test hook (not reference behaviour): 1 = solve each island's contacts in greedy
+ * colour order, to measure what reordering Gauss-Seidel alone changes. */
+int orc_world_set_solve_order(void *w, int mode);
+/* Test hook: the contacts the last step solved, in solve order, as (island,
+ * bodyA, bodyB, dynamic mask) rows; returns the row count. */
+int orc_solve_log(void *w, int32_t *out, int cap);
 int orc_create_body(void *w, const orc_body_def *def);
 int orc_create_fixture(void *w, const orc_fixture_def *def);
 /* Builds a whole scene: bodies in order, then fixtures in order. */

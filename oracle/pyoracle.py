@@ -53,6 +53,8 @@ def lib():
         L.orc_world_destroy.argtypes = [vp]
         L.orc_world_set_flags.argtypes = [vp, i32, i32, i32]
         L.orc_build.argtypes = [vp, vp, i32, vp, i32]
+        L.orc_world_set_solve_order.argtypes = [vp, i32]
+        L.orc_solve_log.argtypes = [vp, vp, i32]
         L.orc_step.argtypes = [vp, f32, i32, i32]
         for name in ("orc_body_count", "orc_fixture_count", "orc_contact_count", "orc_dynamic_body_count"):
             getattr(L, name).argtypes = [vp]
@@ -78,12 +80,13 @@ def _p(a: np.ndarray):
 class OracleWorld:
     """b2World restated on the CPU, built from a scene's definitions."""
 
-    def __init__(self, scene, warm_starting=True, allow_sleep=True):
+    def __init__(self, scene, warm_starting=True, allow_sleep=True, colour_order=False):
         L = lib()
         self.L = L
         self.h = L.orc_world_create(float(scene.gravity[0]), float(scene.gravity[1]))
         if L.orc_world_set_flags(self.h, int(warm_starting), int(allow_sleep), 0) != 0:
             raise RuntimeError("oracle flag rejected")
+        L.orc_world_set_solve_order(self.h, 1 if colour_order else 0)
         self.bodies_def = np.ascontiguousarray(scene.bodies)
         self.fixtures_def = np.ascontiguousarray(scene.fixtures)
         if L.orc_build(self.h, _p(self.bodies_def), len(self.bodies_def), _p(self.fixtures_def), len(self.fixtures_def)) != 0:
@@ -123,6 +126,12 @@ class OracleWorld:
         n = self.L.orc_last_pairs(self.h, None, 0)
         out = np.zeros((max(n, 1), 2), np.int32)
         self.L.orc_last_pairs(self.h, _p(out), n)
+        return out[:n]
+
+    def solve_log(self) -> np.ndarray:
+        n = self.L.orc_solve_log(self.h, None, 0)
+        out = np.zeros((max(n, 1), 4), np.int32)
+        self.L.orc_solve_log(self.h, _p(out), n)
         return out[:n]
 
     def profile(self) -> np.ndarray:

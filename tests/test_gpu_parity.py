@@ -157,15 +157,32 @@ def test_pyramid_manifolds_within_1e5_after_steps():
         assert np.array_equal(cg["id"][k][:n], co["id"][k][:n])
 
 
-@pytest.mark.parametrize("ordered", [True, False])
-def test_pyramid_60_steps_within_tolerance(ordered):
-    print(f"case: pyramid 60 steps ordered={ordered}", flush=True)
-    flags = native.WARM | native.SLEEP | (native.ORDERED if ordered else 0)
-    o, g = _lockstep(scenes.pyramid(), 60, flags)
+def test_pyramid_60_steps_within_tolerance():
+    print("case: pyramid 60 steps in the reference order", flush=True)
+    o, g = _lockstep(scenes.pyramid(), 60, native.WARM | native.SLEEP | native.ORDERED)
     bo, bg = o.bodies(), g.bodies()
     dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"]).max()
     da = angle_diff(bg["a"], bo["a"]).max()
-    print(f"pyramid 60 steps ordered={ordered}: max |dx| {dp:.3e} m, max |da| {da:.3e} rad")
+    print(f"pyramid 60 steps: max |dx| {dp:.3e} m, max |da| {da:.3e} rad")
+    assert dp < 1e-3 and da < 1e-3
+
+
+def test_coloured_schedule_matches_colour_order_oracle():
+    # The greedy-colour schedule solves the same constraints in a different
+    # Gauss-Seidel order.  Checked against the oracle run in that same order; the
+    # distance to the reference order itself is a property of the reordering
+    # (0.11 m on this pyramid at step 60), not of the kernels.
+    print("case: pyramid 60 coloured steps vs colour-order oracle", flush=True)
+    s = scenes.pyramid()
+    o = po.OracleWorld(s, colour_order=True)
+    g = gpu_world([s])
+    for _ in range(60):
+        o.step()
+        g.step(flags=native.WARM | native.SLEEP)
+    bo, bg = o.bodies(), g.bodies()
+    dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"]).max()
+    da = angle_diff(bg["a"], bo["a"]).max()
+    print(f"coloured vs colour-order oracle: max |dx| {dp:.3e} m, max |da| {da:.3e} rad")
     assert dp < 1e-3 and da < 1e-3
 
 
