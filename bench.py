@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--cpu-sample-every", type=int, default=32)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (worlds shard across GPUs with no "
@@ -114,6 +115,11 @@ def main():
     t_gen = time.perf_counter()
     worlds = scenes.batched_worlds(args.worlds)
     log(f"generated {len(worlds)} worlds in {time.perf_counter() - t_gen:.1f}s")
+    if args.subset:
+        kind, n = args.subset.split(":")
+        by_size = sorted(range(len(worlds)), key=lambda i: len(worlds[i].bodies), reverse=(kind == "largest"))
+        worlds = [worlds[i] for i in sorted(by_size[: int(n)])]
+        log(f"subset {args.subset}: bodies per world {[len(w.bodies) for w in worlds][:8]}...")
     h = native.StepWorlds()
     for s in worlds:
         h.add_scene(s)

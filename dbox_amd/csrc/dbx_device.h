@@ -50,6 +50,7 @@ struct WorldDesc {
     int pairBase, pairCap;
     float gx, gy;
     int islandBase, pad;
+    long long islScratchBase; // byte offset of the world's island-search scratch
 };
 
 struct WorldState {
@@ -144,13 +145,15 @@ struct Dev {
     int *wake;              // list position of the first wake in Collide
     unsigned long long *bmask; // colour masks
     // per contact-slot scratch
-    int *adjList;           // 2 entries per contact slot: contact * 2 + side
+    int2 *adjList;          // 2 entries per contact slot: (contact * 2 + side, other body)
     int2 *pairs;
     int *ordC;              // island-ordered contact indices
     int *cbatch;            // batch of each ordered constraint
     int *cisl;              // island of each ordered constraint
     int *sched;             // ordered constraints grouped by batch
     int *batchOff;          // per world: contactCap + 65 batch offsets at contactBase + 65 * world
+    int *batchCur;          // scatter cursors, same layout as batchOff
+    char *islScratch;       // island-search scratch of worlds that do not fit in LDS
     VelCon *vc;
     PosCon *pc;
     // per island scratch (island index < body count)

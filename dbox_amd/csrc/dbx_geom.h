@@ -93,9 +93,11 @@ __device__ __forceinline__ float normalize(float2 &v) {
 // b2Rot.Set (b2math.d:483-488) calls sinf/cosf.  Evaluated in double and
 // rounded once, which gives the correctly rounded float.
 __device__ __forceinline__ Rot rot(float angle) {
+    double sd, cd;
+    sincos((double)angle, &sd, &cd);
     Rot q;
-    q.s = (float)sin((double)angle);
-    q.c = (float)cos((double)angle);
+    q.s = (float)sd;
+    q.c = (float)cd;
     return q;
 }
 __device__ __forceinline__ float2 mul(Rot q, float2 v) { return v2(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }

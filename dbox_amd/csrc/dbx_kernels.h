@@ -10,8 +10,12 @@ namespace dbx {
 constexpr int kMaxBlock = 1024;
 
 hipError_t launch_collide(const Dev &d, int cur, int block, hipStream_t s);
-hipError_t launch_islands(const Dev &d, int cur, int ordered, int block, hipStream_t s);
+size_t islands_scratch_bytes(int bodies, int contacts);
+hipError_t configure_kernels(int maxDynamicLds);
+hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, hipStream_t s);
 hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int block, hipStream_t s);
+hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
+                             hipStream_t s);
 hipError_t launch_broadphase(const Dev &d, int cur, int block, hipStream_t s);
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, int *cnt, int2 *out, int cap, int *total,
                             hipStream_t s);

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -396,6 +397,10 @@ struct dbx_handle {
     float inv_dt0 = 0.0f;
     bool newFixture = true;
     int block = 256;
+    int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
+    int islSmallLds = 24576;      // island-search LDS of ordinary worlds
+    int solveSmallLds = 32768;    // exact-order solver LDS of ordinary worlds
+    int nBigIsl = 0;              // worlds (largest first) launched with ldsMax
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
     std::vector<hipEvent_t> ring;   // 5 events per step since the last sync
@@ -462,6 +467,16 @@ dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
         return DBX_E_HIP;
     }
     for (auto &e : h->ev) (void)hipEventCreate(&e);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->cfg.device) == hipSuccess) {
+        size_t lim = prop.sharedMemPerBlock;
+        if (lim > 160 * 1024) lim = 160 * 1024;
+        if (lim >= 32768) h->ldsMax = (int)lim;
+    }
+    if (configure_kernels(h->ldsMax) != hipSuccess) h->ldsMax = 65536;
+    if (getenv("DBX_VERBOSE"))
+        fprintf(stderr, "dbx: device %d, %d CUs, LDS per workgroup %d bytes\n", h->cfg.device,
+                prop.multiProcessorCount, h->ldsMax);
     *out = h;
     return DBX_OK;
 }
@@ -508,7 +523,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     const int nw = (int)h->pending.size();
     h->nworlds = nw;
     h->W.resize(nw);
-    long long nb = 0, nf = 0, nc = 0, np = 0;
+    long long nb = 0, nf = 0, nc = 0, np = 0, islBytes = 0;
     int cpf = h->cfg.contacts_per_fixture;
     for (int w = 0; w < nw; ++w) {
         const WorldHost &H = h->pending[w];
@@ -525,6 +540,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
         D.gy = H.gy;
         D.islandBase = D.bodyBase;
         D.pad = 0;
+        D.islScratchBase = islBytes;
+        islBytes += (long long)islands_scratch_bytes(D.bodyCount, D.contactCap);
         nb += D.bodyCount;
         nf += D.fixCount;
         nc += D.contactCap;
@@ -540,6 +557,13 @@ dbx_status dbx_finalize(dbx_handle *h) {
     std::vector<int> order(nw);
     for (int w = 0; w < nw; ++w) order[w] = w;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->W[a].fixCount > h->W[b].fixCount; });
+    // Island-search LDS classes: a world whose search state (at four contacts per
+    // fixture) exceeds the small class gets a workgroup with the device's full LDS.
+    h->nBigIsl = 0;
+    for (int i = 0; i < nw; ++i) {
+        const WorldDesc &D = h->W[order[i]];
+        if (islands_scratch_bytes(D.bodyCount, 4 * D.fixCount) > (size_t)h->islSmallLds) h->nBigIsl = i + 1;
+    }
 
     Dev &d = h->d;
     d.nworlds = nw;
@@ -630,6 +654,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.cisl, nc));
     DBX_HIP(h, dalloc(h, &d.sched, nc));
     DBX_HIP(h, dalloc(h, &d.batchOff, nc + 65LL * nw));
+    DBX_HIP(h, dalloc(h, &d.batchCur, nc + 65LL * nw));
+    DBX_HIP(h, dalloc(h, &d.islScratch, islBytes));
     DBX_HIP(h, dalloc(h, &d.vc, nc));
     DBX_HIP(h, dalloc(h, &d.pc, nc));
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
@@ -678,13 +704,21 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     h->cur ^= 1;
     (void)hipEventRecord(ev[1], s);
     if (dt > 0.0f) {
-        DBX_HIP(h, launch_islands(h->d, h->cur, (flags & DBX_STEP_ORDERED) ? 1 : 0, h->block, s));
+        int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
+        DBX_HIP(h, launch_islands(h->d, h->cur, ordered, 0, h->nBigIsl, h->ldsMax, s));
+        DBX_HIP(h, launch_islands(h->d, h->cur, ordered, h->nBigIsl, h->nworlds - h->nBigIsl, h->islSmallLds, s));
         (void)hipEventRecord(ev[2], s);
         if (flags & DBX_STEP_LEARNED) {
             dbx_status st = mlp_step_hook(h->mlp, h->d, h->cur, sp, h->block, s);
             if (st != DBX_OK) return fail(h, st, "learned warm start failed");
         }
-        DBX_HIP(h, launch_solve(h->d, h->cur, sp, h->block, s));
+        if (ordered) {
+            // the fallback path of the kernel keeps a few bytes of static LDS
+            DBX_HIP(h, launch_solve_wave(h->d, h->cur, sp, 0, h->nBigIsl, h->ldsMax - 1024, s));
+            DBX_HIP(h, launch_solve_wave(h->d, h->cur, sp, h->nBigIsl, h->nworlds - h->nBigIsl, h->solveSmallLds, s));
+        } else {
+            DBX_HIP(h, launch_solve(h->d, h->cur, sp, h->block, s));
+        }
         (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(h->d, h->cur, h->block, s));
         (void)hipEventRecord(ev[4], s);
@@ -916,6 +950,35 @@ dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_
         DBX_HIP(h, launch_max_penetration(h->d, h->cur, h->dPen, h->stream));
         DBX_HIP(h, hipMemcpyAsync(max_penetration, h->dPen, h->nworlds * sizeof(float), hipMemcpyDeviceToHost, h->stream));
         DBX_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_schedule(dbx_handle *h, int32_t world, int32_t *out, int32_t cap, int32_t *count) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    int n = S.constraints;
+    if (count) *count = n;
+    if (!out || cap <= 0 || n == 0) return DBX_OK;
+    std::vector<int> ord(n), isl(n), bat(n), fb(D.fixCount);
+    std::vector<Contact> c(S.contactCount);
+    DBX_HIP(h, hipMemcpy(ord.data(), h->d.ordC + D.contactBase, n * sizeof(int), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(isl.data(), h->d.cisl + D.contactBase, n * sizeof(int), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(bat.data(), h->d.cbatch + D.contactBase, n * sizeof(int), hipMemcpyDeviceToHost));
+    DBX_HIP(h, hipMemcpy(fb.data(), h->d.fbody + D.fixBase, D.fixCount * sizeof(int), hipMemcpyDeviceToHost));
+    if (S.contactCount)
+        DBX_HIP(h, hipMemcpy(c.data(), h->d.cont[h->cur] + D.contactBase, S.contactCount * sizeof(Contact),
+                             hipMemcpyDeviceToHost));
+    for (int i = 0; i < n && i < cap; ++i) {
+        const Contact &k = c[ord[i]];
+        out[4 * i] = isl[i];
+        out[4 * i + 1] = fb[k.fa];
+        out[4 * i + 2] = fb[k.fb];
+        out[4 * i + 3] = bat[i];
     }
     return DBX_OK;
 }

@@ -58,7 +58,7 @@ EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
-    "dbx_collide_pairs",
+    "dbx_collide_pairs", "dbx_read_schedule",
 ]
 
 
@@ -91,6 +91,7 @@ def lib():
         L.dbx_read_solver_stats.argtypes = [vp, vp, vp, i32]
         L.dbx_broadphase_pairs.argtypes = [vp, vp, vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_collide_pairs.argtypes = [vp, vp, vp, vp, vp, i32, vp]
+        L.dbx_read_schedule.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("dbx_destroy", "dbx_last_error"):
@@ -194,6 +195,14 @@ class StepWorlds:
         p = DbxProfile()
         self._check(self.L.dbx_profile_read(self.h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in DbxProfile._fields_}
+
+    def schedule(self, world=0) -> np.ndarray:
+        """(island, body A, body B, batch) of each constraint of the last step, in island order."""
+        n = C.c_int32()
+        self._check(self.L.dbx_read_schedule(self.h, world, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 4), np.int32)
+        self._check(self.L.dbx_read_schedule(self.h, world, _p(out), n.value, C.byref(n)))
+        return out[:n.value]
 
     def profile_totals(self, reset=True):
         """Per-phase milliseconds summed over the steps resolved since the last reset."""

@@ -157,6 +157,11 @@ dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
 /* Per-world velocity iterations used by the last step (RESIDUAL mode) and max penetration. */
 dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds);
 
+/* The island solve order of the last step for one world: for each constraint in
+ * the order b2World.Solve's island search adds contacts (b2world.d:963-1076), its
+ * island, body A, body B and solve batch (4 int32 per row).  Returns the row count. */
+dbx_status dbx_read_schedule(dbx_handle *h, int32_t world, int32_t *out, int32_t cap, int32_t *count);
+
 /* Stand-alone broadphase (b2BroadPhase.UpdatePairs semantics) on given fat
  * AABBs and moved flags of n proxies; pairs are written sorted as (i, j), i < j. */
 dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uint8_t *moved, int32_t n,
