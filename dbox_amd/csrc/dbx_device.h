@@ -51,6 +51,7 @@ struct WorldDesc {
     float gx, gy;
     int islandBase, pad;
     long long islScratchBase; // byte offset of the world's island-search scratch
+    long long bpScratchBase;  // byte offset of the world's broadphase scratch
 };
 
 struct WorldState {
@@ -154,6 +155,7 @@ struct Dev {
     int *batchOff;          // per world: contactCap + 65 batch offsets at contactBase + 65 * world
     int *batchCur;          // scatter cursors, same layout as batchOff
     char *islScratch;       // island-search scratch of worlds that do not fit in LDS
+    char *bpScratch;        // broadphase scratch of worlds that do not fit in LDS
     VelCon *vc;
     PosCon *pc;
     // per island scratch (island index < body count)
@@ -165,6 +167,7 @@ struct Dev {
     float *pred;            // 2 predictions per feature row
     int *featCount;         // single counter
     float *residual;        // per world residual of the last iteration
+    long long *ktime;       // optional: phase timestamps of the largest world (s_memtime)
 };
 
 // Order-preserving int image of a float (for atomicMin on floats).

@@ -90,15 +90,58 @@ __device__ __forceinline__ float normalize(float2 &v) {
     return len;
 }
 
-// b2Rot.Set (b2math.d:483-488) calls sinf/cosf.  Evaluated in double and
-// rounded once, which gives the correctly rounded float.
+// b2Rot.Set (b2math.d:483-488) calls sinf/cosf.  Here both are evaluated in
+// double precision and rounded once to float:
+//   x = k * pi/2 + r, |r| <= pi/4, with pi/2 split into three doubles whose
+//   first two have 33 significant bits, so k * P1 and k * P2 are exact for
+//   |k| < 2^20 (Cody-Waite reduction);
+//   sin r and cos r from their Taylor series up to r^21 / r^20, whose truncation
+//   error on |r| <= pi/4 is below 1e-21;
+//   the quadrant k mod 4 picks the signs and the swap of sin and cos.
+// The float results equal the correctly rounded sin and cos except when the exact
+// value lies within a few double ulps of a float rounding boundary.
 __device__ __forceinline__ Rot rot(float angle) {
-    double sd, cd;
-    sincos((double)angle, &sd, &cd);
-    Rot q;
-    q.s = (float)sd;
-    q.c = (float)cd;
-    return q;
+    double x = (double)angle;
+    double s, c;
+    if (fabs(x) < 1.0e6) {
+        const double P1 = 0x1.921fb54400000p+0, P2 = 0x1.0b4611a600000p-34, P3 = 0x1.3198a2e037073p-69;
+        double k = rint(x * 0.6366197723675814);
+        double r = fma(-k, P1, x);
+        r = fma(-k, P2, r);
+        r = fma(-k, P3, r);
+        double z = r * r;
+        double ps = 1.9572941063391263e-20;
+        ps = fma(ps, z, -8.22063524662433e-18);
+        ps = fma(ps, z, 2.8114572543455206e-15);
+        ps = fma(ps, z, -7.647163731819816e-13);
+        ps = fma(ps, z, 1.6059043836821613e-10);
+        ps = fma(ps, z, -2.505210838544172e-08);
+        ps = fma(ps, z, 2.7557319223985893e-06);
+        ps = fma(ps, z, -0.0001984126984126984);
+        ps = fma(ps, z, 0.008333333333333333);
+        ps = fma(ps, z, -0.16666666666666666);
+        ps = fma(ps * z, r, r);
+        double pc = 4.110317623312165e-19;
+        pc = fma(pc, z, -1.5619206968586225e-16);
+        pc = fma(pc, z, 4.779477332387385e-14);
+        pc = fma(pc, z, -1.1470745597729725e-11);
+        pc = fma(pc, z, 2.08767569878681e-09);
+        pc = fma(pc, z, -2.755731922398589e-07);
+        pc = fma(pc, z, 2.48015873015873e-05);
+        pc = fma(pc, z, -0.001388888888888889);
+        pc = fma(pc, z, 0.041666666666666664);
+        pc = fma(pc, z, -0.5);
+        pc = fma(pc, z, 1.0);
+        int q = (int)(long long)k & 3;
+        s = q == 0 ? ps : q == 1 ? pc : q == 2 ? -ps : -pc;
+        c = q == 0 ? pc : q == 1 ? -ps : q == 2 ? -pc : ps;
+    } else {
+        sincos(x, &s, &c);
+    }
+    Rot qr;
+    qr.s = (float)s;
+    qr.c = (float)c;
+    return qr;
 }
 __device__ __forceinline__ float2 mul(Rot q, float2 v) { return v2(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
 __device__ __forceinline__ float2 mulT(Rot q, float2 v) { return v2(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
@@ -187,20 +230,37 @@ struct ClipV {
     float2 v;
     uint32_t id;
 };
-// b2ClipSegmentToLine (b2collision.d:413-446).
-__device__ __forceinline__ int clip_segment(ClipV out[2], const ClipV in[2], float2 normal, float offset, int vertexIndexA) {
-    int n = 0;
-    float d0 = dot(normal, in[0].v) - offset;
-    float d1 = dot(normal, in[1].v) - offset;
-    if (d0 <= 0.0f) out[n++] = in[0];
-    if (d1 <= 0.0f) out[n++] = in[1];
-    if (d0 * d1 < 0.0f) {
+// b2ClipSegmentToLine (b2collision.d:413-446).  The outputs are, in order, the
+// input points behind the plane and then the intersection point; written with
+// selects so that no local array is indexed at run time.
+__device__ __forceinline__ int clip_segment(ClipV &o0, ClipV &o1, const ClipV &i0, const ClipV &i1, float2 normal,
+                                            float offset, int vertexIndexA) {
+    float d0 = dot(normal, i0.v) - offset;
+    float d1 = dot(normal, i1.v) - offset;
+    bool k0 = d0 <= 0.0f, k1 = d1 <= 0.0f, cut = d0 * d1 < 0.0f;
+    ClipV x;
+    if (cut) {
         float interp = d0 / (d0 - d1);
-        out[n].v = add(in[0].v, scl(interp, sub(in[1].v, in[0].v)));
-        out[n].id = fkey((uint32_t)vertexIndexA, (in[0].id >> 8) & 0xffu, kFeatVertex, kFeatFace);
-        ++n;
+        x.v = add(i0.v, scl(interp, sub(i1.v, i0.v)));
+        x.id = fkey((uint32_t)vertexIndexA, (i0.id >> 8) & 0xffu, kFeatVertex, kFeatFace);
+    } else {
+        x = i0;
     }
-    return n;
+    o0 = k0 ? i0 : (k1 ? i1 : x);
+    o1 = (k0 && k1) ? i1 : x;
+    return (int)k0 + (int)k1 + (int)cut;
+}
+
+// Appends a manifold point (count < 2) without run-time indexing.
+__device__ __forceinline__ void push_point(Manifold &m, int &pc, float2 p, uint32_t id) {
+    if (pc == 0) {
+        m.p[0] = p;
+        m.id[0] = id;
+    } else {
+        m.p[1] = p;
+        m.id[1] = id;
+    }
+    ++pc;
 }
 
 // b2FindMaxSeparation (b2collidepolygon.d:29-71): edge normals of p1 against p2 in p2's frame.
@@ -260,11 +320,11 @@ __device__ inline void collide_polygons(Manifold &m, const Shape *__restrict__ A
         }
     }
     int i2 = index + 1 < n2 ? index + 1 : 0;
-    ClipV inc[2];
-    inc[0].v = mul(xf2, p2->v[index]);
-    inc[0].id = fkey(edge1, index, kFeatFace, kFeatVertex);
-    inc[1].v = mul(xf2, p2->v[i2]);
-    inc[1].id = fkey(edge1, i2, kFeatFace, kFeatVertex);
+    ClipV inc0, inc1;
+    inc0.v = mul(xf2, p2->v[index]);
+    inc0.id = fkey(edge1, index, kFeatFace, kFeatVertex);
+    inc1.v = mul(xf2, p2->v[i2]);
+    inc1.id = fkey(edge1, i2, kFeatFace, kFeatVertex);
 
     int n1 = p1->count;
     int iv1 = edge1;
@@ -282,20 +342,16 @@ __device__ inline void collide_polygons(Manifold &m, const Shape *__restrict__ A
     float frontOffset = dot(normal, v11);
     float sideOffset1 = -dot(tangent, v11) + totalRadius;
     float sideOffset2 = dot(tangent, v12) + totalRadius;
-    ClipV c1[2], c2[2];
-    if (clip_segment(c1, inc, neg(tangent), sideOffset1, iv1) < 2) return;
-    if (clip_segment(c2, c1, tangent, sideOffset2, iv2) < 2) return;
+    ClipV a0, a1, b0, b1;
+    if (clip_segment(a0, a1, inc0, inc1, neg(tangent), sideOffset1, iv1) < 2) return;
+    if (clip_segment(b0, b1, a0, a1, tangent, sideOffset2, iv2) < 2) return;
     m.ln = localNormal;
     m.lp = planePoint;
     int pc = 0;
-    for (int i = 0; i < 2; ++i) {
-        float separation = dot(normal, c2[i].v) - frontOffset;
-        if (separation <= totalRadius) {
-            m.p[pc] = mulT(xf2, c2[i].v);
-            m.id[pc] = flip ? fkey_swap(c2[i].id) : c2[i].id;
-            ++pc;
-        }
-    }
+    if (dot(normal, b0.v) - frontOffset <= totalRadius)
+        push_point(m, pc, mulT(xf2, b0.v), flip ? fkey_swap(b0.id) : b0.id);
+    if (dot(normal, b1.v) - frontOffset <= totalRadius)
+        push_point(m, pc, mulT(xf2, b1.v), flip ? fkey_swap(b1.id) : b1.id);
     m.count = pc;
 }
 
@@ -489,19 +545,14 @@ __device__ inline void collide_edge_polygon(Manifold &m, const Shape *__restrict
         front = offset1 >= 0.0f;
         normal = front ? normal1 : nn1; lower = front ? nn1 : normal1; upper = front ? nn1 : normal1;
     }
-    // Polygon B in frame A.
-    int pn = P->count;
-    float2 pv[kMaxPolygonVertices], pnorm[kMaxPolygonVertices];
-    for (int i = 0; i < pn; ++i) {
-        pv[i] = mul(xf, P->v[i]);
-        pnorm[i] = mul(xf.q, P->n[i]);
-    }
+    // Polygon B in frame A: vertices and normals are transformed where used.
+    const int pn = P->count;
     const float radius = 2.0f * kPolygonRadius;
     m.count = 0;
     // ComputeEdgeSeparation (b2collideedge.d:624-642)
     float edgeSep = kMaxFloat;
     for (int i = 0; i < pn; ++i) {
-        float s = dot(normal, sub(pv[i], ev1));
+        float s = dot(normal, sub(mul(xf, P->v[i]), ev1));
         if (s < edgeSep) edgeSep = s;
     }
     if (edgeSep > radius) return;
@@ -510,9 +561,10 @@ __device__ inline void collide_edge_polygon(Manifold &m, const Shape *__restrict
     float polySep = -kMaxFloat;
     float2 perp = v2(-normal.y, normal.x);
     for (int i = 0; i < pn; ++i) {
-        float2 n = neg(pnorm[i]);
-        float s1 = dot(n, sub(pv[i], ev1));
-        float s2 = dot(n, sub(pv[i], ev2));
+        float2 n = neg(mul(xf.q, P->n[i]));
+        float2 pvi = mul(xf, P->v[i]);
+        float s1 = dot(n, sub(pvi, ev1));
+        float s2 = dot(n, sub(pvi, ev2));
         float s = fminr(s1, s2);
         if (s > radius) {
             polyType = 2;
@@ -535,25 +587,25 @@ __device__ inline void collide_edge_polygon(Manifold &m, const Shape *__restrict
     const float k_relativeTol = 0.98f;
     const float k_absoluteTol = 0.001f;
     bool useEdge = polyType == 0 || !(polySep > k_relativeTol * edgeSep + k_absoluteTol);
-    ClipV ie[2];
+    ClipV ie0, ie1;
     int rf_i1, rf_i2;
     float2 rf_v1, rf_v2, rf_normal;
     if (useEdge) {
         m.type = kManFaceA;
         int best = 0;
-        float bestValue = dot(normal, pnorm[0]);
+        float bestValue = dot(normal, mul(xf.q, P->n[0]));
         for (int i = 1; i < pn; ++i) {
-            float value = dot(normal, pnorm[i]);
+            float value = dot(normal, mul(xf.q, P->n[i]));
             if (value < bestValue) {
                 bestValue = value;
                 best = i;
             }
         }
         int j2 = best + 1 < pn ? best + 1 : 0;
-        ie[0].v = pv[best];
-        ie[0].id = fkey(0, best, kFeatFace, kFeatVertex);
-        ie[1].v = pv[j2];
-        ie[1].id = fkey(0, j2, kFeatFace, kFeatVertex);
+        ie0.v = mul(xf, P->v[best]);
+        ie0.id = fkey(0, best, kFeatFace, kFeatVertex);
+        ie1.v = mul(xf, P->v[j2]);
+        ie1.id = fkey(0, j2, kFeatFace, kFeatVertex);
         rf_i1 = front ? 0 : 1;
         rf_i2 = front ? 1 : 0;
         rf_v1 = front ? ev1 : ev2;
@@ -561,23 +613,23 @@ __device__ inline void collide_edge_polygon(Manifold &m, const Shape *__restrict
         rf_normal = front ? normal1 : nn1;
     } else {
         m.type = kManFaceB;
-        ie[0].v = ev1;
-        ie[0].id = fkey(0, polyIndex, kFeatVertex, kFeatFace);
-        ie[1].v = ev2;
-        ie[1].id = fkey(0, polyIndex, kFeatVertex, kFeatFace);
+        ie0.v = ev1;
+        ie0.id = fkey(0, polyIndex, kFeatVertex, kFeatFace);
+        ie1.v = ev2;
+        ie1.id = fkey(0, polyIndex, kFeatVertex, kFeatFace);
         rf_i1 = polyIndex;
         rf_i2 = rf_i1 + 1 < pn ? rf_i1 + 1 : 0;
-        rf_v1 = pv[rf_i1];
-        rf_v2 = pv[rf_i2];
-        rf_normal = pnorm[rf_i1];
+        rf_v1 = mul(xf, P->v[rf_i1]);
+        rf_v2 = mul(xf, P->v[rf_i2]);
+        rf_normal = mul(xf.q, P->n[rf_i1]);
     }
     float2 side1 = v2(rf_normal.y, -rf_normal.x);
     float2 side2 = neg(side1);
     float off1 = dot(side1, rf_v1);
     float off2 = dot(side2, rf_v2);
-    ClipV c1[2], c2[2];
-    if (clip_segment(c1, ie, side1, off1, rf_i1) < 2) return;
-    if (clip_segment(c2, c1, side2, off2, rf_i2) < 2) return;
+    ClipV a0, a1, b0, b1;
+    if (clip_segment(a0, a1, ie0, ie1, side1, off1, rf_i1) < 2) return;
+    if (clip_segment(b0, b1, a0, a1, side2, off2, rf_i2) < 2) return;
     if (useEdge) {
         m.ln = rf_normal;
         m.lp = rf_v1;
@@ -586,18 +638,13 @@ __device__ inline void collide_edge_polygon(Manifold &m, const Shape *__restrict
         m.lp = P->v[rf_i1];
     }
     int pc = 0;
-    for (int i = 0; i < 2; ++i) {
-        float separation = dot(rf_normal, sub(c2[i].v, rf_v1));
-        if (separation <= radius) {
-            if (useEdge) {
-                m.p[pc] = mulT(xf, c2[i].v);
-                m.id[pc] = c2[i].id;
-            } else {
-                m.p[pc] = c2[i].v;
-                m.id[pc] = fkey_swap(c2[i].id);
-            }
-            ++pc;
-        }
+    if (dot(rf_normal, sub(b0.v, rf_v1)) <= radius) {
+        if (useEdge) push_point(m, pc, mulT(xf, b0.v), b0.id);
+        else push_point(m, pc, b0.v, fkey_swap(b0.id));
+    }
+    if (dot(rf_normal, sub(b1.v, rf_v1)) <= radius) {
+        if (useEdge) push_point(m, pc, mulT(xf, b1.v), b1.id);
+        else push_point(m, pc, b1.v, fkey_swap(b1.id));
     }
     m.count = pc;
 }
@@ -630,13 +677,13 @@ struct WorldManifold {
     float2 points[2];
     float sep[2];
 };
-__device__ inline void world_manifold(WorldManifold &w, int type, int count, float2 ln, float2 lp, const float2 *pts,
-                                      const Xf &xfA, float rA, const Xf &xfB, float rB) {
+__device__ __forceinline__ void world_manifold(WorldManifold &w, int type, int count, float2 ln, float2 lp, float2 p0,
+                                               float2 p1, const Xf &xfA, float rA, const Xf &xfB, float rB) {
     if (count == 0) return;
     if (type == kManCircles) {
         w.normal = v2(1.0f, 0.0f);
         float2 pointA = mul(xfA, lp);
-        float2 pointB = mul(xfB, pts[0]);
+        float2 pointB = mul(xfB, p0);
         if (dist2(pointA, pointB) > kEps * kEps) {
             w.normal = sub(pointB, pointA);
             normalize(w.normal);
@@ -648,8 +695,10 @@ __device__ inline void world_manifold(WorldManifold &w, int type, int count, flo
     } else if (type == kManFaceA) {
         w.normal = mul(xfA.q, ln);
         float2 planePoint = mul(xfA, lp);
-        for (int i = 0; i < count; ++i) {
-            float2 clip = mul(xfB, pts[i]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (i >= count) break;
+            float2 clip = mul(xfB, i == 0 ? p0 : p1);
             float2 cA = add(clip, scl(rA - dot(sub(clip, planePoint), w.normal), w.normal));
             float2 cB = sub(clip, scl(rB, w.normal));
             w.points[i] = scl(0.5f, add(cA, cB));
@@ -658,8 +707,10 @@ __device__ inline void world_manifold(WorldManifold &w, int type, int count, flo
     } else {
         w.normal = mul(xfB.q, ln);
         float2 planePoint = mul(xfB, lp);
-        for (int i = 0; i < count; ++i) {
-            float2 clip = mul(xfA, pts[i]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (i >= count) break;
+            float2 clip = mul(xfA, i == 0 ? p0 : p1);
             float2 cB = add(clip, scl(rB - dot(sub(clip, planePoint), w.normal), w.normal));
             float2 cA = sub(clip, scl(rA, w.normal));
             w.points[i] = scl(0.5f, add(cA, cB));

@@ -400,6 +400,7 @@ struct dbx_handle {
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
     int islSmallLds = 24576;      // island-search LDS of ordinary worlds
     int solveSmallLds = 32768;    // exact-order solver LDS of ordinary worlds
+    int bpSmallLds = 16384;       // broadphase LDS of ordinary worlds
     int nBigIsl = 0;              // worlds (largest first) launched with ldsMax
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
@@ -473,7 +474,11 @@ dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
         if (lim > 160 * 1024) lim = 160 * 1024;
         if (lim >= 32768) h->ldsMax = (int)lim;
     }
-    if (configure_kernels(h->ldsMax) != hipSuccess) h->ldsMax = 65536;
+    if (configure_kernels(h->ldsMax) != hipSuccess) {
+        h->ldsMax = 65536;
+        (void)configure_kernels(h->ldsMax);
+    }
+    (void)hipGetLastError(); // clear any error state left by the attribute calls
     if (getenv("DBX_VERBOSE"))
         fprintf(stderr, "dbx: device %d, %d CUs, LDS per workgroup %d bytes\n", h->cfg.device,
                 prop.multiProcessorCount, h->ldsMax);
@@ -523,7 +528,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     const int nw = (int)h->pending.size();
     h->nworlds = nw;
     h->W.resize(nw);
-    long long nb = 0, nf = 0, nc = 0, np = 0, islBytes = 0;
+    long long nb = 0, nf = 0, nc = 0, np = 0, islBytes = 0, bpBytes = 0;
     int cpf = h->cfg.contacts_per_fixture;
     for (int w = 0; w < nw; ++w) {
         const WorldHost &H = h->pending[w];
@@ -542,6 +547,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
         D.pad = 0;
         D.islScratchBase = islBytes;
         islBytes += (long long)islands_scratch_bytes(D.bodyCount, D.contactCap);
+        D.bpScratchBase = bpBytes;
+        bpBytes += (long long)broadphase_scratch_bytes(D.fixCount);
         nb += D.bodyCount;
         nf += D.fixCount;
         nc += D.contactCap;
@@ -656,16 +663,24 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.batchOff, nc + 65LL * nw));
     DBX_HIP(h, dalloc(h, &d.batchCur, nc + 65LL * nw));
     DBX_HIP(h, dalloc(h, &d.islScratch, islBytes));
+    DBX_HIP(h, dalloc(h, &d.bpScratch, bpBytes));
     DBX_HIP(h, dalloc(h, &d.vc, nc));
     DBX_HIP(h, dalloc(h, &d.pc, nc));
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
+    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 16));
     DBX_HIP(h, hipStreamSynchronize(h->stream));
     h->pending.clear();
     h->pending.shrink_to_fit();
     h->finalized = true;
     return DBX_OK;
+}
+
+static hipError_t broadphase(dbx_handle *h, hipStream_t s) {
+    hipError_t e = launch_broadphase(h->d, h->cur, 0, h->nBigIsl, h->ldsMax - 1024, h->block, s);
+    if (e != hipSuccess) return e;
+    return launch_broadphase(h->d, h->cur, h->nBigIsl, h->nworlds - h->nBigIsl, h->bpSmallLds, h->block, s);
 }
 
 dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t flags) {
@@ -697,7 +712,7 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     h->ringUsed++;
     (void)hipEventRecord(ev[0], s);
     if (h->newFixture) { // b2World.Step: FindNewContacts when fixtures were added (b2world.d:372-376)
-        DBX_HIP(h, launch_broadphase(h->d, h->cur, h->block, s));
+        DBX_HIP(h, broadphase(h, s));
         h->newFixture = false;
     }
     DBX_HIP(h, launch_collide(h->d, h->cur, h->block, s));
@@ -720,7 +735,7 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
             DBX_HIP(h, launch_solve(h->d, h->cur, sp, h->block, s));
         }
         (void)hipEventRecord(ev[3], s);
-        DBX_HIP(h, launch_broadphase(h->d, h->cur, h->block, s));
+        DBX_HIP(h, broadphase(h, s));
         (void)hipEventRecord(ev[4], s);
         h->inv_dt0 = sp.inv_dt;
     } else {
@@ -768,6 +783,14 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->sumSteps++;
     }
     h->ringUsed = 0;
+    if (h->d.ktime && getenv("DBX_KTIME")) {
+        long long t[16];
+        if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess)
+            fprintf(stderr, "dbx ktime (shader cycles) islands: stage %lld dfs %lld batch %lld [nc %lld nbatch %lld] | "
+                            "solve: stage+init %lld warm %lld vel %lld pos %lld tail %lld\n",
+                    t[1] - t[0], t[2] - t[1], t[3] - t[2], t[15], t[14], t[5] - t[4], t[6] - t[5], t[7] - t[6],
+                    t[8] - t[7], t[9] - t[8]);
+    }
     return DBX_OK;
 }
 
@@ -990,12 +1013,13 @@ dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uin
     std::vector<int> mv(n);
     for (int i = 0; i < n; ++i) mv[i] = moved[i] ? 1 : 0;
     float4 *dfat = nullptr;
-    int *dmv = nullptr, *dcnt = nullptr, *dtot = nullptr;
+    int *dmv = nullptr, *dtot = nullptr;
+    char *dcnt = nullptr;
     int2 *dout = nullptr;
     size_t m = (size_t)std::max(1, n);
     hipError_t e = hipMalloc(&dfat, m * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&dmv, m * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&dcnt, m * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dcnt, broadphase_scratch_bytes(std::max(1, n)));
     if (e == hipSuccess) e = hipMalloc(&dtot, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&dout, (size_t)std::max(1, cap) * sizeof(int2));
     if (e == hipSuccess && n) e = hipMemcpy(dfat, fat_aabbs, n * sizeof(float4), hipMemcpyHostToDevice);
