@@ -77,6 +77,9 @@ __device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b;
 __device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fabsr(float a) { return a > 0.0f ? a : -a; }
 __device__ __forceinline__ float clampr(float a, float lo, float hi) { return fmaxr(lo, fminr(a, hi)); }
+// Component-wise select; a conditional expression on float2 objects can be
+// lowered to a select of their addresses, which keeps them in scratch memory.
+__device__ __forceinline__ float2 sel(bool c, float2 a, float2 b) { return v2(c ? a.x : b.x, c ? a.y : b.y); }
 __device__ __forceinline__ float2 vminr(float2 a, float2 b) { return v2(fminr(a.x, b.x), fminr(a.y, b.y)); }
 __device__ __forceinline__ float2 vmaxr(float2 a, float2 b) { return v2(fmaxr(a.x, b.x), fmaxr(a.y, b.y)); }
 

@@ -9,15 +9,20 @@ namespace dbx {
 
 constexpr int kMaxBlock = 1024;
 
-hipError_t launch_collide(const Dev &d, int cur, int block, hipStream_t s);
+// Step kernels over the worlds order[worldOffset .. worldOffset + nworlds); lead
+// selects the instantiation that profiles report for the lead (largest) tier.
+hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int block, bool lead, hipStream_t s);
 size_t islands_scratch_bytes(int bodies, int contacts);
+size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands);
 hipError_t configure_kernels(int maxDynamicLds);
-hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, hipStream_t s);
-hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int block, hipStream_t s);
+hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, bool lead,
+                          hipStream_t s);
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block, bool lead,
+                        hipStream_t s);
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
-                             hipStream_t s);
+                             bool lead, hipStream_t s);
 size_t broadphase_scratch_bytes(int proxies);
-hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block,
+hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                              hipStream_t s);
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total,
                             hipStream_t s);

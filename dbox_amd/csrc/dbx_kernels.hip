@@ -151,9 +151,13 @@ __device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbo
     c.id1 = m.id[1];
 }
 
-__global__ void k_collide(Dev d, int cur) {
+// kLead marks the launch of the lead tier (the largest worlds, which set the
+// step time); the two instantiations differ only in name, so that profiles
+// report the lead tier on its own.
+template <bool kLead>
+__global__ void k_collide(Dev d, int cur, int worldOffset) {
     __shared__ int s_tmp[kMaxBlock / 64 + 1];
-    const int w = d.order[blockIdx.x];
+    const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
     const int n = d.S[w].contactCount;
     Contact *src = d.cont[cur] + wd.contactBase;
@@ -540,6 +544,7 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
     }
 }
 
+template <bool kLead>
 __global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
     const int w = d.order[worldOffset + blockIdx.x];
@@ -607,50 +612,48 @@ __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, 
     float wA = va.z, wB = vb.z;
     Xf xfA = xf_from(cA, pa.z, p.lcA);
     Xf xfB = xf_from(cB, pb.z, p.lcB);
-    WorldManifold wm;
+    WorldManifold wm = {};
     world_manifold(wm, type, count, p.localNormal, p.localPoint, p.lp0, p.lp1, xfA, radiusA, xfB, radiusB);
     v.normal = wm.normal;
     float2 tangent = crs(wm.normal, 1.0f);
     float restitution = c.restitution;
-    float nm[2] = {0.0f, 0.0f}, tm[2] = {0.0f, 0.0f}, bias[2] = {0.0f, 0.0f};
-    float2 rAs[2] = {v2(0.0f, 0.0f), v2(0.0f, 0.0f)}, rBs[2] = {v2(0.0f, 0.0f), v2(0.0f, 0.0f)};
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        if (j >= count) break;
-        float2 rA = sub(wm.points[j], cA);
-        float2 rB = sub(wm.points[j], cB);
+    // per point: rA, rB, normal mass, tangent mass, velocity bias (b2contactsolver.d:384-415)
+    auto point = [&](float2 wp, float2 &rA, float2 &rB, float &nmass, float &tmass, float &bias) {
+        rA = sub(wp, cA);
+        rB = sub(wp, cB);
         float rnA = crs(rA, wm.normal);
         float rnB = crs(rB, wm.normal);
         float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
-        nm[j] = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+        nmass = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
         float rtA = crs(rA, tangent);
         float rtB = crs(rB, tangent);
         float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
-        tm[j] = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
-        bias[j] = 0.0f;
+        tmass = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
         float vRel = dot(wm.normal, sub(sub(add(vB, crs(wB, rB)), vA), crs(wA, rA)));
-        if (vRel < -kVelocityThreshold) bias[j] = -restitution * vRel;
-        rAs[j] = rA;
-        rBs[j] = rB;
-    }
-    v.rA0 = rAs[0];
-    v.rB0 = rBs[0];
-    v.nMass0 = nm[0];
-    v.tMass0 = tm[0];
-    v.bias0 = bias[0];
+        bias = vRel < -kVelocityThreshold ? -restitution * vRel : 0.0f;
+    };
+    float2 rA0, rB0, rA1, rB1;
+    float nm0, tm0, bias0, nm1, tm1, bias1;
+    point(wm.points[0], rA0, rB0, nm0, tm0, bias0);
+    point(wm.points[1], rA1, rB1, nm1, tm1, bias1);
+    v.rA0 = rA0;
+    v.rB0 = rB0;
+    v.nMass0 = nm0;
+    v.tMass0 = tm0;
+    v.bias0 = bias0;
     v.pointCount = count;
     v.K = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     v.normalMass = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (count == 2) {
-        v.rA1 = rAs[1];
-        v.rB1 = rBs[1];
-        v.nMass1 = nm[1];
-        v.tMass1 = tm[1];
-        v.bias1 = bias[1];
-        float rn1A = crs(rAs[0], wm.normal);
-        float rn1B = crs(rBs[0], wm.normal);
-        float rn2A = crs(rAs[1], wm.normal);
-        float rn2B = crs(rBs[1], wm.normal);
+        v.rA1 = rA1;
+        v.rB1 = rB1;
+        v.nMass1 = nm1;
+        v.tMass1 = tm1;
+        v.bias1 = bias1;
+        float rn1A = crs(rA0, wm.normal);
+        float rn1B = crs(rB0, wm.normal);
+        float rn2A = crs(rA1, wm.normal);
+        float rn2B = crs(rB1, wm.normal);
         float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
         float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
         float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
@@ -674,189 +677,218 @@ __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, 
     }
 }
 
+// The three per-constraint cores below are written as straight-line code: a
+// level of the exact-order schedule usually holds a handful of constraints of
+// mixed point counts and manifold types in one wave, and branching on those
+// would run every path anyway, with the branch overhead on top.  Each core
+// evaluates both the one-point and the two-point form and selects the results,
+// keeping the reference's floating-point operation order in each form.
+
 // b2ContactSolver.WarmStart (b2contactsolver.d:452-490) for one constraint.
 // imp = (normalImpulse0, tangentImpulse0, normalImpulse1, tangentImpulse1);
 // va, vb = (v.x, v.y, w, unused) of bodies A and B.
 __device__ __forceinline__ void warm_start_core(const VelCon &v, float4 imp, float4 &va, float4 &vb) {
     float2 vA = v2(va.x, va.y), vB = v2(vb.x, vb.y);
     float wA = va.z, wB = vb.z;
-    float2 normal = v.normal;
-    float2 tangent = crs(normal, 1.0f);
-    float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
-    for (int j = 0; j < v.pointCount; ++j) {
-        float nI = j == 0 ? imp.x : imp.z;
-        float tI = j == 0 ? imp.y : imp.w;
-        float2 rA = j == 0 ? v.rA0 : v.rA1;
-        float2 rB = j == 0 ? v.rB0 : v.rB1;
-        float2 P = add(scl(nI, normal), scl(tI, tangent));
-        wA -= iA * crs(rA, P);
+    const float2 normal = v.normal;
+    const float2 tangent = crs(normal, 1.0f);
+    const float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
+    const bool two = v.pointCount > 1;
+    {
+        float2 P = add(scl(imp.x, normal), scl(imp.y, tangent));
+        wA -= iA * crs(v.rA0, P);
         vA = sub(vA, scl(mA, P));
-        wB += iB * crs(rB, P);
+        wB += iB * crs(v.rB0, P);
         vB = add(vB, scl(mB, P));
+    }
+    {
+        float2 P = add(scl(imp.z, normal), scl(imp.w, tangent));
+        float wA1 = wA - iA * crs(v.rA1, P);
+        float2 vA1 = sub(vA, scl(mA, P));
+        float wB1 = wB + iB * crs(v.rB1, P);
+        float2 vB1 = add(vB, scl(mB, P));
+        wA = two ? wA1 : wA;
+        vA = sel(two, vA1, vA);
+        wB = two ? wB1 : wB;
+        vB = sel(two, vB1, vB);
     }
     va = make_float4(vA.x, vA.y, wA, 0.0f);
     vb = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
 // b2ContactSolver.SolveVelocityConstraints (b2contactsolver.d:492-772) for one
-// constraint; returns the constraint's velocity residual when asked for it.
-__device__ float solve_velocity_core(const VelCon &v, float4 &imp, float4 &va, float4 &vb, bool wantResidual) {
+// constraint; returns the constraint's velocity residual when kResidual is set:
+// the largest of |vt| over points whose friction is not saturated and, over
+// points, |vn - bias| for a positive normal impulse, else max(0, -(vn - bias)).
+template <bool kResidual>
+__device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &imp, float4 &va, float4 &vb) {
     float2 vA = v2(va.x, va.y), vB = v2(vb.x, vb.y);
     float wA = va.z, wB = vb.z;
-    float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
-    float2 normal = v.normal;
-    float2 tangent = crs(normal, 1.0f);
-    float friction = v.friction;
-    int pc = v.pointCount;
+    const float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
+    const float2 normal = v.normal;
+    const float2 tangent = crs(normal, 1.0f);
+    const float friction = v.friction;
+    const bool two = v.pointCount > 1;
     float res = 0.0f;
     float nI0 = imp.x, tI0 = imp.y, nI1 = imp.z, tI1 = imp.w;
-    // friction
-    for (int j = 0; j < pc; ++j) {
-        float2 rA = j == 0 ? v.rA0 : v.rA1;
-        float2 rB = j == 0 ? v.rB0 : v.rB1;
-        float tMass = j == 0 ? v.tMass0 : v.tMass1;
-        float nI = j == 0 ? nI0 : nI1;
-        float tI = j == 0 ? tI0 : tI1;
-        float2 dv = sub(sub(add(vB, crs(wB, rB)), vA), crs(wA, rA));
-        float vt = dot(dv, tangent) - v.tangentSpeed;
-        float lambda = tMass * (-vt);
-        float maxFriction = friction * nI;
-        float newImpulse = clampr(tI + lambda, -maxFriction, maxFriction);
-        if (wantResidual && fabsr(tI) < maxFriction) res = fmaxr(res, fabsr(vt));
-        lambda = newImpulse - tI;
-        if (j == 0) tI0 = newImpulse;
-        else tI1 = newImpulse;
-        float2 P = scl(lambda, tangent);
-        vA = sub(vA, scl(mA, P));
-        wA -= iA * crs(rA, P);
-        vB = add(vB, scl(mB, P));
-        wB += iB * crs(rB, P);
-    }
-    if (pc == 1) {
+    // friction, point 0 (b2contactsolver.d:527-553)
+    {
         float2 dv = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
-        float vn = dot(dv, normal);
-        float lambda = -v.nMass0 * (vn - v.bias0);
-        float newImpulse = fmaxr(nI0 + lambda, 0.0f);
-        if (wantResidual) {
-            float e = vn - v.bias0;
-            res = fmaxr(res, nI0 > 0.0f ? fabsr(e) : fmaxr(0.0f, -e));
-        }
-        lambda = newImpulse - nI0;
-        nI0 = newImpulse;
-        float2 P = scl(lambda, normal);
+        float vt = dot(dv, tangent) - v.tangentSpeed;
+        float lambda = v.tMass0 * (-vt);
+        float maxFriction = friction * nI0;
+        float newImpulse = clampr(tI0 + lambda, -maxFriction, maxFriction);
+        if (kResidual && fabsr(tI0) < maxFriction) res = fmaxr(res, fabsr(vt));
+        lambda = newImpulse - tI0;
+        tI0 = newImpulse;
+        float2 P = scl(lambda, tangent);
         vA = sub(vA, scl(mA, P));
         wA -= iA * crs(v.rA0, P);
         vB = add(vB, scl(mB, P));
         wB += iB * crs(v.rB0, P);
-    } else {
-        // 2-point block solver by total enumeration
-        float2 a = v2(nI0, nI1);
-        float2 dv1 = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
-        float2 dv2 = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
-        float vn1 = dot(dv1, normal);
-        float vn2 = dot(dv2, normal);
-        float2 b;
-        b.x = vn1 - v.bias0;
-        b.y = vn2 - v.bias1;
-        if (wantResidual) {
-            res = fmaxr(res, nI0 > 0.0f ? fabsr(b.x) : fmaxr(0.0f, -b.x));
-            res = fmaxr(res, nI1 > 0.0f ? fabsr(b.y) : fmaxr(0.0f, -b.y));
-        }
-        // b -= K * a
-        float Kax = v.K.x * a.x + v.K.z * a.y;
-        float Kay = v.K.y * a.x + v.K.w * a.y;
-        b.x -= Kax;
-        b.y -= Kay;
-        float2 x;
-        bool done = false;
-        // case 1: both points active
-        x = v2(-(v.normalMass.x * b.x + v.normalMass.z * b.y), -(v.normalMass.y * b.x + v.normalMass.w * b.y));
-        if (x.x >= 0.0f && x.y >= 0.0f) done = true;
-        if (!done) { // case 2: point 1 only
-            x.x = -v.nMass0 * b.x;
-            x.y = 0.0f;
-            vn2 = v.K.y * x.x + b.y;
-            if (x.x >= 0.0f && vn2 >= 0.0f) done = true;
-        }
-        if (!done) { // case 3: point 2 only
-            x.x = 0.0f;
-            x.y = -v.nMass1 * b.y;
-            vn1 = v.K.z * x.y + b.x;
-            if (x.y >= 0.0f && vn1 >= 0.0f) done = true;
-        }
-        if (!done) { // case 4: neither
-            x.x = 0.0f;
-            x.y = 0.0f;
-            vn1 = b.x;
-            vn2 = b.y;
-            if (vn1 >= 0.0f && vn2 >= 0.0f) done = true;
-        }
-        if (done) {
-            float2 dd = sub(x, a);
-            float2 P1 = scl(dd.x, normal);
-            float2 P2 = scl(dd.y, normal);
-            vA = sub(vA, scl(mA, add(P1, P2)));
-            wA -= iA * (crs(v.rA0, P1) + crs(v.rA1, P2));
-            vB = add(vB, scl(mB, add(P1, P2)));
-            wB += iB * (crs(v.rB0, P1) + crs(v.rB1, P2));
-            nI0 = x.x;
-            nI1 = x.y;
-        }
     }
+    // friction, point 1
+    {
+        float2 dv = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
+        float vt = dot(dv, tangent) - v.tangentSpeed;
+        float lambda = v.tMass1 * (-vt);
+        float maxFriction = friction * nI1;
+        float newImpulse = clampr(tI1 + lambda, -maxFriction, maxFriction);
+        if (kResidual && two && fabsr(tI1) < maxFriction) res = fmaxr(res, fabsr(vt));
+        lambda = newImpulse - tI1;
+        float2 P = scl(lambda, tangent);
+        float2 vA1 = sub(vA, scl(mA, P));
+        float wA1 = wA - iA * crs(v.rA1, P);
+        float2 vB1 = add(vB, scl(mB, P));
+        float wB1 = wB + iB * crs(v.rB1, P);
+        tI1 = two ? newImpulse : tI1;
+        vA = sel(two, vA1, vA);
+        wA = two ? wA1 : wA;
+        vB = sel(two, vB1, vB);
+        wB = two ? wB1 : wB;
+    }
+    const float2 dv1 = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
+    const float vn1 = dot(dv1, normal);
+    // one point (b2contactsolver.d:559-585)
+    float2 sA, sB;
+    float swA, swB, sI0;
+    {
+        float lambda = -v.nMass0 * (vn1 - v.bias0);
+        float newImpulse = fmaxr(nI0 + lambda, 0.0f);
+        lambda = newImpulse - nI0;
+        sI0 = newImpulse;
+        float2 P = scl(lambda, normal);
+        sA = sub(vA, scl(mA, P));
+        swA = wA - iA * crs(v.rA0, P);
+        sB = add(vB, scl(mB, P));
+        swB = wB + iB * crs(v.rB0, P);
+    }
+    // two points: block solver by total enumeration (b2contactsolver.d:587-764)
+    const float2 dv2 = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
+    const float vn2 = dot(dv2, normal);
+    float bx = vn1 - v.bias0;
+    float by = vn2 - v.bias1;
+    if (kResidual) { // vn1 - bias0 is also the one-point form's residual
+        res = fmaxr(res, nI0 > 0.0f ? fabsr(bx) : fmaxr(0.0f, -bx));
+        float r1 = nI1 > 0.0f ? fabsr(by) : fmaxr(0.0f, -by);
+        res = two ? fmaxr(res, r1) : res;
+    }
+    bx -= v.K.x * nI0 + v.K.z * nI1;
+    by -= v.K.y * nI0 + v.K.w * nI1;
+    // case 1: both points active
+    const float x1x = -(v.normalMass.x * bx + v.normalMass.z * by);
+    const float x1y = -(v.normalMass.y * bx + v.normalMass.w * by);
+    const bool c1 = x1x >= 0.0f && x1y >= 0.0f;
+    // case 2: point 1 only
+    const float x2x = -v.nMass0 * bx;
+    const bool c2 = x2x >= 0.0f && v.K.y * x2x + by >= 0.0f;
+    // case 3: point 2 only
+    const float x3y = -v.nMass1 * by;
+    const bool c3 = x3y >= 0.0f && v.K.z * x3y + bx >= 0.0f;
+    // case 4: neither
+    const bool c4 = bx >= 0.0f && by >= 0.0f;
+    const bool done = c1 || c2 || c3 || c4;
+    const float xx = c1 ? x1x : (c2 ? x2x : 0.0f);
+    const float xy = c1 ? x1y : (c2 ? 0.0f : (c3 ? x3y : 0.0f));
+    float2 tA, tB;
+    float twA, twB;
+    {
+        float2 P1 = scl(xx - nI0, normal);
+        float2 P2 = scl(xy - nI1, normal);
+        tA = sub(vA, scl(mA, add(P1, P2)));
+        twA = wA - iA * (crs(v.rA0, P1) + crs(v.rA1, P2));
+        tB = add(vB, scl(mB, add(P1, P2)));
+        twB = wB + iB * (crs(v.rB0, P1) + crs(v.rB1, P2));
+    }
+    const bool blk = two && done;
+    vA = sel(two, sel(done, tA, vA), sA);
+    wA = two ? (done ? twA : wA) : swA;
+    vB = sel(two, sel(done, tB, vB), sB);
+    wB = two ? (done ? twB : wB) : swB;
+    nI0 = two ? (done ? xx : nI0) : sI0;
+    nI1 = blk ? xy : nI1;
     imp = make_float4(nI0, tI0, nI1, tI1);
     va = make_float4(vA.x, vA.y, wA, 0.0f);
     vb = make_float4(vB.x, vB.y, wB, 0.0f);
     return res;
 }
 
-// b2ContactSolver.SolvePositionConstraints (b2contactsolver.d:73-149) for one
+// b2ContactSolver.SolvePositionConstraints (b2contactsolver.d:73-149) with
+// b2PositionSolverManifold.Initialize (b2contactsolver.d:818-863) for one
 // constraint; pa, pb = (c.x, c.y, a, unused); returns the minimum separation seen.
-__device__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb) {
+// For every manifold type the point pair is (P1, P2) = (X1 * localPoint,
+// X2 * pointLocal[j]) with X1 = xfA, X2 = xfB for circles and faceA and the
+// other way round for faceB, and separation = dot(P2 - P1, normal) - rA - rB;
+// only the normal (normalised P2 - P1 for circles, X1.q * localNormal for the
+// faces) and the contact point (midpoint for circles, P2 for the faces) differ.
+__device__ __forceinline__ float position_point(const PosCon &p, float2 lp, float2 &cA, float &aA, float2 &cB,
+                                                float &aB) {
+    const float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
+    const Xf xfA = xf_from(cA, aA, p.lcA);
+    const Xf xfB = xf_from(cB, aB, p.lcB);
+    const bool circles = p.type == kManCircles, faceB = p.type == kManFaceB;
+    Xf X1, X2;
+    X1.p = sel(faceB, xfB.p, xfA.p);
+    X1.q.s = faceB ? xfB.q.s : xfA.q.s;
+    X1.q.c = faceB ? xfB.q.c : xfA.q.c;
+    X2.p = sel(faceB, xfA.p, xfB.p);
+    X2.q.s = faceB ? xfA.q.s : xfB.q.s;
+    X2.q.c = faceB ? xfA.q.c : xfB.q.c;
+    const float2 P1 = mul(X1, p.localPoint);
+    const float2 P2 = mul(X2, lp);
+    const float2 d = sub(P2, P1);
+    // b2Vec2.Normalize (b2math.d:162-175) of d for circles
+    float2 cn = d;
+    {
+        float len = sqrtf(d.x * d.x + d.y * d.y);
+        float inv = 1.0f / len;
+        bool ok = !(len < kEps);
+        cn = sel(ok, v2(d.x * inv, d.y * inv), d);
+    }
+    float2 normal = sel(circles, cn, mul(X1.q, p.localNormal));
+    const float separation = dot(d, normal) - p.radiusA - p.radiusB;
+    const float2 point = sel(circles, scl(0.5f, add(P1, P2)), P2);
+    normal = sel(faceB, neg(normal), normal);
+    const float2 rA = sub(point, cA);
+    const float2 rB = sub(point, cB);
+    const float C = clampr(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+    const float rnA = crs(rA, normal);
+    const float rnB = crs(rB, normal);
+    const float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+    const float impulse = K > 0.0f ? -C / K : 0.0f;
+    const float2 P = scl(impulse, normal);
+    cA = sub(cA, scl(mA, P));
+    aA -= iA * crs(rA, P);
+    cB = add(cB, scl(mB, P));
+    aB += iB * crs(rB, P);
+    return separation;
+}
+
+__device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb) {
     float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
     float aA = pa.z, aB = pb.z;
-    float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
-    float minSep = 0.0f;
-    for (int j = 0; j < p.pointCount; ++j) {
-        Xf xfA = xf_from(cA, aA, p.lcA);
-        Xf xfB = xf_from(cB, aB, p.lcB);
-        // b2PositionSolverManifold.Initialize (b2contactsolver.d:818-863)
-        float2 normal, point;
-        float separation;
-        if (p.type == kManCircles) {
-            float2 pointA = mul(xfA, p.localPoint);
-            float2 pointB = mul(xfB, p.lp0);
-            normal = sub(pointB, pointA);
-            normalize(normal);
-            point = scl(0.5f, add(pointA, pointB));
-            separation = dot(sub(pointB, pointA), normal) - p.radiusA - p.radiusB;
-        } else if (p.type == kManFaceA) {
-            normal = mul(xfA.q, p.localNormal);
-            float2 planePoint = mul(xfA, p.localPoint);
-            float2 clipPoint = mul(xfB, j == 0 ? p.lp0 : p.lp1);
-            separation = dot(sub(clipPoint, planePoint), normal) - p.radiusA - p.radiusB;
-            point = clipPoint;
-        } else {
-            normal = mul(xfB.q, p.localNormal);
-            float2 planePoint = mul(xfB, p.localPoint);
-            float2 clipPoint = mul(xfA, j == 0 ? p.lp0 : p.lp1);
-            separation = dot(sub(clipPoint, planePoint), normal) - p.radiusA - p.radiusB;
-            point = clipPoint;
-            normal = neg(normal);
-        }
-        float2 rA = sub(point, cA);
-        float2 rB = sub(point, cB);
-        minSep = fminr(minSep, separation);
-        float C = clampr(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
-        float rnA = crs(rA, normal);
-        float rnB = crs(rB, normal);
-        float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
-        float impulse = K > 0.0f ? -C / K : 0.0f;
-        float2 P = scl(impulse, normal);
-        cA = sub(cA, scl(mA, P));
-        aA -= iA * crs(rA, P);
-        cB = add(cB, scl(mB, P));
-        aB += iB * crs(rB, P);
-    }
+    float minSep = fminr(0.0f, position_point(p, p.lp0, cA, aA, cB, aB));
+    if (p.pointCount > 1) minSep = fminr(minSep, position_point(p, p.lp1, cA, aA, cB, aB));
     pa = make_float4(cA.x, cA.y, aA, pa.w);
     pb = make_float4(cB.x, cB.y, aB, pb.w);
     return minSep;
@@ -872,7 +904,7 @@ __device__ __forceinline__ void warm_start(const VelCon &v, float4 *bvel) {
 __device__ float solve_velocity(VelCon &v, float4 *bvel, bool wantResidual) {
     float4 va = bvel[v.indexA], vb = bvel[v.indexB];
     float4 imp = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
-    float r = solve_velocity_core(v, imp, va, vb, wantResidual);
+    float r = wantResidual ? solve_velocity_core<true>(v, imp, va, vb) : solve_velocity_core<false>(v, imp, va, vb);
     v.nImp0 = imp.x;
     v.tImp0 = imp.y;
     v.nImp1 = imp.z;
@@ -1124,7 +1156,10 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void k_solve(Dev d, int cur, StepParams sp) { solve_world_global(d, cur, sp, d.order[blockIdx.x]); }
+template <bool kLead>
+__global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, int worldOffset) {
+    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x]);
+}
 
 // ---------------------------------------------------------------------------
 // Exact-order solver: one wave per world.  Body positions, velocities and the
@@ -1140,6 +1175,119 @@ __host__ __device__ inline size_t solve_lds_bytes(int nb, int nc, int nbatch, in
            isl_align(4 * (size_t)(nbatch + 1)) + isl_align(8 * (size_t)nisl);
 }
 
+// One constraint of a level, on LDS body state.  kMode 0 = warm start,
+// 1 = velocity iteration, 2 = velocity iteration that also returns the residual.
+template <int kMode>
+__device__ __forceinline__ void velocity_one(const VelCon &v, int k, float4 *Lvel, float4 *Limp, float &lres) {
+    float4 va = Lvel[v.indexA], vb = Lvel[v.indexB], im = Limp[k];
+    if (kMode == 0) warm_start_core(v, im, va, vb);
+    else lres = fmaxr(lres, solve_velocity_core<kMode == 2>(v, im, va, vb));
+    if (kMode != 0) Limp[k] = im;
+    if (v.dynMask & 1) Lvel[v.indexA] = va;
+    if (v.dynMask & 2) Lvel[v.indexB] = vb;
+}
+
+// Slot of this lane in level bt of the exact-order schedule (Lsch = constraint
+// per slot, Lbo = level offsets); a lane without a constraint gets slot 0's, so
+// that the loads issued for it stay in bounds.
+__device__ __forceinline__ void level_slot(const int *Lsch, const int *Lbo, int bt, int nbatch, int lane, int &k,
+                                           bool &have) {
+    int j = 0;
+    have = false;
+    if (bt < nbatch) {
+        j = Lbo[bt] + lane;
+        have = j < Lbo[bt + 1];
+    }
+    k = Lsch[have ? j : 0];
+}
+
+template <int kMode>
+__device__ __forceinline__ void velocity_level(const VelCon &v, int k, bool have, int bt, const int *Lsch,
+                                               const int *Lbo, const VelCon *__restrict__ vc, float4 *Lvel,
+                                               float4 *Limp, float &lres) {
+    if (have) velocity_one<kMode>(v, k, Lvel, Limp, lres);
+    const int e = Lbo[bt + 1];
+    for (int c = Lbo[bt] + 64; c < e; c += 64) { // levels wider than the wave
+        const int j2 = c + (threadIdx.x & 63);
+        if (j2 < e) {
+            const int k2 = Lsch[j2];
+            velocity_one<kMode>(vc[k2], k2, Lvel, Limp, lres);
+        }
+    }
+}
+
+// One sweep over the levels of the exact-order schedule held in LDS.  Two
+// register sets alternate: while one level is solved, the constraints of the
+// level after the next are already loading into the set just used, so the
+// wait before a level covers loads issued a whole level earlier.
+template <int kMode>
+__device__ __forceinline__ float velocity_sweep(const int *Lsch, const int *Lbo, int nbatch,
+                                                const VelCon *__restrict__ vc, float4 *Lvel, float4 *Limp, int lane) {
+    float lres = 0.0f;
+    int k0, k1;
+    bool h0, h1;
+    level_slot(Lsch, Lbo, 0, nbatch, lane, k0, h0);
+    VelCon c0 = vc[k0];
+    level_slot(Lsch, Lbo, 1, nbatch, lane, k1, h1);
+    VelCon c1 = vc[k1];
+    for (int bt = 0; bt < nbatch; bt += 2) {
+        velocity_level<kMode>(c0, k0, h0, bt, Lsch, Lbo, vc, Lvel, Limp, lres);
+        level_slot(Lsch, Lbo, bt + 2, nbatch, lane, k0, h0);
+        c0 = vc[k0];
+        if (bt + 1 >= nbatch) break;
+        velocity_level<kMode>(c1, k1, h1, bt + 1, Lsch, Lbo, vc, Lvel, Limp, lres);
+        level_slot(Lsch, Lbo, bt + 3, nbatch, lane, k1, h1);
+        c1 = vc[k1];
+    }
+    return lres;
+}
+
+__device__ __forceinline__ void position_one(const PosCon &q, float4 *Lpos, int *Lmin) {
+    float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
+    const float ms = solve_position_core(q, pa, pb);
+    if (q.dynMask & 1) Lpos[q.indexA] = pa;
+    if (q.dynMask & 2) Lpos[q.indexB] = pb;
+    atomicMin(&Lmin[q.island], f2ord(ms));
+}
+
+__device__ __forceinline__ void position_level(const PosCon &q, bool have, int bt, const int *Lsch, const int *Lbo,
+                                               const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
+                                               const int *Lsol) {
+    if (have && !Lsol[q.island]) position_one(q, Lpos, Lmin);
+    const int e = Lbo[bt + 1];
+    for (int c = Lbo[bt] + 64; c < e; c += 64) {
+        const int j2 = c + (threadIdx.x & 63);
+        if (j2 < e) {
+            const PosCon q2 = pc[Lsch[j2]];
+            if (!Lsol[q2.island]) position_one(q2, Lpos, Lmin);
+        }
+    }
+}
+
+// One position iteration over the levels, with the same two-set pipeline;
+// islands already solved are skipped (b2island.d:206-224 stops iterating an
+// island once it is solved).
+__device__ __forceinline__ void position_sweep(const int *Lsch, const int *Lbo, int nbatch,
+                                               const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
+                                               const int *Lsol, int lane) {
+    int k0, k1;
+    bool h0, h1;
+    level_slot(Lsch, Lbo, 0, nbatch, lane, k0, h0);
+    PosCon c0 = pc[k0];
+    level_slot(Lsch, Lbo, 1, nbatch, lane, k1, h1);
+    PosCon c1 = pc[k1];
+    for (int bt = 0; bt < nbatch; bt += 2) {
+        position_level(c0, h0, bt, Lsch, Lbo, pc, Lpos, Lmin, Lsol);
+        level_slot(Lsch, Lbo, bt + 2, nbatch, lane, k0, h0);
+        c0 = pc[k0];
+        if (bt + 1 >= nbatch) break;
+        position_level(c1, h1, bt + 1, Lsch, Lbo, pc, Lpos, Lmin, Lsol);
+        level_slot(Lsch, Lbo, bt + 3, nbatch, lane, k1, h1);
+        c1 = pc[k1];
+    }
+}
+
+template <bool kLead>
 __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
     const int w = d.order[worldOffset + blockIdx.x];
@@ -1226,65 +1374,25 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
 
     if (timed && lane == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
-    // phase 0: warm start, phase 1: velocity iterations
-    for (int phase = 0; phase < 2; ++phase) {
-        if (phase == 0 && !(sp.flags & 1u)) continue;
-        int iters = phase == 0 ? 1 : (residualMode ? sp.maxIterations : sp.velocityIterations);
+    if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+    if (timed && lane == 0) d.ktime[6] = __builtin_amdgcn_s_memtime();
+    {
+        const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
         int used = 0;
         for (int it = 0; it < iters && nbatch > 0; ++it) {
-            float lres = 0.0f;
-            // The loads of the next level are issued unconditionally (a valid
-            // constraint index is always used) so that the wait before a level
-            // only covers that level's own data.
-            int j = Lbo[0] + lane;
-            bool have = j < Lbo[1];
-            int k = Lsch[have ? j : 0];
-            VelCon v = vc[k];
-            int end1 = Lbo[1];
-            for (int bt = 0; bt < nbatch; ++bt) {
-                const int e0 = end1;
-                const int e1 = bt + 1 < nbatch ? Lbo[bt + 2] : e0;
-                int jn = e0 + lane;
-                bool haveN = jn < e1;
-                int kN = Lsch[haveN ? jn : 0];
-                VelCon vN = vc[kN];
-                end1 = e1;
-                if (have) {
-                    float4 va = Lvel[v.indexA], vb = Lvel[v.indexB], im = Limp[k];
-                    if (phase == 0) warm_start_core(v, im, va, vb);
-                    else lres = fmaxr(lres, solve_velocity_core(v, im, va, vb, residualMode));
-                    Limp[k] = im;
-                    if (v.dynMask & 1) Lvel[v.indexA] = va;
-                    if (v.dynMask & 2) Lvel[v.indexB] = vb;
-                }
-                for (int c = Lbo[bt] + 64; c < e0; c += 64) {
-                    int j2 = c + lane;
-                    if (j2 < e0) {
-                        int k2 = Lsch[j2];
-                        VelCon v2_ = vc[k2];
-                        float4 va = Lvel[v2_.indexA], vb = Lvel[v2_.indexB], im = Limp[k2];
-                        if (phase == 0) warm_start_core(v2_, im, va, vb);
-                        else lres = fmaxr(lres, solve_velocity_core(v2_, im, va, vb, residualMode));
-                        Limp[k2] = im;
-                        if (v2_.dynMask & 1) Lvel[v2_.indexA] = va;
-                        if (v2_.dynMask & 2) Lvel[v2_.indexB] = vb;
-                    }
-                }
-                have = haveN;
-                k = kN;
-                v = vN;
-            }
             ++used;
-            if (phase == 1 && residualMode) {
-                float r = lres;
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) r = fmaxr(r, __shfl_xor(r, o, 64));
-                if (r <= sp.residualTol) break;
+            if (!residualMode) {
+                velocity_sweep<1>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+                continue;
             }
+            float r = velocity_sweep<2>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r = fmaxr(r, __shfl_xor(r, o, 64));
+            if (r <= sp.residualTol) break;
         }
-        if (phase == 1 && lane == 0) d.S[w].iterations = used;
-        if (timed && lane == 0) d.ktime[6 + phase] = __builtin_amdgcn_s_memtime();
+        if (lane == 0) d.S[w].iterations = used;
     }
+    if (timed && lane == 0) d.ktime[7] = __builtin_amdgcn_s_memtime();
     // StoreImpulses (b2contactsolver.d:774-787)
     for (int k = lane; k < nc; k += 64) {
         const VelCon &v = vc[k];
@@ -1321,45 +1429,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     for (int it = 0; it < sp.positionIterations; ++it) {
         for (int i = lane; i < nisl; i += 64)
             if (!Lsol[i]) Lmin[i] = f2ord(0.0f);
-        if (nbatch > 0) {
-            int j = Lbo[0] + lane;
-            bool have = j < Lbo[1];
-            int k = Lsch[have ? j : 0];
-            PosCon q = pc[k];
-            int end1 = Lbo[1];
-            for (int bt = 0; bt < nbatch; ++bt) {
-                const int e0 = end1;
-                const int e1 = bt + 1 < nbatch ? Lbo[bt + 2] : e0;
-                int jn = e0 + lane;
-                bool haveN = jn < e1;
-                int kN = Lsch[haveN ? jn : 0];
-                PosCon qN = pc[kN];
-                end1 = e1;
-                if (have && !Lsol[q.island]) {
-                    float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
-                    float ms = solve_position_core(q, pa, pb);
-                    if (q.dynMask & 1) Lpos[q.indexA] = pa;
-                    if (q.dynMask & 2) Lpos[q.indexB] = pb;
-                    atomicMin(&Lmin[q.island], f2ord(ms));
-                }
-                for (int c = Lbo[bt] + 64; c < e0; c += 64) {
-                    int j2 = c + lane;
-                    if (j2 < e0) {
-                        PosCon q2 = pc[Lsch[j2]];
-                        if (!Lsol[q2.island]) {
-                            float4 pa = Lpos[q2.indexA], pb = Lpos[q2.indexB];
-                            float ms = solve_position_core(q2, pa, pb);
-                            if (q2.dynMask & 1) Lpos[q2.indexA] = pa;
-                            if (q2.dynMask & 2) Lpos[q2.indexB] = pb;
-                            atomicMin(&Lmin[q2.island], f2ord(ms));
-                        }
-                    }
-                }
-                have = haveN;
-                k = kN;
-                q = qN;
-            }
-        }
+        if (nbatch > 0) position_sweep(Lsch, Lbo, nbatch, pc, Lpos, Lmin, Lsol, lane);
         bool open = false;
         for (int i = lane; i < nisl; i += 64) {
             if (Lsol[i]) continue;
@@ -1572,6 +1642,7 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
     return total;
 }
 
+template <bool kLead>
 __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_tmp[kMaxBlock / 64 + 1];
@@ -1782,43 +1853,64 @@ __global__ void k_max_penetration(Dev d, int cur, float *out) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_collide(const Dev &d, int cur, int block, hipStream_t s) {
-    hipLaunchKernelGGL(k_collide, dim3(d.nworlds), dim3(block), 0, s, d, cur);
+#define DBX_LAUNCH(kernel, lead, grid, block, lds, s, ...)                                                              \
+    do {                                                                                                              \
+        if (lead) hipLaunchKernelGGL(kernel<true>, grid, block, lds, s, __VA_ARGS__);                                 \
+        else hipLaunchKernelGGL(kernel<false>, grid, block, lds, s, __VA_ARGS__);                                     \
+    } while (0)
+
+hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int block, bool lead, hipStream_t s) {
+    if (nworlds <= 0) return hipSuccess;
+    DBX_LAUNCH(k_collide, lead, dim3(nworlds), dim3(block), 0, s, d, cur, worldOffset);
     return hipGetLastError();
 }
 size_t islands_scratch_bytes(int bodies, int contacts) { return isl_bytes(bodies, contacts); }
+size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands) {
+    return solve_lds_bytes(bodies, constraints, levels, islands);
+}
 
-hipError_t configure_kernels(int maxDynamicLds) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_islands, hipFuncAttributeMaxDynamicSharedMemorySize, maxDynamicLds);
+template <bool kLead>
+static hipError_t configure_tier(int maxDynamicLds) {
+    // k_solve_wave and k_broadphase also hold a few bytes of static LDS
+    hipError_t e = hipFuncSetAttribute((const void *)k_islands<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       maxDynamicLds);
     if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_broadphase, hipFuncAttributeMaxDynamicSharedMemorySize,
+    e = hipFuncSetAttribute((const void *)k_broadphase<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
-    // k_solve_wave also holds a few bytes of static LDS in its fallback path
-    return hipFuncSetAttribute((const void *)k_solve_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+    return hipFuncSetAttribute((const void *)k_solve_wave<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                maxDynamicLds - 1024);
 }
+hipError_t configure_kernels(int maxDynamicLds) {
+    hipError_t e = configure_tier<true>(maxDynamicLds);
+    if (e != hipSuccess) return e;
+    return configure_tier<false>(maxDynamicLds);
+}
 
-hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, hipStream_t s) {
+hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, bool lead,
+                          hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_islands, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, ordered, worldOffset, ldsBytes);
+    DBX_LAUNCH(k_islands, lead, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, ordered, worldOffset, ldsBytes);
     return hipGetLastError();
 }
-hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int block, hipStream_t s) {
-    hipLaunchKernelGGL(k_solve, dim3(d.nworlds), dim3(block), 0, s, d, cur, sp);
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block, bool lead,
+                        hipStream_t s) {
+    if (nworlds <= 0) return hipSuccess;
+    if (block > 256) block = 256; // the kernel's launch bound
+    DBX_LAUNCH(k_solve, lead, dim3(nworlds), dim3(block), 0, s, d, cur, sp, worldOffset);
     return hipGetLastError();
 }
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
-                             hipStream_t s) {
+                             bool lead, hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_solve_wave, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, sp, worldOffset, ldsBytes);
+    DBX_LAUNCH(k_solve_wave, lead, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, sp, worldOffset, ldsBytes);
     return hipGetLastError();
 }
 size_t broadphase_scratch_bytes(int proxies) { return bp_bytes(proxies); }
-hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block,
+hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                              hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_broadphase, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
+    DBX_LAUNCH(k_broadphase, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
 }
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total,

@@ -383,6 +383,21 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
+// Worlds are launched in tiers of similar size, largest first.  A tier's island
+// search, solver and broadphase workgroups get `lds` bytes of LDS (a world whose
+// state does not fit there runs from global memory instead), and each tier
+// runs its whole step chain on its own stream, so that the many small worlds
+// proceed while the few largest ones, which set the step time, are solved.
+struct Tier {
+    int first = 0, count = 0; // range in the launch order
+    int lds = 0;
+    int block = 256;          // threads of its collide, broadphase and coloured-solve workgroups
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool own = false;         // stream and event created for this tier
+};
+constexpr int kStepEvents = 6;
+
 struct dbx_handle {
     dbx_config cfg{};
     hipStream_t stream = nullptr;
@@ -398,13 +413,10 @@ struct dbx_handle {
     bool newFixture = true;
     int block = 256;
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
-    int islSmallLds = 24576;      // island-search LDS of ordinary worlds
-    int solveSmallLds = 32768;    // exact-order solver LDS of ordinary worlds
-    int bpSmallLds = 16384;       // broadphase LDS of ordinary worlds
-    int nBigIsl = 0;              // worlds (largest first) launched with ldsMax
+    std::vector<Tier> tiers;      // tier 0 = the largest worlds, on `stream`
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
-    std::vector<hipEvent_t> ring;   // 5 events per step since the last sync
+    std::vector<hipEvent_t> ring;   // kStepEvents events per step since the last sync
     int ringUsed = 0;
     dbx_profile prof{};             // last step
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
@@ -495,6 +507,12 @@ void dbx_destroy(dbx_handle *h) {
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : h->ring) (void)hipEventDestroy(e);
+    for (Tier &t : h->tiers) {
+        if (!t.own) continue;
+        (void)hipStreamSynchronize(t.stream);
+        (void)hipStreamDestroy(t.stream);
+        (void)hipEventDestroy(t.done);
+    }
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -560,16 +578,46 @@ dbx_status dbx_finalize(dbx_handle *h) {
     h->totalFix = nf;
     h->totalContacts = nc;
     h->totalPairs = np;
-    // launch order: largest world first
+    // Launch order: largest LDS need first (island-search state at three contacts
+    // per fixture, which also bounds the solver's), then split into tiers.
+    std::vector<size_t> need(nw);
+    for (int w = 0; w < nw; ++w)
+        need[w] = islands_scratch_bytes(h->W[w].bodyCount, 3 * h->W[w].fixCount + 16);
     std::vector<int> order(nw);
     for (int w = 0; w < nw; ++w) order[w] = w;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->W[a].fixCount > h->W[b].fixCount; });
-    // Island-search LDS classes: a world whose search state (at four contacts per
-    // fixture) exceeds the small class gets a workgroup with the device's full LDS.
-    h->nBigIsl = 0;
-    for (int i = 0; i < nw; ++i) {
-        const WorldDesc &D = h->W[order[i]];
-        if (islands_scratch_bytes(D.bodyCount, 4 * D.fixCount) > (size_t)h->islSmallLds) h->nBigIsl = i + 1;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need[a] > need[b]; });
+    {
+        std::vector<int> sizes = {h->ldsMax};
+        for (int l : {81920, 40960, 20480})
+            if (l < h->ldsMax) sizes.push_back(l);
+        std::vector<Tier> tiers;
+        for (int i = 0; i < nw; ++i) {
+            size_t n = need[order[i]];
+            int t = 0;
+            while (t + 1 < (int)sizes.size() && (size_t)sizes[t + 1] >= n) ++t;
+            if (tiers.empty() || tiers.back().lds != sizes[t]) {
+                Tier T;
+                T.first = i;
+                T.lds = sizes[t];
+                T.block = h->block;
+                tiers.push_back(T);
+            }
+            tiers.back().count++;
+        }
+        for (size_t t = 0; t < tiers.size(); ++t) {
+            if (t == 0) {
+                tiers[t].stream = h->stream;
+                continue;
+            }
+            DBX_HIP(h, hipStreamCreate(&tiers[t].stream));
+            tiers[t].own = true;
+            DBX_HIP(h, hipEventCreateWithFlags(&tiers[t].done, hipEventDisableTiming));
+        }
+        h->tiers = tiers;
+        if (getenv("DBX_VERBOSE"))
+            for (const Tier &T : h->tiers)
+                fprintf(stderr, "dbx: tier of %d worlds from %d, LDS %d bytes, block %d\n", T.count, T.first, T.lds,
+                        T.block);
     }
 
     Dev &d = h->d;
@@ -677,10 +725,32 @@ dbx_status dbx_finalize(dbx_handle *h) {
     return DBX_OK;
 }
 
-static hipError_t broadphase(dbx_handle *h, hipStream_t s) {
-    hipError_t e = launch_broadphase(h->d, h->cur, 0, h->nBigIsl, h->ldsMax - 1024, h->block, s);
-    if (e != hipSuccess) return e;
-    return launch_broadphase(h->d, h->cur, h->nBigIsl, h->nworlds - h->nBigIsl, h->bpSmallLds, h->block, s);
+// The step chain of one tier (b2World.Step, b2world.d:367-434, for its worlds).
+static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew, hipEvent_t *ev) {
+    const Tier &T = h->tiers[t];
+    const bool lead = t == 0;
+    const int cin = h->cur, cout = cin ^ 1;
+    const int lds1 = std::min(T.lds, h->ldsMax - 1024); // kernels that also hold static LDS
+    hipStream_t s = T.stream;
+    const Dev &d = h->d;
+    if (findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
+        DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s));
+    DBX_HIP(h, launch_collide(d, cin, T.first, T.count, T.block, lead, s));
+    if (lead) (void)hipEventRecord(ev[1], s);
+    if (sp.dt > 0.0f) {
+        const int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
+        DBX_HIP(h, launch_islands(d, cout, ordered, T.first, T.count, T.lds, lead, s));
+        if (lead) (void)hipEventRecord(ev[2], s);
+        if (ordered) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
+        else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, T.block, lead, s));
+        if (lead) (void)hipEventRecord(ev[3], s);
+        DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s));
+    } else if (lead) {
+        (void)hipEventRecord(ev[2], s);
+        (void)hipEventRecord(ev[3], s);
+    }
+    if (lead) (void)hipEventRecord(ev[4], s);
+    return DBX_OK;
 }
 
 dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t flags) {
@@ -701,48 +771,30 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     sp.residualTol = 1e-3f;
     sp.maxIterations = 100;
     hipStream_t s = h->stream;
-    if ((int)h->ring.size() < 5 * (h->ringUsed + 1)) {
-        for (int i = 0; i < 5; ++i) {
+    if ((int)h->ring.size() < kStepEvents * (h->ringUsed + 1)) {
+        for (int i = 0; i < kStepEvents; ++i) {
             hipEvent_t e;
             DBX_HIP(h, hipEventCreate(&e));
             h->ring.push_back(e);
         }
     }
-    hipEvent_t *ev = &h->ring[5 * h->ringUsed];
+    hipEvent_t *ev = &h->ring[kStepEvents * h->ringUsed];
     h->ringUsed++;
     (void)hipEventRecord(ev[0], s);
-    if (h->newFixture) { // b2World.Step: FindNewContacts when fixtures were added (b2world.d:372-376)
-        DBX_HIP(h, broadphase(h, s));
-        h->newFixture = false;
+    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, ev[0], 0));
+    const bool findNew = h->newFixture;
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        dbx_status st = step_tier(h, (int)t, sp, flags, findNew, ev);
+        if (st != DBX_OK) return st;
     }
-    DBX_HIP(h, launch_collide(h->d, h->cur, h->block, s));
+    for (size_t t = 1; t < h->tiers.size(); ++t) {
+        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
+        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    }
+    (void)hipEventRecord(ev[5], s);
+    h->newFixture = false;
     h->cur ^= 1;
-    (void)hipEventRecord(ev[1], s);
-    if (dt > 0.0f) {
-        int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
-        DBX_HIP(h, launch_islands(h->d, h->cur, ordered, 0, h->nBigIsl, h->ldsMax, s));
-        DBX_HIP(h, launch_islands(h->d, h->cur, ordered, h->nBigIsl, h->nworlds - h->nBigIsl, h->islSmallLds, s));
-        (void)hipEventRecord(ev[2], s);
-        if (flags & DBX_STEP_LEARNED) {
-            dbx_status st = mlp_step_hook(h->mlp, h->d, h->cur, sp, h->block, s);
-            if (st != DBX_OK) return fail(h, st, "learned warm start failed");
-        }
-        if (ordered) {
-            // the fallback path of the kernel keeps a few bytes of static LDS
-            DBX_HIP(h, launch_solve_wave(h->d, h->cur, sp, 0, h->nBigIsl, h->ldsMax - 1024, s));
-            DBX_HIP(h, launch_solve_wave(h->d, h->cur, sp, h->nBigIsl, h->nworlds - h->nBigIsl, h->solveSmallLds, s));
-        } else {
-            DBX_HIP(h, launch_solve(h->d, h->cur, sp, h->block, s));
-        }
-        (void)hipEventRecord(ev[3], s);
-        DBX_HIP(h, broadphase(h, s));
-        (void)hipEventRecord(ev[4], s);
-        h->inv_dt0 = sp.inv_dt;
-    } else {
-        (void)hipEventRecord(ev[2], s);
-        (void)hipEventRecord(ev[3], s);
-        (void)hipEventRecord(ev[4], s);
-    }
+    if (dt > 0.0f) h->inv_dt0 = sp.inv_dt;
     return DBX_OK;
 }
 
@@ -761,9 +813,10 @@ dbx_status dbx_sync(dbx_handle *h) {
         }
     }
     for (int k = 0; k < h->ringUsed; ++k) {
-        hipEvent_t *ev = &h->ring[5 * k];
-        float ms[4] = {0, 0, 0, 0};
+        hipEvent_t *ev = &h->ring[kStepEvents * k];
+        float ms[4] = {0, 0, 0, 0}, total = 0.0f;
         for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        (void)hipEventElapsedTime(&total, ev[0], ev[5]);
         dbx_profile p{};
         p.collide = ms[0];
         p.solveInit = ms[1];
@@ -772,7 +825,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         p.solve = ms[1] + ms[2] + ms[3];
         p.broadphase = ms[3];
         p.solveTOI = 0.0f;
-        p.step = ms[0] + ms[1] + ms[2] + ms[3];
+        p.step = total;
         h->prof = p;
         h->sum.step += p.step;
         h->sum.collide += p.collide;
@@ -791,6 +844,20 @@ dbx_status dbx_sync(dbx_handle *h) {
                     t[1] - t[0], t[2] - t[1], t[3] - t[2], t[15], t[14], t[5] - t[4], t[6] - t[5], t[7] - t[6],
                     t[8] - t[7], t[9] - t[8]);
     }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_tiers(dbx_handle *h, int32_t *tiers, int32_t cap, int32_t *order, int32_t *count) {
+    if (!h || !count || (cap > 0 && !tiers)) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    *count = (int32_t)h->tiers.size();
+    for (int i = 0; i < cap && i < (int)h->tiers.size(); ++i) {
+        tiers[4 * i + 0] = h->tiers[i].first;
+        tiers[4 * i + 1] = h->tiers[i].count;
+        tiers[4 * i + 2] = h->tiers[i].lds;
+        tiers[4 * i + 3] = h->tiers[i].block;
+    }
+    if (order) DBX_HIP(h, hipMemcpy(order, h->d.order, h->nworlds * sizeof(int), hipMemcpyDeviceToHost));
     return DBX_OK;
 }
 

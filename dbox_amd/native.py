@@ -58,7 +58,7 @@ EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
-    "dbx_collide_pairs", "dbx_read_schedule",
+    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers",
 ]
 
 
@@ -210,6 +210,14 @@ class StepWorlds:
         n = C.c_int32()
         self._check(self.L.dbx_profile_totals(self.h, C.byref(p), C.byref(n), int(reset)))
         return {k: getattr(p, k) for k, _ in DbxProfile._fields_}, n.value
+
+    def tiers(self):
+        """Launch tiers as rows (first launch slot, worlds, LDS bytes, workgroup size) and the launch order."""
+        n = C.c_int32()
+        rows = np.zeros((8, 4), np.int32)
+        order = np.zeros(self.world_count, np.int32)
+        self._check(self.L.dbx_read_tiers(self.h, _p(rows), 8, _p(order), C.byref(n)))
+        return rows[:n.value], order
 
     def solver_stats(self):
         nw = self.world_count

@@ -157,6 +157,14 @@ dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
 /* Per-world velocity iterations used by the last step (RESIDUAL mode) and max penetration. */
 dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds);
 
+/* Launch tiers (no counterpart in the reference; worlds are independent there):
+ * worlds are launched largest first, in tiers of similar size that each run
+ * their step chain on a stream of their own.  Writes (first launch slot, world
+ * count, LDS bytes, workgroup size) per tier into tiers[4 * i .. 4 * i + 3] for
+ * up to cap tiers, the launch order (world index per slot) into order when it
+ * is not null (nworlds entries), and the tier count into count. */
+dbx_status dbx_read_tiers(dbx_handle *h, int32_t *tiers, int32_t cap, int32_t *order, int32_t *count);
+
 /* The island solve order of the last step for one world: for each constraint in
  * the order b2World.Solve's island search adds contacts (b2world.d:963-1076), its
  * island, body A, body B and solve batch (4 int32 per row).  Returns the row count. */
