@@ -12,9 +12,11 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "dbx_geom.h"
+#include "dbx_mlp_dev.h"
 
 namespace dbx {
 
@@ -93,6 +95,7 @@ struct alignas(16) VelCon {
     int pointCount, contact, island, dynMask; // dynMask bit0/1: body A/B is dynamic
 };
 static_assert(sizeof(VelCon) == 160, "velocity constraint layout");
+static_assert(offsetof(VelCon, nImp0) == 32, "impulses form one float4");
 
 // b2ContactPositionConstraint (b2contactsolver.d:801-814) plus body indices, 96 bytes.
 struct alignas(16) PosCon {
@@ -161,12 +164,16 @@ struct Dev {
     // per island scratch (island index < body count)
     int *islMin;            // order-preserving int image of a float minimum
     int *islSolved;
-    // learned warm start (optional)
-    float *feat;            // 14 features per new manifold point
-    int *featSlot;          // constraint slot * 2 + point for each feature row
-    float *pred;            // 2 predictions per feature row
-    int *featCount;         // single counter
-    float *residual;        // per world residual of the last iteration
+    // learned warm start and its training log (null unless enabled)
+    MlpDev mlp;
+    float *mlpFeat;         // staged feature rows: 2 per contact slot from 2 * contactBase
+    int2 *mlpList;          // per staged row: (constraint * 2 + point, log row or -1)
+    int *mlpNew;            // per world: rows staged in the current step
+    float2 *mlpBody;        // per body: sum and count of matched warm-start normal impulses
+    float *logX;            // log rows of kMlpFeatStride features
+    float *logT;            // 2 labels (normal, tangent impulse) per log row
+    unsigned long long *logCount;
+    long long logCap;
     long long *ktime;       // optional: phase timestamps of the largest world (s_memtime)
 };
 

@@ -894,6 +894,144 @@ __device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa
     return minSep;
 }
 
+// ---------------------------------------------------------------------------
+// Learned warm start and its training log (no counterpart in the reference).
+// A new manifold point is one that b2Contact.Update (b2contact.d:306-324)
+// could not match to a point of the previous manifold; its impulse starts at
+// zero.  With the LEARNED flag the network's prediction replaces that zero
+// before the warm start; with the LOG flag the point's features and, after the
+// velocity iterations, its accumulated impulses are appended to the log.
+// ---------------------------------------------------------------------------
+struct ImpRef { // accumulated impulses (nImp0, tImp0, nImp1, tImp1) of constraint k
+    float4 *base;
+    int stride; // in float4
+    __device__ __forceinline__ float4 &operator[](int k) const { return base[(size_t)k * stride]; }
+};
+
+// The 14 features of point j of a constraint: relative normal and tangent
+// velocity, normal, separation, inverse masses and inertias, mean matched
+// normal impulse of the contacts of both bodies, rA x n, rB x n, friction and
+// the point's normal mass.
+__device__ void mlp_features(float *f, const VelCon &v, const PosCon &q, int j, float4 va, float4 vb, float4 pa,
+                             float4 pb, float2 nbA, float2 nbB) {
+    const float2 rA = sel(j != 0, v.rA1, v.rA0), rB = sel(j != 0, v.rB1, v.rB0);
+    const float2 n = v.normal, t = crs(n, 1.0f);
+    const float2 dv = sub(sub(add(v2(vb.x, vb.y), crs(vb.z, rB)), v2(va.x, va.y)), crs(va.z, rA));
+    f[0] = dot(dv, n);
+    f[1] = dot(dv, t);
+    f[2] = n.x;
+    f[3] = n.y;
+    float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
+    float aA = pa.z, aB = pb.z; // copies: only the separation is wanted
+    f[4] = position_point(q, sel(j != 0, q.lp1, q.lp0), cA, aA, cB, aB);
+    f[5] = v.invMassA;
+    f[6] = v.invMassB;
+    f[7] = v.invIA;
+    f[8] = v.invIB;
+    const float cnt = nbA.y + nbB.y;
+    f[9] = cnt > 0.0f ? (nbA.x + nbB.x) / cnt : 0.0f;
+    f[10] = crs(rA, n);
+    f[11] = crs(rB, n);
+    f[12] = v.friction;
+    f[13] = j ? v.nMass1 : v.nMass0;
+    f[14] = 0.0f;
+    f[15] = 0.0f;
+}
+
+// Workgroup-level: stage the new points of world w, predict their impulses
+// (LEARNED) and log their features (LOG).  vel / pos are the bodies' integrated
+// velocities and positions, imp the constraints' accumulated impulses.
+__device__ __forceinline__ void mlp_new_points(const Dev &d, int w, int nc, uint32_t flags, const VelCon *vc,
+                                            const PosCon *pc, const Contact *cont, ImpRef imp, const float4 *vel,
+                                            const float4 *pos) {
+    const WorldDesc wd = d.W[w];
+    const bool learned = (flags & (1u << 4)) != 0, logging = (flags & (1u << 6)) != 0;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    float2 *nbs = d.mlpBody + wd.bodyBase;
+    float *feat = d.mlpFeat + (size_t)2 * wd.contactBase * kMlpFeatStride;
+    int2 *list = d.mlpList + (size_t)2 * wd.contactBase;
+    for (int b = tid; b < wd.bodyCount; b += BS) nbs[b] = make_float2(0.0f, 0.0f);
+    if (tid == 0) d.mlpNew[w] = 0;
+    __syncthreads();
+    for (int k = tid; k < nc; k += BS) {
+        const VelCon &v = vc[k];
+        const uint32_t fl = cont[v.contact].flags;
+        const float4 im = imp[k];
+        for (int j = 0; j < v.pointCount; ++j) {
+            if (fl & (j ? kContactNewPt1 : kContactNewPt0)) continue;
+            const float li = j ? im.z : im.x;
+            atomicAdd(&nbs[v.indexA].x, li);
+            atomicAdd(&nbs[v.indexA].y, 1.0f);
+            atomicAdd(&nbs[v.indexB].x, li);
+            atomicAdd(&nbs[v.indexB].y, 1.0f);
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < nc; k += BS) {
+        const VelCon &v = vc[k];
+        const uint32_t fl = cont[v.contact].flags;
+        for (int j = 0; j < v.pointCount; ++j) {
+            if (!(fl & (j ? kContactNewPt1 : kContactNewPt0))) continue;
+            const int row = atomicAdd(&d.mlpNew[w], 1); // at most two rows per constraint
+            float f[kMlpFeatStride];
+            mlp_features(f, v, pc[k], j, vel[v.indexA], vel[v.indexB], pos[v.indexA], pos[v.indexB], nbs[v.indexA],
+                         nbs[v.indexB]);
+            float *dst = feat + (size_t)row * kMlpFeatStride;
+            for (int i = 0; i < kMlpFeatStride; ++i) dst[i] = f[i];
+            int logRow = -1;
+            if (logging) {
+                const unsigned long long r = atomicAdd(d.logCount, 1ull);
+                if (r < (unsigned long long)d.logCap) {
+                    logRow = (int)r;
+                    float *lx = d.logX + r * kMlpFeatStride;
+                    for (int i = 0; i < kMlpFeatStride; ++i) lx[i] = f[i];
+                }
+            }
+            list[row] = make_int2(2 * k + j, logRow);
+        }
+    }
+    __syncthreads();
+    if (!learned) return;
+    const int n = d.mlpNew[w];
+    const int lane = tid & 63, g = lane >> 4, col = lane & 15, wave = tid >> 6, nwaves = (BS + 63) >> 6;
+    for (int base = 16 * wave; base < n; base += 16 * nwaves) { // wave-uniform: the MFMAs need every lane
+        const int r = base + col;
+        const bf16x8 x = mlp_input_operand(d.mlp, r < n ? feat + (size_t)r * kMlpFeatStride : nullptr, g);
+        MlpFwd F;
+        mlp_forward16(d.mlp, x, F);
+        if (g == 0 && r < n) {
+            const int2 e = list[r];
+            const int k = e.x >> 1, j = e.x & 1;
+            float ln = F.y[0] * d.mlp.norm[kNormOutScale] + d.mlp.norm[kNormOutMean];
+            float lt = F.y[1] * d.mlp.norm[kNormOutScale + 1] + d.mlp.norm[kNormOutMean + 1];
+            // the plain solver's own bounds: normal impulse >= 0, |tangent| <= friction * normal
+            if (!(ln >= 0.0f) || !(ln < kMaxFloat)) ln = 0.0f;
+            const float mt = vc[k].friction * ln;
+            if (!(lt == lt)) lt = 0.0f;
+            lt = clampr(lt, -mt, mt);
+            float *ip = reinterpret_cast<float *>(&imp[k]);
+            ip[2 * j] = ln;
+            ip[2 * j + 1] = lt;
+        }
+    }
+    __syncthreads();
+}
+
+// Labels of the logged points: their accumulated impulses after the velocity iterations.
+__device__ __forceinline__ void mlp_log_labels(const Dev &d, int w, ImpRef imp) {
+    const WorldDesc wd = d.W[w];
+    const int n = d.mlpNew[w];
+    const int2 *list = d.mlpList + (size_t)2 * wd.contactBase;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const int2 e = list[r];
+        if (e.y < 0) continue;
+        const float4 im = imp[e.x >> 1];
+        const bool p1 = (e.x & 1) != 0;
+        d.logT[2 * (size_t)e.y] = p1 ? im.z : im.x;
+        d.logT[2 * (size_t)e.y + 1] = p1 ? im.w : im.y;
+    }
+}
+
 // Global-memory wrappers (workgroup-synchronised schedules).
 __device__ __forceinline__ void warm_start(const VelCon &v, float4 *bvel) {
     float4 va = bvel[v.indexA], vb = bvel[v.indexB];
@@ -985,6 +1123,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         pc[k] = p;
     }
     __syncthreads();
+    const ImpRef gimp{reinterpret_cast<float4 *>(&vc[0].nImp0), (int)(sizeof(VelCon) / sizeof(float4))};
+    if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, gimp, bvel, bpos);
 
     if (sp.flags & 1u) {
         for (int bt = 0; bt < nbatch; ++bt) {
@@ -1024,6 +1164,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         }
     }
     if (tid == 0) d.S[w].iterations = used;
+    if (sp.flags & (1u << 6)) mlp_log_labels(d, w, gimp);
 
     // StoreImpulses (b2contactsolver.d:774-787)
     for (int k = tid; k < nc; k += BS) {
@@ -1371,6 +1512,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         Limp[k] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
     }
     __syncthreads();
+    if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, ImpRef{Limp, 1}, Lvel, Lpos);
 
     if (timed && lane == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
@@ -1393,6 +1535,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         if (lane == 0) d.S[w].iterations = used;
     }
     if (timed && lane == 0) d.ktime[7] = __builtin_amdgcn_s_memtime();
+    if (sp.flags & (1u << 6)) mlp_log_labels(d, w, ImpRef{Limp, 1});
     // StoreImpulses (b2contactsolver.d:774-787)
     for (int k = lane; k < nc; k += 64) {
         const VelCon &v = vc[k];

@@ -32,9 +32,10 @@ hipError_t mlp_upload(MlpState &m, const float *params, const float *norm, hipSt
 hipError_t mlp_forward(MlpState &m, const float *dX, int n, float *dY, hipStream_t s);
 // Accumulates into m.grad the gradient of the mean squared error over `batch`
 // samples (rows drawn with replacement by `seed`, or rows 0..batch-1 if seed is 0)
-// and writes the loss to m.loss.
+// and adds the loss to *dLoss (a zeroed device float), or writes it to m.loss if dLoss is null.
 hipError_t mlp_gradients(MlpState &m, const float *dX, const float *dT, long long n, long long batch, uint64_t seed,
-                         hipStream_t s);
+                         float *dLoss, hipStream_t s);
+hipError_t mlp_fill(float *p, long long n, float v, hipStream_t s);
 hipError_t mlp_adam_step(MlpState &m, float lr, hipStream_t s);
 hipError_t mlp_clear_grad(MlpState &m, hipStream_t s);
 

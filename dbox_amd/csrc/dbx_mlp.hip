@@ -295,12 +295,20 @@ hipError_t mlp_forward(MlpState &m, const float *dX, int n, float *dY, hipStream
 }
 
 hipError_t mlp_gradients(MlpState &m, const float *dX, const float *dT, long long n, long long batch, uint64_t seed,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, m.loss, 1LL, 0.0f);
+                         float *dLoss, hipStream_t s) {
+    if (!dLoss) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, m.loss, 1LL, 0.0f);
+        dLoss = m.loss;
+    }
     const long long rounds = (batch + 31) / 32;
     const long long waves = rounds < 2048 ? rounds : 2048;
     hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)waves), dim3(64), 0, s, mlp_dev(m), dX, dT, n, batch, seed, m.grad,
-                       m.loss);
+                       dLoss);
+    return hipGetLastError();
+}
+
+hipError_t mlp_fill(float *p, long long n, float v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 

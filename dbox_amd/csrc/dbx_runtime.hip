@@ -424,6 +424,12 @@ struct dbx_handle {
     long long totalBodies = 0, totalFix = 0, totalContacts = 0, totalPairs = 0;
     float *dPen = nullptr;
     MlpState mlp{};
+    float residualTol = 1e-3f;
+    int maxIterations = 100;
+    bool staged = false;            // learned / log staging buffers allocated
+    float *logX = nullptr, *logT = nullptr;
+    unsigned long long *logCount = nullptr;
+    long long logCap = 0;
 };
 
 namespace {
@@ -725,6 +731,17 @@ dbx_status dbx_finalize(dbx_handle *h) {
     return DBX_OK;
 }
 
+// Staging buffers of the learned warm start / training log, sized by the worlds.
+static dbx_status stage_mlp(dbx_handle *h) {
+    Dev &d = h->d;
+    DBX_HIP(h, dalloc(h, &d.mlpFeat, 2 * h->totalContacts * kMlpFeatStride));
+    DBX_HIP(h, dalloc(h, &d.mlpList, 2 * h->totalContacts));
+    DBX_HIP(h, dalloc(h, &d.mlpNew, h->nworlds));
+    DBX_HIP(h, dalloc(h, &d.mlpBody, h->totalBodies));
+    h->staged = true;
+    return DBX_OK;
+}
+
 // The step chain of one tier (b2World.Step, b2world.d:367-434, for its worlds).
 static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew, hipEvent_t *ev) {
     const Tier &T = h->tiers[t];
@@ -759,6 +776,9 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     if (flags & DBX_STEP_CCD) return fail(h, DBX_E_UNSUPPORTED, "continuous physics (TOI) is not implemented");
     if (!std::isfinite(dt) || vel < 0 || pos < 0) return fail(h, DBX_E_INVALID_ARG, "bad step parameters");
     if ((flags & DBX_STEP_LEARNED) && !h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    if ((flags & DBX_STEP_LEARNED) && !(flags & DBX_STEP_WARM))
+        return fail(h, DBX_E_INVALID_ARG, "the learned warm start needs warm starting");
+    if ((flags & DBX_STEP_LOG) && !h->logX) return fail(h, DBX_E_STATE, "the training log is not enabled");
     DBX_HIP(h, hipSetDevice(h->cfg.device));
     StepParams sp{};
     sp.dt = dt;
@@ -767,9 +787,14 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     sp.velocityIterations = vel;
     sp.positionIterations = pos;
     sp.flags = ((flags & DBX_STEP_WARM) ? 1u : 0u) | ((flags & DBX_STEP_SLEEP) ? 2u : 0u) |
-               ((flags & DBX_STEP_RESIDUAL) ? (1u << 5) : 0u) | ((flags & DBX_STEP_LEARNED) ? (1u << 4) : 0u);
-    sp.residualTol = 1e-3f;
-    sp.maxIterations = 100;
+               ((flags & DBX_STEP_RESIDUAL) ? (1u << 5) : 0u) | ((flags & DBX_STEP_LEARNED) ? (1u << 4) : 0u) |
+               ((flags & DBX_STEP_LOG) ? (1u << 6) : 0u);
+    if ((flags & (DBX_STEP_LEARNED | DBX_STEP_LOG)) && !h->staged) {
+        dbx_status st = stage_mlp(h);
+        if (st != DBX_OK) return st;
+    }
+    sp.residualTol = h->residualTol;
+    sp.maxIterations = h->maxIterations;
     hipStream_t s = h->stream;
     if ((int)h->ring.size() < kStepEvents * (h->ringUsed + 1)) {
         for (int i = 0; i < kStepEvents; ++i) {
@@ -780,6 +805,11 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     }
     hipEvent_t *ev = &h->ring[kStepEvents * h->ringUsed];
     h->ringUsed++;
+    h->d.mlp = mlp_dev(h->mlp);
+    h->d.logX = h->logX;
+    h->d.logT = h->logT;
+    h->d.logCount = h->logCount;
+    h->d.logCap = h->logCap;
     (void)hipEventRecord(ev[0], s);
     for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, ev[0], 0));
     const bool findNew = h->newFixture;
@@ -1141,3 +1171,239 @@ dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const floa
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Residual mode and the learned warm start.
+// ---------------------------------------------------------------------------
+dbx_status dbx_set_residual(dbx_handle *h, float tolerance, int32_t max_iterations) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!(tolerance >= 0.0f) || !std::isfinite(tolerance) || max_iterations < 1 || max_iterations > 100000)
+        return fail(h, DBX_E_INVALID_ARG, "bad residual settings");
+    h->residualTol = tolerance;
+    h->maxIterations = max_iterations;
+    return DBX_OK;
+}
+
+static uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+dbx_status dbx_mlp_init(dbx_handle *h, uint64_t seed) {
+    if (!h) return DBX_E_INVALID_ARG;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    std::vector<float> P(kMlpParams, 0.0f), N(kMlpNorm, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&](float lim) { return (float)(((splitmix(st) >> 11) * 0x1.0p-53) * 2.0 - 1.0) * lim; };
+    const float l1 = std::sqrt(6.0f / kMlpIn), l2 = std::sqrt(6.0f / kMlpHidden), l3 = std::sqrt(1.0f / kMlpHidden);
+    for (int i = 0; i < kMlpHidden * kMlpIn; ++i) P[kOffW1 + i] = uni(l1);
+    for (int i = 0; i < kMlpHidden * kMlpHidden; ++i) P[kOffW2 + i] = uni(l2);
+    for (int i = 0; i < kMlpOut * kMlpHidden; ++i) P[kOffW3 + i] = uni(l3);
+    for (int i = 0; i < kMlpIn; ++i) N[kNormInScale + i] = 1.0f;
+    N[kNormOutScale] = N[kNormOutScale + 1] = 1.0f;
+    DBX_HIP(h, mlp_upload(h->mlp, P.data(), N.data(), h->stream));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_set(dbx_handle *h, const float *params, const float *norm) {
+    if (!h || (!params && !norm)) return DBX_E_INVALID_ARG;
+    if (!h->mlp.ready && (!params || !norm)) return fail(h, DBX_E_STATE, "the first network needs parameters and normalisation");
+    if (params)
+        for (int i = 0; i < kMlpParams; ++i)
+            if (!std::isfinite(params[i])) return fail(h, DBX_E_INVALID_ARG, "parameter is not finite");
+    if (norm)
+        for (int i = 0; i < kMlpNorm; ++i)
+            if (!std::isfinite(norm[i])) return fail(h, DBX_E_INVALID_ARG, "normalisation is not finite");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, mlp_upload(h->mlp, params, norm, h->stream));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_get(dbx_handle *h, float *params, float *norm) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    if (params) DBX_HIP(h, hipMemcpy(params, h->mlp.params, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    if (norm) DBX_HIP(h, hipMemcpy(norm, h->mlp.norm, kMlpNorm * sizeof(float), hipMemcpyDeviceToHost));
+    return DBX_OK;
+}
+
+// Device copy of n feature rows of DBX_MLP_FEATURES floats, padded to kMlpFeatStride.
+static dbx_status upload_rows(dbx_handle *h, const float *x, int32_t n, float **dX) {
+    std::vector<float> pad((size_t)n * kMlpFeatStride, 0.0f);
+    for (int32_t i = 0; i < n; ++i)
+        for (int j = 0; j < kMlpIn; ++j) pad[(size_t)i * kMlpFeatStride + j] = x[(size_t)i * kMlpIn + j];
+    DBX_HIP(h, hipMalloc((void **)dX, pad.size() * sizeof(float)));
+    hipError_t e = hipMemcpy(*dX, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(*dX);
+        return hip_fail(h, e, "hipMemcpy");
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_forward(dbx_handle *h, const float *x, int32_t n, float *y) {
+    if (!h || n < 0 || (n > 0 && (!x || !y))) return DBX_E_INVALID_ARG;
+    if (!h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    if (n == 0) return DBX_OK;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    float *dX = nullptr, *dY = nullptr;
+    dbx_status st = upload_rows(h, x, n, &dX);
+    if (st != DBX_OK) return st;
+    hipError_t e = hipMalloc((void **)&dY, 2 * (size_t)n * sizeof(float));
+    if (e == hipSuccess) e = mlp_forward(h->mlp, dX, n, dY, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(y, dY, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dX);
+    if (dY) (void)hipFree(dY);
+    if (e != hipSuccess) return hip_fail(h, e, "mlp forward");
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_gradients(dbx_handle *h, const float *x, const float *t, int32_t n, float *grad, float *loss) {
+    if (!h || n <= 0 || !x || !t || !grad) return DBX_E_INVALID_ARG;
+    if (!h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    float *dX = nullptr, *dT = nullptr;
+    dbx_status st = upload_rows(h, x, n, &dX);
+    if (st != DBX_OK) return st;
+    hipError_t e = hipMalloc((void **)&dT, 2 * (size_t)n * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(dT, t, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_clear_grad(h->mlp, h->stream);
+    if (e == hipSuccess) e = mlp_gradients(h->mlp, dX, dT, n, n, 0, nullptr, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(grad, h->mlp.grad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, h->mlp.loss, sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = mlp_clear_grad(h->mlp, h->stream);
+    (void)hipFree(dX);
+    if (dT) (void)hipFree(dT);
+    if (e != hipSuccess) return hip_fail(h, e, "mlp gradients");
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_log_enable(dbx_handle *h, int64_t capacity) {
+    if (!h || capacity <= 0 || capacity > INT32_MAX) return DBX_E_INVALID_ARG;
+    if (h->logX) return fail(h, DBX_E_STATE, "the training log is already enabled");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    DBX_HIP(h, dalloc(h, &h->logX, capacity * kMlpFeatStride));
+    DBX_HIP(h, dalloc(h, &h->logT, 2 * capacity));
+    DBX_HIP(h, dalloc(h, &h->logCount, 1));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    h->logCap = capacity;
+    return DBX_OK;
+}
+
+static dbx_status log_rows(dbx_handle *h, long long *count) {
+    if (!h->logX) return fail(h, DBX_E_STATE, "the training log is not enabled");
+    unsigned long long c = 0;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&c, h->logCount, sizeof c, hipMemcpyDeviceToHost));
+    *count = (long long)std::min<unsigned long long>(c, (unsigned long long)h->logCap);
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_log_count(dbx_handle *h, int64_t *count) {
+    if (!h || !count) return DBX_E_INVALID_ARG;
+    long long c = 0;
+    dbx_status st = log_rows(h, &c);
+    if (st != DBX_OK) return st;
+    *count = c;
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_log_read(dbx_handle *h, int64_t first, int64_t n, float *x, float *t) {
+    if (!h || first < 0 || n < 0) return DBX_E_INVALID_ARG;
+    long long c = 0;
+    dbx_status st = log_rows(h, &c);
+    if (st != DBX_OK) return st;
+    if (first + n > c) return fail(h, DBX_E_INVALID_ARG, "log rows out of range");
+    if (n == 0) return DBX_OK;
+    std::vector<float> rows((size_t)n * kMlpFeatStride);
+    DBX_HIP(h, hipMemcpy(rows.data(), h->logX + first * kMlpFeatStride, rows.size() * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    if (x)
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < kMlpIn; ++j) x[i * kMlpIn + j] = rows[(size_t)i * kMlpFeatStride + j];
+    if (t) DBX_HIP(h, hipMemcpy(t, h->logT + 2 * first, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_log_clear(dbx_handle *h) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->logX) return fail(h, DBX_E_STATE, "the training log is not enabled");
+    DBX_HIP(h, hipMemsetAsync(h->logCount, 0, sizeof(unsigned long long), h->stream));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    return DBX_OK;
+}
+
+dbx_status dbx_mlp_fit_normalization(dbx_handle *h) {
+    if (!h) return DBX_E_INVALID_ARG;
+    long long c = 0;
+    dbx_status st = log_rows(h, &c);
+    if (st != DBX_OK) return st;
+    if (c < 2) return fail(h, DBX_E_STATE, "the training log has fewer than two rows");
+    std::vector<float> x((size_t)c * kMlpIn), t(2 * (size_t)c);
+    st = dbx_mlp_log_read(h, 0, c, x.data(), t.data());
+    if (st != DBX_OK) return st;
+    std::vector<float> N(kMlpNorm, 0.0f);
+    auto stats = [&](const float *v, int stride, int col, float &mean, float &sd) {
+        double s = 0.0, s2 = 0.0;
+        for (long long i = 0; i < c; ++i) s += v[i * stride + col];
+        const double m = s / (double)c;
+        for (long long i = 0; i < c; ++i) {
+            const double dv = v[i * stride + col] - m;
+            s2 += dv * dv;
+        }
+        mean = (float)m;
+        sd = (float)std::sqrt(s2 / (double)c);
+        if (!(sd > 1e-6f)) sd = 1.0f; // a constant feature carries no information
+    };
+    for (int j = 0; j < kMlpIn; ++j) {
+        float m, sd;
+        stats(x.data(), kMlpIn, j, m, sd);
+        N[kNormInMean + j] = m;
+        N[kNormInScale + j] = 1.0f / sd;
+    }
+    for (int o = 0; o < kMlpOut; ++o) {
+        float m, sd;
+        stats(t.data(), 2, o, m, sd);
+        N[kNormOutMean + o] = m;
+        N[kNormOutScale + o] = sd;
+    }
+    if (!h->mlp.ready) {
+        st = dbx_mlp_init(h, 0x5EED);
+        if (st != DBX_OK) return st;
+    }
+    return dbx_mlp_set(h, nullptr, N.data());
+}
+
+dbx_status dbx_mlp_train(dbx_handle *h, int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss) {
+    if (!h || steps < 0 || batch <= 0 || !(lr > 0.0f)) return DBX_E_INVALID_ARG;
+    if (!h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
+    long long c = 0;
+    dbx_status st = log_rows(h, &c);
+    if (st != DBX_OK) return st;
+    if (c < 1) return fail(h, DBX_E_STATE, "the training log is empty");
+    if (steps == 0) return DBX_OK;
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    float *dLoss = nullptr;
+    DBX_HIP(h, hipMalloc((void **)&dLoss, (size_t)steps * sizeof(float)));
+    hipError_t e = mlp_fill(dLoss, steps, 0.0f, h->stream);
+    for (int32_t i = 0; i < steps && e == hipSuccess; ++i) {
+        const uint64_t sd = (seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1))) | 1ull; // non-zero: sampled rows
+        e = mlp_gradients(h->mlp, h->logX, h->logT, c, batch, sd, dLoss + i, h->stream);
+        if (e == hipSuccess) e = mlp_adam_step(h->mlp, lr, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, dLoss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dLoss);
+    if (e != hipSuccess) return hip_fail(h, e, "mlp training");
+    return DBX_OK;
+}

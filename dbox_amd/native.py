@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdbox_hip.so")
 
 DBX_OK, DBX_E_INVALID_ARG, DBX_E_CAPACITY, DBX_E_UNSUPPORTED, DBX_E_HIP, DBX_E_LOCKED, DBX_E_STATE = 0, -1, -2, -3, -4, -5, -6
-WARM, SLEEP, CCD, ORDERED, LEARNED, RESIDUAL = 1, 2, 4, 8, 16, 32
+WARM, SLEEP, CCD, ORDERED, LEARNED, RESIDUAL, LOG = 1, 2, 4, 8, 16, 32, 64
 DEFAULT_FLAGS = WARM | SLEEP
 
 BODY_STATE_DTYPE = np.dtype([
@@ -58,8 +58,14 @@ EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
-    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers",
+    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual",
+    "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
+    "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
+
+MLP_PARAMS = 5250
+MLP_NORM = 32
+MLP_FEATURES = 14
 
 
 def lib():
@@ -92,6 +98,20 @@ def lib():
         L.dbx_broadphase_pairs.argtypes = [vp, vp, vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_collide_pairs.argtypes = [vp, vp, vp, vp, vp, i32, vp]
         L.dbx_read_schedule.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
+        L.dbx_read_tiers.argtypes = [vp, vp, i32, vp, C.POINTER(i32)]
+        i64, u64 = C.c_int64, C.c_uint64
+        L.dbx_set_residual.argtypes = [vp, f32, i32]
+        L.dbx_mlp_init.argtypes = [vp, u64]
+        L.dbx_mlp_set.argtypes = [vp, vp, vp]
+        L.dbx_mlp_get.argtypes = [vp, vp, vp]
+        L.dbx_mlp_forward.argtypes = [vp, vp, i32, vp]
+        L.dbx_mlp_gradients.argtypes = [vp, vp, vp, i32, vp, vp]
+        L.dbx_mlp_log_enable.argtypes = [vp, i64]
+        L.dbx_mlp_log_count.argtypes = [vp, C.POINTER(i64)]
+        L.dbx_mlp_log_read.argtypes = [vp, i64, i64, vp, vp]
+        L.dbx_mlp_log_clear.argtypes = [vp]
+        L.dbx_mlp_fit_normalization.argtypes = [vp]
+        L.dbx_mlp_train.argtypes = [vp, i32, i32, f32, u64, vp]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("dbx_destroy", "dbx_last_error"):
@@ -210,6 +230,67 @@ class StepWorlds:
         n = C.c_int32()
         self._check(self.L.dbx_profile_totals(self.h, C.byref(p), C.byref(n), int(reset)))
         return {k: getattr(p, k) for k, _ in DbxProfile._fields_}, n.value
+
+    # ---- residual mode and learned warm start ----
+    def set_residual(self, tolerance=1e-3, max_iterations=100):
+        self._check(self.L.dbx_set_residual(self.h, float(tolerance), int(max_iterations)))
+
+    def mlp_init(self, seed=1):
+        self._check(self.L.dbx_mlp_init(self.h, int(seed)))
+
+    def mlp_set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        assert p is None or p.size == MLP_PARAMS
+        assert n is None or n.size == MLP_NORM
+        self._check(self.L.dbx_mlp_set(self.h, None if p is None else _p(p), None if n is None else _p(n)))
+
+    def mlp_get(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        n = np.zeros(MLP_NORM, np.float32)
+        self._check(self.L.dbx_mlp_get(self.h, _p(p), _p(n)))
+        return p, n
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_FEATURES)
+        y = np.zeros((len(x), 2), np.float32)
+        self._check(self.L.dbx_mlp_forward(self.h, _p(x), len(x), _p(y)))
+        return y
+
+    def mlp_gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.L.dbx_mlp_gradients(self.h, _p(x), _p(t), len(x), _p(g), _p(loss)))
+        return g, float(loss[0])
+
+    def mlp_log_enable(self, capacity):
+        self._check(self.L.dbx_mlp_log_enable(self.h, int(capacity)))
+
+    def mlp_log_count(self) -> int:
+        c = C.c_int64()
+        self._check(self.L.dbx_mlp_log_count(self.h, C.byref(c)))
+        return c.value
+
+    def mlp_log_read(self, first=0, n=None):
+        if n is None:
+            n = self.mlp_log_count() - first
+        x = np.zeros((n, MLP_FEATURES), np.float32)
+        t = np.zeros((n, 2), np.float32)
+        self._check(self.L.dbx_mlp_log_read(self.h, int(first), int(n), _p(x), _p(t)))
+        return x, t
+
+    def mlp_log_clear(self):
+        self._check(self.L.dbx_mlp_log_clear(self.h))
+
+    def mlp_fit_normalization(self):
+        self._check(self.L.dbx_mlp_fit_normalization(self.h))
+
+    def mlp_train(self, steps, batch=4096, lr=1e-3, seed=1):
+        loss = np.zeros(max(steps, 1), np.float32)
+        self._check(self.L.dbx_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed), _p(loss)))
+        return loss[:steps]
 
     def tiers(self):
         """Launch tiers as rows (first launch slot, worlds, LDS bytes, workgroup size) and the launch order."""

@@ -1,0 +1,98 @@
+"""Learned warm start: collecting training data, training and evaluating the network.
+
+No counterpart in the reference.  b2Contact.Update (b2contact.d:306-324) gives a
+manifold point without a match in the previous manifold a zero impulse; the
+native library can replace that zero by the prediction of a small network
+(14 features -> 64 -> 64 -> 2, bf16 matrix cores with fp32 accumulation).  The
+training data are the features of new points and the impulses the plain solver
+converges to for them, logged by the solver kernels themselves (DBX_STEP_LOG)
+while they iterate to a tight residual; the network is trained by the library's
+own forward/backward kernels with Adam.  Everything here runs on the GPU; this
+module only drives it.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import native, scenes
+
+DT = 1.0 / 60.0
+VEL, POS = 8, 3
+PLAIN = native.WARM | native.SLEEP | native.ORDERED | native.RESIDUAL
+
+
+def _handle(scene_list) -> native.StepWorlds:
+    h = native.StepWorlds()
+    for s in scene_list:
+        h.add_scene(s)
+    h.finalize()
+    return h
+
+
+def collect(scene_list, steps: int, capacity: int = 1 << 22, tolerance: float = 1e-5,
+            max_iterations: int = 200) -> native.StepWorlds:
+    """Steps the scenes with the plain solver iterating to `tolerance` and logs
+    every new point's features and converged impulses.  Returns the handle,
+    whose log holds the samples."""
+    h = _handle(scene_list)
+    h.mlp_log_enable(capacity)
+    h.set_residual(tolerance, max_iterations)
+    h.step(DT, VEL, POS, PLAIN | native.LOG, n=steps)
+    h.sync()
+    return h
+
+
+def train(h: native.StepWorlds, steps: int = 2000, batch: int = 4096, lr: float = 2e-3, seed: int = 1,
+          init_seed: int = 7):
+    """Fits the normalisation to the log, initialises and trains the network.
+    Returns (params, norm, per-step loss)."""
+    h.mlp_init(init_seed)
+    h.mlp_fit_normalization()
+    loss = h.mlp_train(steps, batch, lr, seed)
+    params, norm = h.mlp_get()
+    return params, norm, loss
+
+
+@dataclasses.dataclass
+class IterationStats:
+    mean_iterations: float      # over (world, step) pairs that solved at least one constraint
+    p95_iterations: float
+    capped_fraction: float      # share of those pairs that hit max_iterations
+    max_penetration: float      # over all worlds and steps
+    world_steps: int
+
+
+def iterations(scene_list, steps: int, params=None, norm=None, tolerance: float = 1e-3,
+               max_iterations: int = 100) -> IterationStats:
+    """Velocity iterations needed to reach `tolerance`, per world and step, with
+    the plain warm start (params None) or the learned one."""
+    h = _handle(scene_list)
+    h.set_residual(tolerance, max_iterations)
+    flags = PLAIN
+    if params is not None:
+        h.mlp_set(params, norm)
+        flags |= native.LEARNED
+    its, pen = [], 0.0
+    for _ in range(steps):
+        h.step(DT, VEL, POS, flags)
+        h.sync()
+        it, p = h.solver_stats()
+        its.append(it[it > 0])
+        pen = max(pen, float(p.max()))
+    h.close()
+    a = np.concatenate(its) if its else np.zeros(0)
+    if len(a) == 0:
+        return IterationStats(0.0, 0.0, 0.0, pen, 0)
+    return IterationStats(float(a.mean()), float(np.percentile(a, 95)), float(np.mean(a >= max_iterations)), pen,
+                          int(len(a)))
+
+
+def training_scenes(seed: int = 11, pyramids: int = 8, stacks: int = 48, mixed: int = 8):
+    """Plain-solver data sources: seeded pyramids (the 20-row benchmark scene),
+    stack worlds and small mixed containers."""
+    out = [scenes.pyramid(20, seed=seed * 1000 + i) for i in range(pyramids)]
+    out += [scenes.stacks_world(seed * 7919 + i) for i in range(stacks)]
+    out += [scenes.mixed_container(200, 40.0, 80.0, seed * 104729 + i) for i in range(mixed)]
+    return out

@@ -47,7 +47,8 @@ enum {
     DBX_STEP_CCD = 1u << 2,     /* continuous physics: not implemented -> DBX_E_UNSUPPORTED */
     DBX_STEP_ORDERED = 1u << 3, /* solve in the reference's exact island order (dependency levels) */
     DBX_STEP_LEARNED = 1u << 4, /* MLP warm start for new contact points */
-    DBX_STEP_RESIDUAL = 1u << 5 /* iterate velocities to a residual instead of a fixed count */
+    DBX_STEP_RESIDUAL = 1u << 5, /* iterate velocities to a residual instead of a fixed count */
+    DBX_STEP_LOG = 1u << 6       /* append new points' features and solved impulses to the training log */
 };
 
 /* Body definition, same fields as b2BodyDef (src/dbox/dynamics/b2body.d:51-104). */
@@ -181,6 +182,54 @@ dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uin
  * world normal (2), world points (4), separations (2), 8 unused. */
 dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const float *xa, const dbx_fixture_def *b,
                              const float *xb, int32_t n, float *out);
+
+/* Residual mode (DBX_STEP_RESIDUAL): velocity iterations stop once the residual
+ * (the largest over points of the normal complementarity error |vn - bias| for a
+ * positive impulse, else max(0, bias - vn), and of |vt| where friction is not
+ * saturated) is at most tolerance, or after max_iterations.  Defaults 1e-3, 100.
+ * No counterpart in the reference, whose iteration count is fixed. */
+dbx_status dbx_set_residual(dbx_handle *h, float tolerance, int32_t max_iterations);
+
+/* ---- Learned warm start (no counterpart in the reference) ----
+ * A 14 -> 64 -> 64 -> 2 ReLU network, evaluated and trained on the GPU's bf16
+ * matrix cores with fp32 accumulation, predicts the normal and tangent impulse
+ * of a manifold point that b2Contact.Update (b2contact.d:306-324) found no match
+ * for, in place of the zero such a point starts with; matched points keep the
+ * previous step's impulses.  DBX_STEP_LEARNED needs DBX_STEP_WARM and a network.
+ * Parameters: DBX_MLP_PARAMS floats, W1[64][14], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2].  Normalisation: DBX_MLP_NORM floats, input mean[14], input
+ * scale[14] (the network sees (x - mean) * scale), output mean[2], output
+ * scale[2] (impulse = y * scale + mean).  The DBX_MLP_FEATURES features of a
+ * point, in order: relative normal velocity, relative tangent velocity, normal
+ * x, normal y, separation, invMassA, invMassB, invIA, invIB, mean matched normal
+ * impulse over the contacts of both bodies, rA x n, rB x n, friction, the
+ * point's normal mass.  Predicted impulses are clamped to the solver's bounds
+ * (normal >= 0, |tangent| <= friction * normal). */
+#define DBX_MLP_PARAMS 5250
+#define DBX_MLP_NORM 32
+#define DBX_MLP_FEATURES 14
+/* He-uniform random weights from seed, zero biases, identity normalisation. */
+dbx_status dbx_mlp_init(dbx_handle *h, uint64_t seed);
+/* Either pointer may be null to keep the current values. */
+dbx_status dbx_mlp_set(dbx_handle *h, const float *params, const float *norm);
+dbx_status dbx_mlp_get(dbx_handle *h, float *params, float *norm);
+/* Predicted impulses y[2 i], y[2 i + 1] for feature rows x[14 i .. 14 i + 13]. */
+dbx_status dbx_mlp_forward(dbx_handle *h, const float *x, int32_t n, float *y);
+/* Gradient (DBX_MLP_PARAMS floats) and value of the loss
+ * 1/n sum_i sum_o (y_o(i) - (t[2 i + o] - out_mean_o) / out_scale_o)^2. */
+dbx_status dbx_mlp_gradients(dbx_handle *h, const float *x, const float *t, int32_t n, float *grad, float *loss);
+/* Training log of up to capacity rows, filled by steps with DBX_STEP_LOG: the
+ * features of each new point and its accumulated impulses after the velocity
+ * iterations (the labels). */
+dbx_status dbx_mlp_log_enable(dbx_handle *h, int64_t capacity);
+dbx_status dbx_mlp_log_count(dbx_handle *h, int64_t *count);
+dbx_status dbx_mlp_log_read(dbx_handle *h, int64_t first, int64_t n, float *x, float *t);
+dbx_status dbx_mlp_log_clear(dbx_handle *h);
+/* Sets the normalisation from the mean and standard deviation of the log. */
+dbx_status dbx_mlp_fit_normalization(dbx_handle *h);
+/* steps Adam updates (learning rate lr) on mini-batches of batch log rows drawn
+ * with replacement; loss (steps floats, may be null) receives each step's loss. */
+dbx_status dbx_mlp_train(dbx_handle *h, int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,130 @@
+"""GPU tests of the learned warm start: the bf16 matrix-core network against the
+NumPy reference, gradients, training, and what the trained network gains.
+
+Tolerances: the reference rounds to bf16 at the same points as the kernels and
+accumulates in float64, the kernels in fp32; what is left is accumulation order
+and the occasional bf16 rounding flip of an activation that lies on a rounding
+boundary (one bf16 ulp, 2^-8 relative), hence a tight median and a loose max.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dbox_amd import native, scenes, warmstart
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _norm(rng):
+    n = np.zeros(mlp_ref.N_NORM, np.float32)
+    n[0:14] = rng.normal(0, 1, 14)
+    n[14:28] = rng.uniform(0.5, 2.0, 14)
+    n[28:30] = (0.1, -0.2)
+    n[30:32] = (2.0, 0.5)
+    return n
+
+
+def test_mlp_forward_matches_reference():
+    print("case: mlp forward, 1000 random rows", flush=True)
+    rng = np.random.default_rng(3)
+    p = mlp_ref.he_init(5)
+    norm = _norm(rng)
+    x = rng.normal(0, 2, (1000, 14)).astype(np.float32)
+    h = native.StepWorlds()
+    h.mlp_set(p, norm)
+    y = h.mlp_forward(x)
+    ref = mlp_ref.forward(p, norm, x)["out"]
+    err = np.abs(y - ref)
+    scale = np.abs(ref).max()
+    assert np.median(err) < 1e-4 * scale, np.median(err)
+    assert err.max() < 2e-2 * scale, err.max()
+    # and the exact (unrounded) network is close too: bf16 costs well under a percent
+    exact = mlp_ref.forward(p, norm, x, mlp_ref.exact)["out"]
+    assert np.median(np.abs(y - exact)) < 1e-2 * scale
+
+
+def test_mlp_gradients_match_reference():
+    print("case: mlp gradients, 200 rows", flush=True)
+    rng = np.random.default_rng(4)
+    p = mlp_ref.he_init(6)
+    norm = _norm(rng)
+    x = rng.normal(0, 1.5, (200, 14)).astype(np.float32)
+    t = rng.normal(0, 1, (200, 2)).astype(np.float32)
+    h = native.StepWorlds()
+    h.mlp_set(p, norm)
+    g, loss = h.mlp_gradients(x, t)
+    gref, lref = mlp_ref.gradients(p, norm, x, t)
+    assert abs(loss - lref) < 1e-3 * max(1.0, lref), (loss, lref)
+    blocks = [(mlp_ref.OFF_W1, mlp_ref.OFF_B1), (mlp_ref.OFF_B1, mlp_ref.OFF_W2), (mlp_ref.OFF_W2, mlp_ref.OFF_B2),
+              (mlp_ref.OFF_B2, mlp_ref.OFF_W3), (mlp_ref.OFF_W3, mlp_ref.OFF_B3), (mlp_ref.OFF_B3, mlp_ref.N_PARAMS)]
+    for a, b in blocks:
+        d = np.abs(g[a:b] - gref[a:b])
+        s = np.abs(gref[a:b]).max()
+        assert d.max() <= 2e-2 * s + 1e-7, (a, b, d.max(), s)
+        assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-2 * np.linalg.norm(gref[a:b]) + 1e-7, (a, b)
+
+
+def test_step_flags_of_the_learned_path_are_checked():
+    print("case: learned / log flag validation", flush=True)
+    h = native.StepWorlds()
+    h.add_scene(scenes.pyramid(4))
+    h.finalize()
+    with pytest.raises(native.DbxError) as e:
+        h.step(flags=native.WARM | native.LEARNED)
+    assert e.value.status == native.DBX_E_STATE
+    with pytest.raises(native.DbxError) as e:
+        h.step(flags=native.WARM | native.LOG)
+    assert e.value.status == native.DBX_E_STATE
+    h.mlp_init(1)
+    with pytest.raises(native.DbxError) as e:
+        h.step(flags=native.LEARNED)
+    assert e.value.status == native.DBX_E_INVALID_ARG
+
+
+def test_logging_does_not_change_the_plain_step():
+    print("case: LOG flag leaves the trajectory bit-identical", flush=True)
+    sc = scenes.pyramid(10)
+    out = []
+    for log in (False, True):
+        h = native.StepWorlds()
+        h.add_scene(sc)
+        h.finalize()
+        if log:
+            h.mlp_log_enable(1 << 16)
+        flags = native.WARM | native.SLEEP | native.ORDERED | (native.LOG if log else 0)
+        h.step(flags=flags, n=40)
+        h.sync()
+        out.append(h.bodies(0).tobytes())
+        if log:
+            n = h.mlp_log_count()
+            x, t = h.mlp_log_read(0, n)
+            assert n > 50  # the first contacts of every box are new
+            assert np.all(np.isfinite(x)) and np.all(np.isfinite(t))
+            assert np.all(t[:, 0] >= 0.0)  # accumulated normal impulses are never negative
+            assert np.all(np.abs(t[:, 1]) <= x[:, 12] * t[:, 0] * (1 + 1e-5) + 1e-7)  # friction cone
+    assert out[0] == out[1]
+
+
+def test_training_reduces_the_loss():
+    print("case: mlp training on a logged stack-world set", flush=True)
+    h = warmstart.collect([scenes.stacks_world(500 + i) for i in range(16)], steps=60, capacity=1 << 18)
+    n = h.mlp_log_count()
+    assert n > 500
+    _, _, loss = warmstart.train(h, steps=400, batch=2048, lr=2e-3)
+    assert np.all(np.isfinite(loss))
+    assert loss[-20:].mean() < 0.5 * loss[:20].mean(), (loss[:20].mean(), loss[-20:].mean())
+
+
+def test_learned_warm_start_needs_fewer_iterations():
+    print("case: learned vs plain warm start, iterations to a 1e-3 residual", flush=True)
+    h = warmstart.collect(warmstart.training_scenes(seed=11, pyramids=4, stacks=48, mixed=4), steps=150)
+    params, norm, loss = warmstart.train(h, steps=1500, batch=4096, lr=2e-3)
+    h.close()
+    test = [scenes.stacks_world(90000 + i) for i in range(48)]
+    plain = warmstart.iterations(test, steps=150)
+    learned = warmstart.iterations(test, steps=150, params=params, norm=norm)
+    print(f"plain {plain}\nlearned {learned}", flush=True)
+    assert learned.mean_iterations < plain.mean_iterations
+    assert learned.max_penetration <= plain.max_penetration + 0.005  # b2_linearSlop
