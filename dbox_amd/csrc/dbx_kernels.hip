@@ -2056,6 +2056,27 @@ hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds
     DBX_LAUNCH(k_broadphase, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
 }
+// b2Body.SetTransform's b2Fixture.Synchronize(xf, xf) (b2body.d:430-454,
+// b2fixture.d:480-502): after the host rewrote body poses, every fixture whose
+// fat AABB no longer contains its tight AABB gets a new fat AABB (no
+// displacement term) and enters the move buffer.
+__global__ void k_refit(Dev d, int w) {
+    const WorldDesc wd = d.W[w];
+    for (int f = threadIdx.x; f < wd.fixCount; f += blockDim.x) {
+        const int b = d.fbody[wd.fixBase + f];
+        const Xf x = load_xf(d.bxf + wd.bodyBase, b);
+        const float4 aabb = shape_aabb(d.fshape + wd.fixBase + f, x);
+        const float4 fa = d.ffat[wd.fixBase + f];
+        if (fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w) continue;
+        d.ffat[wd.fixBase + f] = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                             aabb.w + kAabbExtension);
+        d.fmoved[wd.fixBase + f] = 1;
+    }
+}
+hipError_t launch_refit(const Dev &d, int w, hipStream_t s) {
+    hipLaunchKernelGGL(k_refit, dim3(1), dim3(256), 0, s, d, w);
+    return hipGetLastError();
+}
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total,
                             hipStream_t s) {
     hipLaunchKernelGGL(k_bp_single, dim3(1), dim3(256), 0, s, fat, moved, n, scratch, out, cap, total);

@@ -954,6 +954,46 @@ dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, in
     return DBX_OK;
 }
 
+dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *in, const float *force3, int32_t n) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    if (!in || n != D.bodyCount) return fail(h, DBX_E_INVALID_ARG, "one state per body is required");
+    const int nb = D.bodyCount, b0 = D.bodyBase;
+    std::vector<uint32_t> fl(nb);
+    std::vector<float4> xf(nb), pos(nb), pos0(nb), vel(nb), frc;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(fl.data(), h->d.bflags + b0, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; ++i) {
+        const dbx_body_state &s = in[i];
+        const float v[] = {s.px, s.py, s.qs, s.qc, s.cx, s.cy, s.a, s.sleep_time, s.vx, s.vy, s.w, s.c0x, s.c0y, s.a0};
+        for (float x : v)
+            if (!std::isfinite(x)) return fail(h, DBX_E_INVALID_ARG, "body state is not finite");
+        xf[i] = make_float4(s.px, s.py, s.qs, s.qc);
+        pos[i] = make_float4(s.cx, s.cy, s.a, s.sleep_time);
+        pos0[i] = make_float4(s.c0x, s.c0y, s.a0, 0.0f);
+        vel[i] = make_float4(s.vx, s.vy, s.w, 0.0f);
+        fl[i] = (fl[i] & ~(uint32_t)kBodyAwake) | ((uint32_t)s.flags & kBodyAwake);
+    }
+    if (force3) {
+        frc.resize(nb);
+        for (int i = 0; i < nb; ++i) {
+            if (!std::isfinite(force3[3 * i]) || !std::isfinite(force3[3 * i + 1]) || !std::isfinite(force3[3 * i + 2]))
+                return fail(h, DBX_E_INVALID_ARG, "force is not finite");
+            frc[i] = make_float4(force3[3 * i], force3[3 * i + 1], force3[3 * i + 2], 0.0f);
+        }
+    }
+    DBX_HIP(h, hipMemcpy(h->d.bflags + b0, fl.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice));
+    DBX_HIP(h, hipMemcpy(h->d.bxf + b0, xf.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    DBX_HIP(h, hipMemcpy(h->d.bpos + b0, pos.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    DBX_HIP(h, hipMemcpy(h->d.bpos0 + b0, pos0.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    DBX_HIP(h, hipMemcpy(h->d.bvel + b0, vel.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    if (force3) DBX_HIP(h, hipMemcpy(h->d.bforce + b0, frc.data(), nb * sizeof(float4), hipMemcpyHostToDevice));
+    DBX_HIP(h, launch_refit(h->d, world, h->stream));
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    return DBX_OK;
+}
+
 dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count) {
     dbx_status st = check_world(h, world);
     if (st) return st;

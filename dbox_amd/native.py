@@ -58,7 +58,7 @@ EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
-    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual",
+    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
@@ -101,6 +101,7 @@ def lib():
         L.dbx_read_tiers.argtypes = [vp, vp, i32, vp, C.POINTER(i32)]
         i64, u64 = C.c_int64, C.c_uint64
         L.dbx_set_residual.argtypes = [vp, f32, i32]
+        L.dbx_write_bodies.argtypes = [vp, i32, vp, vp, i32]
         L.dbx_mlp_init.argtypes = [vp, u64]
         L.dbx_mlp_set.argtypes = [vp, vp, vp]
         L.dbx_mlp_get.argtypes = [vp, vp, vp]
@@ -230,6 +231,12 @@ class StepWorlds:
         n = C.c_int32()
         self._check(self.L.dbx_profile_totals(self.h, C.byref(p), C.byref(n), int(reset)))
         return {k: getattr(p, k) for k, _ in DbxProfile._fields_}, n.value
+
+    def write_bodies(self, world, states, forces=None):
+        """Overwrite body poses, velocities, sleep times, awake bits and (optionally) force/torque."""
+        st = np.ascontiguousarray(states)
+        f = None if forces is None else np.ascontiguousarray(forces, np.float32).reshape(-1, 3)
+        self._check(self.L.dbx_write_bodies(self.h, world, _p(st), None if f is None else _p(f), len(st)))
 
     # ---- residual mode and learned warm start ----
     def set_residual(self, tolerance=1e-3, max_iterations=100):

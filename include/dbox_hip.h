@@ -141,6 +141,15 @@ dbx_status dbx_world_count(const dbx_handle *h, int32_t *count);
 /* body count, fixture count, contact count, touching contacts */
 dbx_status dbx_world_info(dbx_handle *h, int32_t world, int32_t out[4]);
 dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, int32_t n);
+/* Host-side mutations between steps (b2Body.SetTransform, SetLinearVelocity,
+ * SetAngularVelocity, SetAwake, ApplyForce/ApplyTorque: b2body.d:430-454,
+ * 570-640, 827-846): overwrites every body's pose, sweep, velocities, sleep
+ * time and awake bit (flags & 0x2) from `in` (n = body count, in creation
+ * order), and, when force3 is not null, its force and torque (3 floats per body,
+ * the accumulators the next step integrates and then clears).  Fixtures whose
+ * fat AABB no longer contains the new pose are re-fattened and moved, as
+ * b2Fixture.Synchronize(xf, xf) does after SetTransform. */
+dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *in, const float *force3, int32_t n);
 /* Contacts of one world in b2World contact-list order. */
 dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count);
 /* Fat AABBs of each fixture's proxy (lower x, lower y, upper x, upper y). */

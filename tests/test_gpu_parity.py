@@ -12,7 +12,7 @@ import pytest
 
 from dbox_amd import native, scenes
 from oracle import pyoracle as po
-from tests.helpers import box, circle, edge, random_shape
+from tests.helpers import box, circle, edge, random_shape, scene_of
 
 pytestmark = pytest.mark.gpu
 DT = 1.0 / 60.0
@@ -341,3 +341,42 @@ def test_invalid_inputs_are_rejected():
     with pytest.raises(native.DbxError) as e:
         g.add_scene(s)
     assert e.value.status == native.DBX_E_LOCKED
+
+
+@pytest.mark.gpu
+def test_write_bodies_forces_velocities_and_teleport():
+    """dbx_write_bodies: a force integrates as v += h * (g * gs + invMass * F) (b2island.d:94-101), a written
+    velocity is used by the next step, and a teleported body's fixtures are re-fattened, moved and paired."""
+    print("case: write_bodies force / velocity / teleport", flush=True)
+    f0, f1 = box(0.5, 0.5), box(0.5, 0.5)
+    f0["body"], f1["body"] = 0, 1
+    sc = scene_of([(2, 0.0, 0.0, 0.0), (2, 10.0, 0.0, 0.0)], [f0, f1])
+    sc.gravity = (0.0, 0.0)
+    h = native.StepWorlds()
+    h.add_scene(sc)
+    h.finalize()
+    h.step(n=1)
+    st = h.bodies(0)
+    st["vx"][1] = -3.0
+    force = np.zeros((2, 3), np.float32)
+    force[0] = (2.0, -1.0, 0.5)
+    h.write_bodies(0, st, force)
+    h.step(n=1)
+    out = h.bodies(0)
+    dt = np.float32(1.0 / 60.0)
+    im = st["inv_mass"][0]
+    assert out["vx"][0] == np.float32(dt * np.float32(im * np.float32(2.0)))
+    assert out["vy"][0] == np.float32(dt * np.float32(im * np.float32(-1.0)))
+    assert out["vx"][1] == np.float32(-3.0)
+    assert out["cx"][1] == np.float32(np.float32(10.0) + dt * np.float32(-3.0))
+    # forces are cleared after the step
+    h.step(n=1)
+    assert h.bodies(0)["vx"][0] == out["vx"][0]
+    # teleport body 1 onto body 0: the next step's broadphase creates the pair
+    st = h.bodies(0)
+    for k, v in (("px", 0.6), ("cx", 0.6), ("c0x", 0.6), ("py", 0.0), ("cy", 0.0), ("c0y", 0.0)):
+        st[k][1] = v
+    st["vx"][1] = 0.0
+    h.write_bodies(0, st)
+    h.step(n=2)
+    assert len(h.contacts(0)) == 1
