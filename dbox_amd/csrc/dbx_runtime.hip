@@ -394,8 +394,19 @@ struct Tier {
     int block = 256;          // threads of its collide, broadphase and coloured-solve workgroups
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr; // chain timing while autotuning
     bool own = false;         // stream and event created for this tier
 };
+
+// A captured step: the launch sequence of one step for one combination of
+// device state, step parameters, buffer parity and launch shapes.
+struct GraphEntry {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+};
+constexpr int kMaxGraphs = 8;
+constexpr int kTuneBlocks[] = {64, 128, 256, 512, 1024};
+constexpr int kTuneCandidates = 5;
 constexpr int kStepEvents = 6;
 
 struct dbx_handle {
@@ -417,6 +428,11 @@ struct dbx_handle {
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
     std::vector<hipEvent_t> ring;   // kStepEvents events per step since the last sync
+    std::vector<char> ringGraph;    // per ring entry: the step ran as a graph (phase events unrecorded)
+    std::vector<GraphEntry> graphs;
+    hipEvent_t fork = nullptr;      // fork event inside captures
+    int tuneSteps = 0, tuneDone = 0, tunePer = 0; // autotuner progress
+    std::vector<double> tuneMs;     // [tier][candidate] summed chain milliseconds
     int ringUsed = 0;
     dbx_profile prof{};             // last step
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
@@ -462,6 +478,12 @@ hipError_t upload(dbx_handle *h, T *dst, const std::vector<T> &src) {
 }
 
 } // namespace
+
+template <class T>
+static void key_append(std::vector<unsigned char> &k, const T &v) {
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
 
 extern "C" {
 
@@ -513,7 +535,11 @@ void dbx_destroy(dbx_handle *h) {
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : h->ring) (void)hipEventDestroy(e);
+    for (GraphEntry &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    if (h->fork) (void)hipEventDestroy(h->fork);
     for (Tier &t : h->tiers) {
+        if (t.t0) (void)hipEventDestroy(t.t0);
+        if (t.t1) (void)hipEventDestroy(t.t1);
         if (!t.own) continue;
         (void)hipStreamSynchronize(t.stream);
         (void)hipStreamDestroy(t.stream);
@@ -743,9 +769,10 @@ static dbx_status stage_mlp(dbx_handle *h) {
 }
 
 // The step chain of one tier (b2World.Step, b2world.d:367-434, for its worlds).
-static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew, hipEvent_t *ev) {
+static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew,
+                            const hipEvent_t *ev) {
     const Tier &T = h->tiers[t];
-    const bool lead = t == 0;
+    bool lead = t == 0;
     const int cin = h->cur, cout = cin ^ 1;
     const int lds1 = std::min(T.lds, h->ldsMax - 1024); // kernels that also hold static LDS
     hipStream_t s = T.stream;
@@ -753,20 +780,113 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     if (findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
         DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s));
     DBX_HIP(h, launch_collide(d, cin, T.first, T.count, T.block, lead, s));
-    if (lead) (void)hipEventRecord(ev[1], s);
+    const bool timed = lead && ev; // phase events: lead tier, eager steps only
+    if (timed) (void)hipEventRecord(ev[1], s);
     if (sp.dt > 0.0f) {
         const int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
         DBX_HIP(h, launch_islands(d, cout, ordered, T.first, T.count, T.lds, lead, s));
-        if (lead) (void)hipEventRecord(ev[2], s);
+        if (timed) (void)hipEventRecord(ev[2], s);
         if (ordered) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
         else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, T.block, lead, s));
-        if (lead) (void)hipEventRecord(ev[3], s);
+        if (timed) (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s));
-    } else if (lead) {
+    } else if (timed) {
         (void)hipEventRecord(ev[2], s);
         (void)hipEventRecord(ev[3], s);
     }
-    if (lead) (void)hipEventRecord(ev[4], s);
+    if (timed) (void)hipEventRecord(ev[4], s);
+    return DBX_OK;
+}
+
+// Launches one step: fork the tier streams from the handle's stream, run each
+// tier's chain, join.  ev (timing events of this step) is null while capturing.
+static dbx_status issue_step(dbx_handle *h, const StepParams &sp, uint32_t flags, bool findNew, const hipEvent_t *ev,
+                             bool timeTiers) {
+    hipStream_t s = h->stream;
+    hipEvent_t fork = ev ? ev[0] : h->fork;
+    DBX_HIP(h, hipEventRecord(fork, s));
+    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, fork, 0));
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        Tier &T = h->tiers[t];
+        if (timeTiers) DBX_HIP(h, hipEventRecord(T.t0, T.stream));
+        dbx_status st = step_tier(h, (int)t, sp, flags, findNew, ev);
+        if (st != DBX_OK) return st;
+        if (timeTiers) DBX_HIP(h, hipEventRecord(T.t1, T.stream));
+    }
+    for (size_t t = 1; t < h->tiers.size(); ++t) {
+        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
+        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    }
+    if (ev) DBX_HIP(h, hipEventRecord(ev[5], s));
+    return DBX_OK;
+}
+
+// The step as a hipGraph: captured once per distinct (device pointers, step
+// parameters, buffer parity, launch shapes) and replayed.  Kernel arguments are
+// baked into a graph, so anything that changes them selects another graph.
+static dbx_status step_graph(dbx_handle *h, const StepParams &sp, uint32_t flags, bool findNew, const hipEvent_t *ev) {
+    std::vector<unsigned char> key;
+    key_append(key, h->d);
+    key_append(key, sp);
+    key_append(key, flags);
+    key_append(key, h->cur);
+    key_append(key, findNew);
+    for (const Tier &T : h->tiers) key_append(key, T.block);
+    hipGraphExec_t exec = nullptr;
+    for (GraphEntry &g : h->graphs)
+        if (g.key == key) exec = g.exec;
+    hipStream_t s = h->stream;
+    if (!exec) {
+        if (!h->fork) DBX_HIP(h, hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+        DBX_HIP(h, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        dbx_status st = issue_step(h, sp, flags, findNew, nullptr, false);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (st != DBX_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+        if ((int)h->graphs.size() >= kMaxGraphs) {
+            (void)hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(h->graphs.front().exec);
+            h->graphs.erase(h->graphs.begin());
+        }
+        h->graphs.push_back(GraphEntry{key, exec});
+    }
+    DBX_HIP(h, hipEventRecord(ev[0], s));
+    DBX_HIP(h, hipGraphLaunch(exec, s));
+    DBX_HIP(h, hipEventRecord(ev[5], s));
+    return DBX_OK;
+}
+
+// Autotuner bookkeeping after a tuning step: add each tier's chain time to its
+// candidate, and after the last candidate keep each tier's fastest size.
+static dbx_status tune_account(dbx_handle *h) {
+    const int c = h->tuneDone / h->tunePer;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        float ms = 0.0f;
+        DBX_HIP(h, hipEventElapsedTime(&ms, h->tiers[t].t0, h->tiers[t].t1));
+        h->tuneMs[t * kTuneCandidates + c] += ms;
+    }
+    if (++h->tuneDone < h->tuneSteps) return DBX_OK;
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        int best = 0;
+        for (int k = 1; k < kTuneCandidates; ++k)
+            if (h->tuneMs[t * kTuneCandidates + k] < h->tuneMs[t * kTuneCandidates + best]) best = k;
+        h->tiers[t].block = kTuneBlocks[best];
+    }
+    if (getenv("DBX_VERBOSE"))
+        for (size_t t = 0; t < h->tiers.size(); ++t) {
+            fprintf(stderr, "dbx: autotune tier %zu:", t);
+            for (int k = 0; k < kTuneCandidates; ++k)
+                fprintf(stderr, " %d:%.3fms", kTuneBlocks[k], h->tuneMs[t * kTuneCandidates + k] / h->tunePer);
+            fprintf(stderr, " -> %d\n", h->tiers[t].block);
+        }
     return DBX_OK;
 }
 
@@ -802,29 +922,34 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
             DBX_HIP(h, hipEventCreate(&e));
             h->ring.push_back(e);
         }
+        h->ringGraph.push_back(0);
     }
     hipEvent_t *ev = &h->ring[kStepEvents * h->ringUsed];
-    h->ringUsed++;
     h->d.mlp = mlp_dev(h->mlp);
     h->d.logX = h->logX;
     h->d.logT = h->logT;
     h->d.logCount = h->logCount;
     h->d.logCap = h->logCap;
-    (void)hipEventRecord(ev[0], s);
-    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, ev[0], 0));
     const bool findNew = h->newFixture;
-    for (size_t t = 0; t < h->tiers.size(); ++t) {
-        dbx_status st = step_tier(h, (int)t, sp, flags, findNew, ev);
-        if (st != DBX_OK) return st;
+    const bool tuning = h->tuneDone < h->tuneSteps;
+    if (tuning) {
+        const int c = h->tuneDone / h->tunePer;
+        for (Tier &T : h->tiers) T.block = kTuneBlocks[c];
     }
-    for (size_t t = 1; t < h->tiers.size(); ++t) {
-        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
-        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    dbx_status st = DBX_OK;
+    if (h->cfg.use_graph && !tuning) {
+        st = step_graph(h, sp, flags, findNew, ev);
+        h->ringGraph[h->ringUsed] = 1;
+    } else {
+        st = issue_step(h, sp, flags, findNew, ev, tuning);
+        h->ringGraph[h->ringUsed] = 0;
     }
-    (void)hipEventRecord(ev[5], s);
+    if (st != DBX_OK) return st;
+    h->ringUsed++;
     h->newFixture = false;
     h->cur ^= 1;
     if (dt > 0.0f) h->inv_dt0 = sp.inv_dt;
+    if (tuning) return tune_account(h);
     return DBX_OK;
 }
 
@@ -845,7 +970,8 @@ dbx_status dbx_sync(dbx_handle *h) {
     for (int k = 0; k < h->ringUsed; ++k) {
         hipEvent_t *ev = &h->ring[kStepEvents * k];
         float ms[4] = {0, 0, 0, 0}, total = 0.0f;
-        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        if (!h->ringGraph[k]) // a replayed graph records only the step's ends
+            for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
         (void)hipEventElapsedTime(&total, ev[0], ev[5]);
         dbx_profile p{};
         p.collide = ms[0];
@@ -1445,5 +1571,34 @@ dbx_status dbx_mlp_train(dbx_handle *h, int32_t steps, int32_t batch, float lr, 
     if (e == hipSuccess && loss) e = hipMemcpy(loss, dLoss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(dLoss);
     if (e != hipSuccess) return hip_fail(h, e, "mlp training");
+    return DBX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner.
+// ---------------------------------------------------------------------------
+dbx_status dbx_autotune(dbx_handle *h, int32_t steps_per_candidate) {
+    if (!h || steps_per_candidate < 1 || steps_per_candidate > 1000) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    for (Tier &T : h->tiers) {
+        if (!T.t0) DBX_HIP(h, hipEventCreate(&T.t0));
+        if (!T.t1) DBX_HIP(h, hipEventCreate(&T.t1));
+    }
+    h->tunePer = steps_per_candidate;
+    h->tuneSteps = steps_per_candidate * kTuneCandidates;
+    h->tuneDone = 0;
+    h->tuneMs.assign(h->tiers.size() * kTuneCandidates, 0.0);
+    return DBX_OK;
+}
+
+dbx_status dbx_set_tier_blocks(dbx_handle *h, const int32_t *blocks, int32_t n) {
+    if (!h || !blocks) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (n != (int32_t)h->tiers.size()) return fail(h, DBX_E_INVALID_ARG, "one size per tier is required");
+    for (int32_t i = 0; i < n; ++i)
+        if (blocks[i] < 64 || blocks[i] > kMaxBlock || (blocks[i] & 63))
+            return fail(h, DBX_E_INVALID_ARG, "workgroup size must be a multiple of 64 in [64, 1024]");
+    for (int32_t i = 0; i < n; ++i) h->tiers[i].block = blocks[i];
     return DBX_OK;
 }

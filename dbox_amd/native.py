@@ -59,6 +59,7 @@ EXPORTS = [
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
+    "dbx_autotune", "dbx_set_tier_blocks",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
@@ -102,6 +103,8 @@ def lib():
         i64, u64 = C.c_int64, C.c_uint64
         L.dbx_set_residual.argtypes = [vp, f32, i32]
         L.dbx_write_bodies.argtypes = [vp, i32, vp, vp, i32]
+        L.dbx_autotune.argtypes = [vp, i32]
+        L.dbx_set_tier_blocks.argtypes = [vp, vp, i32]
         L.dbx_mlp_init.argtypes = [vp, u64]
         L.dbx_mlp_set.argtypes = [vp, vp, vp]
         L.dbx_mlp_get.argtypes = [vp, vp, vp]
@@ -128,10 +131,10 @@ def _p(a: np.ndarray):
 class StepWorlds:
     """One device handle holding a batch of independent worlds."""
 
-    def __init__(self, device=0, contacts_per_fixture=0, block_size=0):
+    def __init__(self, device=0, contacts_per_fixture=0, block_size=0, use_graph=False):
         L = lib()
         self.L = L
-        cfg = DbxConfig(device, contacts_per_fixture, block_size, 0)
+        cfg = DbxConfig(device, contacts_per_fixture, block_size, int(use_graph))
         h = C.c_void_p()
         st = L.dbx_create(C.byref(cfg), C.byref(h))
         if st != DBX_OK:
@@ -298,6 +301,13 @@ class StepWorlds:
         loss = np.zeros(max(steps, 1), np.float32)
         self._check(self.L.dbx_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed), _p(loss)))
         return loss[:steps]
+
+    def autotune(self, steps_per_candidate=4):
+        self._check(self.L.dbx_autotune(self.h, int(steps_per_candidate)))
+
+    def set_tier_blocks(self, blocks):
+        b = np.ascontiguousarray(blocks, np.int32)
+        self._check(self.L.dbx_set_tier_blocks(self.h, _p(b), len(b)))
 
     def tiers(self):
         """Launch tiers as rows (first launch slot, worlds, LDS bytes, workgroup size) and the launch order."""

@@ -112,7 +112,8 @@ typedef struct {
     int32_t device;                 /* HIP device ordinal */
     int32_t contacts_per_fixture;   /* contact capacity per fixture (0 = 8) */
     int32_t block_size;             /* threads per world workgroup (0 = autotuned default) */
-    int32_t use_graph;              /* capture the step in a hipGraph */
+    int32_t use_graph;              /* replay each step as a captured hipGraph (one per distinct
+                                       step parameters and buffer parity) */
 } dbx_config;
 
 typedef struct dbx_handle dbx_handle;
@@ -174,6 +175,18 @@ dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_
  * up to cap tiers, the launch order (world index per slot) into order when it
  * is not null (nworlds entries), and the tier count into count. */
 dbx_status dbx_read_tiers(dbx_handle *h, int32_t *tiers, int32_t cap, int32_t *order, int32_t *count);
+
+/* Launch-shape autotuner (no counterpart in the reference).  The next
+ * 5 * steps_per_candidate calls of dbx_step run every tier's collide,
+ * broadphase and coloured-solve kernels with workgroups of 64, 128, 256, 512
+ * and 1024 threads in turn, timing each tier's step chain on its own stream;
+ * after the last of them each tier keeps its fastest size (dbx_read_tiers shows
+ * it).  Results never depend on the workgroup size, so tuning steps are
+ * ordinary steps.  Those steps are not captured as graphs. */
+dbx_status dbx_autotune(dbx_handle *h, int32_t steps_per_candidate);
+/* Sets the workgroup size of each tier (n = tier count), e.g. from a cache of
+ * earlier autotuning. */
+dbx_status dbx_set_tier_blocks(dbx_handle *h, const int32_t *blocks, int32_t n);
 
 /* The island solve order of the last step for one world: for each constraint in
  * the order b2World.Solve's island search adds contacts (b2world.d:963-1076), its

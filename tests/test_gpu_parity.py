@@ -380,3 +380,47 @@ def test_write_bodies_forces_velocities_and_teleport():
     h.write_bodies(0, st)
     h.step(n=2)
     assert len(h.contacts(0)) == 1
+
+
+def _run_batch(scs, steps, flags, use_graph=False, tune=0):
+    h = native.StepWorlds(use_graph=use_graph)
+    for s in scs:
+        h.add_scene(s)
+    h.finalize()
+    if tune:
+        h.autotune(tune)
+    for i in range(steps):
+        h.step(flags=flags)
+        if i % 7 == 3:
+            h.sync()  # host work between replays
+    h.sync()
+    out = b"".join(h.bodies(w).tobytes() for w in range(len(scs)))
+    tiers, _ = h.tiers()
+    return out, tiers
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ordered", [True, False])
+def test_graph_replay_matches_eager(ordered):
+    """The captured step replayed as a hipGraph gives the eager step's bytes, over many replays
+    of each captured parity and across several tiers."""
+    print(f"case: graph vs eager, ordered={ordered}", flush=True)
+    sizes = np.array([700, 300, 150, 90, 40, 12] * 4)
+    scs = scenes.batched_worlds(len(sizes), seed=77, sizes=sizes)
+    flags = native.WARM | native.SLEEP | (native.ORDERED if ordered else 0)
+    eager, tiers = _run_batch(scs, 40, flags)
+    graph, _ = _run_batch(scs, 40, flags, use_graph=True)
+    assert len(tiers) >= 3
+    assert graph == eager
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results_and_picks_sizes():
+    print("case: autotune 2 steps per candidate", flush=True)
+    sizes = np.array([600, 200, 100, 30] * 4)
+    scs = scenes.batched_worlds(len(sizes), seed=78, sizes=sizes)
+    flags = native.WARM | native.SLEEP | native.ORDERED
+    plain, _ = _run_batch(scs, 20, flags)
+    tuned, tiers = _run_batch(scs, 20, flags, tune=2)
+    assert tuned == plain
+    assert all(int(b) in (64, 128, 256, 512, 1024) for b in tiers[:, 3])
