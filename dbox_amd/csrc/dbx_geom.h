@@ -98,8 +98,8 @@ __device__ __forceinline__ float normalize(float2 &v) {
 //   x = k * pi/2 + r, |r| <= pi/4, with pi/2 split into three doubles whose
 //   first two have 33 significant bits, so k * P1 and k * P2 are exact for
 //   |k| < 2^20 (Cody-Waite reduction);
-//   sin r and cos r from their Taylor series up to r^21 / r^20, whose truncation
-//   error on |r| <= pi/4 is below 1e-21;
+//   sin r and cos r from their Taylor series up to r^15 / r^16, whose truncation
+//   error on |r| <= pi/4 is below 5e-17 (about half a double ulp of the result);
 //   the quadrant k mod 4 picks the signs and the swap of sin and cos.
 // The float results equal the correctly rounded sin and cos except when the exact
 // value lies within a few double ulps of a float rounding boundary.
@@ -113,10 +113,7 @@ __device__ __forceinline__ Rot rot(float angle) {
         r = fma(-k, P2, r);
         r = fma(-k, P3, r);
         double z = r * r;
-        double ps = 1.9572941063391263e-20;
-        ps = fma(ps, z, -8.22063524662433e-18);
-        ps = fma(ps, z, 2.8114572543455206e-15);
-        ps = fma(ps, z, -7.647163731819816e-13);
+        double ps = -7.647163731819816e-13;
         ps = fma(ps, z, 1.6059043836821613e-10);
         ps = fma(ps, z, -2.505210838544172e-08);
         ps = fma(ps, z, 2.7557319223985893e-06);
@@ -124,9 +121,7 @@ __device__ __forceinline__ Rot rot(float angle) {
         ps = fma(ps, z, 0.008333333333333333);
         ps = fma(ps, z, -0.16666666666666666);
         ps = fma(ps * z, r, r);
-        double pc = 4.110317623312165e-19;
-        pc = fma(pc, z, -1.5619206968586225e-16);
-        pc = fma(pc, z, 4.779477332387385e-14);
+        double pc = 4.779477332387385e-14;
         pc = fma(pc, z, -1.1470745597729725e-11);
         pc = fma(pc, z, 2.08767569878681e-09);
         pc = fma(pc, z, -2.755731922398589e-07);

@@ -857,15 +857,15 @@ __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, floa
     const float2 P1 = mul(X1, p.localPoint);
     const float2 P2 = mul(X2, lp);
     const float2 d = sub(P2, P1);
-    // b2Vec2.Normalize (b2math.d:162-175) of d for circles
-    float2 cn = d;
-    {
-        float len = sqrtf(d.x * d.x + d.y * d.y);
-        float inv = 1.0f / len;
-        bool ok = !(len < kEps);
-        cn = sel(ok, v2(d.x * inv, d.y * inv), d);
+    // b2Vec2.Normalize (b2math.d:162-175) of d for circles; skipped by waves
+    // without circle manifolds (the square root and division are the longest
+    // part of the point)
+    float2 normal = mul(X1.q, p.localNormal);
+    if (circles) {
+        const float len = sqrtf(d.x * d.x + d.y * d.y);
+        const float inv = 1.0f / len;
+        normal = len < kEps ? d : v2(d.x * inv, d.y * inv);
     }
-    float2 normal = sel(circles, cn, mul(X1.q, p.localNormal));
     const float separation = dot(d, normal) - p.radiusA - p.radiusB;
     const float2 point = sel(circles, scl(0.5f, add(P1, P2)), P2);
     normal = sel(faceB, neg(normal), normal);
