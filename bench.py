@@ -108,7 +108,18 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 20 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_mib:
+        import torch
+        x = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(20):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "copies": 20}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (worlds shard across GPUs with no "
               "exchange; see DESIGN.md)", file=sys.stderr)
