@@ -51,7 +51,7 @@ struct WorldDesc {
     int contactBase, contactCap;
     int pairBase, pairCap;
     float gx, gy;
-    int islandBase, pad;
+    int islandBase, pad;      // pad bit 0: a body of this world has several fixtures
     long long islScratchBase; // byte offset of the world's island-search scratch
     long long bpScratchBase;  // byte offset of the world's broadphase scratch
 };

@@ -384,6 +384,8 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
 
     if (timed && lane == 0) d.ktime[1] = __builtin_amdgcn_s_memtime();
     int nc = 0, nisl = 0, maxBatch = -1, overflow = 0;
+    int laneMax = -1; // ordered: this lane's largest level (reduced over the wave after the search)
+    const bool dupPossible = (wd.pad & 1) != 0;
     // Seeds in body-list order (newest body first), 64 candidates at a time.
     for (int top = nb - 1; top >= 0; top -= 64) {
         int lo = top - 63 < 0 ? 0 : top - 63;
@@ -442,8 +444,7 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
                                 __syncthreads();
                             }
                         }
-                        int mx = wave_last(wave_incl_max(elig ? lvl : -1));
-                        if (mx > maxBatch) maxBatch = mx;
+                        if (elig && lvl > laneMax) laneMax = lvl;
                     } else {
                         // greedy colours, one eligible edge after another
                         unsigned long long rem = em;
@@ -473,11 +474,14 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
                     nc += ne;
                     // Push the other bodies in edge order; the first edge to a body wins.
                     bool pc = elig && M.bisl[other] < 0 && !(M.bfl[other] & kIslStatic);
-                    if (pc) atomicMin(&M.claim[other], lane);
-                    __syncthreads();
-                    bool push = pc && M.claim[other] == lane;
-                    __syncthreads();
-                    if (pc) M.claim[other] = 64;
+                    bool push = pc;
+                    if (dupPossible) { // with one fixture per body every edge leads to a different body
+                        if (pc) atomicMin(&M.claim[other], lane);
+                        __syncthreads();
+                        push = pc && M.claim[other] == lane;
+                        __syncthreads();
+                        if (pc) M.claim[other] = 64;
+                    }
                     unsigned long long pmask = __ballot(push);
                     int prank = __popcll(pmask & ltmask);
                     if (push) {
@@ -495,6 +499,10 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
             }
             ++nisl;
         }
+    }
+    if (ordered) {
+        const int mx = wave_last(wave_incl_max(laneMax));
+        if (mx > maxBatch) maxBatch = mx;
     }
     if (timed && lane == 0) {
         d.ktime[2] = __builtin_amdgcn_s_memtime();
