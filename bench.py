@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--cpu-sample-every", type=int, default=32)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="after the timed eager steps, time this many more steps replayed as hipGraphs")
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 20 device-to-device copies of this many MiB (HBM counter calibration) and exit")
@@ -150,6 +153,9 @@ def main():
     except Exception:  # torch is only used for an extra device barrier
         have_torch = False
 
+    # warm-up; its first 5 x 2 steps let the autotuner pick each tier's workgroup size
+    if not args.no_autotune and args.warmup >= 10:
+        h.autotune(2)
     h.step(DT, VEL, POS, flags, n=args.warmup)
     h.sync()
     if have_torch:
@@ -191,6 +197,22 @@ def main():
     log("lead-tier ms/step: " + ", ".join(f"{k} {v / max(nsteps, 1):.3f}" for k, v in kern_ms.items()) +
         f"; tiers {tier_rows.tolist()}")
 
+    graph = None
+    if args.graph_steps > 0:
+        h.set_graph(True)
+        h.step(DT, VEL, POS, flags, n=4)  # capture both buffer parities
+        h.sync()
+        if have_torch:
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.step(DT, VEL, POS, flags, n=args.graph_steps)
+        h.sync()
+        if have_torch:
+            torch.cuda.synchronize()
+        tg = time.perf_counter() - t0
+        h.set_graph(False)
+        graph = {"value": ndyn * args.graph_steps / tg, "ms_per_step": 1e3 * tg / args.graph_steps,
+                 "steps": f"{args.warmup + args.steps + 5}-{args.warmup + args.steps + 4 + args.graph_steps}"}
     result = {
         "metric": "body-steps/s at 4096 worlds",
         "value": value,
@@ -233,6 +255,8 @@ def main():
             "avg_kernel_ms": avg_ms,
         },
         "lead_tier_phase_ms_per_step": {k: v / max(nsteps, 1) for k, v in kern_ms.items()},
+        "tiers": [{"worlds": int(r[1]), "lds": int(r[2]), "block": int(r[3])} for r in tier_rows],
+        "graph_replay": graph,
         "step_algorithmic_gbs": step_bytes / (elapsed / args.steps) / 1e9,
     }
     if not args.no_cpu:

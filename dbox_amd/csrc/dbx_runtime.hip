@@ -1609,3 +1609,9 @@ dbx_status dbx_set_tier_blocks(dbx_handle *h, const int32_t *blocks, int32_t n) 
     for (int32_t i = 0; i < n; ++i) h->tiers[i].block = blocks[i];
     return DBX_OK;
 }
+
+dbx_status dbx_set_graph(dbx_handle *h, int32_t enabled) {
+    if (!h) return DBX_E_INVALID_ARG;
+    h->cfg.use_graph = enabled ? 1 : 0;
+    return DBX_OK;
+}

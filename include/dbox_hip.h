@@ -184,6 +184,9 @@ dbx_status dbx_read_tiers(dbx_handle *h, int32_t *tiers, int32_t cap, int32_t *o
  * it).  Results never depend on the workgroup size, so tuning steps are
  * ordinary steps.  Those steps are not captured as graphs. */
 dbx_status dbx_autotune(dbx_handle *h, int32_t steps_per_candidate);
+/* Turns the hipGraph replay of dbx_config.use_graph on or off for later steps.
+ * Replayed steps record only the step's total time in dbx_profile. */
+dbx_status dbx_set_graph(dbx_handle *h, int32_t enabled);
 /* Sets the workgroup size of each tier (n = tier count), e.g. from a cache of
  * earlier autotuning. */
 dbx_status dbx_set_tier_blocks(dbx_handle *h, const int32_t *blocks, int32_t n);
