@@ -65,6 +65,8 @@ struct WorldState {
     int touching;
     int iterations;   // velocity iterations used by the last solve
     int constraints;  // constraints solved by the last step
+    float residual;   // velocity residual after the last iteration (RESIDUAL mode), else -1
+    int pad[3];
 };
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).

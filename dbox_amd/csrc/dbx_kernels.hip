@@ -917,9 +917,9 @@ struct ImpRef { // accumulated impulses (nImp0, tImp0, nImp1, tImp1) of constrai
 };
 
 // The 14 features of point j of a constraint: relative normal and tangent
-// velocity, normal, separation, inverse masses and inertias, mean matched
-// normal impulse of the contacts of both bodies, rA x n, rB x n, friction and
-// the point's normal mass.
+// velocity, normal, separation, inverse masses and inertias, the summed matched
+// normal impulses on body A's and on body B's contacts and their number, friction
+// and the point's normal mass.
 __device__ void mlp_features(float *f, const VelCon &v, const PosCon &q, int j, float4 va, float4 vb, float4 pa,
                              float4 pb, float2 nbA, float2 nbB) {
     const float2 rA = sel(j != 0, v.rA1, v.rA0), rB = sel(j != 0, v.rB1, v.rB0);
@@ -936,10 +936,9 @@ __device__ void mlp_features(float *f, const VelCon &v, const PosCon &q, int j, 
     f[6] = v.invMassB;
     f[7] = v.invIA;
     f[8] = v.invIB;
-    const float cnt = nbA.y + nbB.y;
-    f[9] = cnt > 0.0f ? (nbA.x + nbB.x) / cnt : 0.0f;
-    f[10] = crs(rA, n);
-    f[11] = crs(rB, n);
+    f[9] = nbA.x;
+    f[10] = nbB.x;
+    f[11] = nbA.y + nbB.y;
     f[12] = v.friction;
     f[13] = j ? v.nMass1 : v.nMass0;
     f[14] = 0.0f;
@@ -1147,6 +1146,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
     int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
     int used = 0;
+    float lastRes = residualMode ? 0.0f : -1.0f;
     for (int it = 0; it < iters; ++it) {
         if (tid == 0) s_res = 0.0f;
         __syncthreads();
@@ -1168,10 +1168,14 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
             __syncthreads();
             float r = s_res;
             __syncthreads();
+            lastRes = r;
             if (r <= sp.residualTol) break;
         }
     }
-    if (tid == 0) d.S[w].iterations = used;
+    if (tid == 0) {
+        d.S[w].iterations = used;
+        d.S[w].residual = lastRes;
+    }
     if (sp.flags & (1u << 6)) mlp_log_labels(d, w, gimp);
 
     // StoreImpulses (b2contactsolver.d:774-787)
@@ -1529,6 +1533,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     {
         const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
         int used = 0;
+        float lastRes = residualMode ? 0.0f : -1.0f;
         for (int it = 0; it < iters && nbatch > 0; ++it) {
             ++used;
             if (!residualMode) {
@@ -1538,9 +1543,13 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
             float r = velocity_sweep<2>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) r = fmaxr(r, __shfl_xor(r, o, 64));
+            lastRes = r;
             if (r <= sp.residualTol) break;
         }
-        if (lane == 0) d.S[w].iterations = used;
+        if (lane == 0) {
+            d.S[w].iterations = used;
+            d.S[w].residual = lastRes;
+        }
     }
     if (timed && lane == 0) d.ktime[7] = __builtin_amdgcn_s_memtime();
     if (sp.flags & (1u << 6)) mlp_log_labels(d, w, ImpRef{Limp, 1});

@@ -828,44 +828,65 @@ static dbx_status issue_step(dbx_handle *h, const StepParams &sp, uint32_t flags
     return DBX_OK;
 }
 
-// The step as a hipGraph: captured once per distinct (device pointers, step
-// parameters, buffer parity, launch shapes) and replayed.  Kernel arguments are
-// baked into a graph, so anything that changes them selects another graph.
-static dbx_status step_graph(dbx_handle *h, const StepParams &sp, uint32_t flags, bool findNew, const hipEvent_t *ev) {
+// The step as hipGraphs, one per tier: each tier's chain is captured once per
+// distinct (device pointers, step parameters, buffer parity, launch shape) and
+// replayed on the tier's own stream, so the tiers still run concurrently (one
+// graph holding all tiers executes its branches one after another).  Kernel
+// arguments are baked into a graph, so anything that changes them selects
+// another graph.
+static dbx_status tier_graph(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew,
+                             hipGraphExec_t *out) {
     std::vector<unsigned char> key;
+    key_append(key, t);
     key_append(key, h->d);
     key_append(key, sp);
     key_append(key, flags);
     key_append(key, h->cur);
     key_append(key, findNew);
-    for (const Tier &T : h->tiers) key_append(key, T.block);
-    hipGraphExec_t exec = nullptr;
+    key_append(key, h->tiers[t].block);
     for (GraphEntry &g : h->graphs)
-        if (g.key == key) exec = g.exec;
-    hipStream_t s = h->stream;
-    if (!exec) {
-        if (!h->fork) DBX_HIP(h, hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
-        DBX_HIP(h, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-        dbx_status st = issue_step(h, sp, flags, findNew, nullptr, false);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(s, &graph);
-        if (st != DBX_OK) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return st;
+        if (g.key == key) {
+            *out = g.exec;
+            return DBX_OK;
         }
-        if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
-        if ((int)h->graphs.size() >= kMaxGraphs) {
-            (void)hipStreamSynchronize(s);
-            (void)hipGraphExecDestroy(h->graphs.front().exec);
-            h->graphs.erase(h->graphs.begin());
-        }
-        h->graphs.push_back(GraphEntry{key, exec});
+    hipStream_t s = h->tiers[t].stream;
+    DBX_HIP(h, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    dbx_status st = step_tier(h, t, sp, flags, findNew, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st != DBX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
     }
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+    if ((int)h->graphs.size() >= kMaxGraphs * (int)h->tiers.size()) {
+        DBX_HIP(h, hipDeviceSynchronize());
+        (void)hipGraphExecDestroy(h->graphs.front().exec);
+        h->graphs.erase(h->graphs.begin());
+    }
+    h->graphs.push_back(GraphEntry{key, exec});
+    *out = exec;
+    return DBX_OK;
+}
+
+static dbx_status step_graph(dbx_handle *h, const StepParams &sp, uint32_t flags, bool findNew, const hipEvent_t *ev) {
+    std::vector<hipGraphExec_t> execs(h->tiers.size());
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        dbx_status st = tier_graph(h, (int)t, sp, flags, findNew, &execs[t]);
+        if (st != DBX_OK) return st;
+    }
+    hipStream_t s = h->stream;
     DBX_HIP(h, hipEventRecord(ev[0], s));
-    DBX_HIP(h, hipGraphLaunch(exec, s));
+    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, ev[0], 0));
+    for (size_t t = 0; t < h->tiers.size(); ++t) DBX_HIP(h, hipGraphLaunch(execs[t], h->tiers[t].stream));
+    for (size_t t = 1; t < h->tiers.size(); ++t) {
+        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
+        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    }
     DBX_HIP(h, hipEventRecord(ev[5], s));
     return DBX_OK;
 }
@@ -1244,6 +1265,17 @@ dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_
         DBX_HIP(h, hipMemcpyAsync(max_penetration, h->dPen, h->nworlds * sizeof(float), hipMemcpyDeviceToHost, h->stream));
         DBX_HIP(h, hipStreamSynchronize(h->stream));
     }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_residuals(dbx_handle *h, float *residual, int32_t nworlds) {
+    if (!h || !residual) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (nworlds < h->nworlds) return fail(h, DBX_E_INVALID_ARG, "output too small");
+    std::vector<WorldState> S(h->nworlds);
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
+    for (int w = 0; w < h->nworlds; ++w) residual[w] = S[w].residual;
     return DBX_OK;
 }
 

@@ -32,7 +32,7 @@ def _handle(scene_list) -> native.StepWorlds:
 
 
 def collect(scene_list, steps: int, capacity: int = 1 << 22, tolerance: float = 1e-5,
-            max_iterations: int = 200) -> native.StepWorlds:
+            max_iterations: int = 500) -> native.StepWorlds:
     """Steps the scenes with the plain solver iterating to `tolerance` and logs
     every new point's features and converged impulses.  Returns the handle,
     whose log holds the samples."""
@@ -62,6 +62,7 @@ class IterationStats:
     capped_fraction: float      # share of those pairs that hit max_iterations
     max_penetration: float      # over all worlds and steps
     world_steps: int
+    mean_residual: float        # velocity residual after the last iteration, mean over those pairs
 
 
 def iterations(scene_list, steps: int, params=None, norm=None, tolerance: float = 1e-3,
@@ -74,19 +75,21 @@ def iterations(scene_list, steps: int, params=None, norm=None, tolerance: float 
     if params is not None:
         h.mlp_set(params, norm)
         flags |= native.LEARNED
-    its, pen = [], 0.0
+    its, res, pen = [], [], 0.0
     for _ in range(steps):
         h.step(DT, VEL, POS, flags)
         h.sync()
         it, p = h.solver_stats()
+        r = h.residuals()
         its.append(it[it > 0])
+        res.append(r[it > 0])
         pen = max(pen, float(p.max()))
     h.close()
     a = np.concatenate(its) if its else np.zeros(0)
     if len(a) == 0:
-        return IterationStats(0.0, 0.0, 0.0, pen, 0)
+        return IterationStats(0.0, 0.0, 0.0, pen, 0, 0.0)
     return IterationStats(float(a.mean()), float(np.percentile(a, 95)), float(np.mean(a >= max_iterations)), pen,
-                          int(len(a)))
+                          int(len(a)), float(np.concatenate(res).mean()))
 
 
 def training_scenes(seed: int = 11, pyramids: int = 8, stacks: int = 48, mixed: int = 8):

@@ -168,6 +168,11 @@ dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
 /* Per-world velocity iterations used by the last step (RESIDUAL mode) and max penetration. */
 dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds);
 
+/* Per-world velocity residual after the last velocity iteration of the last
+ * step in RESIDUAL mode (-1 otherwise).  With dbx_set_residual(h, 0, n) a step
+ * runs exactly n iterations, so this is the residual after n iterations. */
+dbx_status dbx_read_residuals(dbx_handle *h, float *residual, int32_t nworlds);
+
 /* Launch tiers (no counterpart in the reference; worlds are independent there):
  * worlds are launched largest first, in tiers of similar size that each run
  * their step chain on a stream of their own.  Writes (first launch slot, world
@@ -226,9 +231,9 @@ dbx_status dbx_set_residual(dbx_handle *h, float tolerance, int32_t max_iteratio
  * scale[14] (the network sees (x - mean) * scale), output mean[2], output
  * scale[2] (impulse = y * scale + mean).  The DBX_MLP_FEATURES features of a
  * point, in order: relative normal velocity, relative tangent velocity, normal
- * x, normal y, separation, invMassA, invMassB, invIA, invIB, mean matched normal
- * impulse over the contacts of both bodies, rA x n, rB x n, friction, the
- * point's normal mass.  Predicted impulses are clamped to the solver's bounds
+ * x, normal y, separation, invMassA, invMassB, invIA, invIB, the sum of the
+ * matched (warm-start) normal impulses on body A's contacts, the same for body
+ * B, the number of those matched points, friction, the point's normal mass.  Predicted impulses are clamped to the solver's bounds
  * (normal >= 0, |tangent| <= friction * normal). */
 #define DBX_MLP_PARAMS 5250
 #define DBX_MLP_NORM 32
