@@ -1324,8 +1324,8 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, in
 // being solved.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline size_t solve_lds_bytes(int nb, int nc, int nbatch, int nisl) {
-    return isl_align(32 * (size_t)nb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)nc) +
-           isl_align(4 * (size_t)(nbatch + 1)) + isl_align(8 * (size_t)nisl);
+    return isl_align(32 * (size_t)nb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)(nbatch + 1)) +
+           isl_align(8 * (size_t)nisl);
 }
 
 // One constraint of a level, on LDS body state.  kMode 0 = warm start,
@@ -1340,56 +1340,75 @@ __device__ __forceinline__ void velocity_one(const VelCon &v, int k, float4 *Lve
     if (v.dynMask & 2) Lvel[v.indexB] = vb;
 }
 
-// Slot of this lane in level bt of the exact-order schedule (Lsch = constraint
-// per slot, Lbo = level offsets); a lane without a constraint gets slot 0's, so
-// that the loads issued for it stay in bounds.
-__device__ __forceinline__ void level_slot(const int *Lsch, const int *Lbo, int bt, int nbatch, int lane, int &k,
-                                           bool &have) {
-    int j = 0;
-    have = false;
+// Level offsets of the exact-order schedule, read through a window of 64
+// entries held one per lane: an offset is a v_readlane with a wave-uniform index
+// instead of an LDS round trip on the per-level critical path.
+struct LevelWindow {
+    const int *Lbo; // nbatch + 1 offsets in LDS
+    int n, base, reg;
+    __device__ __forceinline__ void load(int b) {
+        base = b;
+        const int i = b + (int)(threadIdx.x & 63);
+        reg = i < n ? Lbo[i] : 0;
+    }
+    // Makes entries [first, first + 4] readable (first is wave-uniform).
+    __device__ __forceinline__ void cover(int first) {
+        if (first + 4 >= base + 64) load(first);
+    }
+    __device__ __forceinline__ int at(int i) const { return __builtin_amdgcn_readlane(reg, i - base); }
+};
+
+// This lane's slot in level bt: constraints are stored in schedule order, so
+// the slot is the constraint; a lane without one reads slot 0 so that the loads
+// issued for it stay in bounds.
+__device__ __forceinline__ void level_slot(const LevelWindow &W, int bt, int nbatch, int lane, int &k, bool &have,
+                                           int &lo, int &hi) {
+    lo = 0;
+    hi = 0;
     if (bt < nbatch) {
-        j = Lbo[bt] + lane;
-        have = j < Lbo[bt + 1];
+        lo = W.at(bt);
+        hi = W.at(bt + 1);
     }
-    k = Lsch[have ? j : 0];
+    const int j = lo + lane;
+    have = j < hi;
+    k = have ? j : 0;
 }
 
 template <int kMode>
-__device__ __forceinline__ void velocity_level(const VelCon &v, int k, bool have, int bt, const int *Lsch,
-                                               const int *Lbo, const VelCon *__restrict__ vc, float4 *Lvel,
-                                               float4 *Limp, float &lres) {
+__device__ __forceinline__ void velocity_level(const VelCon &v, int k, bool have, int lo, int hi,
+                                               const VelCon *__restrict__ vc, float4 *Lvel, float4 *Limp,
+                                               float &lres) {
     if (have) velocity_one<kMode>(v, k, Lvel, Limp, lres);
-    const int e = Lbo[bt + 1];
-    for (int c = Lbo[bt] + 64; c < e; c += 64) { // levels wider than the wave
-        const int j2 = c + (threadIdx.x & 63);
-        if (j2 < e) {
-            const int k2 = Lsch[j2];
-            velocity_one<kMode>(vc[k2], k2, Lvel, Limp, lres);
-        }
+    for (int c = lo + 64; c < hi; c += 64) { // levels wider than the wave
+        const int j2 = c + (int)(threadIdx.x & 63);
+        if (j2 < hi) velocity_one<kMode>(vc[j2], j2, Lvel, Limp, lres);
     }
 }
 
-// One sweep over the levels of the exact-order schedule held in LDS.  Two
-// register sets alternate: while one level is solved, the constraints of the
-// level after the next are already loading into the set just used, so the
-// wait before a level covers loads issued a whole level earlier.
+// One sweep over the levels of the exact-order schedule.  Two register sets
+// alternate: while one level is solved, the constraints of the level after the
+// next are already loading into the set just used, so the wait before a level
+// covers loads issued a whole level earlier.
 template <int kMode>
-__device__ __forceinline__ float velocity_sweep(const int *Lsch, const int *Lbo, int nbatch,
-                                                const VelCon *__restrict__ vc, float4 *Lvel, float4 *Limp, int lane) {
+__device__ __forceinline__ float velocity_sweep(const int *Lbo, int nbatch, const VelCon *__restrict__ vc,
+                                                float4 *Lvel, float4 *Limp, int lane) {
     float lres = 0.0f;
-    int k0, k1;
+    LevelWindow W{Lbo, nbatch + 1, 0, 0};
+    W.load(0);
+    int k0, k1, lo0, hi0, lo1, hi1;
     bool h0, h1;
-    level_slot(Lsch, Lbo, 0, nbatch, lane, k0, h0);
+    level_slot(W, 0, nbatch, lane, k0, h0, lo0, hi0);
     VelCon c0 = vc[k0];
-    level_slot(Lsch, Lbo, 1, nbatch, lane, k1, h1);
+    level_slot(W, 1, nbatch, lane, k1, h1, lo1, hi1);
     VelCon c1 = vc[k1];
     for (int bt = 0; bt < nbatch; bt += 2) {
-        velocity_level<kMode>(c0, k0, h0, bt, Lsch, Lbo, vc, Lvel, Limp, lres);
-        level_slot(Lsch, Lbo, bt + 2, nbatch, lane, k0, h0);
+        W.cover(bt);
+        velocity_level<kMode>(c0, k0, h0, lo0, hi0, vc, Lvel, Limp, lres);
+        level_slot(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
         c0 = vc[k0];
         if (bt + 1 >= nbatch) break;
-        velocity_level<kMode>(c1, k1, h1, bt + 1, Lsch, Lbo, vc, Lvel, Limp, lres);
-        level_slot(Lsch, Lbo, bt + 3, nbatch, lane, k1, h1);
+        velocity_level<kMode>(c1, k1, h1, lo1, hi1, vc, Lvel, Limp, lres);
+        level_slot(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
         c1 = vc[k1];
     }
     return lres;
@@ -1403,15 +1422,14 @@ __device__ __forceinline__ void position_one(const PosCon &q, float4 *Lpos, int 
     atomicMin(&Lmin[q.island], f2ord(ms));
 }
 
-__device__ __forceinline__ void position_level(const PosCon &q, bool have, int bt, const int *Lsch, const int *Lbo,
+__device__ __forceinline__ void position_level(const PosCon &q, bool have, int lo, int hi,
                                                const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
                                                const int *Lsol) {
     if (have && !Lsol[q.island]) position_one(q, Lpos, Lmin);
-    const int e = Lbo[bt + 1];
-    for (int c = Lbo[bt] + 64; c < e; c += 64) {
-        const int j2 = c + (threadIdx.x & 63);
-        if (j2 < e) {
-            const PosCon q2 = pc[Lsch[j2]];
+    for (int c = lo + 64; c < hi; c += 64) {
+        const int j2 = c + (int)(threadIdx.x & 63);
+        if (j2 < hi) {
+            const PosCon q2 = pc[j2];
             if (!Lsol[q2.island]) position_one(q2, Lpos, Lmin);
         }
     }
@@ -1420,22 +1438,24 @@ __device__ __forceinline__ void position_level(const PosCon &q, bool have, int b
 // One position iteration over the levels, with the same two-set pipeline;
 // islands already solved are skipped (b2island.d:206-224 stops iterating an
 // island once it is solved).
-__device__ __forceinline__ void position_sweep(const int *Lsch, const int *Lbo, int nbatch,
-                                               const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
-                                               const int *Lsol, int lane) {
-    int k0, k1;
+__device__ __forceinline__ void position_sweep(const int *Lbo, int nbatch, const PosCon *__restrict__ pc,
+                                               float4 *Lpos, int *Lmin, const int *Lsol, int lane) {
+    LevelWindow W{Lbo, nbatch + 1, 0, 0};
+    W.load(0);
+    int k0, k1, lo0, hi0, lo1, hi1;
     bool h0, h1;
-    level_slot(Lsch, Lbo, 0, nbatch, lane, k0, h0);
+    level_slot(W, 0, nbatch, lane, k0, h0, lo0, hi0);
     PosCon c0 = pc[k0];
-    level_slot(Lsch, Lbo, 1, nbatch, lane, k1, h1);
+    level_slot(W, 1, nbatch, lane, k1, h1, lo1, hi1);
     PosCon c1 = pc[k1];
     for (int bt = 0; bt < nbatch; bt += 2) {
-        position_level(c0, h0, bt, Lsch, Lbo, pc, Lpos, Lmin, Lsol);
-        level_slot(Lsch, Lbo, bt + 2, nbatch, lane, k0, h0);
+        W.cover(bt);
+        position_level(c0, h0, lo0, hi0, pc, Lpos, Lmin, Lsol);
+        level_slot(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
         c0 = pc[k0];
         if (bt + 1 >= nbatch) break;
-        position_level(c1, h1, bt + 1, Lsch, Lbo, pc, Lpos, Lmin, Lsol);
-        level_slot(Lsch, Lbo, bt + 3, nbatch, lane, k1, h1);
+        position_level(c1, h1, lo1, hi1, pc, Lpos, Lmin, Lsol);
+        level_slot(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
         c1 = pc[k1];
     }
 }
@@ -1457,7 +1477,6 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     float4 *Lvel = (float4 *)p; p += isl_align(16 * (size_t)nb);
     float4 *Lpos = (float4 *)p; p += isl_align(16 * (size_t)nb);
     float4 *Limp = (float4 *)p; p += isl_align(16 * (size_t)nc);
-    int *Lsch = (int *)p; p += isl_align(4 * (size_t)nc);
     int *Lbo = (int *)p; p += isl_align(4 * (size_t)(nbatch + 1));
     int *Lmin = (int *)p; p += isl_align(4 * (size_t)nisl);
     int *Lsol = (int *)p;
@@ -1488,7 +1507,6 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
 
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0;
     if (timed && lane == 0) d.ktime[4] = __builtin_amdgcn_s_memtime();
-    for (int i = lane; i < nc; i += 64) Lsch[i] = sched[i];
     for (int i = lane; i <= nbatch; i += 64) Lbo[i] = boff[i];
     // Integrate velocities with damping (b2island.d:82-116) while staging the bodies.
     for (int b = lane; b < nb; b += 64) {
@@ -1514,21 +1532,24 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         Lvel[b] = make_float4(vx, vy, wv, vw);
     }
     __syncthreads();
-    for (int k = lane; k < nc; k += 64) {
-        int ci = ordC[k];
+    // Constraints are stored in schedule order (slot j holds constraint sched[j]),
+    // so a level's constraints are a contiguous range of slots.
+    for (int j = lane; j < nc; j += 64) {
+        const int k = sched[j];
+        const int ci = ordC[k];
         VelCon v;
         PosCon q;
         init_constraint(v, q, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, Lpos, Lvel, sp);
-        vc[k] = v;
-        pc[k] = q;
-        Limp[k] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+        vc[j] = v;
+        pc[j] = q;
+        Limp[j] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
     }
     __syncthreads();
     if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, ImpRef{Limp, 1}, Lvel, Lpos);
 
     if (timed && lane == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
-    if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+    if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lbo, nbatch, vc, Lvel, Limp, lane);
     if (timed && lane == 0) d.ktime[6] = __builtin_amdgcn_s_memtime();
     {
         const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
@@ -1537,10 +1558,10 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         for (int it = 0; it < iters && nbatch > 0; ++it) {
             ++used;
             if (!residualMode) {
-                velocity_sweep<1>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+                velocity_sweep<1>(Lbo, nbatch, vc, Lvel, Limp, lane);
                 continue;
             }
-            float r = velocity_sweep<2>(Lsch, Lbo, nbatch, vc, Lvel, Limp, lane);
+            float r = velocity_sweep<2>(Lbo, nbatch, vc, Lvel, Limp, lane);
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) r = fmaxr(r, __shfl_xor(r, o, 64));
             lastRes = r;
@@ -1589,7 +1610,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     for (int it = 0; it < sp.positionIterations; ++it) {
         for (int i = lane; i < nisl; i += 64)
             if (!Lsol[i]) Lmin[i] = f2ord(0.0f);
-        if (nbatch > 0) position_sweep(Lsch, Lbo, nbatch, pc, Lpos, Lmin, Lsol, lane);
+        if (nbatch > 0) position_sweep(Lbo, nbatch, pc, Lpos, Lmin, Lsol, lane);
         bool open = false;
         for (int i = lane; i < nisl; i += 64) {
             if (Lsol[i]) continue;
