@@ -685,12 +685,11 @@ __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, 
     }
 }
 
-// The three per-constraint cores below are written as straight-line code: a
-// level of the exact-order schedule usually holds a handful of constraints of
-// mixed point counts and manifold types in one wave, and branching on those
-// would run every path anyway, with the branch overhead on top.  Each core
-// evaluates both the one-point and the two-point form and selects the results,
-// keeping the reference's floating-point operation order in each form.
+// The per-constraint cores below branch only on the point count (and the
+// position core on the manifold type), and are straight-line code inside each
+// branch: a wave runs a form only when one of its lanes needs it, which a level
+// of the exact-order schedule (a handful of constraints) often does not.  Each
+// form keeps the reference's floating-point operation order.
 
 // b2ContactSolver.WarmStart (b2contactsolver.d:452-490) for one constraint.
 // imp = (normalImpulse0, tangentImpulse0, normalImpulse1, tangentImpulse1);
@@ -701,24 +700,19 @@ __device__ __forceinline__ void warm_start_core(const VelCon &v, float4 imp, flo
     const float2 normal = v.normal;
     const float2 tangent = crs(normal, 1.0f);
     const float mA = v.invMassA, mB = v.invMassB, iA = v.invIA, iB = v.invIB;
-    const bool two = v.pointCount > 1;
     {
-        float2 P = add(scl(imp.x, normal), scl(imp.y, tangent));
+        const float2 P = add(scl(imp.x, normal), scl(imp.y, tangent));
         wA -= iA * crs(v.rA0, P);
         vA = sub(vA, scl(mA, P));
         wB += iB * crs(v.rB0, P);
         vB = add(vB, scl(mB, P));
     }
-    {
-        float2 P = add(scl(imp.z, normal), scl(imp.w, tangent));
-        float wA1 = wA - iA * crs(v.rA1, P);
-        float2 vA1 = sub(vA, scl(mA, P));
-        float wB1 = wB + iB * crs(v.rB1, P);
-        float2 vB1 = add(vB, scl(mB, P));
-        wA = two ? wA1 : wA;
-        vA = sel(two, vA1, vA);
-        wB = two ? wB1 : wB;
-        vB = sel(two, vB1, vB);
+    if (v.pointCount > 1) {
+        const float2 P = add(scl(imp.z, normal), scl(imp.w, tangent));
+        wA -= iA * crs(v.rA1, P);
+        vA = sub(vA, scl(mA, P));
+        wB += iB * crs(v.rB1, P);
+        vB = add(vB, scl(mB, P));
     }
     va = make_float4(vA.x, vA.y, wA, 0.0f);
     vb = make_float4(vB.x, vB.y, wB, 0.0f);
@@ -741,100 +735,84 @@ __device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &im
     float nI0 = imp.x, tI0 = imp.y, nI1 = imp.z, tI1 = imp.w;
     // friction, point 0 (b2contactsolver.d:527-553)
     {
-        float2 dv = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
-        float vt = dot(dv, tangent) - v.tangentSpeed;
+        const float2 dv = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
+        const float vt = dot(dv, tangent) - v.tangentSpeed;
         float lambda = v.tMass0 * (-vt);
-        float maxFriction = friction * nI0;
-        float newImpulse = clampr(tI0 + lambda, -maxFriction, maxFriction);
+        const float maxFriction = friction * nI0;
+        const float newImpulse = clampr(tI0 + lambda, -maxFriction, maxFriction);
         if (kResidual && fabsr(tI0) < maxFriction) res = fmaxr(res, fabsr(vt));
         lambda = newImpulse - tI0;
         tI0 = newImpulse;
-        float2 P = scl(lambda, tangent);
+        const float2 P = scl(lambda, tangent);
         vA = sub(vA, scl(mA, P));
         wA -= iA * crs(v.rA0, P);
         vB = add(vB, scl(mB, P));
         wB += iB * crs(v.rB0, P);
     }
-    // friction, point 1
-    {
-        float2 dv = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
-        float vt = dot(dv, tangent) - v.tangentSpeed;
+    if (two) { // friction, point 1
+        const float2 dv = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
+        const float vt = dot(dv, tangent) - v.tangentSpeed;
         float lambda = v.tMass1 * (-vt);
-        float maxFriction = friction * nI1;
-        float newImpulse = clampr(tI1 + lambda, -maxFriction, maxFriction);
-        if (kResidual && two && fabsr(tI1) < maxFriction) res = fmaxr(res, fabsr(vt));
+        const float maxFriction = friction * nI1;
+        const float newImpulse = clampr(tI1 + lambda, -maxFriction, maxFriction);
+        if (kResidual && fabsr(tI1) < maxFriction) res = fmaxr(res, fabsr(vt));
         lambda = newImpulse - tI1;
-        float2 P = scl(lambda, tangent);
-        float2 vA1 = sub(vA, scl(mA, P));
-        float wA1 = wA - iA * crs(v.rA1, P);
-        float2 vB1 = add(vB, scl(mB, P));
-        float wB1 = wB + iB * crs(v.rB1, P);
-        tI1 = two ? newImpulse : tI1;
-        vA = sel(two, vA1, vA);
-        wA = two ? wA1 : wA;
-        vB = sel(two, vB1, vB);
-        wB = two ? wB1 : wB;
+        tI1 = newImpulse;
+        const float2 P = scl(lambda, tangent);
+        vA = sub(vA, scl(mA, P));
+        wA -= iA * crs(v.rA1, P);
+        vB = add(vB, scl(mB, P));
+        wB += iB * crs(v.rB1, P);
     }
     const float2 dv1 = sub(sub(add(vB, crs(wB, v.rB0)), vA), crs(wA, v.rA0));
     const float vn1 = dot(dv1, normal);
-    // one point (b2contactsolver.d:559-585)
-    float2 sA, sB;
-    float swA, swB, sI0;
-    {
+    if (kResidual) { // vn1 - bias0 is point 0's error in both forms
+        const float e = vn1 - v.bias0;
+        res = fmaxr(res, nI0 > 0.0f ? fabsr(e) : fmaxr(0.0f, -e));
+    }
+    if (!two) { // one point (b2contactsolver.d:559-585)
         float lambda = -v.nMass0 * (vn1 - v.bias0);
-        float newImpulse = fmaxr(nI0 + lambda, 0.0f);
+        const float newImpulse = fmaxr(nI0 + lambda, 0.0f);
         lambda = newImpulse - nI0;
-        sI0 = newImpulse;
-        float2 P = scl(lambda, normal);
-        sA = sub(vA, scl(mA, P));
-        swA = wA - iA * crs(v.rA0, P);
-        sB = add(vB, scl(mB, P));
-        swB = wB + iB * crs(v.rB0, P);
+        nI0 = newImpulse;
+        const float2 P = scl(lambda, normal);
+        vA = sub(vA, scl(mA, P));
+        wA -= iA * crs(v.rA0, P);
+        vB = add(vB, scl(mB, P));
+        wB += iB * crs(v.rB0, P);
+    } else { // two points: block solver by total enumeration (b2contactsolver.d:587-764)
+        const float2 dv2 = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
+        const float vn2 = dot(dv2, normal);
+        float bx = vn1 - v.bias0;
+        float by = vn2 - v.bias1;
+        if (kResidual) res = fmaxr(res, nI1 > 0.0f ? fabsr(by) : fmaxr(0.0f, -by));
+        bx -= v.K.x * nI0 + v.K.z * nI1;
+        by -= v.K.y * nI0 + v.K.w * nI1;
+        // case 1: both points active
+        const float x1x = -(v.normalMass.x * bx + v.normalMass.z * by);
+        const float x1y = -(v.normalMass.y * bx + v.normalMass.w * by);
+        const bool c1 = x1x >= 0.0f && x1y >= 0.0f;
+        // case 2: point 1 only
+        const float x2x = -v.nMass0 * bx;
+        const bool c2 = x2x >= 0.0f && v.K.y * x2x + by >= 0.0f;
+        // case 3: point 2 only
+        const float x3y = -v.nMass1 * by;
+        const bool c3 = x3y >= 0.0f && v.K.z * x3y + bx >= 0.0f;
+        // case 4: neither
+        const bool c4 = bx >= 0.0f && by >= 0.0f;
+        if (c1 || c2 || c3 || c4) {
+            const float xx = c1 ? x1x : (c2 ? x2x : 0.0f);
+            const float xy = c1 ? x1y : (c2 ? 0.0f : (c3 ? x3y : 0.0f));
+            const float2 P1 = scl(xx - nI0, normal);
+            const float2 P2 = scl(xy - nI1, normal);
+            vA = sub(vA, scl(mA, add(P1, P2)));
+            wA -= iA * (crs(v.rA0, P1) + crs(v.rA1, P2));
+            vB = add(vB, scl(mB, add(P1, P2)));
+            wB += iB * (crs(v.rB0, P1) + crs(v.rB1, P2));
+            nI0 = xx;
+            nI1 = xy;
+        }
     }
-    // two points: block solver by total enumeration (b2contactsolver.d:587-764)
-    const float2 dv2 = sub(sub(add(vB, crs(wB, v.rB1)), vA), crs(wA, v.rA1));
-    const float vn2 = dot(dv2, normal);
-    float bx = vn1 - v.bias0;
-    float by = vn2 - v.bias1;
-    if (kResidual) { // vn1 - bias0 is also the one-point form's residual
-        res = fmaxr(res, nI0 > 0.0f ? fabsr(bx) : fmaxr(0.0f, -bx));
-        float r1 = nI1 > 0.0f ? fabsr(by) : fmaxr(0.0f, -by);
-        res = two ? fmaxr(res, r1) : res;
-    }
-    bx -= v.K.x * nI0 + v.K.z * nI1;
-    by -= v.K.y * nI0 + v.K.w * nI1;
-    // case 1: both points active
-    const float x1x = -(v.normalMass.x * bx + v.normalMass.z * by);
-    const float x1y = -(v.normalMass.y * bx + v.normalMass.w * by);
-    const bool c1 = x1x >= 0.0f && x1y >= 0.0f;
-    // case 2: point 1 only
-    const float x2x = -v.nMass0 * bx;
-    const bool c2 = x2x >= 0.0f && v.K.y * x2x + by >= 0.0f;
-    // case 3: point 2 only
-    const float x3y = -v.nMass1 * by;
-    const bool c3 = x3y >= 0.0f && v.K.z * x3y + bx >= 0.0f;
-    // case 4: neither
-    const bool c4 = bx >= 0.0f && by >= 0.0f;
-    const bool done = c1 || c2 || c3 || c4;
-    const float xx = c1 ? x1x : (c2 ? x2x : 0.0f);
-    const float xy = c1 ? x1y : (c2 ? 0.0f : (c3 ? x3y : 0.0f));
-    float2 tA, tB;
-    float twA, twB;
-    {
-        float2 P1 = scl(xx - nI0, normal);
-        float2 P2 = scl(xy - nI1, normal);
-        tA = sub(vA, scl(mA, add(P1, P2)));
-        twA = wA - iA * (crs(v.rA0, P1) + crs(v.rA1, P2));
-        tB = add(vB, scl(mB, add(P1, P2)));
-        twB = wB + iB * (crs(v.rB0, P1) + crs(v.rB1, P2));
-    }
-    const bool blk = two && done;
-    vA = sel(two, sel(done, tA, vA), sA);
-    wA = two ? (done ? twA : wA) : swA;
-    vB = sel(two, sel(done, tB, vB), sB);
-    wB = two ? (done ? twB : wB) : swB;
-    nI0 = two ? (done ? xx : nI0) : sI0;
-    nI1 = blk ? xy : nI1;
     imp = make_float4(nI0, tI0, nI1, tI1);
     va = make_float4(vA.x, vA.y, wA, 0.0f);
     vb = make_float4(vB.x, vB.y, wB, 0.0f);
