@@ -130,12 +130,18 @@ __device__ __forceinline__ Rot rot(float angle) {
         pc = fma(pc, z, 0.041666666666666664);
         pc = fma(pc, z, -0.5);
         pc = fma(pc, z, 1.0);
-        int q = (int)(long long)k & 3;
-        s = q == 0 ? ps : q == 1 ? pc : q == 2 ? -ps : -pc;
-        c = q == 0 ? pc : q == 1 ? -ps : q == 2 ? -pc : ps;
-    } else {
-        sincos(x, &s, &c);
+        // quadrant q: sin = (ps, pc, -ps, -pc)[q], cos = (pc, -ps, -pc, ps)[q]; rounding to
+        // float commutes with negation, so the swap and signs are applied to the floats
+        const int q = (int)(long long)k & 3;
+        const float fs = (float)ps, fc = (float)pc;
+        Rot qr;
+        qr.s = (q & 1) ? fc : fs;
+        qr.c = (q & 1) ? fs : fc;
+        qr.s = (q & 2) ? -qr.s : qr.s;
+        qr.c = ((q + 1) & 2) ? -qr.c : qr.c;
+        return qr;
     }
+    sincos(x, &s, &c);
     Rot qr;
     qr.s = (float)s;
     qr.c = (float)c;

@@ -827,11 +827,9 @@ __device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &im
 // other way round for faceB, and separation = dot(P2 - P1, normal) - rA - rB;
 // only the normal (normalised P2 - P1 for circles, X1.q * localNormal for the
 // faces) and the contact point (midpoint for circles, P2 for the faces) differ.
-__device__ __forceinline__ float position_point(const PosCon &p, float2 lp, float2 &cA, float &aA, float2 &cB,
-                                                float &aB) {
+__device__ __forceinline__ float position_point(const PosCon &p, float2 lp, const Xf &xfA, const Xf &xfB, float2 &cA,
+                                                float &aA, float2 &cB, float &aB) {
     const float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
-    const Xf xfA = xf_from(cA, aA, p.lcA);
-    const Xf xfB = xf_from(cB, aB, p.lcB);
     const bool circles = p.type == kManCircles, faceB = p.type == kManFaceB;
     Xf X1, X2;
     X1.p = sel(faceB, xfB.p, xfA.p);
@@ -870,11 +868,26 @@ __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, floa
     return separation;
 }
 
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
+
+// The reference rebuilds both transforms before every point (b2contactsolver.d:101-104);
+// when point 0 left both bodies' positions bit-identical, point 1's transforms are
+// the ones already computed, so they are reused.
 __device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb) {
     float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
     float aA = pa.z, aB = pb.z;
-    float minSep = fminr(0.0f, position_point(p, p.lp0, cA, aA, cB, aB));
-    if (p.pointCount > 1) minSep = fminr(minSep, position_point(p, p.lp1, cA, aA, cB, aB));
+    Xf xfA = xf_from(cA, aA, p.lcA);
+    Xf xfB = xf_from(cB, aB, p.lcB);
+    float minSep = fminr(0.0f, position_point(p, p.lp0, xfA, xfB, cA, aA, cB, aB));
+    if (p.pointCount > 1) {
+        const bool moved = !(same_bits(cA.x, pa.x) && same_bits(cA.y, pa.y) && same_bits(aA, pa.z) &&
+                             same_bits(cB.x, pb.x) && same_bits(cB.y, pb.y) && same_bits(aB, pb.z));
+        if (moved) {
+            xfA = xf_from(cA, aA, p.lcA);
+            xfB = xf_from(cB, aB, p.lcB);
+        }
+        minSep = fminr(minSep, position_point(p, p.lp1, xfA, xfB, cA, aA, cB, aB));
+    }
     pa = make_float4(cA.x, cA.y, aA, pa.w);
     pb = make_float4(cB.x, cB.y, aB, pb.w);
     return minSep;
@@ -909,7 +922,8 @@ __device__ void mlp_features(float *f, const VelCon &v, const PosCon &q, int j, 
     f[3] = n.y;
     float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
     float aA = pa.z, aB = pb.z; // copies: only the separation is wanted
-    f[4] = position_point(q, sel(j != 0, q.lp1, q.lp0), cA, aA, cB, aB);
+    const Xf xfA = xf_from(cA, aA, q.lcA), xfB = xf_from(cB, aB, q.lcB);
+    f[4] = position_point(q, sel(j != 0, q.lp1, q.lp0), xfA, xfB, cA, aA, cB, aB);
     f[5] = v.invMassA;
     f[6] = v.invMassB;
     f[7] = v.invIA;
