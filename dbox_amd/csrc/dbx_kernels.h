@@ -11,7 +11,8 @@ constexpr int kMaxBlock = 1024;
 
 // Step kernels over the worlds order[worldOffset .. worldOffset + nworlds); lead
 // selects the instantiation that profiles report for the lead (largest) tier.
-hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int block, bool lead, hipStream_t s);
+hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
+                          hipStream_t s);
 size_t islands_scratch_bytes(int bodies, int contacts);
 size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands);
 hipError_t configure_kernels(int maxDynamicLds);

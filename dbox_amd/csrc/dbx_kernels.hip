@@ -155,7 +155,8 @@ __device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbo
 // step time); the two instantiations differ only in name, so that profiles
 // report the lead tier on its own.
 template <bool kLead>
-__global__ void k_collide(Dev d, int cur, int worldOffset) {
+__global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
+    extern __shared__ __align__(16) char lds[];
     __shared__ int s_tmp[kMaxBlock / 64 + 1];
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
@@ -171,6 +172,8 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
     int *wake = d.wake + wd.bodyBase;
     const int nb = wd.bodyCount;
     const int tid = threadIdx.x, BS = blockDim.x;
+    const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && tid == 0;
+    if (timed) d.ktime[16] = __builtin_amdgcn_s_memtime();
 
     for (int b = tid; b < nb; b += BS) wake[b] = INT_MAX;
     __syncthreads();
@@ -197,6 +200,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
     // earlier contact in the list woke one of its non-static bodies: iterate to
     // the fixed point of that rule.
     int more = __syncthreads_or(dormant);
+    if (timed) d.ktime[17] = __builtin_amdgcn_s_memtime();
     while (more) {
         int changed = 0;
         for (int i = tid; i < n; i += BS) {
@@ -217,6 +221,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
     for (int b = tid; b < nb; b += BS)
         if (wake[b] != INT_MAX) wake_body(bfl, bpos, b);
     __syncthreads();
+    if (timed) d.ktime[18] = __builtin_amdgcn_s_memtime();
 
     // Stable compaction of the surviving contacts into the other buffer.
     int m = 0, touching = 0;
@@ -240,10 +245,16 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
         touching += t2;
     }
 
+    if (timed) d.ktime[19] = __builtin_amdgcn_s_memtime();
     // Contact adjacency of each body in list order (newest contact first),
-    // the order of b2Body.m_contactList.
-    int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
-    int2 *adj = d.adjList + 2 * wd.contactBase;
+    // the order of b2Body.m_contactList: counted, scattered and sorted per body
+    // in LDS when the world's lists fit there, else in place in global memory.
+    int *gOff = d.adjOff + wd.bodyBase, *gCnt = d.adjCnt + wd.bodyBase;
+    int2 *gAdj = d.adjList + 2 * wd.contactBase;
+    const bool inLds = 8 * (size_t)nb + 16 * (size_t)m <= (size_t)ldsBytes;
+    int *adjOff = inLds ? (int *)lds : gOff;
+    int *adjCnt = inLds ? (int *)lds + nb : gCnt;
+    int2 *adj = inLds ? (int2 *)(lds + 8 * (size_t)nb) : gAdj;
     for (int b = tid; b < nb; b += BS) adjCnt[b] = 0;
     __syncthreads();
     for (int k = tid; k < m; k += BS) {
@@ -262,6 +273,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
         adj[adjOff[bb] + atomicAdd(&adjCnt[bb], 1)] = make_int2(2 * k + 1, ba);
     }
     __syncthreads();
+    if (timed) d.ktime[20] = __builtin_amdgcn_s_memtime();
     for (int b = tid; b < nb; b += BS) {
         int o = adjOff[b], cnt = adjCnt[b];
         for (int i = 1; i < cnt; ++i) {
@@ -274,10 +286,19 @@ __global__ void k_collide(Dev d, int cur, int worldOffset) {
             adj[o + j + 1] = v;
         }
     }
+    if (inLds) {
+        __syncthreads();
+        for (int b = tid; b < nb; b += BS) {
+            gOff[b] = adjOff[b];
+            gCnt[b] = adjCnt[b];
+        }
+        for (int e = tid; e < 2 * m; e += BS) gAdj[e] = adj[e];
+    }
     if (tid == 0) {
         d.S[w].contactCount = m;
         d.S[w].touching = touching;
     }
+    if (timed) d.ktime[21] = __builtin_amdgcn_s_memtime();
 }
 
 // ---------------------------------------------------------------------------
@@ -1718,7 +1739,10 @@ __device__ __forceinline__ unsigned int f2key(float f) {
 }
 
 // Returns the pair count; pairs are written only when they fit in cap.
-__device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, int n, BpMem M, int2 *out, int cap, int *s_tmp) {
+// Pairs are emitted and sorted in `stage` (LDS, stageCap pairs) when they fit
+// there, and copied to `out` afterwards; otherwise they are sorted in `out`.
+__device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, int n, BpMem M, int2 *out, int cap,
+                                        int2 *stage, int stageCap, int *s_tmp) {
     const int tid = threadIdx.x, BS = blockDim.x;
     __shared__ int s_ext[4];
     if (tid < 4) s_ext[tid] = (tid < 2) ? f2ord(kMaxFloat) : f2ord(-kMaxFloat);
@@ -1769,6 +1793,7 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
         }
     }
     // count, scan, emit
+    int2 *dstp = out;
     for (int pass = 0; pass < 2; ++pass) {
         for (int p = tid; p < n; p += BS) {
             int i = (int)(M.keys[p] & 0xffffffffu);
@@ -1783,7 +1808,7 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
                 if ((mi || M.moved[j]) && overlap(fi, fj)) {
                     int a = i < j ? i : j, b = i < j ? j : i;
                     if (pass == 0) atomicAdd(&M.cnt[a], 1);
-                    else out[M.cur[a] + atomicAdd(&M.cnt[a], 1)] = make_int2(a, b);
+                    else dstp[M.cur[a] + atomicAdd(&M.cnt[a], 1)] = make_int2(a, b);
                 }
             }
         }
@@ -1793,25 +1818,29 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
             __syncthreads();
             int total = block_scan_array(M.cur, n, s_tmp);
             if (total > cap) return total;
+            if (stage && total <= stageCap) dstp = stage;
             for (int i = tid; i < n; i += BS) M.cnt[i] = 0;
             __syncthreads();
         }
     }
-    int total = 0;
     for (int a = tid; a < n; a += BS) {
         int o = M.cur[a], c = M.cnt[a];
         for (int i = 1; i < c; ++i) {
-            int2 v = out[o + i];
+            int2 v = dstp[o + i];
             int j = i - 1;
-            while (j >= 0 && out[o + j].y > v.y) {
-                out[o + j + 1] = out[o + j];
+            while (j >= 0 && dstp[o + j].y > v.y) {
+                dstp[o + j + 1] = dstp[o + j];
                 --j;
             }
-            out[o + j + 1] = v;
+            dstp[o + j + 1] = v;
         }
     }
-    if (n > 0) total = M.cur[n - 1] + M.cnt[n - 1];
+    const int total = n > 0 ? M.cur[n - 1] + M.cnt[n - 1] : 0;
     __syncthreads();
+    if (dstp != out) {
+        for (int e = tid; e < total; e += BS) out[e] = dstp[e];
+        __syncthreads();
+    }
     return total;
 }
 
@@ -1835,12 +1864,18 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
     const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
     const int2 *adj = d.adjList + 2 * wd.contactBase;
     const int tid = threadIdx.x, BS = blockDim.x;
+    const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && tid == 0;
+    if (timed) d.ktime[22] = __builtin_amdgcn_s_memtime();
 
     int total;
-    if (bp_bytes(nf) <= (size_t)ldsBytes)
-        total = bp_pairs(fat, moved, nf, bp_carve(lds, nf), pairs, wd.pairCap, s_tmp);
+    const size_t need = bp_bytes(nf);
+    if (need <= (size_t)ldsBytes)
+        total = bp_pairs(fat, moved, nf, bp_carve(lds, nf), pairs, wd.pairCap, (int2 *)(lds + need),
+                         (int)(((size_t)ldsBytes - need) / sizeof(int2)), s_tmp);
     else
-        total = bp_pairs(fat, moved, nf, bp_carve(d.bpScratch + wd.bpScratchBase, nf), pairs, wd.pairCap, s_tmp);
+        total = bp_pairs(fat, moved, nf, bp_carve(d.bpScratch + wd.bpScratchBase, nf), pairs, wd.pairCap, (int2 *)lds,
+                         ldsBytes / (int)sizeof(int2), s_tmp);
+    if (timed) d.ktime[23] = __builtin_amdgcn_s_memtime();
     if (total > wd.pairCap) {
         if (tid == 0) {
             d.S[w].error |= 2;
@@ -1925,6 +1960,7 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
         d.S[w].contactCount = nc;
         d.S[w].pairCount = total;
     }
+    if (timed) d.ktime[24] = __builtin_amdgcn_s_memtime();
 }
 
 // ---------------------------------------------------------------------------
@@ -1932,7 +1968,7 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
 // ---------------------------------------------------------------------------
 __global__ void k_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total) {
     __shared__ int s_tmp[kMaxBlock / 64 + 1];
-    int t = bp_pairs(fat, moved, n, bp_carve(scratch, n), out, cap, s_tmp);
+    int t = bp_pairs(fat, moved, n, bp_carve(scratch, n), out, cap, nullptr, 0, s_tmp);
     if (threadIdx.x == 0) *total = t;
 }
 
@@ -2032,9 +2068,10 @@ __global__ void k_max_penetration(Dev d, int cur, float *out) {
         else hipLaunchKernelGGL(kernel<false>, grid, block, lds, s, __VA_ARGS__);                                     \
     } while (0)
 
-hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int block, bool lead, hipStream_t s) {
+hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
+                          hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    DBX_LAUNCH(k_collide, lead, dim3(nworlds), dim3(block), 0, s, d, cur, worldOffset);
+    DBX_LAUNCH(k_collide, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
 }
 size_t islands_scratch_bytes(int bodies, int contacts) { return isl_bytes(bodies, contacts); }
@@ -2049,6 +2086,9 @@ static hipError_t configure_tier(int maxDynamicLds) {
                                        maxDynamicLds);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)k_broadphase<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            maxDynamicLds - 1024);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_collide<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_solve_wave<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,

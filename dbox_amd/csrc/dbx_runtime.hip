@@ -756,7 +756,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
-    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 16));
+    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
     DBX_HIP(h, hipStreamSynchronize(h->stream));
     h->pending.clear();
     h->pending.shrink_to_fit();
@@ -786,7 +786,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     const Dev &d = h->d;
     if (findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
         DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s));
-    DBX_HIP(h, launch_collide(d, cin, T.first, T.count, T.block, lead, s));
+    DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s));
     const bool timed = lead && ev; // phase events: lead tier, eager steps only
     if (timed) (void)hipEventRecord(ev[1], s);
     if (sp.dt > 0.0f) {
@@ -1021,12 +1021,17 @@ dbx_status dbx_sync(dbx_handle *h) {
     }
     h->ringUsed = 0;
     if (h->d.ktime && getenv("DBX_KTIME")) {
-        long long t[16];
-        if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess)
+        long long t[32];
+        if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess) {
             fprintf(stderr, "dbx ktime (shader cycles) islands: stage %lld dfs %lld batch %lld [nc %lld nbatch %lld] | "
                             "solve: stage+init %lld warm %lld vel %lld pos %lld tail %lld\n",
                     t[1] - t[0], t[2] - t[1], t[3] - t[2], t[15], t[14], t[5] - t[4], t[6] - t[5], t[7] - t[6],
                     t[8] - t[7], t[9] - t[8]);
+            fprintf(stderr, "dbx ktime collide: narrowphase %lld dormant+wake %lld compact %lld adj-build %lld "
+                            "adj-sort %lld | broadphase: pairs %lld addpair %lld\n",
+                    t[17] - t[16], t[18] - t[17], t[19] - t[18], t[20] - t[19], t[21] - t[20], t[23] - t[22],
+                    t[24] - t[23]);
+        }
     }
     return DBX_OK;
 }
