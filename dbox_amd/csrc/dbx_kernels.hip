@@ -274,8 +274,10 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
     }
     __syncthreads();
     if (timed) d.ktime[20] = __builtin_amdgcn_s_memtime();
+    // (a static body's list is only searched, never walked in order: the
+    // island search does not expand static bodies)
     for (int b = tid; b < nb; b += BS) {
-        int o = adjOff[b], cnt = adjCnt[b];
+        int o = adjOff[b], cnt = body_type(bfl[b]) == kStatic ? 0 : adjCnt[b];
         for (int i = 1; i < cnt; ++i) {
             int2 v = adj[o + i];
             int j = i - 1;
@@ -1792,6 +1794,80 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
             __syncthreads();
         }
     }
+    // Load-balanced sweep: proxy p (in sorted order) is tested against the sorted
+    // proxies p+1 .. end(p)-1, where end(p) is the first whose lower bound
+    // exceeds p's upper bound (binary search); the candidate tests of all
+    // proxies are dealt to the threads evenly, so a proxy with a long extent
+    // along the axis (a container wall) does not serialise the sweep.  Found
+    // pairs are collected as (a << 32 | b) keys and sorted, which is the
+    // b2PairLessThan order.
+    if (stage) {
+        __shared__ int s_np;
+        if (tid == 0) s_np = 0;
+        for (int p = tid; p < n; p += BS) {
+            const float4 fi = M.fat[(int)(M.keys[p] & 0xffffffffu)];
+            const unsigned int hk = f2key(axis == 0 ? fi.z : fi.w);
+            int lo = p + 1, hi = n; // first q in [p+1, n) with key(q) > hk
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((unsigned int)(M.keys[mid] >> 32) > hk) hi = mid;
+                else lo = mid + 1;
+            }
+            M.cnt[p] = lo - p - 1;
+            M.cur[p] = lo - p - 1;
+        }
+        __syncthreads();
+        const int C = block_scan_array(M.cur, n, s_tmp);
+        unsigned long long *skeys = (unsigned long long *)stage;
+        for (int t = tid; t < C; t += BS) {
+            int lo = 0, hi = n; // last p with cur[p] <= t
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (M.cur[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const int p = lo, q = p + 1 + (t - M.cur[p]);
+            const int i = (int)(M.keys[p] & 0xffffffffu), j = (int)(M.keys[q] & 0xffffffffu);
+            if ((M.moved[i] || M.moved[j]) && overlap(M.fat[i], M.fat[j])) {
+                const int slot = atomicAdd(&s_np, 1);
+                const unsigned int a = i < j ? i : j, b = i < j ? j : i;
+                if (slot < stageCap) skeys[slot] = ((unsigned long long)a << 32) | b;
+            }
+        }
+        __syncthreads();
+        const int np = s_np;
+        __syncthreads();
+        if (np > cap) return np;
+        const int P2 = bp_pow2(np > 0 ? np : 1);
+        if (P2 <= stageCap) {
+            for (int i = np + tid; i < P2; i += BS) skeys[i] = ~0ull;
+            __syncthreads();
+            for (int k = 2; k <= P2; k <<= 1) {
+                for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                    for (int i = tid; i < P2; i += BS) {
+                        const int ixj = i ^ jj;
+                        if (ixj > i) {
+                            const unsigned long long x = skeys[i], y = skeys[ixj];
+                            if ((x > y) == ((i & k) == 0)) {
+                                skeys[i] = y;
+                                skeys[ixj] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int e = tid; e < np; e += BS) {
+                const unsigned long long key = skeys[e];
+                out[e] = make_int2((int)(key >> 32), (int)(key & 0xffffffffu));
+            }
+            __syncthreads();
+            return np;
+        }
+        // too many pairs for the staging space: per-proxy lists below
+        for (int i = tid; i < n; i += BS) M.cnt[i] = 0;
+        __syncthreads();
+    }
     // count, scan, emit
     int2 *dstp = out;
     for (int pass = 0; pass < 2; ++pass) {
@@ -1818,7 +1894,6 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
             __syncthreads();
             int total = block_scan_array(M.cur, n, s_tmp);
             if (total > cap) return total;
-            if (stage && total <= stageCap) dstp = stage;
             for (int i = tid; i < n; i += BS) M.cnt[i] = 0;
             __syncthreads();
         }
@@ -1837,10 +1912,6 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
     }
     const int total = n > 0 ? M.cur[n - 1] + M.cnt[n - 1] : 0;
     __syncthreads();
-    if (dstp != out) {
-        for (int e = tid; e < total; e += BS) out[e] = dstp[e];
-        __syncthreads();
-    }
     return total;
 }
 
@@ -1898,8 +1969,10 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
             fb = pr.y;
             int ba = fbody[fa], bb = fbody[fb];
             accept = ba != bb;
-            if (accept) { // does a contact already exist? (scan of bodyB's contact list)
-                int o = adjOff[bb], n = adjCnt[bb];
+            if (accept) { // does a contact already exist?  The reference scans bodyB's
+                          // contact list; a contact is on both lists, so the shorter one is scanned
+                const int eb = adjCnt[bb] <= adjCnt[ba] ? bb : ba;
+                int o = adjOff[eb], n = adjCnt[eb];
                 for (int e = 0; e < n; ++e) {
                     const Contact &c = cont[adj[o + e].x >> 1];
                     if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) {
