@@ -1463,7 +1463,16 @@ __device__ __forceinline__ void position_one(const PosCon &q, float4 *Lpos, int 
 __device__ __forceinline__ void position_level(const PosCon &q, bool have, int lo, int hi,
                                                const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
                                                const int *Lsol) {
-    if (have && !Lsol[q.island]) position_one(q, Lpos, Lmin);
+    // the bodies are read together with the island's solved flag (one LDS round
+    // trip instead of two); a lane without a constraint reads slot 0's bodies
+    float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
+    const int solved = Lsol[q.island];
+    if (have && !solved) {
+        const float ms = solve_position_core(q, pa, pb);
+        if (q.dynMask & 1) Lpos[q.indexA] = pa;
+        if (q.dynMask & 2) Lpos[q.indexB] = pb;
+        atomicMin(&Lmin[q.island], f2ord(ms));
+    }
     for (int c = lo + 64; c < hi; c += 64) {
         const int j2 = c + (int)(threadIdx.x & 63);
         if (j2 < hi) {

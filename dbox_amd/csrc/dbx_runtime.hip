@@ -943,7 +943,6 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     }
     sp.residualTol = h->residualTol;
     sp.maxIterations = h->maxIterations;
-    hipStream_t s = h->stream;
     if ((int)h->ring.size() < kStepEvents * (h->ringUsed + 1)) {
         for (int i = 0; i < kStepEvents; ++i) {
             hipEvent_t e;
