@@ -197,6 +197,13 @@ def main():
     log("lead-tier ms/step: " + ", ".join(f"{k} {v / max(nsteps, 1):.3f}" for k, v in kern_ms.items()) +
         f"; tiers {tier_rows.tolist()}")
 
+    # HBM bytes per launch of the lead-tier solver from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # summarised in profiles/ (counters cannot be read from inside the timed run)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_solve_wave_lead_4096worlds_traffic.json")
+    if dom == "solve" and not args.coloured and not args.subset and len(worlds) == 4096 and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes"]
     graph = None
     if args.graph_steps > 0:
         h.set_graph(True)
@@ -250,7 +257,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes": abytes,
             "avg_kernel_ms": avg_ms,
         },
