@@ -103,48 +103,58 @@ __device__ __forceinline__ float normalize(float2 &v) {
 //   the quadrant k mod 4 picks the signs and the swap of sin and cos.
 // The float results equal the correctly rounded sin and cos except when the exact
 // value lies within a few double ulps of a float rounding boundary.
+//
+// Each Horner step is one v_fma_f64 whose addend is the coefficient held in a
+// register: written as fma(), the compiler turns the step into the two-operand
+// v_fmac_f64 and copies the loop-invariant coefficient before every step.  The
+// instruction computes exactly what fma() computes.
+__device__ __forceinline__ double fma_v(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 __device__ __forceinline__ Rot rot(float angle) {
-    double x = (double)angle;
-    double s, c;
-    if (fabs(x) < 1.0e6) {
-        const double P1 = 0x1.921fb54400000p+0, P2 = 0x1.0b4611a600000p-34, P3 = 0x1.3198a2e037073p-69;
-        double k = rint(x * 0.6366197723675814);
-        double r = fma(-k, P1, x);
-        r = fma(-k, P2, r);
-        r = fma(-k, P3, r);
-        double z = r * r;
-        double ps = -7.647163731819816e-13;
-        ps = fma(ps, z, 1.6059043836821613e-10);
-        ps = fma(ps, z, -2.505210838544172e-08);
-        ps = fma(ps, z, 2.7557319223985893e-06);
-        ps = fma(ps, z, -0.0001984126984126984);
-        ps = fma(ps, z, 0.008333333333333333);
-        ps = fma(ps, z, -0.16666666666666666);
-        ps = fma(ps * z, r, r);
-        double pc = 4.779477332387385e-14;
-        pc = fma(pc, z, -1.1470745597729725e-11);
-        pc = fma(pc, z, 2.08767569878681e-09);
-        pc = fma(pc, z, -2.755731922398589e-07);
-        pc = fma(pc, z, 2.48015873015873e-05);
-        pc = fma(pc, z, -0.001388888888888889);
-        pc = fma(pc, z, 0.041666666666666664);
-        pc = fma(pc, z, -0.5);
-        pc = fma(pc, z, 1.0);
-        // quadrant q: sin = (ps, pc, -ps, -pc)[q], cos = (pc, -ps, -pc, ps)[q]; rounding to
-        // float commutes with negation, so the swap and signs are applied to the floats
-        const int q = (int)(long long)k & 3;
-        const float fs = (float)ps, fc = (float)pc;
-        Rot qr;
-        qr.s = (q & 1) ? fc : fs;
-        qr.c = (q & 1) ? fs : fc;
-        qr.s = (q & 2) ? -qr.s : qr.s;
-        qr.c = ((q + 1) & 2) ? -qr.c : qr.c;
-        return qr;
-    }
-    sincos(x, &s, &c);
+    const double x = (double)angle;
+    const double P1 = 0x1.921fb54400000p+0, P2 = 0x1.0b4611a600000p-34, P3 = 0x1.3198a2e037073p-69;
+    const double k = rint(x * 0.6366197723675814);
+    double r = fma(-k, P1, x);
+    r = fma(-k, P2, r);
+    r = fma(-k, P3, r);
+    const double z = r * r;
+    double ps = fma_v(-7.647163731819816e-13, z, 1.6059043836821613e-10);
+    ps = fma_v(ps, z, -2.505210838544172e-08);
+    ps = fma_v(ps, z, 2.7557319223985893e-06);
+    ps = fma_v(ps, z, -0.0001984126984126984);
+    ps = fma_v(ps, z, 0.008333333333333333);
+    ps = fma_v(ps, z, -0.16666666666666666);
+    ps = fma(ps * z, r, r);
+    double pc = fma_v(4.779477332387385e-14, z, -1.1470745597729725e-11);
+    pc = fma_v(pc, z, 2.08767569878681e-09);
+    pc = fma_v(pc, z, -2.755731922398589e-07);
+    pc = fma_v(pc, z, 2.48015873015873e-05);
+    pc = fma_v(pc, z, -0.001388888888888889);
+    pc = fma_v(pc, z, 0.041666666666666664);
+    pc = fma(pc, z, -0.5);
+    pc = fma(pc, z, 1.0);
+    // quadrant q: sin = (ps, pc, -ps, -pc)[q], cos = (pc, -ps, -pc, ps)[q]; rounding to
+    // float commutes with negation, so the swap and signs are applied to the floats,
+    // the signs as a flip of the sign bit.  |k| < 2^20 wherever this result is kept,
+    // so k converts to int32 exactly.
+    const unsigned q = (unsigned)__double2int_rn(k);
+    const float fs = (float)ps, fc = (float)pc;
+    const bool odd = (q & 1u) != 0u;
     Rot qr;
-    qr.s = (float)s;
-    qr.c = (float)c;
+    qr.s = __uint_as_float(__float_as_uint(odd ? fc : fs) ^ ((q & 2u) << 30));
+    qr.c = __uint_as_float(__float_as_uint(odd ? fs : fc) ^ (((q + 1u) & 2u) << 30));
+    // |angle| >= 1e6 (or not finite): libm's sincos.  The branch is taken by the
+    // whole wave or not at all, so the common path carries no divergent branch.
+    const bool big = !(fabs(x) < 1.0e6);
+    if (__ballot(big) != 0ull) {
+        double s, c;
+        sincos(x, &s, &c);
+        qr.s = big ? (float)s : qr.s;
+        qr.c = big ? (float)c : qr.c;
+    }
     return qr;
 }
 __device__ __forceinline__ float2 mul(Rot q, float2 v) { return v2(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }

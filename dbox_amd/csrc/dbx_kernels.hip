@@ -835,20 +835,20 @@ __device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &im
         bx -= v.K.x * nI0 + v.K.z * nI1;
         by -= v.K.y * nI0 + v.K.w * nI1;
         // case 1: both points active
-        const float x1x = -(v.normalMass.x * bx + v.normalMass.z * by);
-        const float x1y = -(v.normalMass.y * bx + v.normalMass.w * by);
-        const bool c1 = x1x >= 0.0f && x1y >= 0.0f;
-        // case 2: point 1 only
-        const float x2x = -v.nMass0 * bx;
-        const bool c2 = x2x >= 0.0f && v.K.y * x2x + by >= 0.0f;
-        // case 3: point 2 only
-        const float x3y = -v.nMass1 * by;
-        const bool c3 = x3y >= 0.0f && v.K.z * x3y + bx >= 0.0f;
-        // case 4: neither
-        const bool c4 = bx >= 0.0f && by >= 0.0f;
-        if (c1 || c2 || c3 || c4) {
-            const float xx = c1 ? x1x : (c2 ? x2x : 0.0f);
-            const float xy = c1 ? x1y : (c2 ? 0.0f : (c3 ? x3y : 0.0f));
+        float xx = -(v.normalMass.x * bx + v.normalMass.z * by);
+        float xy = -(v.normalMass.y * bx + v.normalMass.w * by);
+        bool ok = xx >= 0.0f && xy >= 0.0f;
+        if (!ok) { // cases 2-4, evaluated only by waves with a lane that needs them
+            const float x2x = -v.nMass0 * bx; // case 2: point 1 only
+            const float x3y = -v.nMass1 * by; // case 3: point 2 only
+            const bool c2 = x2x >= 0.0f && v.K.y * x2x + by >= 0.0f;
+            const bool c3 = x3y >= 0.0f && v.K.z * x3y + bx >= 0.0f;
+            const bool c4 = bx >= 0.0f && by >= 0.0f; // case 4: neither
+            ok = c2 || c3 || c4;
+            xx = c2 ? x2x : 0.0f;
+            xy = c2 ? 0.0f : (c3 ? x3y : 0.0f);
+        }
+        if (ok) {
             const float2 P1 = scl(xx - nI0, normal);
             const float2 P2 = scl(xy - nI1, normal);
             vA = sub(vA, scl(mA, add(P1, P2)));
@@ -914,11 +914,9 @@ __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, cons
     return separation;
 }
 
-__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
-
-// The reference rebuilds both transforms before every point (b2contactsolver.d:101-104);
-// when point 0 left both bodies' positions bit-identical, point 1's transforms are
-// the ones already computed, so they are reused.
+// The reference rebuilds both transforms before every point (b2contactsolver.d:101-104).
+// Point 0 almost always moves a body, so point 1's transforms are rebuilt without
+// first testing whether they changed.
 __device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb) {
     float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
     float aA = pa.z, aB = pb.z;
@@ -926,12 +924,8 @@ __device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa
     Xf xfB = xf_from(cB, aB, p.lcB);
     float minSep = fminr(0.0f, position_point(p, p.lp0, xfA, xfB, cA, aA, cB, aB));
     if (p.pointCount > 1) {
-        const bool moved = !(same_bits(cA.x, pa.x) && same_bits(cA.y, pa.y) && same_bits(aA, pa.z) &&
-                             same_bits(cB.x, pb.x) && same_bits(cB.y, pb.y) && same_bits(aB, pb.z));
-        if (moved) {
-            xfA = xf_from(cA, aA, p.lcA);
-            xfB = xf_from(cB, aB, p.lcB);
-        }
+        xfA = xf_from(cA, aA, p.lcA);
+        xfB = xf_from(cB, aB, p.lcB);
         minSep = fminr(minSep, position_point(p, p.lp1, xfA, xfB, cA, aA, cB, aB));
     }
     pa = make_float4(cA.x, cA.y, aA, pa.w);
@@ -1368,14 +1362,21 @@ __host__ __device__ inline size_t solve_lds_bytes(int nb, int nc, int nbatch, in
 
 // One constraint of a level, on LDS body state.  kMode 0 = warm start,
 // 1 = velocity iteration, 2 = velocity iteration that also returns the residual.
+// Both bodies are written back whatever their type, as b2ContactSolver does
+// (b2contactsolver.d:766-769): a static or kinematic body has zero inverse mass
+// and inertia, so v - 0 * P leaves its velocity unchanged, and constraints of one
+// level that share such a body write the same value.
 template <int kMode>
 __device__ __forceinline__ void velocity_one(const VelCon &v, int k, float4 *Lvel, float4 *Limp, float &lres) {
     float4 va = Lvel[v.indexA], vb = Lvel[v.indexB], im = Limp[k];
+    // the three reads are issued back to back (one LDS round trip): the empty asm
+    // uses a value of each before any arithmetic is scheduled
+    asm volatile("" ::"v"(va.x), "v"(vb.x), "v"(im.x));
     if (kMode == 0) warm_start_core(v, im, va, vb);
     else lres = fmaxr(lres, solve_velocity_core<kMode == 2>(v, im, va, vb));
     if (kMode != 0) Limp[k] = im;
-    if (v.dynMask & 1) Lvel[v.indexA] = va;
-    if (v.dynMask & 2) Lvel[v.indexB] = vb;
+    Lvel[v.indexA] = va;
+    Lvel[v.indexB] = vb;
 }
 
 // Level offsets of the exact-order schedule, read through a window of 64
@@ -1452,11 +1453,13 @@ __device__ __forceinline__ float velocity_sweep(const int *Lbo, int nbatch, cons
     return lres;
 }
 
+// As for velocities, both bodies are written back (b2contactsolver.d:141-144);
+// zero inverse mass and inertia leave a static or kinematic body where it is.
 __device__ __forceinline__ void position_one(const PosCon &q, float4 *Lpos, int *Lmin) {
     float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
     const float ms = solve_position_core(q, pa, pb);
-    if (q.dynMask & 1) Lpos[q.indexA] = pa;
-    if (q.dynMask & 2) Lpos[q.indexB] = pb;
+    Lpos[q.indexA] = pa;
+    Lpos[q.indexB] = pb;
     atomicMin(&Lmin[q.island], f2ord(ms));
 }
 
@@ -1464,13 +1467,16 @@ __device__ __forceinline__ void position_level(const PosCon &q, bool have, int l
                                                const PosCon *__restrict__ pc, float4 *Lpos, int *Lmin,
                                                const int *Lsol) {
     // the bodies are read together with the island's solved flag (one LDS round
-    // trip instead of two); a lane without a constraint reads slot 0's bodies
+    // trip instead of two); a lane without a constraint reads slot 0's bodies.
+    // The empty asm uses the loaded values before the branch, so the compiler
+    // cannot move the body reads behind the test of the flag.
     float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
     const int solved = Lsol[q.island];
+    asm volatile("" ::"v"(pa.x), "v"(pb.x), "v"(solved));
     if (have && !solved) {
         const float ms = solve_position_core(q, pa, pb);
-        if (q.dynMask & 1) Lpos[q.indexA] = pa;
-        if (q.dynMask & 2) Lpos[q.indexB] = pb;
+        Lpos[q.indexA] = pa;
+        Lpos[q.indexB] = pb;
         atomicMin(&Lmin[q.island], f2ord(ms));
     }
     for (int c = lo + 64; c < hi; c += 64) {
