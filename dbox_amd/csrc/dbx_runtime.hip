@@ -433,6 +433,8 @@ struct dbx_handle {
     hipEvent_t fork = nullptr;      // fork event inside captures
     int tuneSteps = 0, tuneDone = 0, tunePer = 0; // autotuner progress
     std::vector<double> tuneMs;     // [tier][candidate] summed chain milliseconds
+    std::vector<double> tierMs;     // DBX_TIER_TIMES: summed chain milliseconds per tier
+    int tierSteps = 0;
     int ringUsed = 0;
     dbx_profile prof{};             // last step
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
@@ -963,15 +965,34 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
         const int c = h->tuneDone / h->tunePer;
         for (Tier &T : h->tiers) T.block = kTuneBlocks[c];
     }
+    // DBX_TIER_TIMES (diagnostics): time every tier's chain; each step then waits
+    // for its own end, so the steps no longer overlap the host
+    static const bool tierTimes = getenv("DBX_TIER_TIMES") != nullptr;
+    const bool timeTiers = tuning || (tierTimes && !h->cfg.use_graph);
+    if (timeTiers)
+        for (Tier &T : h->tiers) {
+            if (!T.t0) DBX_HIP(h, hipEventCreate(&T.t0));
+            if (!T.t1) DBX_HIP(h, hipEventCreate(&T.t1));
+        }
     dbx_status st = DBX_OK;
     if (h->cfg.use_graph && !tuning) {
         st = step_graph(h, sp, flags, findNew, ev);
         h->ringGraph[h->ringUsed] = 1;
     } else {
-        st = issue_step(h, sp, flags, findNew, ev, tuning);
+        st = issue_step(h, sp, flags, findNew, ev, timeTiers);
         h->ringGraph[h->ringUsed] = 0;
     }
     if (st != DBX_OK) return st;
+    if (timeTiers && !tuning) {
+        DBX_HIP(h, hipStreamSynchronize(h->stream));
+        h->tierMs.resize(h->tiers.size(), 0.0);
+        for (size_t t = 0; t < h->tiers.size(); ++t) {
+            float ms = 0.0f;
+            DBX_HIP(h, hipEventElapsedTime(&ms, h->tiers[t].t0, h->tiers[t].t1));
+            h->tierMs[t] += ms;
+        }
+        h->tierSteps++;
+    }
     h->ringUsed++;
     h->newFixture = false;
     h->cur ^= 1;
@@ -1019,6 +1040,13 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->sumSteps++;
     }
     h->ringUsed = 0;
+    if (h->tierSteps > 0) {
+        fprintf(stderr, "dbx tier chain ms/step over %d steps:", h->tierSteps);
+        for (size_t t = 0; t < h->tierMs.size(); ++t) fprintf(stderr, " [%zu] %.3f", t, h->tierMs[t] / h->tierSteps);
+        fprintf(stderr, "\n");
+        h->tierMs.assign(h->tierMs.size(), 0.0);
+        h->tierSteps = 0;
+    }
     if (h->d.ktime && getenv("DBX_KTIME")) {
         long long t[32];
         if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess) {
