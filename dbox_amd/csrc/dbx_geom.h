@@ -104,14 +104,35 @@ __device__ __forceinline__ float normalize(float2 &v) {
 // The float results equal the correctly rounded sin and cos except when the exact
 // value lies within a few double ulps of a float rounding boundary.
 //
-// Each Horner step is one v_fma_f64 whose addend is the coefficient held in a
-// register: written as fma(), the compiler turns the step into the two-operand
-// v_fmac_f64 and copies the loop-invariant coefficient before every step.  The
-// instruction computes exactly what fma() computes.
-__device__ __forceinline__ double fma_v(double a, double b, double c) {
-    double r;
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
+// Both Horner chains are written out as one block of v_fma_f64 instructions, the
+// two chains interleaved: written as fma(), the compiler turns each step into the
+// two-operand v_fmac_f64 and copies the loop-invariant coefficient before it, and
+// separate asm statements get a wait state between them.  Each instruction
+// computes exactly what the corresponding fma() computes, in the same order.
+__device__ __forceinline__ void sincos_poly(double z, double &ps, double &pc) {
+    const double s0 = -7.647163731819816e-13, s1 = 1.6059043836821613e-10, s2 = -2.505210838544172e-08,
+                 s3 = 2.7557319223985893e-06, s4 = -0.0001984126984126984, s5 = 0.008333333333333333,
+                 s6 = -0.16666666666666666;
+    const double c0 = 4.779477332387385e-14, c1 = -1.1470745597729725e-11, c2 = 2.08767569878681e-09,
+                 c3 = -2.755731922398589e-07, c4 = 2.48015873015873e-05, c5 = -0.001388888888888889,
+                 c6 = 0.041666666666666664;
+    asm("v_fma_f64 %0, %3, %2, %4\n\t"
+        "v_fma_f64 %1, %10, %2, %11\n\t"
+        "v_fma_f64 %0, %0, %2, %5\n\t"
+        "v_fma_f64 %1, %1, %2, %12\n\t"
+        "v_fma_f64 %0, %0, %2, %6\n\t"
+        "v_fma_f64 %1, %1, %2, %13\n\t"
+        "v_fma_f64 %0, %0, %2, %7\n\t"
+        "v_fma_f64 %1, %1, %2, %14\n\t"
+        "v_fma_f64 %0, %0, %2, %8\n\t"
+        "v_fma_f64 %1, %1, %2, %15\n\t"
+        "v_fma_f64 %0, %0, %2, %9\n\t"
+        "v_fma_f64 %1, %1, %2, %16\n\t"
+        "v_fma_f64 %1, %1, %2, -0.5\n\t"
+        "v_fma_f64 %1, %1, %2, 1.0"
+        : "=&v"(ps), "=&v"(pc)
+        : "v"(z), "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(s4), "v"(s5), "v"(s6), "v"(c0), "v"(c1), "v"(c2),
+          "v"(c3), "v"(c4), "v"(c5), "v"(c6));
 }
 __device__ __forceinline__ Rot rot(float angle) {
     const double x = (double)angle;
@@ -121,21 +142,9 @@ __device__ __forceinline__ Rot rot(float angle) {
     r = fma(-k, P2, r);
     r = fma(-k, P3, r);
     const double z = r * r;
-    double ps = fma_v(-7.647163731819816e-13, z, 1.6059043836821613e-10);
-    ps = fma_v(ps, z, -2.505210838544172e-08);
-    ps = fma_v(ps, z, 2.7557319223985893e-06);
-    ps = fma_v(ps, z, -0.0001984126984126984);
-    ps = fma_v(ps, z, 0.008333333333333333);
-    ps = fma_v(ps, z, -0.16666666666666666);
+    double ps, pc;
+    sincos_poly(z, ps, pc); // ps = s0..s6 in z; pc = c0..c6, -1/2, 1 in z
     ps = fma(ps * z, r, r);
-    double pc = fma_v(4.779477332387385e-14, z, -1.1470745597729725e-11);
-    pc = fma_v(pc, z, 2.08767569878681e-09);
-    pc = fma_v(pc, z, -2.755731922398589e-07);
-    pc = fma_v(pc, z, 2.48015873015873e-05);
-    pc = fma_v(pc, z, -0.001388888888888889);
-    pc = fma_v(pc, z, 0.041666666666666664);
-    pc = fma(pc, z, -0.5);
-    pc = fma(pc, z, 1.0);
     // quadrant q: sin = (ps, pc, -ps, -pc)[q], cos = (pc, -ps, -pc, ps)[q]; rounding to
     // float commutes with negation, so the swap and signs are applied to the floats,
     // the signs as a flip of the sign bit.  |k| < 2^20 wherever this result is kept,
