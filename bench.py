@@ -7,7 +7,8 @@ U containers, stepped together at dt = 1/60 with 8 velocity and 3 position
 iterations, warm starting and sleeping on, continuous physics off.
 
 Prints one JSON line.  value = dynamic bodies (awake or asleep) x steps / wall
-seconds over the timed steps, with the worlds already resident in HBM.
+seconds over steps 1..K of freshly built worlds (after a separate warm-up run
+that also picks the workgroup sizes), with the worlds already resident in HBM.
 """
 from __future__ import annotations
 
@@ -60,47 +61,42 @@ def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int,
     return c * 12 + nbodies * 8
 
 
-def run_cpu_baseline(worlds, warmup, steps, threads, sample_every):
+def run_cpu_baseline(worlds, steps, threads, sample_every):
+    """Steps 1..steps of every sample_every-th world on the oracle (C++ restatement
+    of dbox), single-threaded and on a pool of threads, one world per task."""
     from oracle import pyoracle as po
     sample = worlds[::sample_every]
-    ows = [po.OracleWorld(s) for s in sample]
-    dyn = sum(w.dynamic_count for w in ows)
+    dyn = sum(s.dynamic_count for s in sample)
 
     def run(ws, n):
         for _ in range(n):
             for w in ws:
                 w.step(DT, VEL, POS)
 
-    def timed(nthreads):
-        parts = [ows[i::nthreads] for i in range(nthreads)]
-        ts = [threading.Thread(target=run, args=(p, warmup)) for p in parts]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
-        t0 = time.perf_counter()
-        ts = [threading.Thread(target=run, args=(p, steps)) for p in parts]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
-        return time.perf_counter() - t0
-
-    # single thread on a fresh copy, then all threads on the sample itself
     single = [po.OracleWorld(s) for s in sample]
-    run(single, warmup)
     t0 = time.perf_counter()
     run(single, steps)
     t1 = time.perf_counter() - t0
-    tn = timed(threads)
+
+    ows = [po.OracleWorld(s) for s in sample]
+    parts = [ows[i::threads] for i in range(threads)]
+    ts = [threading.Thread(target=run, args=(p, steps)) for p in parts]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    tn = time.perf_counter() - t0
     return dyn, t1, tn, len(sample)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=600,
+                    help="timed steps, from step 1 of freshly built worlds")
+    ap.add_argument("--warmup", type=int, default=30,
+                    help="steps of the separate warm-up run (which also autotunes the workgroup sizes)")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--coloured", action="store_true",
                     help="time the greedy-coloured solver instead of the reference's exact Gauss-Seidel order")
@@ -108,8 +104,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--graph-steps", type=int, default=100,
-                    help="after the timed eager steps, time this many more steps replayed as hipGraphs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the second timed run whose steps are replayed from captured hipGraphs")
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 20 device-to-device copies of this many MiB (HBM counter calibration) and exit")
@@ -136,10 +132,13 @@ def main():
         by_size = sorted(range(len(worlds)), key=lambda i: len(worlds[i].bodies), reverse=(kind == "largest"))
         worlds = [worlds[i] for i in sorted(by_size[: int(n)])]
         log(f"subset {args.subset}: bodies per world {[len(w.bodies) for w in worlds][:8]}...")
-    h = native.StepWorlds()
-    for s in worlds:
-        h.add_scene(s)
-    h.finalize()
+    def build():
+        handle = native.StepWorlds()
+        for sc in worlds:
+            handle.add_scene(sc)
+        handle.finalize()
+        return handle
+
     ndyn = sum(s.dynamic_count for s in worlds)
     nbodies = sum(len(s.bodies) for s in worlds)
     nprox = sum(len(s.fixtures) for s in worlds)
@@ -153,10 +152,20 @@ def main():
     except Exception:  # torch is only used for an extra device barrier
         have_torch = False
 
-    # warm-up; its first 5 x 2 steps let the autotuner pick each tier's workgroup size
+    # Warm-up run on its own copy of the worlds; its first 5 x 2 steps let the
+    # autotuner pick each tier's workgroup size, which the timed run reuses.
+    h = build()
     if not args.no_autotune and args.warmup >= 10:
         h.autotune(2)
-    h.step(DT, VEL, POS, flags, n=args.warmup)
+    if args.warmup > 0:
+        h.step(DT, VEL, POS, flags, n=args.warmup)
+    h.sync()
+    blocks = [int(r[3]) for r in h.tiers()[0]]
+    h.close()
+
+    # Timed run: steps 1..K of freshly built worlds.
+    h = build()
+    h.set_tier_blocks(blocks)
     h.sync()
     if have_torch:
         torch.cuda.synchronize()
@@ -205,21 +214,24 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes"]
     graph = None
-    if args.graph_steps > 0:
+    if not args.no_graph:
+        # the same steps 1..K once more on fresh worlds, each tier's step replayed
+        # from a captured hipGraph (captured during steps 1 and 2)
+        h.close()
+        h = build()
+        h.set_tier_blocks(blocks)
         h.set_graph(True)
-        h.step(DT, VEL, POS, flags, n=4)  # capture both buffer parities
         h.sync()
         if have_torch:
             torch.cuda.synchronize()
         t0 = time.perf_counter()
-        h.step(DT, VEL, POS, flags, n=args.graph_steps)
+        h.step(DT, VEL, POS, flags, n=args.steps)
         h.sync()
         if have_torch:
             torch.cuda.synchronize()
         tg = time.perf_counter() - t0
-        h.set_graph(False)
-        graph = {"value": ndyn * args.graph_steps / tg, "ms_per_step": 1e3 * tg / args.graph_steps,
-                 "steps": f"{args.warmup + args.steps + 5}-{args.warmup + args.steps + 4 + args.graph_steps}"}
+        graph = {"value": ndyn * args.steps / tg, "ms_per_step": 1e3 * tg / args.steps,
+                 "steps": f"1-{args.steps}, capture included"}
     result = {
         "metric": "body-steps/s at 4096 worlds",
         "value": value,
@@ -268,15 +280,15 @@ def main():
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.warmup, args.steps, threads, args.cpu_sample_every)
+        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.steps, threads, args.cpu_sample_every)
         result["cpu_baseline"] = {
             "value": dyn * args.steps / tn,
             "unit": "body-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"every {args.cpu_sample_every}th world ({nsample} worlds, {dyn} dynamic bodies), same "
-                      f"{args.warmup} warm-up + {args.steps} timed steps, C++ restatement of dbox -O2 "
-                      "-ffp-contract=off, one world per task on a thread pool",
+            "sample": f"every {args.cpu_sample_every}th world ({nsample} worlds, {dyn} dynamic bodies), the same "
+                      f"steps 1-{args.steps}, C++ restatement of dbox -O2 -ffp-contract=off, one world per task on "
+                      "a thread pool",
             "single_thread_value": dyn * args.steps / t1,
         }
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
