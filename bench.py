@@ -90,6 +90,38 @@ def run_cpu_baseline(worlds, steps, threads, sample_every):
     return dyn, t1, tn, len(sample)
 
 
+def learned_vs_plain(n_eval: int, steps: int) -> dict:
+    """Velocity iterations to a 1e-3 m/s residual (cap 100), plain warm start vs the
+    learned one, on the first n_eval worlds of the 16,384-world stack set (tall
+    uneven stacks, unseen in training).  The network is trained here, on the GPU,
+    on impulses logged from plain-solver runs of other seeds (pyramids, stacks and
+    containers iterated to a 1e-5 residual)."""
+    from dbox_amd import warmstart
+    t0 = time.perf_counter()
+    h = warmstart.collect(warmstart.training_scenes(seed=11, pyramids=4, stacks=48, mixed=16), steps=150)
+    samples = h.mlp_log_count()
+    params, norm, loss = warmstart.train(h, steps=2000, batch=4096, lr=2e-3)
+    h.close()
+    test = scenes.stack_worlds(n_eval)
+    plain = warmstart.iterations(test, steps)
+    learned = warmstart.iterations(test, steps, params=params, norm=norm)
+
+    def row(st):
+        return {"mean_iterations": st.mean_iterations, "p95_iterations": st.p95_iterations,
+                "capped_fraction": st.capped_fraction, "max_penetration_m": st.max_penetration,
+                "world_steps": st.world_steps}
+
+    return {
+        "workload": f"first {n_eval} worlds of the 16,384 stack-world set, steps 1-{steps}",
+        "residual_tolerance": 1e-3, "max_iterations": 100,
+        "plain": row(plain), "learned": row(learned),
+        "iteration_reduction": 1.0 - learned.mean_iterations / max(plain.mean_iterations, 1e-9),
+        "training": {"samples": int(samples), "adam_steps": 2000, "batch": 4096,
+                     "loss_first_20": float(np.mean(loss[:20])), "loss_last_20": float(np.mean(loss[-20:]))},
+        "seconds": time.perf_counter() - t0,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +136,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain warm-start comparison (iterations to a fixed residual)")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the second timed run whose steps are replayed from captured hipGraphs")
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
@@ -233,7 +267,7 @@ def main():
         graph = {"value": ndyn * args.steps / tg, "ms_per_step": 1e3 * tg / args.steps,
                  "steps": f"1-{args.steps}, capture included"}
     result = {
-        "metric": "body-steps/s at 4096 worlds",
+        "metric": "body-steps/s at 4096 worlds; velocity iters to fixed residual, learned vs plain",
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
@@ -278,6 +312,10 @@ def main():
         "graph_replay": graph,
         "step_algorithmic_gbs": step_bytes / (elapsed / args.steps) / 1e9,
     }
+    if not args.no_learned and not args.subset:
+        h.close()
+        log("learned vs plain warm start")
+        result["learned_vs_plain"] = learned_vs_plain(256, 150)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.steps, threads, args.cpu_sample_every)
