@@ -729,6 +729,22 @@ __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, 
         v.tMass1 = 0.0f;
         v.bias1 = 0.0f;
     }
+    // A faceB manifold's reference face belongs to body B, and the position
+    // solver reads it from B's transform (b2contactsolver.d:849-861).  Its
+    // position constraint stores the bodies in the order (B, A), so the solver
+    // always finds the reference face on its first body; the radii keep their
+    // order (position_point).
+    if (type == kManFaceB) {
+        p.indexA = bb;
+        p.indexB = ba;
+        p.lcA = v2(mb.z, mb.w);
+        p.lcB = v2(ma.z, ma.w);
+        p.invMassA = mb.x;
+        p.invMassB = ma.x;
+        p.invIA = mb.y;
+        p.invIB = ma.y;
+        p.dynMask = ((v.dynMask & 1) << 1) | ((v.dynMask & 2) >> 1);
+    }
 }
 
 // The per-constraint cores below branch only on the point count (and the
@@ -868,43 +884,44 @@ __device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &im
 // b2ContactSolver.SolvePositionConstraints (b2contactsolver.d:73-149) with
 // b2PositionSolverManifold.Initialize (b2contactsolver.d:818-863) for one
 // constraint; pa, pb = (c.x, c.y, a, unused); returns the minimum separation seen.
-// For every manifold type the point pair is (P1, P2) = (X1 * localPoint,
-// X2 * pointLocal[j]) with X1 = xfA, X2 = xfB for circles and faceA and the
-// other way round for faceB, and separation = dot(P2 - P1, normal) - rA - rB;
-// only the normal (normalised P2 - P1 for circles, X1.q * localNormal for the
-// faces) and the contact point (midpoint for circles, P2 for the faces) differ.
+// For every manifold type the point pair is (P1, P2) = (xfA * localPoint,
+// xfB * pointLocal[j]) and separation = dot(P2 - P1, normal) - radiusA - radiusB:
+// a faceB constraint has its bodies stored as (B, A) (init_constraint), so its
+// reference face is on the first body here too.  Only the normal (normalised
+// P2 - P1 for circles, xfA.q * localNormal for the faces) and the contact point
+// (midpoint for circles, P2 for the faces) differ.
+//
+// For faceB the reference negates the normal and applies the impulse to (A, B);
+// with the bodies swapped and the normal kept, every update is the same number
+// (-(x * y) == x * (-y) and a - (-b) == a + b exactly), except that the two
+// rotational terms of K = mA + mB + iA rnA^2 + iB rnB^2 are added in the
+// reference's order, which is the swapped one.
 __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, const Xf &xfA, const Xf &xfB, float2 &cA,
                                                 float &aA, float2 &cB, float &aB) {
     const float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
-    const bool circles = p.type == kManCircles, faceB = p.type == kManFaceB;
-    Xf X1, X2;
-    X1.p = sel(faceB, xfB.p, xfA.p);
-    X1.q.s = faceB ? xfB.q.s : xfA.q.s;
-    X1.q.c = faceB ? xfB.q.c : xfA.q.c;
-    X2.p = sel(faceB, xfA.p, xfB.p);
-    X2.q.s = faceB ? xfA.q.s : xfB.q.s;
-    X2.q.c = faceB ? xfA.q.c : xfB.q.c;
-    const float2 P1 = mul(X1, p.localPoint);
-    const float2 P2 = mul(X2, lp);
+    const bool circles = p.type == kManCircles, flip = p.type == kManFaceB;
+    const float2 P1 = mul(xfA, p.localPoint);
+    const float2 P2 = mul(xfB, lp);
     const float2 d = sub(P2, P1);
+    float2 normal = mul(xfA.q, p.localNormal);
+    float2 point = P2;
     // b2Vec2.Normalize (b2math.d:162-175) of d for circles; skipped by waves
     // without circle manifolds (the square root and division are the longest
     // part of the point)
-    float2 normal = mul(X1.q, p.localNormal);
     if (circles) {
         const float len = sqrtf(d.x * d.x + d.y * d.y);
         const float inv = 1.0f / len;
         normal = len < kEps ? d : v2(d.x * inv, d.y * inv);
+        point = scl(0.5f, add(P1, P2));
     }
     const float separation = dot(d, normal) - p.radiusA - p.radiusB;
-    const float2 point = sel(circles, scl(0.5f, add(P1, P2)), P2);
-    normal = sel(faceB, neg(normal), normal);
     const float2 rA = sub(point, cA);
     const float2 rB = sub(point, cB);
     const float C = clampr(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
     const float rnA = crs(rA, normal);
     const float rnB = crs(rB, normal);
-    const float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+    const float tA = iA * rnA * rnA, tB = iB * rnB * rnB;
+    const float K = mA + mB + (flip ? tB : tA) + (flip ? tA : tB);
     const float impulse = K > 0.0f ? -C / K : 0.0f;
     const float2 P = scl(impulse, normal);
     cA = sub(cA, scl(mA, P));
@@ -950,7 +967,9 @@ struct ImpRef { // accumulated impulses (nImp0, tImp0, nImp1, tImp1) of constrai
 // The 14 features of point j of a constraint: relative normal and tangent
 // velocity, normal, separation, inverse masses and inertias, the summed matched
 // normal impulses on body A's and on body B's contacts and their number, friction
-// and the point's normal mass.
+// and the point's normal mass.  va, vb are the velocities of the velocity
+// constraint's bodies; pa, pb the positions of the position constraint's (which
+// are swapped for a faceB manifold).
 __device__ void mlp_features(float *f, const VelCon &v, const PosCon &q, int j, float4 va, float4 vb, float4 pa,
                              float4 pb, float2 nbA, float2 nbB) {
     const float2 rA = sel(j != 0, v.rA1, v.rA0), rB = sel(j != 0, v.rB1, v.rB0);
@@ -1013,7 +1032,7 @@ __device__ __forceinline__ void mlp_new_points(const Dev &d, int w, int nc, uint
             if (!(fl & (j ? kContactNewPt1 : kContactNewPt0))) continue;
             const int row = atomicAdd(&d.mlpNew[w], 1); // at most two rows per constraint
             float f[kMlpFeatStride];
-            mlp_features(f, v, pc[k], j, vel[v.indexA], vel[v.indexB], pos[v.indexA], pos[v.indexB], nbs[v.indexA],
+            mlp_features(f, v, pc[k], j, vel[v.indexA], vel[v.indexB], pos[pc[k].indexA], pos[pc[k].indexB], nbs[v.indexA],
                          nbs[v.indexB]);
             float *dst = feat + (size_t)row * kMlpFeatStride;
             for (int i = 0; i < kMlpFeatStride; ++i) dst[i] = f[i];
