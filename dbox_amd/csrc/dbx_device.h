@@ -43,6 +43,7 @@ enum : uint32_t {
     kContactEnded = 0x800,     // stopped touching this step (EndContact)
     kContactNewPt0 = 0x1000,   // manifold point 0 has no match in the previous manifold
     kContactNewPt1 = 0x2000,
+    kContactSolvedOne = 0x4000, // a two-point manifold solved as one point (block-solver downgrade)
 };
 
 struct WorldDesc {
@@ -66,7 +67,8 @@ struct WorldState {
     int iterations;   // velocity iterations used by the last solve
     int constraints;  // constraints solved by the last step
     float residual;   // velocity residual after the last iteration (RESIDUAL mode), else -1
-    int pad[3];
+    int collideCount; // contacts in the list the last Collide walked (its event flags stay there)
+    int pad[2];
 };
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).

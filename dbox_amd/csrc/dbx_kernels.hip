@@ -179,7 +179,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
     __syncthreads();
 
     const uint32_t transient = kContactDestroyed | kContactDormant | kContactBegan | kContactEnded | kContactIsland |
-                               kContactNewPt0 | kContactNewPt1;
+                               kContactNewPt0 | kContactNewPt1 | kContactSolvedOne;
     int dormant = 0;
     for (int i = tid; i < n; i += BS) {
         Contact c = src[i];
@@ -299,6 +299,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
     if (tid == 0) {
         d.S[w].contactCount = m;
         d.S[w].touching = touching;
+        d.S[w].collideCount = n;
     }
     if (timed) d.ktime[21] = __builtin_amdgcn_s_memtime();
 }
@@ -1238,6 +1239,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         if (v.pointCount > 1) {
             c->pt1.z = v.nImp1;
             c->pt1.w = v.tImp1;
+        } else if (man_count(c->meta) > 1) {
+            c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
         }
     }
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
@@ -1656,6 +1659,8 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         if (v.pointCount > 1) {
             c->pt1.z = im.z;
             c->pt1.w = im.w;
+        } else if (man_count(c->meta) > 1) {
+            c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
         }
     }
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).

@@ -1180,6 +1180,61 @@ dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *
     return DBX_OK;
 }
 
+dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out, int32_t cap, int32_t *count) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    if (!count) return fail(h, DBX_E_INVALID_ARG, "count is required");
+    const WorldDesc &D = h->W[world];
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    // The last step's Collide walked the list held in the other buffer, where its
+    // flags stay; the solved contacts are in the current list, in island order.
+    const int n0 = S.collideCount, m = S.contactCount, nc = S.constraints;
+    std::vector<Contact> walked(n0), now(m);
+    std::vector<int32_t> ord(nc);
+    if (n0) DBX_HIP(h, hipMemcpy(walked.data(), h->d.cont[h->cur ^ 1] + D.contactBase, n0 * sizeof(Contact),
+                                 hipMemcpyDeviceToHost));
+    if (m) DBX_HIP(h, hipMemcpy(now.data(), h->d.cont[h->cur] + D.contactBase, m * sizeof(Contact), hipMemcpyDeviceToHost));
+    if (nc) DBX_HIP(h, hipMemcpy(ord.data(), h->d.ordC + D.contactBase, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<dbx_contact_event> ev;
+    ev.reserve(nc + 16);
+    for (int i = n0 - 1; i >= 0; --i) { // list order: newest first
+        const Contact &c = walked[i];
+        dbx_contact_event e{};
+        e.fixture_a = c.fa;
+        e.fixture_b = c.fb;
+        if (c.flags & kContactBegan) {
+            e.type = DBX_EVENT_BEGIN;
+            ev.push_back(e);
+        } else if (c.flags & kContactEnded) {
+            e.type = DBX_EVENT_END;
+            ev.push_back(e);
+        }
+    }
+    for (int k = 0; k < nc; ++k) {
+        if (ord[k] < 0 || ord[k] >= m) return fail(h, DBX_E_STATE, "solve order out of range");
+        const Contact &c = now[ord[k]];
+        dbx_contact_event e{};
+        e.type = DBX_EVENT_POST_SOLVE;
+        e.fixture_a = c.fa;
+        e.fixture_b = c.fb;
+        e.point_count = (c.flags & kContactSolvedOne) ? 1 : ((c.meta >> 4) & 0xf);
+        e.normal_impulse[0] = c.pt0.z;
+        e.tangent_impulse[0] = c.pt0.w;
+        if (e.point_count > 1) {
+            e.normal_impulse[1] = c.pt1.z;
+            e.tangent_impulse[1] = c.pt1.w;
+        }
+        ev.push_back(e);
+    }
+    *count = (int32_t)ev.size();
+    if (!out) return DBX_OK;
+    if (cap < (int32_t)ev.size()) return fail(h, DBX_E_CAPACITY, "event buffer too small");
+    std::copy(ev.begin(), ev.end(), out);
+    return DBX_OK;
+}
+
 dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count) {
     dbx_status st = check_world(h, world);
     if (st) return st;

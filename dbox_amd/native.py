@@ -34,6 +34,12 @@ CONTACT_DTYPE = np.dtype([
     ("friction", "<f4"), ("restitution", "<f4"),
 ])
 
+EVENT_BEGIN, EVENT_END, EVENT_POST_SOLVE = 1, 2, 3
+EVENT_DTYPE = np.dtype([
+    ("type", "<i4"), ("fixture_a", "<i4"), ("fixture_b", "<i4"), ("point_count", "<i4"),
+    ("normal_impulse", "<f4", (2,)), ("tangent_impulse", "<f4", (2,)),
+])
+
 
 class DbxConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("contacts_per_fixture", C.c_int32), ("block_size", C.c_int32),
@@ -56,7 +62,7 @@ _lib = None
 # Every entry point declared in include/dbox_hip.h.
 EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
-    "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_fat_aabbs",
+    "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
     "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals",
@@ -90,6 +96,7 @@ def lib():
         L.dbx_world_info.argtypes = [vp, i32, vp]
         L.dbx_read_bodies.argtypes = [vp, i32, vp, i32]
         L.dbx_read_contacts.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
+        L.dbx_read_events.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_read_fat_aabbs.argtypes = [vp, i32, vp, i32]
         L.dbx_read_pairs.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_profile_read.argtypes = [vp, C.POINTER(DbxProfile)]
@@ -193,6 +200,14 @@ class StepWorlds:
         self._check(self.L.dbx_read_contacts(self.h, world, None, 0, C.byref(n)))
         out = np.zeros(max(n.value, 1), CONTACT_DTYPE)
         self._check(self.L.dbx_read_contacts(self.h, world, _p(out), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def events(self, world=0) -> np.ndarray:
+        """The b2ContactListener callbacks of the last step (dbx_read_events), in dbox's order."""
+        n = C.c_int32()
+        self._check(self.L.dbx_read_events(self.h, world, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), EVENT_DTYPE)
+        self._check(self.L.dbx_read_events(self.h, world, _p(out), n.value, C.byref(n)))
         return out[:n.value]
 
     def fat_aabbs(self, world=0) -> np.ndarray:

@@ -102,6 +102,20 @@ typedef struct {
     float friction, restitution;
 } dbx_contact;
 
+/* One b2ContactListener callback of the last step.  BEGIN and END come from
+ * b2Contact.Update (src/dbox/dynamics/contacts/b2contact.d:342-350) and, for END,
+ * from b2ContactManager.Destroy of a touching contact (src/dbox/dynamics/
+ * b2contactmanager.d:190-193); POST_SOLVE from b2Island.Report
+ * (src/dbox/dynamics/b2island.d:438-461), with b2ContactImpulse's count and
+ * impulses. */
+enum { DBX_EVENT_BEGIN = 1, DBX_EVENT_END = 2, DBX_EVENT_POST_SOLVE = 3 };
+typedef struct {
+    int32_t type;                 /* DBX_EVENT_* */
+    int32_t fixture_a, fixture_b; /* as in dbx_contact */
+    int32_t point_count;          /* POST_SOLVE: b2ContactImpulse.count (1 for a downgraded two-point manifold) */
+    float normal_impulse[2], tangent_impulse[2];
+} dbx_contact_event;
+
 /* b2Profile (src/dbox/dynamics/b2timestep.d:37-46), milliseconds of the last step,
  * measured with HIP events around each phase. */
 typedef struct {
@@ -153,6 +167,16 @@ dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, in
 dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *in, const float *force3, int32_t n);
 /* Contacts of one world in b2World contact-list order. */
 dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count);
+/* The contact-listener callbacks dbox would have made during the last dbx_step of
+ * one world, in its order: BEGIN / END as b2ContactManager.Collide walks the
+ * contact list (b2contactmanager.d:251-317; a contact destroyed while touching
+ * gives END at its place in the list), then POST_SOLVE for every solved contact
+ * in island order (b2world.d:1085, b2island.d:239).  With more than one step per
+ * dbx_step call only the last step's callbacks are kept.  PreSolve is not
+ * delivered (it would need the step to stop between Collide and Solve).
+ * *count = number of events; DBX_E_CAPACITY when cap is smaller (nothing else
+ * written). */
+dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out, int32_t cap, int32_t *count);
 /* Fat AABBs of each fixture's proxy (lower x, lower y, upper x, upper y). */
 dbx_status dbx_read_fat_aabbs(dbx_handle *h, int32_t world, float *out, int32_t n);
 /* Pairs the last broadphase pass handed to AddPair, as (fixture, fixture) in the

@@ -1586,6 +1586,7 @@ struct World {
     void destroyContact(int ci) {
         Contact &c = contacts[ci];
         int bodyA = fixtures[c.fixtureA].body, bodyB = fixtures[c.fixtureB].body;
+        if (c.flags & cf_touching) recordEvent(2, c, nullptr); /* b2contactmanager.d:190-193 */
         if (c.prev != kNull) contacts[c.prev].next = c.next;
         if (c.next != kNull) contacts[c.next].prev = c.prev;
         if (ci == contactList) contactList = c.next;
@@ -1641,6 +1642,9 @@ struct World {
         }
         if (touching) c.flags |= cf_touching;
         else c.flags &= ~cf_touching;
+        /* listener calls b2contact.d:342-350 (PreSolve is not recorded) */
+        if (!wasTouching && touching) recordEvent(1, c, nullptr);
+        if (wasTouching && !touching) recordEvent(2, c, nullptr);
     }
     /* b2ContactManager.Collide b2contactmanager.d:251-317 */
     void collide() {
@@ -1928,6 +1932,21 @@ struct World {
         cp1.normalImpulse = x.x;
         cp2.normalImpulse = x.y;
     }
+    std::vector<orc_event> events; /* listener callbacks of the current step */
+    void recordEvent(int type, const Contact &c, const VelocityConstraint *vc) {
+        orc_event e = {};
+        e.type = type;
+        e.fixture_a = c.fixtureA;
+        e.fixture_b = c.fixtureB;
+        if (vc) {
+            e.point_count = vc->pointCount;
+            for (int j = 0; j < vc->pointCount; ++j) {
+                e.normal_impulse[j] = vc->points[j].normalImpulse;
+                e.tangent_impulse[j] = vc->points[j].tangentImpulse;
+            }
+        }
+        events.push_back(e);
+    }
     void storeImpulses() {
         for (size_t i = 0; i < vcs.size(); ++i) {
             const VelocityConstraint &vc = vcs[i];
@@ -2086,6 +2105,8 @@ struct World {
             b.synchronizeTransform();
         }
         profile[5] += msSince(t3);
+        /* b2Island.Report b2island.d:239,438-461 */
+        for (size_t i = 0; i < vcs.size(); ++i) recordEvent(3, contacts[islandContacts[vcs[i].contactIndex]], &vcs[i]);
         if (allowSleep) {
             float minSleepTime = kMaxFloat;
             const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
@@ -2205,6 +2226,7 @@ struct World {
     /* b2World.Step b2world.d:367-434 */
     void step(float dt, int velocityIterations, int positionIterations) {
         auto t0 = Clock::now();
+        events.clear();
         if (newFixture) {
             findNewContacts();
             newFixture = false;
@@ -2302,6 +2324,12 @@ void orc_step(void *w, float dt, int vel, int pos) { ((World *)w)->step(dt, vel,
 int orc_body_count(void *w) { return (int)((World *)w)->bodies.size(); }
 int orc_fixture_count(void *w) { return (int)((World *)w)->fixtures.size(); }
 int orc_contact_count(void *w) { return ((World *)w)->contactCount; }
+int orc_events(void *w, orc_event *out, int cap) {
+    const std::vector<orc_event> &ev = ((World *)w)->events;
+    int n = (int)ev.size();
+    for (int i = 0; i < n && i < cap; ++i) out[i] = ev[i];
+    return n;
+}
 int orc_dynamic_body_count(void *w) {
     int n = 0;
     for (const Body &b : ((World *)w)->bodies) n += b.type == b_dynamic;

@@ -72,6 +72,16 @@ typedef struct {
     float friction, restitution;
 } orc_contact; /* 84 bytes */
 
+/* One b2ContactListener callback recorded during the last orc_step: type 1
+ * BeginContact, 2 EndContact (b2contact.d:342-350, b2contactmanager.d:190-193),
+ * 3 PostSolve with b2ContactImpulse (b2island.d:438-461). */
+typedef struct {
+    int32_t type;
+    int32_t fixture_a, fixture_b;
+    int32_t point_count;
+    float normal_impulse[2], tangent_impulse[2];
+} orc_event; /* 32 bytes */
+
 /* Manifold produced by one collide call (b2Manifold, b2collision.d:96-114)
  * together with its b2WorldManifold (b2collision.d:123-191). */
 typedef struct {
@@ -104,6 +114,8 @@ int orc_contact_count(void *w);
 void orc_get_bodies(void *w, orc_body_state *out);
 /* Contacts in m_contactList order; returns the number written (<= cap). */
 int orc_get_contacts(void *w, orc_contact *out, int cap);
+/* Listener callbacks of the last orc_step in call order; returns the count. */
+int orc_events(void *w, orc_event *out, int cap);
 /* World manifold of contact-list entry i at the bodies' current transforms. */
 int orc_contact_world_manifold(void *w, int list_index, orc_manifold *out);
 /* Fat AABB of each fixture's proxy, 4 floats each (lower.x, lower.y, upper.x, upper.y). */

@@ -26,6 +26,10 @@ CONTACT_DTYPE = np.dtype([
     ("normal_impulse", "<f4", (2,)), ("tangent_impulse", "<f4", (2,)), ("id", "<u4", (2,)),
     ("friction", "<f4"), ("restitution", "<f4"),
 ])
+EVENT_DTYPE = np.dtype([
+    ("type", "<i4"), ("fixture_a", "<i4"), ("fixture_b", "<i4"), ("point_count", "<i4"),
+    ("normal_impulse", "<f4", (2,)), ("tangent_impulse", "<f4", (2,)),
+])
 MANIFOLD_DTYPE = np.dtype([
     ("type", "<i4"), ("point_count", "<i4"), ("swapped", "<i4"), ("pad", "<i4"),
     ("local_normal", "<f4", (2,)), ("local_point", "<f4", (2,)), ("point_local", "<f4", (4,)),
@@ -60,6 +64,7 @@ def lib():
             getattr(L, name).argtypes = [vp]
         L.orc_get_bodies.argtypes = [vp, vp]
         L.orc_get_contacts.argtypes = [vp, vp, i32]
+        L.orc_events.argtypes = [vp, vp, i32]
         L.orc_contact_world_manifold.argtypes = [vp, i32, vp]
         L.orc_get_fat_aabbs.argtypes = [vp, vp]
         L.orc_last_pairs.argtypes = [vp, vp, i32]
@@ -111,6 +116,14 @@ class OracleWorld:
         out = np.zeros(n, CONTACT_DTYPE)
         got = self.L.orc_get_contacts(self.h, _p(out), n)
         return out[:got]
+
+    def events(self) -> np.ndarray:
+        """b2ContactListener callbacks of the last step (BeginContact 1, EndContact 2,
+        PostSolve 3), in call order."""
+        n = self.L.orc_events(self.h, None, 0)
+        out = np.zeros(max(n, 1), EVENT_DTYPE)
+        self.L.orc_events(self.h, _p(out), n)
+        return out[:n]
 
     def world_manifold(self, list_index: int) -> np.ndarray:
         out = np.zeros(1, MANIFOLD_DTYPE)

@@ -424,3 +424,52 @@ def test_autotune_keeps_results_and_picks_sizes():
     tuned, tiers = _run_batch(scs, 20, flags, tune=2)
     assert tuned == plain
     assert all(int(b) in (64, 128, 256, 512, 1024) for b in tiers[:, 3])
+
+
+@pytest.mark.parametrize("scene_name", ["pyramid", "mixed"])
+def test_listener_events_match_oracle(scene_name):
+    # dbx_read_events: the BeginContact / EndContact / PostSolve callbacks dbox makes
+    # during b2World.Step (b2contact.d:342-350, b2contactmanager.d:190-193,
+    # b2island.d:438-461), in its order, against the oracle's record of the same
+    # callbacks.  Kinds, pairs and counts must be identical; PostSolve impulses agree
+    # to 2e-3 N s absolute + 1e-3 relative (the trajectories differ by sin/cos rounding).
+    # The pyramid is compared for 80 steps: from about step 85 its resting two-point
+    # contacts shift load between their points with the last bits of the positions
+    # (per-contact totals still agree to 0.06 N s), so single points stop being
+    # comparable while the kinds and order stay identical.
+    print(f"case: listener events {scene_name}", flush=True)
+    s = scenes.pyramid() if scene_name == "pyramid" else scenes.mixed_container(300, 40.0, 80.0, seed=5)
+    o = po.OracleWorld(s)
+    g = gpu_world([s])
+    compared = kinds = 0
+    worst = 0.0
+    for i in range(80 if scene_name == "pyramid" else 120):
+        o.step()
+        g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
+        bo, bg = o.bodies(), g.bodies()
+        if np.abs(bo["cx"] - bg["cx"]).max() > 1e-4 or np.abs(bo["cy"] - bg["cy"]).max() > 1e-4:
+            print(f"states diverged at step {i + 1}")
+            break
+        eo, eg = o.events(), g.events()
+        assert len(eg) == len(eo), (i, len(eg), len(eo))
+        for f in ("type", "fixture_a", "fixture_b", "point_count"):
+            assert np.array_equal(eg[f], eo[f]), (i, f)
+        post = eo["type"] == native.EVENT_POST_SOLVE
+        for f in ("normal_impulse", "tangent_impulse"):
+            d = np.abs(eg[f][post] - eo[f][post])
+            if d.size:
+                worst = max(worst, float(d.max()))
+            np.testing.assert_allclose(eg[f][post], eo[f][post], rtol=1e-3, atol=2e-3)
+        compared += len(eo)
+        kinds |= int(np.bitwise_or.reduce(1 << eo["type"])) if len(eo) else 0
+    print(f"events compared {compared}, largest impulse difference {worst:.2e}")
+    assert compared > 500
+    assert kinds & (1 << native.EVENT_BEGIN) and kinds & (1 << native.EVENT_POST_SOLVE)
+    # a buffer smaller than the event count is refused and left untouched
+    n = native.C.c_int32()
+    assert g.L.dbx_read_events(g.h, 0, None, 0, native.C.byref(n)) == native.DBX_OK
+    if n.value > 1:
+        small = np.zeros(n.value - 1, native.EVENT_DTYPE)
+        st = g.L.dbx_read_events(g.h, 0, native._p(small), n.value - 1, native.C.byref(n))
+        assert st == native.DBX_E_CAPACITY
+        assert not small.view(np.uint8).any()
