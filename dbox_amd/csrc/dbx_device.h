@@ -52,7 +52,8 @@ struct WorldDesc {
     int contactBase, contactCap;
     int pairBase, pairCap;
     float gx, gy;
-    int islandBase, pad;      // pad bit 0: a body of this world has several fixtures
+    int islandBase, pad;      // pad bit 0: a body of this world has several fixtures;
+                              // bit 1: pairs come from the device-wide broadphase pass
     long long islScratchBase; // byte offset of the world's island-search scratch
     long long bpScratchBase;  // byte offset of the world's broadphase scratch
 };
@@ -68,7 +69,8 @@ struct WorldState {
     int constraints;  // constraints solved by the last step
     float residual;   // velocity residual after the last iteration (RESIDUAL mode), else -1
     int collideCount; // contacts in the list the last Collide walked (its event flags stay there)
-    int pad[2];
+    int widePairs;    // pairs found by the device-wide broadphase pass (worlds too big for one workgroup)
+    int pad;
 };
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).

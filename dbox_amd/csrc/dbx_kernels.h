@@ -23,8 +23,29 @@ hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOf
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
                              bool lead, hipStream_t s);
 size_t broadphase_scratch_bytes(int proxies);
+
+// A world whose broadphase scratch does not fit in one workgroup's LDS: its pairs
+// are found by a pass spread over the whole device (sort, sweep and pair sort in
+// global memory) before k_broadphase adds them as contacts.
+struct WideBp {
+    int world;              // world index
+    WorldDesc desc;         // its descriptor (fixture and pair ranges)
+    int n;                  // proxies (fixtures)
+    int P;                  // power of two >= n: sorted proxy keys
+    int cap;                // the world's pair capacity
+    int Pcap;               // power of two >= cap: sorted pair keys
+    int nblk;               // 256-proxy blocks of the sweep count
+    unsigned long long *keys; // P
+    int *cur;               // n + 1 candidate offsets
+    int *bsum;              // nblk block sums
+    float4 *part;           // bounds partials, one per bounds workgroup
+    int *ctr;               // [0] pairs found, [1] candidates, [2] sweep axis
+    unsigned long long *pkeys; // Pcap
+};
+size_t wide_scratch_bytes(int proxies, int pairCap);
+void wide_carve(WideBp &b, char *p);
 hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
-                             hipStream_t s);
+                             hipStream_t s, const WideBp *wide = nullptr, int nwide = 0);
 hipError_t launch_refit(const Dev &d, int world, hipStream_t s);
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total,
                             hipStream_t s);

@@ -2002,7 +2002,9 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
 
     int total;
     const size_t need = bp_bytes(nf);
-    if (need <= (size_t)ldsBytes)
+    if (wd.pad & 2) // found by the device-wide pass, already in pairs[] in order
+        total = d.S[w].widePairs;
+    else if (need <= (size_t)ldsBytes)
         total = bp_pairs(fat, moved, nf, bp_carve(lds, nf), pairs, wd.pairCap, (int2 *)(lds + need),
                          (int)(((size_t)ldsBytes - need) / sizeof(int2)), s_tmp);
     else
@@ -2096,6 +2098,307 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
         d.S[w].pairCount = total;
     }
     if (timed) d.ktime[24] = __builtin_amdgcn_s_memtime();
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide broadphase of one big world: the pair set and order of bp_pairs
+// (every overlapping (i < j) with i or j moved, sorted by (i, j) as
+// b2PairLessThan, b2broadphase.d:139-195), with each stage spread over many
+// workgroups instead of one.  Stages: bounds partials -> sweep axis and keys;
+// bitonic sort of the keys (chunks of kWideChunk in LDS, the wider merge
+// steps in global memory); per proxy the count of sweep candidates (binary
+// search for the first lower bound beyond its upper bound) and their scan; the
+// candidate tests dealt out in equal runs, hits appended with one atomic per
+// wave; bitonic sort of the found pair keys over the power of two above their
+// count, which the sort kernels read on the device.
+// ---------------------------------------------------------------------------
+constexpr int kWideChunk = 4096;       // keys sorted per workgroup in LDS
+constexpr int kWideBoundsBlocks = 512; // workgroups of the bounds pass
+constexpr int kWideEmitBlocks = 2048;  // workgroups of the candidate pass
+constexpr int kWideEmitRun = 16;       // consecutive candidates per thread
+
+size_t wide_scratch_bytes(int n, int cap) {
+    const int P = bp_pow2(n > 0 ? n : 1), Pc = bp_pow2(cap > 0 ? cap : 1), nblk = (n + 255) / 256 + 1;
+    return isl_align(8 * (size_t)P) + isl_align(4 * (size_t)(n + 1)) + isl_align(4 * (size_t)nblk) +
+           isl_align(16 * (size_t)kWideBoundsBlocks) + isl_align(16) + isl_align(8 * (size_t)Pc);
+}
+void wide_carve(WideBp &b, char *p) {
+    b.P = bp_pow2(b.n > 0 ? b.n : 1);
+    b.Pcap = bp_pow2(b.cap > 0 ? b.cap : 1);
+    b.nblk = (b.n + 255) / 256;
+    b.keys = (unsigned long long *)p; p += isl_align(8 * (size_t)b.P);
+    b.cur = (int *)p; p += isl_align(4 * (size_t)(b.n + 1));
+    b.bsum = (int *)p; p += isl_align(4 * (size_t)(b.nblk + 1));
+    b.part = (float4 *)p; p += isl_align(16 * (size_t)kWideBoundsBlocks);
+    b.ctr = (int *)p; p += isl_align(16);
+    b.pkeys = (unsigned long long *)p;
+}
+
+__device__ __forceinline__ float4 bounds_join(float4 a, float4 b) {
+    return make_float4(fminr(a.x, b.x), fminr(a.y, b.y), fmaxr(a.z, b.z), fmaxr(a.w, b.w));
+}
+// Workgroup (256 threads) reduction of AABB bounds; the result is returned to every thread.
+__device__ __forceinline__ float4 block_bounds(float4 v) {
+    __shared__ float4 s[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float4 t = make_float4(__shfl_xor(v.x, o, 64), __shfl_xor(v.y, o, 64), __shfl_xor(v.z, o, 64),
+                                     __shfl_xor(v.w, o, 64));
+        v = bounds_join(v, t);
+    }
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return bounds_join(bounds_join(s[0], s[1]), bounds_join(s[2], s[3]));
+}
+__device__ __forceinline__ float4 empty_bounds() { return make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat); }
+
+__global__ __launch_bounds__(256) void k_bpw_bounds(const float4 *fat, int n, float4 *part) {
+    float4 b = empty_bounds();
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b = bounds_join(b, fat[i]);
+    b = block_bounds(b);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+// Sweep axis from the bounds (the axis of larger spread) and the keys
+// (lower bound << 32 | proxy), padded with ~0 to P.
+__global__ __launch_bounds__(256) void k_bpw_keys(const float4 *fat, int n, const float4 *part, int nparts,
+                                                  unsigned long long *keys, int P, int *ctr) {
+    float4 b = empty_bounds();
+    for (int k = threadIdx.x; k < nparts; k += 256) b = bounds_join(b, part[k]);
+    b = block_bounds(b);
+    const int axis = (b.z - b.x) >= (b.w - b.y) ? 0 : 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[2] = axis;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < P; i += gridDim.x * 256) {
+        unsigned long long key = ~0ull;
+        if (i < n) {
+            const float4 f = fat[i];
+            key = ((unsigned long long)f2key(axis == 0 ? f.x : f.y) << 32) | (unsigned int)i;
+        }
+        keys[i] = key;
+    }
+}
+
+// Length of a sort: the static power of two P, or (dyn set) the power of two
+// above the count *dyn found on the device, 0 when that count exceeds cap.
+__device__ __forceinline__ int wide_len(int P, const int *dyn, int cap) {
+    if (!dyn) return P;
+    const int c = *dyn;
+    if (c > cap) return 0;
+    return bp_pow2(c > 0 ? c : 1);
+}
+// One compare-exchange of the bitonic network: element pair t of the merge
+// step (k, j), in a block of keys whose first element has global index base.
+__device__ __forceinline__ void bitonic_cx(unsigned long long *a, int t, int k, int j, int base) {
+    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+    const unsigned long long x = a[i], y = a[i + j];
+    if ((x > y) == (((base + i) & k) == 0)) {
+        a[i] = y;
+        a[i + j] = x;
+    }
+}
+// Sorts each chunk of kWideChunk keys (or the whole sort, when shorter) with
+// the directions of the full network.
+__global__ __launch_bounds__(1024) void k_bpw_lsort(unsigned long long *keys, int P, const int *dyn, int cap) {
+    __shared__ unsigned long long s[kWideChunk];
+    const int Pe = wide_len(P, dyn, cap);
+    const int base = blockIdx.x * kWideChunk;
+    if (base >= Pe) return;
+    const int Te = Pe < kWideChunk ? Pe : kWideChunk;
+    for (int e = threadIdx.x; e < Te; e += blockDim.x) s[e] = keys[base + e];
+    __syncthreads();
+    for (int k = 2; k <= Te; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < Te / 2; t += blockDim.x) bitonic_cx(s, t, k, j, base);
+            __syncthreads();
+        }
+    for (int e = threadIdx.x; e < Te; e += blockDim.x) keys[base + e] = s[e];
+}
+// Merge steps j = kWideChunk/2 .. 1 of stage k > kWideChunk, per chunk in LDS.
+__global__ __launch_bounds__(1024) void k_bpw_lmerge(unsigned long long *keys, int P, int k, const int *dyn, int cap) {
+    __shared__ unsigned long long s[kWideChunk];
+    const int Pe = wide_len(P, dyn, cap);
+    const int base = blockIdx.x * kWideChunk;
+    if (k > Pe || base >= Pe) return;
+    for (int e = threadIdx.x; e < kWideChunk; e += blockDim.x) s[e] = keys[base + e];
+    __syncthreads();
+    for (int j = kWideChunk >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < kWideChunk / 2; t += blockDim.x) bitonic_cx(s, t, k, j, base);
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kWideChunk; e += blockDim.x) keys[base + e] = s[e];
+}
+// One merge step (k, j >= kWideChunk) in global memory, one compare per thread.
+__global__ __launch_bounds__(256) void k_bpw_gstep(unsigned long long *keys, int P, int k, int j, const int *dyn,
+                                                   int cap) {
+    const int Pe = wide_len(P, dyn, cap);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (k > Pe || t >= Pe / 2) return;
+    bitonic_cx(keys, t, k, j, 0);
+}
+
+// Per sorted proxy p: its candidates are the sorted proxies p+1 .. end(p)-1,
+// end(p) the first whose lower bound exceeds p's upper bound.  Writes each
+// workgroup's exclusive scan of the counts and the workgroup total.
+__global__ __launch_bounds__(256) void k_bpw_count(const float4 *fat, const unsigned long long *keys, int n,
+                                                   const int *ctr, int *cur, int *bsum) {
+    __shared__ int s_tmp[256 / 64 + 1];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int axis = ctr[2];
+    int c = 0;
+    if (p < n) {
+        const float4 fi = fat[(int)(keys[p] & 0xffffffffu)];
+        const unsigned int hk = f2key(axis == 0 ? fi.z : fi.w);
+        int lo = p + 1, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((unsigned int)(keys[mid] >> 32) > hk) hi = mid;
+            else lo = mid + 1;
+        }
+        c = lo - p - 1;
+    }
+    int tot;
+    const int off = block_excl_scan(c, tot, s_tmp);
+    if (p < n) cur[p] = off;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+// Exclusive scan of the workgroup totals (one workgroup of 1024 threads, in 64
+// bits); cur[n] = candidate total.  A total beyond int range cannot be dealt
+// out: the pass then reports more pairs than any capacity.
+__global__ __launch_bounds__(1024) void k_bpw_scan(int *bsum, int nblk, int *cur, int n, int *ctr) {
+    __shared__ long long s_w[1024 / 64 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int per = (nblk + 1023) / 1024, b0 = tid * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+    long long sum = 0;
+    for (int b = b0; b < b1; ++b) sum += bsum[b];
+    long long inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        long long a = 0;
+        for (int i = 0; i < 16; ++i) {
+            const long long t = s_w[i];
+            s_w[i] = a;
+            a += t;
+        }
+        s_w[16] = a;
+    }
+    __syncthreads();
+    const long long total = s_w[16];
+    long long run = inc - sum + s_w[wid];
+    const bool fits = total <= (long long)INT_MAX;
+    for (int b = b0; b < b1; ++b) {
+        const int v = bsum[b];
+        bsum[b] = fits ? (int)run : 0;
+        run += v;
+    }
+    if (tid == 0) {
+        cur[n] = fits ? (int)total : 0;
+        ctr[1] = fits ? (int)total : 0;
+        ctr[0] = fits ? 0 : INT_MAX;
+    }
+}
+__global__ __launch_bounds__(256) void k_bpw_offsets(int *cur, int n, const int *bsum) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < n) cur[p] += bsum[blockIdx.x];
+}
+// The candidate tests, kWideEmitRun consecutive ones per thread; a hit appends
+// its pair key (min << 32 | max).
+__global__ __launch_bounds__(256) void k_bpw_emit(const float4 *fat, const int *moved, const unsigned long long *keys,
+                                                  const int *cur, int n, int *ctr, unsigned long long *pkeys, int cap) {
+    const int C = ctr[1];
+    const int lane = threadIdx.x & 63;
+    const long long stride = (long long)gridDim.x * 256 * kWideEmitRun;
+    for (long long t0 = ((long long)blockIdx.x * 256 + threadIdx.x) * kWideEmitRun; t0 < C; t0 += stride) {
+        // the proxy whose candidates hold t0: last p with cur[p] <= t0 (cur[n] = C > t0)
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cur[mid] <= t0) lo = mid;
+            else hi = mid;
+        }
+        int p = lo, start = cur[p], next = cur[p + 1];
+        int i = (int)(keys[p] & 0xffffffffu);
+        float4 fi = fat[i];
+        int mi = moved[i];
+        const int t1 = t0 + kWideEmitRun < C ? (int)t0 + kWideEmitRun : C;
+        for (int t = (int)t0; t < t1; ++t) {
+            if (next <= t) {
+                do {
+                    ++p;
+                    start = next;
+                    next = cur[p + 1];
+                } while (next <= t);
+                i = (int)(keys[p] & 0xffffffffu);
+                fi = fat[i];
+                mi = moved[i];
+            }
+            const int j = (int)(keys[p + 1 + (t - start)] & 0xffffffffu);
+            const bool hit = (mi || moved[j]) && overlap(fi, fat[j]);
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                int b = 0;
+                if (lane == leader) b = atomicAdd(&ctr[0], __popcll(m));
+                b = __shfl(b, leader, 64);
+                if (hit) {
+                    const int slot = b + __popcll(m & ((1ull << lane) - 1ull));
+                    const unsigned int a = i < j ? i : j, c = i < j ? j : i;
+                    if (slot < cap) pkeys[slot] = ((unsigned long long)a << 32) | c;
+                }
+            }
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_bpw_pad(unsigned long long *pkeys, const int *ctr, int cap) {
+    const int np = ctr[0];
+    if (np > cap) return;
+    const int Pe = bp_pow2(np > 0 ? np : 1);
+    for (int i = np + blockIdx.x * 256 + threadIdx.x; i < Pe; i += gridDim.x * 256) pkeys[i] = ~0ull;
+}
+__global__ __launch_bounds__(256) void k_bpw_out(const unsigned long long *pkeys, const int *ctr, int cap, int2 *pairs,
+                                                 WorldState *S) {
+    const int np = ctr[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->widePairs = np;
+    if (np > cap) return;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < np; e += gridDim.x * 256) {
+        const unsigned long long key = pkeys[e];
+        pairs[e] = make_int2((int)(key >> 32), (int)(key & 0xffffffffu));
+    }
+}
+static void wide_sort(unsigned long long *keys, int P, const int *dyn, int cap, hipStream_t s) {
+    const int T = P < kWideChunk ? P : kWideChunk;
+    hipLaunchKernelGGL(k_bpw_lsort, dim3(P / T), dim3(1024), 0, s, keys, P, dyn, cap);
+    for (int k = 2 * T; k <= P; k <<= 1) {
+        for (int j = k >> 1; j >= T; j >>= 1)
+            hipLaunchKernelGGL(k_bpw_gstep, dim3(P / 512), dim3(256), 0, s, keys, P, k, j, dyn, cap);
+        hipLaunchKernelGGL(k_bpw_lmerge, dim3(P / T), dim3(1024), 0, s, keys, P, k, dyn, cap);
+    }
+}
+static hipError_t launch_wide(const Dev &d, const WideBp &b, hipStream_t s) {
+    const WorldDesc &wd = b.desc;
+    const float4 *fat = d.ffat + wd.fixBase;
+    const int *moved = d.fmoved + wd.fixBase;
+    const int nb = b.nblk > 0 ? b.nblk : 1;
+    const int G = nb < kWideBoundsBlocks ? nb : kWideBoundsBlocks;
+    const int grid = b.P / 256 > 0 ? b.P / 256 : 1;
+    hipLaunchKernelGGL(k_bpw_bounds, dim3(G), dim3(256), 0, s, fat, b.n, b.part);
+    hipLaunchKernelGGL(k_bpw_keys, dim3(grid), dim3(256), 0, s, fat, b.n, b.part, G, b.keys, b.P, b.ctr);
+    wide_sort(b.keys, b.P, nullptr, 0, s);
+    hipLaunchKernelGGL(k_bpw_count, dim3(nb), dim3(256), 0, s, fat, b.keys, b.n, b.ctr, b.cur, b.bsum);
+    hipLaunchKernelGGL(k_bpw_scan, dim3(1), dim3(1024), 0, s, b.bsum, b.nblk, b.cur, b.n, b.ctr);
+    hipLaunchKernelGGL(k_bpw_offsets, dim3(nb), dim3(256), 0, s, b.cur, b.n, b.bsum);
+    hipLaunchKernelGGL(k_bpw_emit, dim3(kWideEmitBlocks), dim3(256), 0, s, fat, moved, b.keys, b.cur, b.n, b.ctr,
+                       b.pkeys, b.cap);
+    const int pg = b.Pcap / 256 < 1024 ? (b.Pcap / 256 > 0 ? b.Pcap / 256 : 1) : 1024;
+    hipLaunchKernelGGL(k_bpw_pad, dim3(pg), dim3(256), 0, s, b.pkeys, b.ctr, b.cap);
+    wide_sort(b.pkeys, b.Pcap, b.ctr, b.cap, s);
+    hipLaunchKernelGGL(k_bpw_out, dim3(pg), dim3(256), 0, s, b.pkeys, b.ctr, b.cap, d.pairs + wd.pairBase,
+                       d.S + b.world);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -2256,8 +2559,12 @@ hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int wo
 }
 size_t broadphase_scratch_bytes(int proxies) { return bp_bytes(proxies); }
 hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
-                             hipStream_t s) {
+                             hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
+    for (int i = 0; i < nwide; ++i) {
+        const hipError_t e = launch_wide(d, wide[i], s);
+        if (e != hipSuccess) return e;
+    }
     DBX_LAUNCH(k_broadphase, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
 }

@@ -396,6 +396,7 @@ struct Tier {
     hipEvent_t done = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr; // chain timing while autotuning
     bool own = false;         // stream and event created for this tier
+    int wideFirst = 0, wideCount = 0; // its worlds whose broadphase runs device-wide (range in the handle's list)
 };
 
 // A captured step: the launch sequence of one step for one combination of
@@ -425,6 +426,7 @@ struct dbx_handle {
     int block = 256;
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
     std::vector<Tier> tiers;      // tier 0 = the largest worlds, on `stream`
+    std::vector<WideBp> wide;     // worlds whose broadphase scratch does not fit in LDS, by tier
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
     std::vector<hipEvent_t> ring;   // kStepEvents events per step since the last sync
@@ -654,6 +656,23 @@ dbx_status dbx_finalize(dbx_handle *h) {
             tiers[t].own = true;
             DBX_HIP(h, hipEventCreateWithFlags(&tiers[t].done, hipEventDisableTiming));
         }
+        // A world whose broadphase scratch does not fit in its tier's LDS has its
+        // pairs found by the device-wide pass.
+        for (Tier &T : tiers) {
+            const int lds1 = std::min(T.lds, h->ldsMax - 1024);
+            T.wideFirst = (int)h->wide.size();
+            for (int i = T.first; i < T.first + T.count; ++i) {
+                WorldDesc &D = h->W[order[i]];
+                if (broadphase_scratch_bytes(D.fixCount) <= (size_t)lds1) continue;
+                D.pad |= 2;
+                WideBp b{};
+                b.world = order[i];
+                b.n = D.fixCount;
+                b.cap = D.pairCap;
+                h->wide.push_back(b);
+            }
+            T.wideCount = (int)h->wide.size() - T.wideFirst;
+        }
         h->tiers = tiers;
         if (getenv("DBX_VERBOSE"))
             for (const Tier &T : h->tiers)
@@ -753,6 +772,17 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.batchCur, nc + 65LL * nw));
     DBX_HIP(h, dalloc(h, &d.islScratch, islBytes));
     DBX_HIP(h, dalloc(h, &d.bpScratch, bpBytes));
+    {
+        size_t wideBytes = 0;
+        for (const WideBp &b : h->wide) wideBytes += wide_scratch_bytes(b.n, b.cap);
+        char *wp = nullptr;
+        if (wideBytes) DBX_HIP(h, dalloc(h, &wp, wideBytes));
+        for (WideBp &b : h->wide) {
+            b.desc = h->W[b.world];
+            wide_carve(b, wp);
+            wp += wide_scratch_bytes(b.n, b.cap);
+        }
+    }
     DBX_HIP(h, dalloc(h, &d.vc, nc));
     DBX_HIP(h, dalloc(h, &d.pc, nc));
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
@@ -786,8 +816,9 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     const int lds1 = std::min(T.lds, h->ldsMax - 1024); // kernels that also hold static LDS
     hipStream_t s = T.stream;
     const Dev &d = h->d;
+    const WideBp *wide = T.wideCount ? &h->wide[T.wideFirst] : nullptr;
     if (findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
-        DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s));
+        DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
     DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s));
     const bool timed = lead && ev; // phase events: lead tier, eager steps only
     if (timed) (void)hipEventRecord(ev[1], s);
@@ -798,7 +829,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
         if (ordered) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
         else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, T.block, lead, s));
         if (timed) (void)hipEventRecord(ev[3], s);
-        DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s));
+        DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
     } else if (timed) {
         (void)hipEventRecord(ev[2], s);
         (void)hipEventRecord(ev[3], s);
