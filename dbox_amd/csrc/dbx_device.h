@@ -70,7 +70,7 @@ struct WorldState {
     float residual;   // velocity residual after the last iteration (RESIDUAL mode), else -1
     int collideCount; // contacts in the list the last Collide walked (its event flags stay there)
     int widePairs;    // pairs found by the device-wide broadphase pass (worlds too big for one workgroup)
-    int pad;
+    int toiEvents;    // TOI events (sub-step solves) of the last step's continuous pass
 };
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).
@@ -181,6 +181,9 @@ struct Dev {
     unsigned long long *logCount;
     long long logCap;
     long long *ktime;       // optional: phase timestamps of the largest world (s_memtime)
+    // continuous pass (b2World.SolveTOI) state, valid during that pass only
+    int2 *toiC;             // per contact slot: (cached TOI as float bits, toiCount | TOI and island flags)
+    int2 *toiB;             // per body: (sweep alpha0 as float bits, island flag)
 };
 
 // Order-preserving int image of a float (for atomicMin on floats).

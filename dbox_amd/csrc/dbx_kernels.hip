@@ -12,6 +12,7 @@
 
 #include "dbx_device.h"
 #include "dbx_kernels.h"
+#include "dbx_toi.h"
 
 namespace dbx {
 
@@ -87,21 +88,14 @@ __device__ __forceinline__ void wake_body(uint32_t *bfl, float4 *bpos, int b) {
 // Collide: b2ContactManager.Collide (b2contactmanager.d:251-317) for one contact.
 // pos is the contact's position in the reference's list (head = 0).
 // ---------------------------------------------------------------------------
-__device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbody, const Shape *__restrict__ shp,
-                                const float4 *__restrict__ fat, const float4 *__restrict__ bxf, int *wake) {
+// b2Contact.Update (b2contact.d:270-356) at the bodies' current transforms:
+// narrowphase, impulses carried over to points with a matching feature key,
+// touching and begin/end flags.  Returns whether touching changed (the
+// reference then wakes both bodies).
+__device__ __forceinline__ bool contact_update(Contact &c, const int *__restrict__ fbody, const Shape *__restrict__ shp,
+                                               const float4 *__restrict__ bxf) {
     int ba = fbody[c.fa], bb = fbody[c.fb];
     int count = man_count(c.meta);
-    if (!overlap(fat[c.fa], fat[c.fb])) {
-        // b2ContactManager.Destroy -> b2Contact.Destroy wakes both bodies of a touching manifold
-        c.flags |= kContactDestroyed;
-        if (c.flags & kContactTouching) c.flags |= kContactEnded;
-        if (count > 0) {
-            atomicMin(&wake[ba], pos);
-            atomicMin(&wake[bb], pos);
-        }
-        return;
-    }
-    // b2Contact.Update (b2contact.d:270-356)
     c.flags |= kContactEnabled;
     bool wasTouching = (c.flags & kContactTouching) != 0;
     Xf xfA = load_xf(bxf, ba), xfB = load_xf(bxf, bb);
@@ -134,10 +128,6 @@ __device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbo
             newPts |= i == 0 ? kContactNewPt0 : kContactNewPt1;
         }
     }
-    if (touching != wasTouching) {
-        atomicMin(&wake[ba], pos);
-        atomicMin(&wake[bb], pos);
-    }
     uint32_t f = c.flags & ~(kContactTouching | kContactNewPt0 | kContactNewPt1);
     if (touching) f |= kContactTouching;
     if (!wasTouching && touching) f |= kContactBegan;
@@ -149,6 +139,27 @@ __device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbo
     c.pt1 = make_float4(m.p[1].x, m.p[1].y, ni[1], ti[1]);
     c.id0 = m.id[0];
     c.id1 = m.id[1];
+    return touching != wasTouching;
+}
+
+__device__ void process_contact(Contact &c, int pos, const int *__restrict__ fbody, const Shape *__restrict__ shp,
+                                const float4 *__restrict__ fat, const float4 *__restrict__ bxf, int *wake) {
+    int ba = fbody[c.fa], bb = fbody[c.fb];
+    int count = man_count(c.meta);
+    if (!overlap(fat[c.fa], fat[c.fb])) {
+        // b2ContactManager.Destroy -> b2Contact.Destroy wakes both bodies of a touching manifold
+        c.flags |= kContactDestroyed;
+        if (c.flags & kContactTouching) c.flags |= kContactEnded;
+        if (count > 0) {
+            atomicMin(&wake[ba], pos);
+            atomicMin(&wake[bb], pos);
+        }
+        return;
+    }
+    if (contact_update(c, fbody, shp, bxf)) {
+        atomicMin(&wake[ba], pos);
+        atomicMin(&wake[bb], pos);
+    }
 }
 
 // kLead marks the launch of the lead tier (the largest worlds, which set the
@@ -897,8 +908,9 @@ __device__ __forceinline__ float solve_velocity_core(const VelCon &v, float4 &im
 // (-(x * y) == x * (-y) and a - (-b) == a + b exactly), except that the two
 // rotational terms of K = mA + mB + iA rnA^2 + iB rnB^2 are added in the
 // reference's order, which is the swapped one.
+// beta: the Baumgarte factor (b2_baumgarte, or b2_toiBaugarte in the TOI sub-step).
 __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, const Xf &xfA, const Xf &xfB, float2 &cA,
-                                                float &aA, float2 &cB, float &aB) {
+                                                float &aA, float2 &cB, float &aB, float beta = kBaumgarte) {
     const float mA = p.invMassA, mB = p.invMassB, iA = p.invIA, iB = p.invIB;
     const bool circles = p.type == kManCircles, flip = p.type == kManFaceB;
     const float2 P1 = mul(xfA, p.localPoint);
@@ -918,7 +930,7 @@ __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, cons
     const float separation = dot(d, normal) - p.radiusA - p.radiusB;
     const float2 rA = sub(point, cA);
     const float2 rB = sub(point, cB);
-    const float C = clampr(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+    const float C = clampr(beta * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
     const float rnA = crs(rA, normal);
     const float rnB = crs(rB, normal);
     const float tA = iA * rnA * rnA, tB = iB * rnB * rnB;
@@ -935,16 +947,17 @@ __device__ __forceinline__ float position_point(const PosCon &p, float2 lp, cons
 // The reference rebuilds both transforms before every point (b2contactsolver.d:101-104).
 // Point 0 almost always moves a body, so point 1's transforms are rebuilt without
 // first testing whether they changed.
-__device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb) {
+__device__ __forceinline__ float solve_position_core(const PosCon &p, float4 &pa, float4 &pb,
+                                                     float beta = kBaumgarte) {
     float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
     float aA = pa.z, aB = pb.z;
     Xf xfA = xf_from(cA, aA, p.lcA);
     Xf xfB = xf_from(cB, aB, p.lcB);
-    float minSep = fminr(0.0f, position_point(p, p.lp0, xfA, xfB, cA, aA, cB, aB));
+    float minSep = fminr(0.0f, position_point(p, p.lp0, xfA, xfB, cA, aA, cB, aB, beta));
     if (p.pointCount > 1) {
         xfA = xf_from(cA, aA, p.lcA);
         xfB = xf_from(cB, aB, p.lcB);
-        minSep = fminr(minSep, position_point(p, p.lp1, xfA, xfB, cA, aA, cB, aB));
+        minSep = fminr(minSep, position_point(p, p.lp1, xfA, xfB, cA, aA, cB, aB, beta));
     }
     pa = make_float4(cA.x, cA.y, aA, pa.w);
     pb = make_float4(cB.x, cB.y, aB, pb.w);
@@ -2101,6 +2114,493 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
 }
 
 // ---------------------------------------------------------------------------
+// Continuous physics: b2World.SolveTOI (b2world.d:1127-1452), one wave per
+// world, after the discrete step's broadphase.
+//
+// The search for the first time of impact computes the TOI of every contact
+// that needs one on its own lane (b2TimeOfImpact, dbx_toi.h).  While walking the
+// contact list the reference advances a contact's body whose sweep is at an
+// earlier time than the other body's (which happens only after an event); a
+// search that meets such a contact walks the list in order on one lane instead,
+// so that every TOI sees the sweeps the reference's walk gives it.  The event
+// itself -- advancing the two bodies, b2Contact.Update, the island of at most
+// b2_maxTOIContacts contacts, its sub-step solve (b2Island.SolveTOI,
+// b2island.d:282-416) -- runs on one lane in the reference's order; fixture
+// synchronisation and the broadphase update (FindNewContacts for the proxies
+// the event moved) use the whole wave.
+// ---------------------------------------------------------------------------
+constexpr int kMaxSubSteps = 8;         // b2settings.d:105
+constexpr int kMaxToiContacts = 32;     // b2settings.d:110
+constexpr float kToiBaumgarte = 0.75f;  // b2settings.d:138
+constexpr int kToiPairCap = 512;        // pairs one event's broadphase update may find
+constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17;
+
+struct ToiWorld {
+    Contact *cont;
+    int2 *tc, *tb;
+    const int *fbody;
+    const Shape *shp;
+    const float4 *fmat;
+    const int2 *ffilter;
+    float4 *fat;
+    int *moved;
+    uint32_t *bfl;
+    float4 *bpos, *bpos0, *bvel, *bxf;
+    const float4 *bmass;
+};
+
+__device__ __forceinline__ Sweep load_sweep(const ToiWorld &W, int b) {
+    Sweep s;
+    const float4 m = W.bmass[b], p0 = W.bpos0[b], p = W.bpos[b];
+    s.lc = v2(m.z, m.w);
+    s.c0 = v2(p0.x, p0.y);
+    s.a0 = p0.z;
+    s.c = v2(p.x, p.y);
+    s.a = p.z;
+    s.alpha0 = __int_as_float(W.tb[b].x);
+    return s;
+}
+__device__ __forceinline__ void store_sweep(const ToiWorld &W, int b, const Sweep &s) {
+    W.bpos0[b] = make_float4(s.c0.x, s.c0.y, s.a0, W.bpos0[b].w);
+    W.bpos[b] = make_float4(s.c.x, s.c.y, s.a, W.bpos[b].w);
+    W.tb[b].x = __float_as_int(s.alpha0);
+}
+// b2Body.SynchronizeTransform (b2body.d:1143-1147)
+__device__ __forceinline__ void sync_transform(const ToiWorld &W, int b) {
+    const float4 p = W.bpos[b], m = W.bmass[b];
+    const Xf x = xf_from(v2(p.x, p.y), p.z, v2(m.z, m.w));
+    W.bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+}
+// b2Body.Advance (b2body.d:1172-1180)
+__device__ __forceinline__ void body_advance(const ToiWorld &W, int b, float alpha) {
+    Sweep s = load_sweep(W, b);
+    sweep_advance(s, alpha);
+    s.c = s.c0;
+    s.a = s.a0;
+    store_sweep(W, b, s);
+    sync_transform(W, b);
+}
+// b2Contact.Update with the wake on a change of touching.
+__device__ __forceinline__ void toi_update(const ToiWorld &W, Contact &c) {
+    if (contact_update(c, W.fbody, W.shp, W.bxf)) {
+        wake_body(W.bfl, W.bpos, W.fbody[c.fa]);
+        wake_body(W.bfl, W.bpos, W.fbody[c.fb]);
+    }
+}
+
+// What the search does with a contact (b2world.d:1161-1200): 0 skip it
+// (disabled, over b2_maxSubSteps, both bodies inactive, or two non-bullet
+// dynamic bodies), 1 use its cached TOI, 2 compute its TOI.
+__device__ __forceinline__ int toi_status(const ToiWorld &W, const Contact &c, int tcy) {
+    if ((c.flags & kContactEnabled) == 0) return 0;
+    if ((tcy & kTcCount) > kMaxSubSteps) return 0;
+    if (tcy & kTcToi) return 1;
+    const uint32_t fa = W.bfl[W.fbody[c.fa]], fb = W.bfl[W.fbody[c.fb]];
+    const int ta = body_type(fa), tb = body_type(fb);
+    const bool activeA = (fa & kBodyAwake) && ta != kStatic;
+    const bool activeB = (fb & kBodyAwake) && tb != kStatic;
+    if (!activeA && !activeB) return 0;
+    const bool collideA = (fa & kBodyBullet) || ta != kDynamic;
+    const bool collideB = (fb & kBodyBullet) || tb != kDynamic;
+    if (!collideA && !collideB) return 0;
+    return 2;
+}
+// The contact's TOI as a fraction of the step (b2world.d:1208-1238); its two
+// sweeps are at the same alpha0 here.
+__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb) {
+    const Sweep sA = load_sweep(W, W.fbody[fa]), sB = load_sweep(W, W.fbody[fb]);
+    const float alpha0 = sA.alpha0;
+    const DProxy pA = make_proxy(W.shp + fa), pB = make_proxy(W.shp + fb);
+    float beta;
+    const int state = time_of_impact(beta, pA, pB, sA, sB, 1.0f);
+    return state == kToiTouching ? fminr(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
+}
+
+// One TOI event (b2world.d:1270-1405) on one lane: advance both bodies to the
+// TOI, update the contact, build the island and solve it (b2island.d:282-416).
+// Returns the island body count, 0 when the contact was not touching at the TOI.
+__device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float minAlpha, const StepParams &sp,
+                                      int *ibod, VelCon *vcs, PosCon *pcs, float4 *imp) {
+    Contact *cont = W.cont;
+    const int ia = W.fbody[cont[ci].fa], ib = W.fbody[cont[ci].fb];
+    const Sweep backup1 = load_sweep(W, ia), backup2 = load_sweep(W, ib);
+    body_advance(W, ia, minAlpha);
+    body_advance(W, ib, minAlpha);
+    Contact mc = cont[ci];
+    toi_update(W, mc);
+    cont[ci] = mc;
+    const int2 t0 = W.tc[ci];
+    W.tc[ci].y = (((t0.y & kTcCount) + 1) & kTcCount) | (t0.y & kTcIsland);
+    if (!(mc.flags & kContactEnabled) || !(mc.flags & kContactTouching)) {
+        cont[ci].flags = mc.flags & ~kContactEnabled;
+        store_sweep(W, ia, backup1);
+        store_sweep(W, ib, backup2);
+        sync_transform(W, ia);
+        sync_transform(W, ib);
+        return 0;
+    }
+    wake_body(W.bfl, W.bpos, ia);
+    wake_body(W.bfl, W.bpos, ib);
+    int icon[kMaxToiContacts];
+    int nbod = 0, ncon = 0;
+    ibod[nbod++] = ia;
+    ibod[nbod++] = ib;
+    icon[ncon++] = ci;
+    W.tb[ia].y = 1;
+    W.tb[ib].y = 1;
+    W.tc[ci].y |= kTcIsland;
+    // contacts of the two bodies with static, kinematic or bullet bodies that
+    // touch at the TOI, each body's list newest first
+    for (int side = 0; side < 2; ++side) {
+        const int body = side ? ib : ia;
+        const uint32_t bf = W.bfl[body];
+        if (body_type(bf) != kDynamic) continue;
+        for (int j = n - 1; j >= 0; --j) {
+            const int ja = W.fbody[cont[j].fa], jb = W.fbody[cont[j].fb];
+            if (ja != body && jb != body) continue;
+            if (nbod == 2 * kMaxToiContacts || ncon == kMaxToiContacts) break;
+            if (W.tc[j].y & kTcIsland) continue;
+            const int other = ja == body ? jb : ja;
+            const uint32_t of = W.bfl[other];
+            if (body_type(of) == kDynamic && !(bf & kBodyBullet) && !(of & kBodyBullet)) continue;
+            const Sweep backup = load_sweep(W, other);
+            if (W.tb[other].y == 0) body_advance(W, other, minAlpha);
+            Contact cu = cont[j];
+            toi_update(W, cu);
+            cont[j] = cu;
+            if (!(cu.flags & kContactEnabled) || !(cu.flags & kContactTouching)) {
+                store_sweep(W, other, backup);
+                sync_transform(W, other);
+                continue;
+            }
+            W.tc[j].y |= kTcIsland;
+            icon[ncon++] = j;
+            if (W.tb[other].y) continue;
+            W.tb[other].y = 1;
+            if (body_type(W.bfl[other]) != kStatic) wake_body(W.bfl, W.bpos, other);
+            ibod[nbod++] = other;
+        }
+    }
+    // the sub-step: the rest of the step, no warm starting (b2world.d:1391-1398)
+    StepParams sub = sp;
+    sub.dt = (1.0f - minAlpha) * sp.dt;
+    sub.inv_dt = 1.0f / sub.dt;
+    sub.dtRatio = 1.0f;
+    sub.flags = sp.flags & ~1u; // no warm starting
+    // The solver works on the bodies' own sweep positions, which the island's
+    // position array starts from and which nothing else reads until the end.
+    for (int k = 0; k < ncon; ++k)
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, sub);
+    // SolveTOIPositionConstraints (b2contactsolver.d:152-242): only the two TOI
+    // bodies move
+    for (int it = 0; it < 20; ++it) {
+        float minSep = 0.0f;
+        for (int k = 0; k < ncon; ++k) {
+            PosCon p = pcs[k];
+            if (p.indexA != ia && p.indexA != ib) {
+                p.invMassA = 0.0f;
+                p.invIA = 0.0f;
+            }
+            if (p.indexB != ia && p.indexB != ib) {
+                p.invMassB = 0.0f;
+                p.invIB = 0.0f;
+            }
+            float4 pa = W.bpos[p.indexA], pb = W.bpos[p.indexB];
+            minSep = fminr(minSep, solve_position_core(p, pa, pb, kToiBaumgarte));
+            W.bpos[p.indexA] = pa;
+            W.bpos[p.indexB] = pb;
+        }
+        if (minSep >= -1.5f * kLinearSlop) break;
+    }
+    // leap of faith to the new safe state
+    {
+        const float4 pa = W.bpos[ia], pb = W.bpos[ib];
+        W.bpos0[ia] = make_float4(pa.x, pa.y, pa.z, W.bpos0[ia].w);
+        W.bpos0[ib] = make_float4(pb.x, pb.y, pb.z, W.bpos0[ib].w);
+    }
+    for (int k = 0; k < ncon; ++k) {
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, sub);
+        imp[k] = make_float4(vcs[k].nImp0, vcs[k].tImp0, vcs[k].nImp1, vcs[k].tImp1);
+    }
+    for (int it = 0; it < sp.velocityIterations; ++it)
+        for (int k = 0; k < ncon; ++k) {
+            const VelCon &v = vcs[k];
+            float4 va = W.bvel[v.indexA], vb = W.bvel[v.indexB];
+            float4 im = imp[k];
+            solve_velocity_core<false>(v, im, va, vb);
+            imp[k] = im;
+            W.bvel[v.indexA] = va;
+            W.bvel[v.indexB] = vb;
+        }
+    // integrate positions (b2island.d:370-413)
+    const float h = sub.dt;
+    for (int i = 0; i < nbod; ++i) {
+        const int b = ibod[i];
+        const float4 p = W.bpos[b], vel = W.bvel[b];
+        float2 c = v2(p.x, p.y), v = v2(vel.x, vel.y);
+        float a = p.z, wv = vel.z;
+        const float2 translation = scl(h, v);
+        if (dot(translation, translation) > kMaxTranslationSquared) {
+            const float ratio = kMaxTranslation / sqrtf(translation.x * translation.x + translation.y * translation.y);
+            v = v2(v.x * ratio, v.y * ratio);
+        }
+        const float rotation = h * wv;
+        if (rotation * rotation > kMaxRotationSquared) {
+            const float ratio = kMaxRotation / fabsr(rotation);
+            wv *= ratio;
+        }
+        c = add(c, scl(h, v));
+        a += h * wv;
+        W.bpos[b] = make_float4(c.x, c.y, a, p.w);
+        W.bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+        sync_transform(W, b);
+    }
+    return nbod;
+}
+
+// b2ContactManager.AddPair filters and b2Contact.Create (b2contactmanager.d:52-176)
+// for a pair that has no contact yet; appends the contact at slot n.
+__device__ bool toi_add_pair(const ToiWorld &W, int fa, int fb, int n, int cap) {
+    const int ba = W.fbody[fa], bb = W.fbody[fb];
+    if (ba == bb) return false;
+    if (!(body_type(W.bfl[ba]) == kDynamic || body_type(W.bfl[bb]) == kDynamic)) return false;
+    const int2 A = W.ffilter[fa], B = W.ffilter[fb];
+    bool accept;
+    if (A.y == B.y && A.y != 0) accept = A.y > 0;
+    else accept = ((A.x >> 16) & (B.x & 0xffff)) != 0 && ((A.x & 0xffff) & (B.x >> 16)) != 0;
+    if (!accept || !pair_has_type(W.shp[fa].type, W.shp[fb].type)) return false;
+    if (n >= cap) return true; // counted; the caller reports the capacity error
+    if (!pair_is_primary(W.shp[fa].type, W.shp[fb].type)) {
+        const int t = fa;
+        fa = fb;
+        fb = t;
+    }
+    Contact c;
+    c.fa = fa;
+    c.fb = fb;
+    c.flags = kContactEnabled;
+    c.meta = 0;
+    c.friction = sqrtf(W.fmat[fa].x * W.fmat[fb].x);
+    const float ra = W.fmat[fa].y, rb = W.fmat[fb].y;
+    c.restitution = ra > rb ? ra : rb;
+    c.id0 = 0;
+    c.id1 = 0;
+    c.man = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.cont[n] = c;
+    W.tc[n] = make_int2(__float_as_int(1.0f), 0);
+    wake_body(W.bfl, W.bpos, ba);
+    wake_body(W.bfl, W.bpos, bb);
+    return true;
+}
+
+template <bool kLead>
+__global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
+    __shared__ int s_ibod[2 * kMaxToiContacts];
+    __shared__ VelCon s_vc[kMaxToiContacts];
+    __shared__ PosCon s_pc[kMaxToiContacts];
+    __shared__ float4 s_imp[kMaxToiContacts];
+    __shared__ unsigned long long s_pairs[kToiPairCap];
+    __shared__ int s_nb, s_np, s_n;
+    const int w = d.order[worldOffset + blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int lane = threadIdx.x;
+    ToiWorld W;
+    W.cont = d.cont[cur] + wd.contactBase;
+    W.tc = d.toiC + wd.contactBase;
+    W.tb = d.toiB + wd.bodyBase;
+    W.fbody = d.fbody + wd.fixBase;
+    W.shp = d.fshape + wd.fixBase;
+    W.fmat = d.fmat + wd.fixBase;
+    W.ffilter = d.ffilter + wd.fixBase;
+    W.fat = d.ffat + wd.fixBase;
+    W.moved = d.fmoved + wd.fixBase;
+    W.bfl = d.bflags + wd.bodyBase;
+    W.bpos = d.bpos + wd.bodyBase;
+    W.bpos0 = d.bpos0 + wd.bodyBase;
+    W.bvel = d.bvel + wd.bodyBase;
+    W.bxf = d.bxf + wd.bodyBase;
+    W.bmass = d.bmass + wd.bodyBase;
+    const int nb = wd.bodyCount, nf = wd.fixCount;
+    int n = d.S[w].contactCount;
+    // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
+    for (int b = lane; b < nb; b += 64) W.tb[b] = make_int2(0, 0);
+    for (int i = lane; i < n; i += 64) W.tc[i] = make_int2(__float_as_int(1.0f), 0);
+    __syncthreads();
+    int events = 0, error = 0;
+    // each pass of the loop either ends it or raises one contact's toiCount,
+    // and a contact stops taking part past b2_maxSubSteps
+    const int guard = (kMaxSubSteps + 1) * wd.contactCap + 1;
+    for (int pass = 0; pass < guard; ++pass) {
+        // the first TOI over the contact list (b2world.d:1148-1261)
+        int serial = 0;
+        for (int i = lane; i < n; i += 64) {
+            const Contact &c = W.cont[i];
+            if (toi_status(W, c, W.tc[i].y) != 2) continue;
+            const float xa = __int_as_float(W.tb[W.fbody[c.fa]].x), xb = __int_as_float(W.tb[W.fbody[c.fb]].x);
+            if (xa != xb) serial = 1;
+        }
+        serial = __syncthreads_or(serial);
+        if (!serial) {
+            for (int i = lane; i < n; i += 64) {
+                const Contact &c = W.cont[i];
+                const int tcy = W.tc[i].y;
+                if (toi_status(W, c, tcy) != 2) continue;
+                W.tc[i] = make_int2(__float_as_int(contact_toi(W, c.fa, c.fb)), tcy | kTcToi);
+            }
+        } else if (lane == 0) {
+            for (int i = n - 1; i >= 0; --i) { // list order
+                const Contact &c = W.cont[i];
+                const int tcy = W.tc[i].y;
+                if (toi_status(W, c, tcy) != 2) continue;
+                const int ba = W.fbody[c.fa], bb = W.fbody[c.fb];
+                const float xa = __int_as_float(W.tb[ba].x), xb = __int_as_float(W.tb[bb].x);
+                if (xa < xb || xb < xa) { // b2Sweep.Advance of the body at the earlier time
+                    const int b = xa < xb ? ba : bb;
+                    Sweep sw = load_sweep(W, b);
+                    sweep_advance(sw, xa < xb ? xb : xa);
+                    store_sweep(W, b, sw);
+                }
+                W.tc[i] = make_int2(__float_as_int(contact_toi(W, c.fa, c.fb)), tcy | kTcToi);
+            }
+        }
+        __syncthreads();
+        float bestA = 1.0f;
+        int bestPos = INT_MAX;
+        for (int i = lane; i < n; i += 64) {
+            const int2 t = W.tc[i];
+            if (!(t.y & kTcToi) || (t.y & kTcCount) > kMaxSubSteps || !(W.cont[i].flags & kContactEnabled)) continue;
+            const float a = __int_as_float(t.x);
+            const int pos = n - 1 - i;
+            if (a < bestA || (a == bestA && pos < bestPos)) {
+                bestA = a;
+                bestPos = pos;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float oa = __shfl_xor(bestA, o, 64);
+            const int op = __shfl_xor(bestPos, o, 64);
+            if (oa < bestA || (oa == bestA && op < bestPos)) {
+                bestA = oa;
+                bestPos = op;
+            }
+        }
+        if (bestPos == INT_MAX || 1.0f - 10.0f * kEps < bestA) break;
+        const int ci = n - 1 - bestPos;
+        if (lane == 0) s_nb = toi_event(W, ci, n, bestA, sp, s_ibod, s_vc, s_pc, s_imp);
+        __syncthreads();
+        const int nbod = s_nb;
+        if (nbod == 0) continue;
+        ++events;
+        // SynchronizeFixtures of the island's dynamic bodies and their contacts'
+        // cached TOIs dropped (b2world.d:1401-1429)
+        for (int f = lane; f < nf; f += 64) {
+            const int b = W.fbody[f];
+            if (W.tb[b].y == 0 || body_type(W.bfl[b]) != kDynamic) continue;
+            const float4 p0 = W.bpos0[b], ms = W.bmass[b];
+            const Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
+            const Xf x2 = load_xf(W.bxf, b);
+            const float4 a1 = shape_aabb(W.shp + f, x1), a2 = shape_aabb(W.shp + f, x2);
+            const float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+            const float2 disp = sub(x2.p, x1.p);
+            const float4 fa = W.fat[f];
+            if (fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w) continue;
+            float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                     aabb.w + kAabbExtension);
+            const float2 dd = scl(kAabbMultiplier, disp);
+            if (dd.x < 0.0f) nbx.x += dd.x;
+            else nbx.z += dd.x;
+            if (dd.y < 0.0f) nbx.y += dd.y;
+            else nbx.w += dd.y;
+            W.fat[f] = nbx;
+            W.moved[f] = 1;
+        }
+        for (int i = lane; i < n; i += 64) {
+            const int ba = W.fbody[W.cont[i].fa], bb = W.fbody[W.cont[i].fb];
+            const bool da = W.tb[ba].y && body_type(W.bfl[ba]) == kDynamic;
+            const bool db = W.tb[bb].y && body_type(W.bfl[bb]) == kDynamic;
+            if (da || db) W.tc[i].y &= ~(kTcToi | kTcIsland);
+        }
+        if (lane == 0) s_np = 0;
+        __syncthreads();
+        for (int i = lane; i < nbod; i += 64) W.tb[s_ibod[i]].y = 0;
+        // FindNewContacts (b2world.d:1434): every proxy whose fat AABB overlaps a
+        // moved one's, as sorted unique (smaller, larger) pairs
+        for (int m0 = 0; m0 < nf; m0 += 64) {
+            unsigned long long mm = __ballot(m0 + lane < nf && W.moved[m0 + lane] != 0);
+            while (mm) {
+                const int m = m0 + __ffsll((long long)mm) - 1;
+                mm &= mm - 1;
+                const float4 fm = W.fat[m];
+                for (int p = lane; p < nf; p += 64) {
+                    if (p == m || !overlap(fm, W.fat[p])) continue;
+                    const int slot = atomicAdd(&s_np, 1);
+                    const unsigned int lo = p < m ? p : m, hi = p < m ? m : p;
+                    if (slot < kToiPairCap) s_pairs[slot] = ((unsigned long long)lo << 32) | hi;
+                }
+            }
+        }
+        __syncthreads();
+        int np = s_np;
+        if (np > kToiPairCap) {
+            error |= 4;
+            np = 0;
+        }
+        if (lane == 0) { // sort and drop duplicates
+            for (int i = 1; i < np; ++i) {
+                const unsigned long long v = s_pairs[i];
+                int j = i - 1;
+                while (j >= 0 && s_pairs[j] > v) {
+                    s_pairs[j + 1] = s_pairs[j];
+                    --j;
+                }
+                s_pairs[j + 1] = v;
+            }
+            int u = 0;
+            for (int i = 0; i < np; ++i)
+                if (u == 0 || s_pairs[i] != s_pairs[u - 1]) s_pairs[u++] = s_pairs[i];
+            s_np = u;
+        }
+        __syncthreads();
+        np = np ? s_np : 0;
+        // pairs that already have a contact are marked ~0
+        for (int e = 0; e < np; ++e) {
+            const int fa = (int)(s_pairs[e] >> 32), fb = (int)(s_pairs[e] & 0xffffffffu);
+            int found = 0;
+            for (int i = lane; i < n; i += 64) {
+                const Contact &c = W.cont[i];
+                if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) found = 1;
+            }
+            found = __syncthreads_or(found);
+            if (found && lane == 0) s_pairs[e] = ~0ull;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int m = n;
+            for (int e = 0; e < np; ++e) {
+                if (s_pairs[e] == ~0ull) continue;
+                if (toi_add_pair(W, (int)(s_pairs[e] >> 32), (int)(s_pairs[e] & 0xffffffffu), m, wd.contactCap)) ++m;
+            }
+            if (m > wd.contactCap) {
+                error |= 1;
+                m = wd.contactCap;
+            }
+            s_n = m;
+        }
+        for (int f = lane; f < nf; f += 64) W.moved[f] = 0;
+        __syncthreads();
+        n = s_n;
+    }
+    if (lane == 0) {
+        d.S[w].toiEvents = events;
+        d.S[w].contactCount = n;
+        if (error) d.S[w].error |= error;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Device-wide broadphase of one big world: the pair set and order of bp_pairs
 // (every overlapping (i < j) with i or j moved, sorted by (i, j) as
 // b2PairLessThan, b2broadphase.d:139-195), with each stage spread over many
@@ -2558,6 +3058,12 @@ hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int wo
     return hipGetLastError();
 }
 size_t broadphase_scratch_bytes(int proxies) { return bp_bytes(proxies); }
+hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, bool lead,
+                      hipStream_t s) {
+    if (nworlds <= 0) return hipSuccess;
+    DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(64), 0, s, d, cur, sp, worldOffset);
+    return hipGetLastError();
+}
 hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                              hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;

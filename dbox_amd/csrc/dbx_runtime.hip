@@ -408,7 +408,7 @@ struct GraphEntry {
 constexpr int kMaxGraphs = 8;
 constexpr int kTuneBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kTuneCandidates = 5;
-constexpr int kStepEvents = 6;
+constexpr int kStepEvents = 7;
 
 struct dbx_handle {
     dbx_config cfg{};
@@ -786,6 +786,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.vc, nc));
     DBX_HIP(h, dalloc(h, &d.pc, nc));
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
+    DBX_HIP(h, dalloc(h, &d.toiC, nc));
+    DBX_HIP(h, dalloc(h, &d.toiB, nb));
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
     if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
@@ -830,11 +832,15 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
         else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, T.block, lead, s));
         if (timed) (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
+        if (timed) (void)hipEventRecord(ev[4], s);
+        // b2World.SolveTOI when continuous physics is on (b2world.d:414-419)
+        if (flags & DBX_STEP_CCD) DBX_HIP(h, launch_toi(d, cout, sp, T.first, T.count, lead, s));
     } else if (timed) {
         (void)hipEventRecord(ev[2], s);
         (void)hipEventRecord(ev[3], s);
+        (void)hipEventRecord(ev[4], s);
     }
-    if (timed) (void)hipEventRecord(ev[4], s);
+    if (timed) (void)hipEventRecord(ev[5], s);
     return DBX_OK;
 }
 
@@ -857,7 +863,7 @@ static dbx_status issue_step(dbx_handle *h, const StepParams &sp, uint32_t flags
         DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
         DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
     }
-    if (ev) DBX_HIP(h, hipEventRecord(ev[5], s));
+    if (ev) DBX_HIP(h, hipEventRecord(ev[6], s));
     return DBX_OK;
 }
 
@@ -920,7 +926,7 @@ static dbx_status step_graph(dbx_handle *h, const StepParams &sp, uint32_t flags
         DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
         DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
     }
-    DBX_HIP(h, hipEventRecord(ev[5], s));
+    DBX_HIP(h, hipEventRecord(ev[6], s));
     return DBX_OK;
 }
 
@@ -954,7 +960,6 @@ static dbx_status tune_account(dbx_handle *h) {
 dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t flags) {
     if (!h) return DBX_E_INVALID_ARG;
     if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
-    if (flags & DBX_STEP_CCD) return fail(h, DBX_E_UNSUPPORTED, "continuous physics (TOI) is not implemented");
     if (!std::isfinite(dt) || vel < 0 || pos < 0) return fail(h, DBX_E_INVALID_ARG, "bad step parameters");
     if ((flags & DBX_STEP_LEARNED) && !h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
     if ((flags & DBX_STEP_LEARNED) && !(flags & DBX_STEP_WARM))
@@ -1048,10 +1053,10 @@ dbx_status dbx_sync(dbx_handle *h) {
     }
     for (int k = 0; k < h->ringUsed; ++k) {
         hipEvent_t *ev = &h->ring[kStepEvents * k];
-        float ms[4] = {0, 0, 0, 0}, total = 0.0f;
+        float ms[5] = {0, 0, 0, 0, 0}, total = 0.0f;
         if (!h->ringGraph[k]) // a replayed graph records only the step's ends
-            for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
-        (void)hipEventElapsedTime(&total, ev[0], ev[5]);
+            for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        (void)hipEventElapsedTime(&total, ev[0], ev[6]);
         dbx_profile p{};
         p.collide = ms[0];
         p.solveInit = ms[1];
@@ -1059,7 +1064,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         p.solvePosition = 0.0f;
         p.solve = ms[1] + ms[2] + ms[3];
         p.broadphase = ms[3];
-        p.solveTOI = 0.0f;
+        p.solveTOI = ms[4];
         p.step = total;
         h->prof = p;
         h->sum.step += p.step;
@@ -1068,6 +1073,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->sum.solveInit += p.solveInit;
         h->sum.solveVelocity += p.solveVelocity;
         h->sum.broadphase += p.broadphase;
+        h->sum.solveTOI += p.solveTOI;
         h->sumSteps++;
     }
     h->ringUsed = 0;
@@ -1383,6 +1389,17 @@ dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_
         DBX_HIP(h, hipMemcpyAsync(max_penetration, h->dPen, h->nworlds * sizeof(float), hipMemcpyDeviceToHost, h->stream));
         DBX_HIP(h, hipStreamSynchronize(h->stream));
     }
+    return DBX_OK;
+}
+
+dbx_status dbx_read_toi_events(dbx_handle *h, int32_t *events, int32_t nworlds) {
+    if (!h || !events) return DBX_E_INVALID_ARG;
+    if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (nworlds < h->nworlds) return fail(h, DBX_E_INVALID_ARG, "output too small");
+    std::vector<WorldState> S(h->nworlds);
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
+    for (int w = 0; w < h->nworlds; ++w) events[w] = S[w].toiEvents;
     return DBX_OK;
 }
 

@@ -65,7 +65,7 @@ EXPORTS = [
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
-    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals",
+    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
@@ -114,6 +114,7 @@ def lib():
         L.dbx_set_tier_blocks.argtypes = [vp, vp, i32]
         L.dbx_set_graph.argtypes = [vp, i32]
         L.dbx_read_residuals.argtypes = [vp, vp, i32]
+        L.dbx_read_toi_events.argtypes = [vp, vp, i32]
         L.dbx_mlp_init.argtypes = [vp, u64]
         L.dbx_mlp_set.argtypes = [vp, vp, vp]
         L.dbx_mlp_get.argtypes = [vp, vp, vp]
@@ -328,6 +329,12 @@ class StepWorlds:
     def set_tier_blocks(self, blocks):
         b = np.ascontiguousarray(blocks, np.int32)
         self._check(self.L.dbx_set_tier_blocks(self.h, _p(b), len(b)))
+
+    def toi_events(self):
+        """Per world: TOI events solved by the last step's continuous pass."""
+        e = np.zeros(self.world_count, np.int32)
+        self._check(self.L.dbx_read_toi_events(self.h, _p(e), len(e)))
+        return e
 
     def residuals(self):
         r = np.zeros(self.world_count, np.float32)

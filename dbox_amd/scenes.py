@@ -305,3 +305,59 @@ def single_box_on_ground(y: float = 0.5, hx: float = 0.5, density: float = 1.0, 
     f[1]["density"] = density
     f[1]["friction"] = friction
     return Scene(b, f, name="box_on_ground")
+
+
+def projectile_room(n: int = 40, seed: int | None = None, size: float = 20.0, speed: float = 150.0) -> Scene:
+    """n small fast bodies (circles, boxes and polygons of 0.1-0.3 m) thrown at up
+    to `speed` m/s inside a closed square room of four static edges: the workload
+    of the continuous pass (b2World.SolveTOI), since without it most of them leave
+    the room through its walls within a few steps."""
+    rng = Stream(config_seed(6) if seed is None else seed)
+    kind = rng.integers(n, 0, 2)
+    r = rng.uniform(n, 0.1, 0.3)
+    hx = rng.uniform(n, 0.1, 0.3)
+    hy = rng.uniform(n, 0.1, 0.3)
+    nverts = rng.integers(n, 3, 8)
+    jit = rng.uniform(n * 8, -0.3, 0.3).reshape(n, 8)
+    half = 0.5 * size
+    px = rng.uniform(n, -half + 1.0, half - 1.0)
+    py = rng.uniform(n, 1.0, size - 1.0)
+    heading = rng.uniform(n, 0.0, 2.0 * math.pi)
+    mag = rng.uniform(n, 0.5, 1.0) * speed
+    w = rng.uniform(n, -10.0, 10.0)
+    angle = rng.uniform(n, 0.0, 2.0 * math.pi)
+
+    b = body_defs(n + 1)
+    b[0]["type"] = STATIC
+    d = b[1:]
+    d["type"] = DYNAMIC
+    d["px"] = px
+    d["py"] = py
+    d["angle"] = angle
+    d["vx"] = mag * np.cos(heading)
+    d["vy"] = mag * np.sin(heading)
+    d["w"] = w
+    f = fixture_defs(n + 4)
+    _edge(f[0], 0, -half, 0.0, half, 0.0, 0.6)
+    _edge(f[1], 0, -half, size, half, size, 0.6)
+    _edge(f[2], 0, -half, 0.0, -half, size, 0.6)
+    _edge(f[3], 0, half, 0.0, half, size, 0.6)
+    g = f[4:]
+    g["body"] = np.arange(1, n + 1)
+    g["density"] = 1.0
+    g["friction"] = 0.4
+    g["shape"] = np.choose(kind, [SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON])
+    g["radius"] = np.where(kind == 0, r, 0.0)
+    g["count"] = np.where(kind == 2, nverts, 0)
+    pts = np.zeros((n, 20), np.float32)
+    k = np.arange(8)[None, :]
+    ang = 2.0 * math.pi * (k + jit) / nverts[:, None]
+    valid = k < nverts[:, None]
+    pts[:, 0:16:2] = r[:, None] * np.cos(ang) * valid
+    pts[:, 1:16:2] = r[:, None] * np.sin(ang) * valid
+    pts[kind == 1, 0] = hx[kind == 1]
+    pts[kind == 1, 1] = hy[kind == 1]
+    pts[kind == 1, 2:] = 0.0
+    pts[kind == 0] = 0.0
+    g["p"] = pts
+    return Scene(b, f, name=f"projectiles{n}", meta={"size": size})

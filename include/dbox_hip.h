@@ -44,7 +44,7 @@ enum {
 enum {
     DBX_STEP_WARM = 1u << 0,    /* warm starting (default on in dbox) */
     DBX_STEP_SLEEP = 1u << 1,   /* allow sleeping (default on in dbox) */
-    DBX_STEP_CCD = 1u << 2,     /* continuous physics: not implemented -> DBX_E_UNSUPPORTED */
+    DBX_STEP_CCD = 1u << 2,     /* continuous physics: b2World.SolveTOI after the discrete solve (default on in dbox) */
     DBX_STEP_ORDERED = 1u << 3, /* solve in the reference's exact island order (dependency levels) */
     DBX_STEP_LEARNED = 1u << 4, /* MLP warm start for new contact points */
     DBX_STEP_RESIDUAL = 1u << 5, /* iterate velocities to a residual instead of a fixed count */
@@ -196,6 +196,12 @@ dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_
  * step in RESIDUAL mode (-1 otherwise).  With dbx_set_residual(h, 0, n) a step
  * runs exactly n iterations, so this is the residual after n iterations. */
 dbx_status dbx_read_residuals(dbx_handle *h, float *residual, int32_t nworlds);
+
+/* Per-world number of TOI events the last step's continuous pass solved (the
+ * sub-step islands of b2World.SolveTOI, b2world.d:1127-1452); 0 without
+ * DBX_STEP_CCD.  The reference counts them nowhere; dbox's b2Profile.solveTOI
+ * times the pass (dbx_profile.solveTOI here). */
+dbx_status dbx_read_toi_events(dbx_handle *h, int32_t *events, int32_t nworlds);
 
 /* Launch tiers (no counterpart in the reference; worlds are independent there):
  * worlds are launched largest first, in tiers of similar size that each run

@@ -17,7 +17,14 @@
  *                              momentum conservation, sleep after 0.5 s
  *   multi-step stacking order  parity unpinned against the dbox binary (no D compiler):
  *                              pinned only to this line-by-line restatement.
- *   TOI / SolveTOI             not restated (continuous physics must be off).
+ *   b2Distance                 brute-force polygon/segment/point distance, circle closed form
+ *                              (tests/test_oracle_toi.py)
+ *   b2TimeOfImpact             closed-form impact time of a translating box over an edge;
+ *                              brute-force sampling of the swept distance (no earlier
+ *                              crossing of the target, separation at the returned time)
+ *   SolveTOI                   a fast box over a thin edge stops on it instead of tunnelling;
+ *                              the event's end state against the closed form; the sub-step
+ *                              island order is pinned only to this restatement.
  */
 #include "b2oracle.h"
 
@@ -1250,12 +1257,513 @@ enum {
     bf_island = 0x0001, bf_awake = 0x0002, bf_autoSleep = 0x0004, bf_bullet = 0x0008,
     bf_fixedRotation = 0x0010, bf_active = 0x0020
 };
-enum { cf_island = 0x0001, cf_touching = 0x0002, cf_enabled = 0x0004, cf_filter = 0x0008 };
+enum { cf_island = 0x0001, cf_touching = 0x0002, cf_enabled = 0x0004, cf_filter = 0x0008, cf_toi = 0x0020 };
+constexpr int kMaxSubSteps = 8;     /* b2settings.d:105 */
+constexpr int kMaxTOIContacts = 32; /* b2settings.d:110 */
+constexpr float kToiBaumgarte = 0.75f; /* b2settings.d:138 */
 
 struct Sweep {
     V2 localCenter, c0, c;
     float a0 = 0, a = 0, alpha0 = 0;
+    /* b2Sweep.GetTransform b2math.d:556-564 */
+    void getTransform(Xf &xf, float beta) const {
+        xf.p = (1.0f - beta) * c0 + beta * c;
+        float angle = (1.0f - beta) * a0 + beta * a;
+        xf.q.set(angle);
+        xf.p -= mul(xf.q, localCenter);
+    }
+    /* b2Sweep.Advance b2math.d:568-575 */
+    void advance(float alpha) {
+        float beta = (alpha - alpha0) / (1.0f - alpha0);
+        c0 += beta * (c - c0);
+        a0 += beta * (a - a0);
+        alpha0 = alpha;
+    }
+    /* b2Sweep.Normalize b2math.d:578-584 */
+    void normalize() {
+        float twoPi = 2.0f * kPi;
+        float d = twoPi * floorf(a0 / twoPi);
+        a0 -= d;
+        a -= d;
+    }
 };
+
+/* ---- b2Distance: GJK with a simplex cache (b2distance.d:30-707) ---- */
+struct DistanceProxy {
+    V2 buf[2];
+    const V2 *vertices = nullptr;
+    int count = 0;
+    float radius = 0;
+    DistanceProxy() = default;
+    DistanceProxy(const DistanceProxy &) = delete; /* vertices may point into buf */
+    /* b2DistanceProxy.Set b2distance.d:34-91 (no chain shapes) */
+    void set(const Shape &s) {
+        if (s.type == s_circle) {
+            vertices = &s.p;
+            count = 1;
+        } else if (s.type == s_polygon) {
+            vertices = s.v;
+            count = s.count;
+        } else {
+            buf[0] = s.v1;
+            buf[1] = s.v2;
+            vertices = buf;
+            count = 2;
+        }
+        radius = s.radius;
+    }
+    /* GetSupport b2distance.d:94-111 */
+    int support(V2 d) const {
+        int bestIndex = 0;
+        float bestValue = dot(vertices[0], d);
+        for (int i = 1; i < count; ++i) {
+            float value = dot(vertices[i], d);
+            if (value > bestValue) {
+                bestIndex = i;
+                bestValue = value;
+            }
+        }
+        return bestIndex;
+    }
+};
+struct SimplexCache {
+    float metric = 0;
+    int count = 0;
+    int indexA[3] = {0, 0, 0}, indexB[3] = {0, 0, 0};
+};
+struct SimplexVertex {
+    V2 wA, wB, w;
+    float a = 0;
+    int indexA = 0, indexB = 0;
+};
+struct Simplex {
+    SimplexVertex v[3];
+    int count = 0;
+    /* ReadCache b2distance.d:574-624 */
+    void readCache(const SimplexCache &cache, const DistanceProxy &pA, const Xf &xfA, const DistanceProxy &pB,
+                   const Xf &xfB) {
+        count = cache.count;
+        for (int i = 0; i < count; ++i) {
+            SimplexVertex &s = v[i];
+            s.indexA = cache.indexA[i];
+            s.indexB = cache.indexB[i];
+            s.wA = mul(xfA, pA.vertices[s.indexA]);
+            s.wB = mul(xfB, pB.vertices[s.indexB]);
+            s.w = s.wB - s.wA;
+            s.a = 0.0f;
+        }
+        if (count > 1) {
+            float metric1 = cache.metric;
+            float metric2 = metric();
+            if (metric2 < 0.5f * metric1 || 2.0f * metric1 < metric2 || metric2 < kEps) count = 0;
+        }
+        if (count == 0) {
+            SimplexVertex &s = v[0];
+            s.indexA = 0;
+            s.indexB = 0;
+            s.wA = mul(xfA, pA.vertices[0]);
+            s.wB = mul(xfB, pB.vertices[0]);
+            s.w = s.wB - s.wA;
+            s.a = 1.0f;
+            count = 1;
+        }
+    }
+    /* WriteCache b2distance.d:626-637 */
+    void writeCache(SimplexCache &cache) const {
+        cache.metric = metric();
+        cache.count = count;
+        for (int i = 0; i < count; ++i) {
+            cache.indexA[i] = v[i].indexA;
+            cache.indexB[i] = v[i].indexB;
+        }
+    }
+    /* GetSearchDirection b2distance.d:639-662 */
+    V2 searchDirection() const {
+        if (count == 1) return -v[0].w;
+        V2 e12 = v[1].w - v[0].w;
+        float sgn = cross(e12, -v[0].w);
+        return sgn > 0.0f ? cross(1.0f, e12) : cross(e12, 1.0f);
+    }
+    /* GetClosestPoint b2distance.d:664-684 */
+    V2 closestPoint() const {
+        if (count == 1) return v[0].w;
+        if (count == 2) return v[0].a * v[0].w + v[1].a * v[1].w;
+        return mk(0.0f, 0.0f);
+    }
+    /* GetWitnessPoints b2distance.d:686-708 */
+    void witnessPoints(V2 &pA, V2 &pB) const {
+        if (count == 1) {
+            pA = v[0].wA;
+            pB = v[0].wB;
+        } else if (count == 2) {
+            pA = v[0].a * v[0].wA + v[1].a * v[1].wA;
+            pB = v[0].a * v[0].wB + v[1].a * v[1].wB;
+        } else {
+            pA = v[0].a * v[0].wA + v[1].a * v[1].wA + v[2].a * v[2].wA;
+            pB = pA;
+        }
+    }
+    /* GetMetric b2distance.d:710-728 */
+    float metric() const {
+        if (count == 1) return 0.0f;
+        if (count == 2) return length(v[0].w - v[1].w);
+        return cross(v[1].w - v[0].w, v[2].w - v[0].w);
+    }
+    /* Solve2 b2distance.d:373-406 */
+    void solve2() {
+        V2 w1 = v[0].w, w2 = v[1].w;
+        V2 e12 = w2 - w1;
+        float d12_2 = -dot(w1, e12);
+        if (d12_2 <= 0.0f) {
+            v[0].a = 1.0f;
+            count = 1;
+            return;
+        }
+        float d12_1 = dot(w2, e12);
+        if (d12_1 <= 0.0f) {
+            v[1].a = 1.0f;
+            count = 1;
+            v[0] = v[1];
+            return;
+        }
+        float inv_d12 = 1.0f / (d12_1 + d12_2);
+        v[0].a = d12_1 * inv_d12;
+        v[1].a = d12_2 * inv_d12;
+        count = 2;
+    }
+    /* Solve3 b2distance.d:413-571 */
+    void solve3() {
+        V2 w1 = v[0].w, w2 = v[1].w, w3 = v[2].w;
+        V2 e12 = w2 - w1;
+        float w1e12 = dot(w1, e12), w2e12 = dot(w2, e12);
+        float d12_1 = w2e12, d12_2 = -w1e12;
+        V2 e13 = w3 - w1;
+        float w1e13 = dot(w1, e13), w3e13 = dot(w3, e13);
+        float d13_1 = w3e13, d13_2 = -w1e13;
+        V2 e23 = w3 - w2;
+        float w2e23 = dot(w2, e23), w3e23 = dot(w3, e23);
+        float d23_1 = w3e23, d23_2 = -w2e23;
+        float n123 = cross(e12, e13);
+        float d123_1 = n123 * cross(w2, w3);
+        float d123_2 = n123 * cross(w3, w1);
+        float d123_3 = n123 * cross(w1, w2);
+        if (d12_2 <= 0.0f && d13_2 <= 0.0f) {
+            v[0].a = 1.0f;
+            count = 1;
+            return;
+        }
+        if (d12_1 > 0.0f && d12_2 > 0.0f && d123_3 <= 0.0f) {
+            float inv_d12 = 1.0f / (d12_1 + d12_2);
+            v[0].a = d12_1 * inv_d12;
+            v[1].a = d12_2 * inv_d12;
+            count = 2;
+            return;
+        }
+        if (d13_1 > 0.0f && d13_2 > 0.0f && d123_2 <= 0.0f) {
+            float inv_d13 = 1.0f / (d13_1 + d13_2);
+            v[0].a = d13_1 * inv_d13;
+            v[2].a = d13_2 * inv_d13;
+            count = 2;
+            v[1] = v[2];
+            return;
+        }
+        if (d12_1 <= 0.0f && d23_2 <= 0.0f) {
+            v[1].a = 1.0f;
+            count = 1;
+            v[0] = v[1];
+            return;
+        }
+        if (d13_1 <= 0.0f && d23_1 <= 0.0f) {
+            v[2].a = 1.0f;
+            count = 1;
+            v[0] = v[2];
+            return;
+        }
+        if (d23_1 > 0.0f && d23_2 > 0.0f && d123_1 <= 0.0f) {
+            float inv_d23 = 1.0f / (d23_1 + d23_2);
+            v[1].a = d23_1 * inv_d23;
+            v[2].a = d23_2 * inv_d23;
+            count = 2;
+            v[0] = v[2];
+            return;
+        }
+        float inv_d123 = 1.0f / (d123_1 + d123_2 + d123_3);
+        v[0].a = d123_1 * inv_d123;
+        v[1].a = d123_2 * inv_d123;
+        v[2].a = d123_3 * inv_d123;
+        count = 3;
+    }
+};
+struct DistanceOutput {
+    V2 pointA, pointB;
+    float distance = 0;
+    int iterations = 0;
+};
+/* b2Distance b2distance.d:185-345 */
+static void gjkDistance(DistanceOutput &output, SimplexCache &cache, const DistanceProxy &proxyA, const Xf &transformA,
+                        const DistanceProxy &proxyB, const Xf &transformB, bool useRadii) {
+    Simplex simplex;
+    simplex.readCache(cache, proxyA, transformA, proxyB, transformB);
+    const int k_maxIters = 20;
+    int saveA[3], saveB[3];
+    int iter = 0;
+    while (iter < k_maxIters) {
+        int saveCount = simplex.count;
+        for (int i = 0; i < saveCount; ++i) {
+            saveA[i] = simplex.v[i].indexA;
+            saveB[i] = simplex.v[i].indexB;
+        }
+        if (simplex.count == 2) simplex.solve2();
+        else if (simplex.count == 3) simplex.solve3();
+        if (simplex.count == 3) break;
+        V2 d = simplex.searchDirection();
+        if (lengthSq(d) < kEps * kEps) break;
+        SimplexVertex &vertex = simplex.v[simplex.count];
+        vertex.indexA = proxyA.support(mulT(transformA.q, -d));
+        vertex.wA = mul(transformA, proxyA.vertices[vertex.indexA]);
+        vertex.indexB = proxyB.support(mulT(transformB.q, d));
+        vertex.wB = mul(transformB, proxyB.vertices[vertex.indexB]);
+        vertex.w = vertex.wB - vertex.wA;
+        ++iter;
+        bool duplicate = false;
+        for (int i = 0; i < saveCount; ++i) {
+            if (vertex.indexA == saveA[i] && vertex.indexB == saveB[i]) {
+                duplicate = true;
+                break;
+            }
+        }
+        if (duplicate) break;
+        ++simplex.count;
+    }
+    simplex.witnessPoints(output.pointA, output.pointB);
+    output.distance = length(output.pointA - output.pointB);
+    output.iterations = iter;
+    simplex.writeCache(cache);
+    if (useRadii) {
+        float rA = proxyA.radius, rB = proxyB.radius;
+        if (output.distance > rA + rB && output.distance > kEps) {
+            output.distance -= rA + rB;
+            V2 normal = output.pointB - output.pointA;
+            normalize(normal);
+            output.pointA += rA * normal;
+            output.pointB -= rB * normal;
+        } else {
+            V2 p = 0.5f * (output.pointA + output.pointB);
+            output.pointA = p;
+            output.pointB = p;
+            output.distance = 0.0f;
+        }
+    }
+}
+
+/* ---- b2TimeOfImpact: conservative advancement on separating axes (b2timeofimpact.d:67-530) ---- */
+enum { toi_unknown = 0, toi_failed = 1, toi_overlapped = 2, toi_touching = 3, toi_separated = 4 };
+struct SeparationFunction {
+    enum { e_points, e_faceA, e_faceB };
+    const DistanceProxy *proxyA = nullptr, *proxyB = nullptr;
+    Sweep sweepA, sweepB;
+    int type = e_points;
+    V2 localPoint, axis;
+    /* Initialize b2timeofimpact.d:331-410 */
+    float initialize(const SimplexCache &cache, const DistanceProxy &pA, const Sweep &sA, const DistanceProxy &pB,
+                     const Sweep &sB, float t1) {
+        proxyA = &pA;
+        proxyB = &pB;
+        int count = cache.count;
+        sweepA = sA;
+        sweepB = sB;
+        Xf xfA, xfB;
+        sweepA.getTransform(xfA, t1);
+        sweepB.getTransform(xfB, t1);
+        if (count == 1) {
+            type = e_points;
+            V2 pointA = mul(xfA, pA.vertices[cache.indexA[0]]);
+            V2 pointB = mul(xfB, pB.vertices[cache.indexB[0]]);
+            axis = pointB - pointA;
+            return normalize(axis);
+        } else if (cache.indexA[0] == cache.indexA[1]) {
+            type = e_faceB;
+            V2 localPointB1 = pB.vertices[cache.indexB[0]];
+            V2 localPointB2 = pB.vertices[cache.indexB[1]];
+            axis = cross(localPointB2 - localPointB1, 1.0f);
+            normalize(axis);
+            V2 normal = mul(xfB.q, axis);
+            localPoint = 0.5f * (localPointB1 + localPointB2);
+            V2 pointB = mul(xfB, localPoint);
+            V2 pointA = mul(xfA, pA.vertices[cache.indexA[0]]);
+            float s = dot(pointA - pointB, normal);
+            if (s < 0.0f) {
+                axis = -axis;
+                s = -s;
+            }
+            return s;
+        } else {
+            type = e_faceA;
+            V2 localPointA1 = pA.vertices[cache.indexA[0]];
+            V2 localPointA2 = pA.vertices[cache.indexA[1]];
+            axis = cross(localPointA2 - localPointA1, 1.0f);
+            normalize(axis);
+            V2 normal = mul(xfA.q, axis);
+            localPoint = 0.5f * (localPointA1 + localPointA2);
+            V2 pointA = mul(xfA, localPoint);
+            V2 pointB = mul(xfB, pB.vertices[cache.indexB[0]]);
+            float s = dot(pointB - pointA, normal);
+            if (s < 0.0f) {
+                axis = -axis;
+                s = -s;
+            }
+            return s;
+        }
+    }
+    /* FindMinSeparation b2timeofimpact.d:413-470 */
+    float findMinSeparation(int &indexA, int &indexB, float t) const {
+        Xf xfA, xfB;
+        sweepA.getTransform(xfA, t);
+        sweepB.getTransform(xfB, t);
+        if (type == e_points) {
+            V2 axisA = mulT(xfA.q, axis);
+            V2 axisB = mulT(xfB.q, -axis);
+            indexA = proxyA->support(axisA);
+            indexB = proxyB->support(axisB);
+            V2 pointA = mul(xfA, proxyA->vertices[indexA]);
+            V2 pointB = mul(xfB, proxyB->vertices[indexB]);
+            return dot(pointB - pointA, axis);
+        } else if (type == e_faceA) {
+            V2 normal = mul(xfA.q, axis);
+            V2 pointA = mul(xfA, localPoint);
+            V2 axisB = mulT(xfB.q, -normal);
+            indexA = -1;
+            indexB = proxyB->support(axisB);
+            V2 pointB = mul(xfB, proxyB->vertices[indexB]);
+            return dot(pointB - pointA, normal);
+        } else {
+            V2 normal = mul(xfB.q, axis);
+            V2 pointB = mul(xfB, localPoint);
+            V2 axisA = mulT(xfA.q, -normal);
+            indexB = -1;
+            indexA = proxyA->support(axisA);
+            V2 pointA = mul(xfA, proxyA->vertices[indexA]);
+            return dot(pointA - pointB, normal);
+        }
+    }
+    /* Evaluate b2timeofimpact.d:473-518 */
+    float evaluate(int indexA, int indexB, float t) const {
+        Xf xfA, xfB;
+        sweepA.getTransform(xfA, t);
+        sweepB.getTransform(xfB, t);
+        if (type == e_points) {
+            V2 pointA = mul(xfA, proxyA->vertices[indexA]);
+            V2 pointB = mul(xfB, proxyB->vertices[indexB]);
+            return dot(pointB - pointA, axis);
+        } else if (type == e_faceA) {
+            V2 normal = mul(xfA.q, axis);
+            V2 pointA = mul(xfA, localPoint);
+            V2 pointB = mul(xfB, proxyB->vertices[indexB]);
+            return dot(pointB - pointA, normal);
+        } else {
+            V2 normal = mul(xfB.q, axis);
+            V2 pointB = mul(xfB, localPoint);
+            V2 pointA = mul(xfA, proxyA->vertices[indexA]);
+            return dot(pointA - pointB, normal);
+        }
+    }
+};
+/* b2TimeOfImpact b2timeofimpact.d:67-295; returns the state, t in [0, tMax] */
+static int timeOfImpact(float &tOut, const DistanceProxy &proxyA, const DistanceProxy &proxyB, Sweep sweepA, Sweep sweepB,
+                        float tMax) {
+    int state = toi_unknown;
+    tOut = tMax;
+    sweepA.normalize();
+    sweepB.normalize();
+    float totalRadius = proxyA.radius + proxyB.radius;
+    float target = fmax_(kLinearSlop, totalRadius - 3.0f * kLinearSlop);
+    float tolerance = 0.25f * kLinearSlop;
+    float t1 = 0.0f;
+    const int k_maxIterations = 20;
+    int iter = 0;
+    SimplexCache cache;
+    cache.count = 0;
+    for (;;) {
+        Xf xfA, xfB;
+        sweepA.getTransform(xfA, t1);
+        sweepB.getTransform(xfB, t1);
+        DistanceOutput distanceOutput;
+        gjkDistance(distanceOutput, cache, proxyA, xfA, proxyB, xfB, false);
+        if (distanceOutput.distance <= 0.0f) {
+            state = toi_overlapped;
+            tOut = 0.0f;
+            break;
+        }
+        if (distanceOutput.distance < target + tolerance) {
+            state = toi_touching;
+            tOut = t1;
+            break;
+        }
+        SeparationFunction fcn;
+        fcn.initialize(cache, proxyA, sweepA, proxyB, sweepB, t1);
+        bool done = false;
+        float t2 = tMax;
+        int pushBackIter = 0;
+        for (;;) {
+            int indexA, indexB;
+            float s2 = fcn.findMinSeparation(indexA, indexB, t2);
+            if (s2 > target + tolerance) {
+                state = toi_separated;
+                tOut = tMax;
+                done = true;
+                break;
+            }
+            if (s2 > target - tolerance) {
+                t1 = t2;
+                break;
+            }
+            float s1 = fcn.evaluate(indexA, indexB, t1);
+            if (s1 < target - tolerance) {
+                state = toi_failed;
+                tOut = t1;
+                done = true;
+                break;
+            }
+            if (s1 <= target + tolerance) {
+                state = toi_touching;
+                tOut = t1;
+                done = true;
+                break;
+            }
+            int rootIterCount = 0;
+            float a1 = t1, a2 = t2;
+            for (;;) {
+                float t;
+                if (rootIterCount & 1) t = a1 + (target - s1) * (a2 - a1) / (s2 - s1);
+                else t = 0.5f * (a1 + a2);
+                ++rootIterCount;
+                float s = fcn.evaluate(indexA, indexB, t);
+                if (fabs_(s - target) < tolerance) {
+                    t2 = t;
+                    break;
+                }
+                if (s > target) {
+                    a1 = t;
+                    s1 = s;
+                } else {
+                    a2 = t;
+                    s2 = s;
+                }
+                if (rootIterCount == 50) break;
+            }
+            ++pushBackIter;
+            if (pushBackIter == kMaxPolygonVertices) break;
+        }
+        ++iter;
+        if (done) break;
+        if (iter == k_maxIterations) {
+            state = toi_failed;
+            tOut = t1;
+            break;
+        }
+    }
+    return state;
+}
 struct Body;
 struct Fixture {
     int index = 0;
@@ -1302,6 +1810,14 @@ struct Body {
         xf.q.set(sweep.a);
         xf.p = sweep.c - mul(xf.q, sweep.localCenter);
     }
+    /* b2Body.Advance b2body.d:1172-1180 */
+    void advance(float alpha) {
+        sweep.advance(alpha);
+        sweep.c = sweep.c0;
+        sweep.a = sweep.a0;
+        xf.q.set(sweep.a);
+        xf.p = sweep.c - mul(xf.q, sweep.localCenter);
+    }
 };
 struct ContactEdge {
     int other = kNull;
@@ -1315,6 +1831,8 @@ struct Contact {
     Manifold manifold;
     float friction = 0, restitution = 0, tangentSpeed = 0;
     bool alive = false;
+    int toiCount = 0;
+    float toi = 0;
 };
 
 struct VelocityConstraintPoint {
@@ -1367,6 +1885,8 @@ struct World {
     BroadPhase bp;
     bool warmStarting = true, continuousPhysics = false, allowSleep = true;
     bool newFixture = false;
+    bool stepComplete = true;
+    int toiEvents = 0; /* test hook: TOI events solved by the last step */
     float inv_dt0 = 0.0f;
     int solveOrder = 0;
     int islandCount = 0; /* test hook: 1 = solve island contacts in greedy-colour order */
@@ -2223,6 +2743,266 @@ struct World {
         profile[6] = msSince(t0);
     }
 
+    /* b2ContactSolver.SolveTOIPositionConstraints b2contactsolver.d:152-242 */
+    bool solveTOIPositionConstraints(int toiIndexA, int toiIndexB) {
+        float minSeparation = 0.0f;
+        for (size_t i = 0; i < pcs.size(); ++i) {
+            const PositionConstraint &pc = pcs[i];
+            int indexA = pc.indexA, indexB = pc.indexB;
+            V2 localCenterA = pc.localCenterA, localCenterB = pc.localCenterB;
+            int pointCount = pc.pointCount;
+            float mA = 0.0f, iA = 0.0f;
+            if (indexA == toiIndexA || indexA == toiIndexB) {
+                mA = pc.invMassA;
+                iA = pc.invIA;
+            }
+            float mB = 0.0f, iB = 0.0f;
+            if (indexB == toiIndexA || indexB == toiIndexB) {
+                mB = pc.invMassB;
+                iB = pc.invIB;
+            }
+            V2 cA = positions[indexA].c;
+            float aA = positions[indexA].a;
+            V2 cB = positions[indexB].c;
+            float aB = positions[indexB].a;
+            for (int j = 0; j < pointCount; ++j) {
+                Xf xfA, xfB;
+                xfA.q.set(aA);
+                xfB.q.set(aB);
+                xfA.p = cA - mul(xfA.q, localCenterA);
+                xfB.p = cB - mul(xfB.q, localCenterB);
+                V2 normal, point;
+                float separation = 0.0f;
+                positionManifold(pc, xfA, xfB, j, normal, point, separation);
+                V2 rA = point - cA;
+                V2 rB = point - cB;
+                minSeparation = fmin_(minSeparation, separation);
+                float C = clampf(kToiBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+                float rnA = cross(rA, normal);
+                float rnB = cross(rB, normal);
+                float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+                float impulse = K > 0.0f ? -C / K : 0.0f;
+                V2 P = impulse * normal;
+                cA -= mA * P;
+                aA -= iA * cross(rA, P);
+                cB += mB * P;
+                aB += iB * cross(rB, P);
+            }
+            positions[indexA].c = cA;
+            positions[indexA].a = aA;
+            positions[indexB].c = cB;
+            positions[indexB].a = aB;
+        }
+        return minSeparation >= -1.5f * kLinearSlop;
+    }
+
+    /* b2Island.SolveTOI b2island.d:282-416 */
+    void solveIslandTOI(const TimeStep &subStep, int toiIndexA, int toiIndexB) {
+        int nb = (int)islandBodies.size();
+        positions.resize(nb);
+        velocities.resize(nb);
+        for (int i = 0; i < nb; ++i) {
+            const Body &b = bodies[islandBodies[i]];
+            positions[i].c = b.sweep.c;
+            positions[i].a = b.sweep.a;
+            velocities[i].v = b.v;
+            velocities[i].w = b.w;
+        }
+        solverCreate(subStep);
+        for (int i = 0; i < subStep.positionIterations; ++i)
+            if (solveTOIPositionConstraints(toiIndexA, toiIndexB)) break;
+        /* leap of faith to the new safe state */
+        Body &ta = bodies[islandBodies[toiIndexA]];
+        ta.sweep.c0 = positions[toiIndexA].c;
+        ta.sweep.a0 = positions[toiIndexA].a;
+        Body &tb = bodies[islandBodies[toiIndexB]];
+        tb.sweep.c0 = positions[toiIndexB].c;
+        tb.sweep.a0 = positions[toiIndexB].a;
+        initializeVelocityConstraints();
+        for (int i = 0; i < subStep.velocityIterations; ++i) solveVelocityConstraints();
+        float h = subStep.dt;
+        for (int i = 0; i < nb; ++i) {
+            V2 c = positions[i].c;
+            float a = positions[i].a;
+            V2 v = velocities[i].v;
+            float w = velocities[i].w;
+            V2 translation = h * v;
+            if (dot(translation, translation) > kMaxTranslationSquared) {
+                float ratio = kMaxTranslation / length(translation);
+                v *= ratio;
+            }
+            float rotation = h * w;
+            if (rotation * rotation > kMaxRotationSquared) {
+                float ratio = kMaxRotation / fabs_(rotation);
+                w *= ratio;
+            }
+            c += h * v;
+            a += h * w;
+            positions[i].c = c;
+            positions[i].a = a;
+            velocities[i].v = v;
+            velocities[i].w = w;
+            Body &b = bodies[islandBodies[i]];
+            b.sweep.c = c;
+            b.sweep.a = a;
+            b.v = v;
+            b.w = w;
+            b.synchronizeTransform();
+        }
+        /* b2Island.Report b2island.d:415,438-461 */
+        for (size_t i = 0; i < vcs.size(); ++i) recordEvent(3, contacts[islandContacts[vcs[i].contactIndex]], &vcs[i]);
+    }
+    void addIslandBody(int bi) {
+        bodies[bi].islandIndex = (int)islandBodies.size();
+        islandBodies.push_back(bi);
+    }
+
+    /* b2World.SolveTOI b2world.d:1127-1452 (no sub-stepping, no sensors) */
+    void solveTOI(const TimeStep &step) {
+        if (stepComplete) {
+            for (int b = bodyList; b != kNull; b = bodies[b].next) {
+                bodies[b].flags &= ~bf_island;
+                bodies[b].sweep.alpha0 = 0.0f;
+            }
+            for (int c = contactList; c != kNull; c = contacts[c].next) {
+                contacts[c].flags &= ~(cf_toi | cf_island);
+                contacts[c].toiCount = 0;
+                contacts[c].toi = 1.0f;
+            }
+        }
+        for (;;) {
+            /* the first TOI over the contact list */
+            int minContact = kNull;
+            float minAlpha = 1.0f;
+            for (int ci = contactList; ci != kNull; ci = contacts[ci].next) {
+                Contact &c = contacts[ci];
+                if ((c.flags & cf_enabled) == 0) continue;
+                if (c.toiCount > kMaxSubSteps) continue;
+                float alpha = 1.0f;
+                if (c.flags & cf_toi) {
+                    alpha = c.toi;
+                } else {
+                    const Fixture &fA = fixtures[c.fixtureA];
+                    const Fixture &fB = fixtures[c.fixtureB];
+                    if (fA.sensor || fB.sensor) continue;
+                    Body &bA = bodies[fA.body];
+                    Body &bB = bodies[fB.body];
+                    int typeA = bA.type, typeB = bB.type;
+                    bool activeA = bA.awake() && typeA != b_static;
+                    bool activeB = bB.awake() && typeB != b_static;
+                    if (!activeA && !activeB) continue;
+                    bool collideA = (bA.flags & bf_bullet) != 0 || typeA != b_dynamic;
+                    bool collideB = (bB.flags & bf_bullet) != 0 || typeB != b_dynamic;
+                    if (!collideA && !collideB) continue;
+                    /* put the sweeps onto the same time interval */
+                    float alpha0 = bA.sweep.alpha0;
+                    if (bA.sweep.alpha0 < bB.sweep.alpha0) {
+                        alpha0 = bB.sweep.alpha0;
+                        bA.sweep.advance(alpha0);
+                    } else if (bB.sweep.alpha0 < bA.sweep.alpha0) {
+                        alpha0 = bA.sweep.alpha0;
+                        bB.sweep.advance(alpha0);
+                    }
+                    DistanceProxy pA, pB;
+                    pA.set(fA.shape);
+                    pB.set(fB.shape);
+                    float beta = 0.0f;
+                    int state = timeOfImpact(beta, pA, pB, bA.sweep, bB.sweep, 1.0f);
+                    if (state == toi_touching) alpha = fmin_(alpha0 + (1.0f - alpha0) * beta, 1.0f);
+                    else alpha = 1.0f;
+                    c.toi = alpha;
+                    c.flags |= cf_toi;
+                }
+                if (alpha < minAlpha) {
+                    minContact = ci;
+                    minAlpha = alpha;
+                }
+            }
+            if (minContact == kNull || 1.0f - 10.0f * kEps < minAlpha) {
+                stepComplete = true;
+                break;
+            }
+            /* advance the bodies to the TOI */
+            const int ia = fixtures[contacts[minContact].fixtureA].body;
+            const int ib = fixtures[contacts[minContact].fixtureB].body;
+            Sweep backup1 = bodies[ia].sweep, backup2 = bodies[ib].sweep;
+            bodies[ia].advance(minAlpha);
+            bodies[ib].advance(minAlpha);
+            updateContact(contacts[minContact]);
+            contacts[minContact].flags &= ~cf_toi;
+            ++contacts[minContact].toiCount;
+            if ((contacts[minContact].flags & cf_enabled) == 0 || (contacts[minContact].flags & cf_touching) == 0) {
+                contacts[minContact].flags &= ~cf_enabled;
+                bodies[ia].sweep = backup1;
+                bodies[ib].sweep = backup2;
+                bodies[ia].synchronizeTransform();
+                bodies[ib].synchronizeTransform();
+                continue;
+            }
+            bodies[ia].setAwake(true);
+            bodies[ib].setAwake(true);
+            /* the island: both bodies, the TOI contact, and their contacts with
+             * static, kinematic or bullet bodies that touch at the TOI */
+            islandBodies.clear();
+            islandContacts.clear();
+            addIslandBody(ia);
+            addIslandBody(ib);
+            islandContacts.push_back(minContact);
+            bodies[ia].flags |= bf_island;
+            bodies[ib].flags |= bf_island;
+            contacts[minContact].flags |= cf_island;
+            const int pair[2] = {ia, ib};
+            for (int i = 0; i < 2; ++i) {
+                Body &body = bodies[pair[i]];
+                if (body.type != b_dynamic) continue;
+                for (int e = body.contactList; e != kNull; e = edge(e).next) {
+                    if ((int)islandBodies.size() == 2 * kMaxTOIContacts) break;
+                    if ((int)islandContacts.size() == kMaxTOIContacts) break;
+                    const int ci = edgeContact(e);
+                    Contact &contact = contacts[ci];
+                    if (contact.flags & cf_island) continue;
+                    const int oi = edge(e).other;
+                    Body &other = bodies[oi];
+                    if (other.type == b_dynamic && (body.flags & bf_bullet) == 0 && (other.flags & bf_bullet) == 0) continue;
+                    if (fixtures[contact.fixtureA].sensor || fixtures[contact.fixtureB].sensor) continue;
+                    Sweep backup = other.sweep;
+                    if ((other.flags & bf_island) == 0) other.advance(minAlpha);
+                    updateContact(contact);
+                    if ((contact.flags & cf_enabled) == 0 || (contact.flags & cf_touching) == 0) {
+                        other.sweep = backup;
+                        other.synchronizeTransform();
+                        continue;
+                    }
+                    contact.flags |= cf_island;
+                    islandContacts.push_back(ci);
+                    if (other.flags & bf_island) continue;
+                    other.flags |= bf_island;
+                    if (other.type != b_static) other.setAwake(true);
+                    addIslandBody(oi);
+                }
+            }
+            TimeStep subStep;
+            subStep.dt = (1.0f - minAlpha) * step.dt;
+            subStep.inv_dt = 1.0f / subStep.dt;
+            subStep.dtRatio = 1.0f;
+            subStep.positionIterations = 20;
+            subStep.velocityIterations = step.velocityIterations;
+            subStep.warmStarting = false;
+            solveIslandTOI(subStep, bodies[ia].islandIndex, bodies[ib].islandIndex);
+            /* reset island flags and synchronize broad-phase proxies */
+            for (int bi : islandBodies) {
+                Body &body = bodies[bi];
+                body.flags &= ~bf_island;
+                if (body.type != b_dynamic) continue;
+                synchronizeFixtures(body);
+                for (int e = body.contactList; e != kNull; e = edge(e).next)
+                    contacts[edgeContact(e)].flags &= ~(cf_toi | cf_island);
+            }
+            findNewContacts();
+            ++toiEvents;
+        }
+    }
+
     /* b2World.Step b2world.d:367-434 */
     void step(float dt, int velocityIterations, int positionIterations) {
         auto t0 = Clock::now();
@@ -2241,10 +3021,17 @@ struct World {
         auto t1 = Clock::now();
         collide();
         profile[1] = msSince(t1);
-        if (st.dt > 0.0f) {
+        if (stepComplete && st.dt > 0.0f) {
             auto t2 = Clock::now();
             solve(st);
             profile[2] = msSince(t2);
+        }
+        toiEvents = 0;
+        profile[7] = 0.0f;
+        if (continuousPhysics && st.dt > 0.0f) {
+            auto t3 = Clock::now();
+            solveTOI(st);
+            profile[7] = msSince(t3);
         }
         if (st.dt > 0.0f) inv_dt0 = st.inv_dt;
         for (int b = bodyList; b != kNull; b = bodies[b].next) { /* ClearForces b2world.d:443-450 */
@@ -2294,8 +3081,7 @@ int orc_world_set_flags(void *w, int warm, int sleep, int continuous) {
     World *W = (World *)w;
     W->warmStarting = warm != 0;
     W->allowSleep = sleep != 0;
-    if (continuous) return -1;
-    W->continuousPhysics = false;
+    W->continuousPhysics = continuous != 0;
     return 0;
 }
 int orc_solve_log(void *w, int32_t *out, int cap) {
@@ -2449,6 +3235,51 @@ int orc_collide(const orc_fixture_def *a, const float *xfa3, const orc_fixture_d
     out->swapped = swapped;
     return 0;
 }
+int orc_shape_distance(const orc_fixture_def *a, const float *xfa3, const orc_fixture_def *b, const float *xfb3,
+                       int use_radii, float *out6) {
+    Shape sa, sb;
+    World::buildShape(*a, sa);
+    World::buildShape(*b, sb);
+    Xf xfa, xfb;
+    xfa.p = mk(xfa3[0], xfa3[1]);
+    xfa.q.set(xfa3[2]);
+    xfb.p = mk(xfb3[0], xfb3[1]);
+    xfb.q.set(xfb3[2]);
+    DistanceProxy pa, pb;
+    pa.set(sa);
+    pb.set(sb);
+    SimplexCache cache;
+    DistanceOutput o;
+    gjkDistance(o, cache, pa, xfa, pb, xfb, use_radii != 0);
+    out6[0] = o.pointA.x;
+    out6[1] = o.pointA.y;
+    out6[2] = o.pointB.x;
+    out6[3] = o.pointB.y;
+    out6[4] = o.distance;
+    out6[5] = (float)o.iterations;
+    return 0;
+}
+static Sweep sweepIn(const float *s9) {
+    Sweep sw;
+    sw.localCenter = mk(s9[0], s9[1]);
+    sw.c0 = mk(s9[2], s9[3]);
+    sw.c = mk(s9[4], s9[5]);
+    sw.a0 = s9[6];
+    sw.a = s9[7];
+    sw.alpha0 = s9[8];
+    return sw;
+}
+int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const orc_fixture_def *b, const float *sweep_b9,
+                       float t_max, float *t_out) {
+    Shape sa, sb;
+    World::buildShape(*a, sa);
+    World::buildShape(*b, sb);
+    DistanceProxy pa, pb;
+    pa.set(sa);
+    pb.set(sb);
+    return timeOfImpact(*t_out, pa, pb, sweepIn(sweep_a9), sweepIn(sweep_b9), t_max);
+}
+int orc_toi_events(void *w) { return ((World *)w)->toiEvents; }
 int orc_shape_geometry(const orc_fixture_def *def, float *o, float *mass_out4) {
     Shape s;
     World::buildShape(*def, s);

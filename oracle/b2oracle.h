@@ -94,8 +94,8 @@ typedef struct {
 
 void *orc_world_create(float gx, float gy);
 void orc_world_destroy(void *w);
-/* b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics.  The
- * oracle restates no TOI solver, so continuous must be 0. */
+/* b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics
+ * (continuous = 1 runs b2World.SolveTOI after the discrete solve). */
 int orc_world_set_flags(void *w, int warm_starting, int allow_sleep, int continuous);
 /* Test hook (not reference behaviour): 1 = solve each island's contacts in greedy
  * colour order, to measure what reordering Gauss-Seidel alone changes. */
@@ -149,6 +149,17 @@ int orc_shape_geometry(const orc_fixture_def *def, float *out36, float *mass_out
  * are written as (proxy, proxy) with the smaller index first, sorted. */
 int orc_tree_pairs(const float *fat_aabbs, const uint8_t *moved, int n, int32_t *out, int cap);
 
+/* b2Distance (b2distance.d:185-345) between two shapes at transforms (px, py,
+ * angle), with an empty simplex cache.  out6: pointA, pointB, distance, iterations. */
+int orc_shape_distance(const orc_fixture_def *a, const float *xfa3, const orc_fixture_def *b, const float *xfb3,
+                       int use_radii, float *out6);
+/* b2TimeOfImpact (b2timeofimpact.d:67-295) of two shapes on sweeps given as 9
+ * floats (localCenter.xy, c0.xy, c.xy, a0, a, alpha0).  Returns the output state
+ * (0 unknown, 1 failed, 2 overlapped, 3 touching, 4 separated); *t_out = time. */
+int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const orc_fixture_def *b, const float *sweep_b9,
+                       float t_max, float *t_out);
+/* Test hook: TOI events (sub-step island solves) of the last orc_step. */
+int orc_toi_events(void *w);
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,9 @@ def lib():
         L.orc_collide.argtypes = [vp, vp, vp, vp, vp]
         L.orc_shape_geometry.argtypes = [vp, vp, vp]
         L.orc_tree_pairs.argtypes = [vp, vp, i32, vp, i32]
+        L.orc_shape_distance.argtypes = [vp, vp, vp, vp, i32, vp]
+        L.orc_time_of_impact.argtypes = [vp, vp, vp, vp, f32, vp]
+        L.orc_toi_events.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -85,11 +88,11 @@ def _p(a: np.ndarray):
 class OracleWorld:
     """b2World restated on the CPU, built from a scene's definitions."""
 
-    def __init__(self, scene, warm_starting=True, allow_sleep=True, colour_order=False):
+    def __init__(self, scene, warm_starting=True, allow_sleep=True, colour_order=False, continuous=False):
         L = lib()
         self.L = L
         self.h = L.orc_world_create(float(scene.gravity[0]), float(scene.gravity[1]))
-        if L.orc_world_set_flags(self.h, int(warm_starting), int(allow_sleep), 0) != 0:
+        if L.orc_world_set_flags(self.h, int(warm_starting), int(allow_sleep), int(continuous)) != 0:
             raise RuntimeError("oracle flag rejected")
         L.orc_world_set_solve_order(self.h, 1 if colour_order else 0)
         self.bodies_def = np.ascontiguousarray(scene.bodies)
@@ -155,6 +158,10 @@ class OracleWorld:
     def max_penetration(self) -> float:
         return float(self.L.orc_max_penetration(self.h))
 
+    def toi_events(self) -> int:
+        """TOI events (sub-step island solves) of the last step."""
+        return int(self.L.orc_toi_events(self.h))
+
     @property
     def dynamic_count(self) -> int:
         return self.L.orc_dynamic_body_count(self.h)
@@ -171,6 +178,34 @@ def collide(fix_a, xfa, fix_b, xfb) -> np.ndarray | None:
     if L.orc_collide(_p(a), _p(xa), _p(b), _p(xb), _p(out)) != 0:
         return None
     return out[0]
+
+
+def shape_distance(fix_a, xfa, fix_b, xfb, use_radii=False):
+    """b2Distance with an empty simplex cache: (pointA, pointB, distance, iterations)."""
+    L = lib()
+    a = np.ascontiguousarray(np.array(fix_a).reshape(1))
+    b = np.ascontiguousarray(np.array(fix_b).reshape(1))
+    xa = np.asarray(xfa, np.float32)
+    xb = np.asarray(xfb, np.float32)
+    out = np.zeros(6, np.float32)
+    L.orc_shape_distance(_p(a), _p(xa), _p(b), _p(xb), int(use_radii), _p(out))
+    return out[0:2].copy(), out[2:4].copy(), float(out[4]), int(out[5])
+
+
+TOI_FAILED, TOI_OVERLAPPED, TOI_TOUCHING, TOI_SEPARATED = 1, 2, 3, 4
+
+
+def time_of_impact(fix_a, sweep_a, fix_b, sweep_b, t_max=1.0):
+    """b2TimeOfImpact; sweeps are 9 floats (localCenter.xy, c0.xy, c.xy, a0, a, alpha0).
+    Returns (state, t)."""
+    L = lib()
+    a = np.ascontiguousarray(np.array(fix_a).reshape(1))
+    b = np.ascontiguousarray(np.array(fix_b).reshape(1))
+    sa = np.asarray(sweep_a, np.float32)
+    sb = np.asarray(sweep_b, np.float32)
+    t = np.zeros(1, np.float32)
+    st = L.orc_time_of_impact(_p(a), _p(sa), _p(b), _p(sb), float(t_max), _p(t))
+    return int(st), float(t[0])
 
 
 def shape_geometry(fix):

@@ -336,8 +336,8 @@ def test_invalid_inputs_are_rejected():
     g.add_scene(s)
     g.finalize()
     with pytest.raises(native.DbxError) as e:
-        g.step(flags=native.CCD)
-    assert e.value.status == native.DBX_E_UNSUPPORTED
+        g.step(flags=native.WARM | native.LEARNED)  # no network loaded
+    assert e.value.status == native.DBX_E_STATE
     with pytest.raises(native.DbxError) as e:
         g.add_scene(s)
     assert e.value.status == native.DBX_E_LOCKED

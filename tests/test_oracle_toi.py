@@ -1,0 +1,238 @@
+"""The oracle's b2Distance, b2TimeOfImpact and b2World.SolveTOI, checked against
+what fixes them without running dbox:
+
+* b2Distance (GJK, b2distance.d:185-345): the core distance of two convex shapes
+  against a brute-force float64 distance (vertex-to-segment minimum for separated
+  polygons, segments and points), witness points on the shapes and |pB - pA| =
+  distance, circle pairs against |c1 - c2| - r1 - r2, overlap reported as 0 with
+  radii;
+* b2TimeOfImpact (b2timeofimpact.d:67-295): a box translating onto an edge
+  against the closed-form impact time (gap(t) = target), and rotating sweeps
+  against brute-force sampling of the swept core distance: at the returned time
+  the distance is within the target band, and no earlier sample is deeper than it;
+* SolveTOI (b2world.d:1127-1452): a box moving 2 m per step through a thin
+  static edge tunnels with continuous physics off and stops on the edge with it
+  on, at the height fixed by the target separation and the TOI position solver.
+"""
+from __future__ import annotations
+
+import math
+import zlib
+
+import numpy as np
+import pytest
+
+from dbox_amd import scenes
+from oracle import pyoracle as po
+
+LINEAR_SLOP = 0.005
+POLY_RADIUS = 2 * LINEAR_SLOP
+
+
+def _poly_fix(pts):
+    f = scenes.fixture_defs(1)[0]
+    f["shape"] = scenes.SHAPE_POLYGON
+    f["count"] = len(pts)
+    p = np.zeros(20, np.float32)
+    p[: 2 * len(pts)] = np.asarray(pts, np.float32).ravel()
+    f["p"] = p
+    return f
+
+
+def _box_fix(hx, hy):
+    f = scenes.fixture_defs(1)[0]
+    f["shape"] = scenes.SHAPE_BOX
+    f["p"][:2] = (hx, hy)
+    return f
+
+
+def _circle_fix(r, cx=0.0, cy=0.0):
+    f = scenes.fixture_defs(1)[0]
+    f["shape"] = scenes.SHAPE_CIRCLE
+    f["radius"] = r
+    f["p"][:2] = (cx, cy)
+    return f
+
+
+def _edge_fix(x1, y1, x2, y2):
+    f = scenes.fixture_defs(1)[0]
+    f["shape"] = scenes.SHAPE_EDGE
+    f["p"][:4] = (x1, y1, x2, y2)
+    return f
+
+
+def _core_points(fix, xf):
+    """The shape's core point set (polygon hull vertices, segment ends, circle
+    centre) in world coordinates, float64."""
+    g, _ = po.shape_geometry(fix)
+    if fix["shape"] == scenes.SHAPE_CIRCLE:
+        loc = np.array([[fix["p"][0], fix["p"][1]]], np.float64)
+    elif fix["shape"] == scenes.SHAPE_EDGE:
+        loc = np.array(fix["p"][:4], np.float64).reshape(2, 2)
+    else:
+        n = int(g[0])
+        loc = np.asarray(g[4 : 4 + 2 * n], np.float64).reshape(n, 2)
+    c, s = math.cos(xf[2]), math.sin(xf[2])
+    return np.stack([xf[0] + c * loc[:, 0] - s * loc[:, 1], xf[1] + s * loc[:, 0] + c * loc[:, 1]], 1)
+
+
+def _pt_seg(p, a, b):
+    ab = b - a
+    den = float(ab @ ab)
+    t = 0.0 if den == 0.0 else min(1.0, max(0.0, float((p - a) @ ab) / den))
+    return float(np.linalg.norm(p - (a + t * ab)))
+
+
+def _segments(P):
+    if len(P) == 1:
+        return [(P[0], P[0])]
+    if len(P) == 2:
+        return [(P[0], P[1])]
+    return [(P[i], P[(i + 1) % len(P)]) for i in range(len(P))]
+
+
+def _brute_distance(PA, PB):
+    """Distance between two separated convex point sets' hulls (their vertices in hull order)."""
+    d = math.inf
+    for p in PA:
+        for a, b in _segments(PB):
+            d = min(d, _pt_seg(p, a, b))
+    for p in PB:
+        for a, b in _segments(PA):
+            d = min(d, _pt_seg(p, a, b))
+    return d
+
+
+def _random_shape(rng, kind):
+    if kind == "circle":
+        return _circle_fix(float(rng.uniform(0.2, 0.8)))
+    if kind == "edge":
+        ang = rng.uniform(0, 2 * math.pi)
+        L = rng.uniform(0.5, 3.0)
+        return _edge_fix(0.0, 0.0, float(L * math.cos(ang)), float(L * math.sin(ang)))
+    n = int(rng.integers(3, 9))
+    ang = 2 * math.pi * (np.arange(n) + rng.uniform(-0.3, 0.3, n)) / n
+    r = rng.uniform(0.25, 1.0)
+    return _poly_fix(np.stack([r * np.cos(ang), r * np.sin(ang)], 1))
+
+
+@pytest.mark.parametrize("kinds", [("poly", "poly"), ("poly", "circle"), ("circle", "circle"), ("edge", "poly"),
+                                   ("edge", "circle")])
+def test_distance_matches_brute_force(kinds):
+    rng = np.random.default_rng(zlib.crc32(repr(kinds).encode()))
+    checked = 0
+    for _ in range(200):
+        fa, fb = _random_shape(rng, kinds[0]), _random_shape(rng, kinds[1])
+        xa = (0.0, 0.0, float(rng.uniform(0, 2 * math.pi)))
+        ang = rng.uniform(0, 2 * math.pi)
+        dist = rng.uniform(4.2, 8.0)  # beyond both shapes' reach from their origins (edges up to 3 m): separated
+        xb = (float(dist * math.cos(ang)), float(dist * math.sin(ang)), float(rng.uniform(0, 2 * math.pi)))
+        pA, pB, d, iters = po.shape_distance(fa, xa, fb, xb)
+        PA, PB = _core_points(fa, xa), _core_points(fb, xb)
+        ref = _brute_distance(PA, PB)
+        assert abs(d - ref) <= 1e-4 * max(1.0, ref), (kinds, d, ref)
+        assert abs(float(np.linalg.norm(pB - pA)) - d) <= 1e-5 * max(1.0, d)
+        # the witness points lie on the shapes (distance to their own hull is ~0)
+        assert _brute_distance(np.asarray([pA], np.float64), PA) < 1e-4 or len(PA) == 1
+        assert _brute_distance(np.asarray([pB], np.float64), PB) < 1e-4 or len(PB) == 1
+        assert 1 <= iters <= 20
+        checked += 1
+    assert checked == 200
+
+
+def test_distance_circles_closed_form_and_overlap():
+    a, b = _circle_fix(0.5), _circle_fix(0.75)
+    _, _, d, _ = po.shape_distance(a, (0.0, 0.0, 0.0), b, (3.0, 4.0, 0.3), use_radii=True)
+    assert abs(d - (5.0 - 1.25)) < 1e-6
+    _, _, d, _ = po.shape_distance(a, (0.0, 0.0, 0.0), b, (1.0, 0.0, 0.0), use_radii=True)
+    assert d == 0.0
+    # overlapping polygons: the core distance is zero (the origin is in the simplex)
+    _, _, d, _ = po.shape_distance(_box_fix(1, 1), (0, 0, 0), _box_fix(1, 1), (0.5, 0.3, 0.2))
+    assert d == 0.0
+
+
+def _sweep(c0, c, a0=0.0, a=0.0):
+    return np.array([0.0, 0.0, c0[0], c0[1], c[0], c[1], a0, a, 0.0], np.float32)
+
+
+def test_toi_translating_box_onto_edge_closed_form():
+    # box (half 0.5) from y = 5 to y = -5 over an edge on y = 0: the core gap is
+    # 4.5 - 10 t, and the impact is where the gap reaches the target
+    # max(slop, rA + rB - 3 slop) = 0.005 within the tolerance slop / 4
+    edge = _edge_fix(-10.0, 0.0, 10.0, 0.0)
+    box = _box_fix(0.5, 0.5)
+    st, t = po.time_of_impact(edge, _sweep((0, 0), (0, 0)), box, _sweep((0.3, 5.0), (0.3, -5.0)))
+    target, tol = max(LINEAR_SLOP, 2 * POLY_RADIUS - 3 * LINEAR_SLOP), 0.25 * LINEAR_SLOP
+    assert st == po.TOI_TOUCHING
+    assert abs((4.5 - 10.0 * t) - target) < tol
+    # a sweep that stops short is separated, one that starts inside is overlapped
+    st, t = po.time_of_impact(edge, _sweep((0, 0), (0, 0)), box, _sweep((0.3, 5.0), (0.3, 2.0)))
+    assert st == po.TOI_SEPARATED and t == 1.0
+    st, t = po.time_of_impact(edge, _sweep((0, 0), (0, 0)), box, _sweep((0.3, 0.2), (0.3, -5.0)))
+    assert st == po.TOI_OVERLAPPED and t == 0.0
+
+
+def test_toi_rotating_sweeps_against_sampled_distance():
+    rng = np.random.default_rng(7)
+    touching = 0
+    for trial in range(60):
+        fa = _random_shape(rng, "poly")
+        fb = _random_shape(rng, ["poly", "circle"][trial % 2])
+        # B starts 4 m away and moves through A's centre with a modest spin
+        ang = rng.uniform(0, 2 * math.pi)
+        c0 = (4.0 * math.cos(ang), 4.0 * math.sin(ang))
+        c1 = (-c0[0], -c0[1])
+        a0 = float(rng.uniform(0, 2 * math.pi))
+        a1 = a0 + float(rng.uniform(-0.5, 0.5))
+        st, t = po.time_of_impact(fa, _sweep((0, 0), (0, 0)), fb, _sweep(c0, c1, a0, a1))
+        radius = po.shape_geometry(fa)[0][1] + po.shape_geometry(fb)[0][1]
+        target, tol = max(LINEAR_SLOP, radius - 3 * LINEAR_SLOP), 0.25 * LINEAR_SLOP
+
+        def dist(s):
+            xb = (c0[0] + s * (c1[0] - c0[0]), c0[1] + s * (c1[1] - c0[1]), a0 + s * (a1 - a0))
+            PA, PB = _core_points(fa, (0.0, 0.0, 0.0)), _core_points(fb, xb)
+            return _brute_distance(PA, PB)
+
+        assert st == po.TOI_TOUCHING, (trial, st)
+        touching += 1
+        assert abs(dist(t) - target) < tol + 2e-4, (trial, dist(t), target)
+        # conservative advancement never steps past a deeper configuration
+        for s in np.linspace(0.0, t, 200):
+            assert dist(float(s)) > target - tol - 2e-4, (trial, s, t)
+    assert touching == 60
+
+
+def _fast_box_scene(vy=-120.0):
+    b = scenes.body_defs(2)
+    b[0]["type"] = scenes.STATIC
+    b[1]["type"] = scenes.DYNAMIC
+    b[1]["py"] = 1.2
+    b[1]["vy"] = vy
+    f = scenes.fixture_defs(2)
+    f[0]["body"] = 0
+    f[0]["shape"] = scenes.SHAPE_EDGE
+    f[0]["p"][:4] = (-5.0, 0.0, 5.0, 0.0)
+    f[1]["body"] = 1
+    f[1]["shape"] = scenes.SHAPE_BOX
+    f[1]["p"][:2] = (0.25, 0.25)
+    f[1]["density"] = 1.0
+    return scenes.Scene(b, f, name="fast_box")
+
+
+def test_solve_toi_stops_a_fast_box_on_a_thin_edge():
+    # 120 m/s is clamped to b2_maxTranslation = 2 m per step: the box passes the
+    # edge inside one step, which the discrete solver never sees
+    off = po.OracleWorld(_fast_box_scene(), continuous=False)
+    off.step(n=3)
+    assert off.bodies()["cy"][1] < -1.0
+    on = po.OracleWorld(_fast_box_scene(), continuous=True)
+    on.step()
+    assert on.toi_events() >= 1
+    y = float(on.bodies()["cy"][1])
+    # resting height: half extent + the two skins minus what the TOI position
+    # solver leaves (it stops within 1.5 slop of the target separation)
+    assert 0.25 + POLY_RADIUS - 1.5 * LINEAR_SLOP - 0.02 <= y <= 0.25 + 2 * POLY_RADIUS + 0.01, y
+    for _ in range(60):
+        on.step()
+    y = float(on.bodies()["cy"][1])
+    assert 0.25 <= y <= 0.25 + 2 * POLY_RADIUS + 0.01, y
