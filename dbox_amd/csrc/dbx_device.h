@@ -71,6 +71,7 @@ struct WorldState {
     int collideCount; // contacts in the list the last Collide walked (its event flags stay there)
     int widePairs;    // pairs found by the device-wide broadphase pass (worlds too big for one workgroup)
     int toiEvents;    // TOI events (sub-step solves) of the last step's continuous pass
+    int adjContacts;  // contacts the last Collide kept, which its adjacency lists cover
 };
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).
@@ -182,8 +183,11 @@ struct Dev {
     long long logCap;
     long long *ktime;       // optional: phase timestamps of the largest world (s_memtime)
     // continuous pass (b2World.SolveTOI) state, valid during that pass only
-    int2 *toiC;             // per contact slot: (cached TOI as float bits, toiCount | TOI and island flags)
-    int2 *toiB;             // per body: (sweep alpha0 as float bits, island flag)
+    int4 *toiC;             // per contact slot: (cached TOI as float bits, toiCount | flags, next edge of
+                            // body A's and of body B's list of contacts made since the last Collide)
+    int4 *toiB;             // per body: (sweep alpha0 as float bits, island flag, head edge of that list, 0)
+    const int *bfixOff, *bfixCnt; // per body: its fixtures' range in bfixList
+    const int *bfixList;    // per world from fixBase: fixture indices grouped by body
 };
 
 // Order-preserving int image of a float (for atomicMin on floats).

@@ -311,6 +311,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
         d.S[w].contactCount = m;
         d.S[w].touching = touching;
         d.S[w].collideCount = n;
+        d.S[w].adjContacts = m;
     }
     if (timed) d.ktime[21] = __builtin_amdgcn_s_memtime();
 }
@@ -2117,27 +2118,33 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
 // Continuous physics: b2World.SolveTOI (b2world.d:1127-1452), one wave per
 // world, after the discrete step's broadphase.
 //
-// The search for the first time of impact computes the TOI of every contact
-// that needs one on its own lane (b2TimeOfImpact, dbx_toi.h).  While walking the
-// contact list the reference advances a contact's body whose sweep is at an
-// earlier time than the other body's (which happens only after an event); a
-// search that meets such a contact walks the list in order on one lane instead,
-// so that every TOI sees the sweeps the reference's walk gives it.  The event
-// itself -- advancing the two bodies, b2Contact.Update, the island of at most
+// The reference walks the whole contact list for every TOI event.  Here the
+// first search computes every contact's TOI on its own lane; after an event
+// only the contacts the event can have changed are looked at again (those of
+// the island's bodies and of bodies it woke, and the contacts it created):
+// every other contact keeps its cached TOI or stays skipped, as the
+// reference's walk would find it.  Those contacts are walked in list order on
+// one lane, which advances a body's sweep where the reference does
+// (b2world.d:1208-1220) and records the sweeps each contact sees; their TOIs
+// are then computed on separate lanes.  A body's contacts are walked in its
+// list order (newest first): the contacts made since the last Collide through
+// a linked list, then the Collide's adjacency list.  The event itself --
+// advancing the two bodies, b2Contact.Update, the island of at most
 // b2_maxTOIContacts contacts, its sub-step solve (b2Island.SolveTOI,
-// b2island.d:282-416) -- runs on one lane in the reference's order; fixture
-// synchronisation and the broadphase update (FindNewContacts for the proxies
-// the event moved) use the whole wave.
+// b2island.d:282-416), fixture synchronisation and AddPair for the new pairs --
+// runs on one lane in the reference's order; the broadphase query of the moved
+// proxies uses the whole wave.
 // ---------------------------------------------------------------------------
 constexpr int kMaxSubSteps = 8;         // b2settings.d:105
 constexpr int kMaxToiContacts = 32;     // b2settings.d:110
 constexpr float kToiBaumgarte = 0.75f;  // b2settings.d:138
-constexpr int kToiPairCap = 512;        // pairs one event's broadphase update may find
-constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17;
+constexpr int kToiPairCap = 512;        // pairs one event's broadphase query may find
+constexpr int kToiListCap = 128;        // woken bodies / moved proxies per event, sweep snapshots per batch
+constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty = 1 << 18;
 
 struct ToiWorld {
     Contact *cont;
-    int2 *tc, *tb;
+    int4 *tc, *tb;
     const int *fbody;
     const Shape *shp;
     const float4 *fmat;
@@ -2147,6 +2154,16 @@ struct ToiWorld {
     uint32_t *bfl;
     float4 *bpos, *bpos0, *bvel, *bxf;
     const float4 *bmass;
+    const int *adjOff, *adjCnt;
+    const int2 *adj;
+    const int *bfixOff, *bfixCnt, *bfixList;
+};
+// Per-event lists kept in LDS; overflow bit 0: more woken bodies than the list
+// holds (the next search then looks at every contact), bit 1: more moved
+// proxies than it holds (reported as a pair capacity error).
+struct ToiLists {
+    int *woken, *moved;
+    int nwoken, nmoved, overflow;
 };
 
 __device__ __forceinline__ Sweep load_sweep(const ToiWorld &W, int b) {
@@ -2180,12 +2197,48 @@ __device__ __forceinline__ void body_advance(const ToiWorld &W, int b, float alp
     store_sweep(W, b, s);
     sync_transform(W, b);
 }
+// b2Body.SetAwake(true); a non-static body that was asleep is recorded, since
+// its contacts may now take part in the search.
+__device__ __forceinline__ void toi_wake(const ToiWorld &W, ToiLists &L, int b) {
+    const uint32_t f = W.bfl[b];
+    if (f & kBodyAwake) return;
+    W.bfl[b] = f | kBodyAwake;
+    W.bpos[b].w = 0.0f;
+    if (body_type(f) == kStatic) return;
+    if (L.nwoken < kToiListCap) L.woken[L.nwoken++] = b;
+    else L.overflow |= 1;
+}
 // b2Contact.Update with the wake on a change of touching.
-__device__ __forceinline__ void toi_update(const ToiWorld &W, Contact &c) {
+__device__ __forceinline__ void toi_update(const ToiWorld &W, ToiLists &L, Contact &c) {
     if (contact_update(c, W.fbody, W.shp, W.bxf)) {
-        wake_body(W.bfl, W.bpos, W.fbody[c.fa]);
-        wake_body(W.bfl, W.bpos, W.fbody[c.fb]);
+        toi_wake(W, L, W.fbody[c.fa]);
+        toi_wake(W, L, W.fbody[c.fb]);
     }
+}
+// Body b's contacts in its list order (newest first): f(contact, other body)
+// returns false to stop the walk.
+template <class F>
+__device__ __forceinline__ void body_contacts(const ToiWorld &W, int b, F &&f) {
+    for (int e = W.tb[b].z; e >= 0;) {
+        const int j = e >> 1;
+        const int4 t = W.tc[j];
+        const int other = W.fbody[(e & 1) ? W.cont[j].fa : W.cont[j].fb];
+        if (!f(j, other)) return;
+        e = (e & 1) ? t.w : t.z;
+    }
+    const int o = W.adjOff[b], cnt = W.adjCnt[b];
+    for (int k = 0; k < cnt; ++k) {
+        const int2 a = W.adj[o + k];
+        if (!f(a.x >> 1, a.y)) return;
+    }
+}
+// Contact j enters the lists of its two bodies (it is the newest of both).
+__device__ __forceinline__ void link_contact(const ToiWorld &W, int j) {
+    const int ba = W.fbody[W.cont[j].fa], bb = W.fbody[W.cont[j].fb];
+    W.tc[j].z = W.tb[ba].z;
+    W.tb[ba].z = 2 * j;
+    W.tc[j].w = W.tb[bb].z;
+    W.tb[bb].z = 2 * j + 1;
 }
 
 // What the search does with a contact (b2world.d:1161-1200): 0 skip it
@@ -2205,21 +2258,48 @@ __device__ __forceinline__ int toi_status(const ToiWorld &W, const Contact &c, i
     if (!collideA && !collideB) return 0;
     return 2;
 }
-// The contact's TOI as a fraction of the step (b2world.d:1208-1238); its two
-// sweeps are at the same alpha0 here.
-__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb) {
-    const Sweep sA = load_sweep(W, W.fbody[fa]), sB = load_sweep(W, W.fbody[fb]);
+// The start of each sweep (c0, a0, alpha0) that one contact's TOI sees.
+struct SweepSnap {
+    int contact;
+    float ax, ay, aa, aalpha, bx, by, ba, balpha;
+};
+// The contact's TOI as a fraction of the step (b2world.d:1221-1238) from the
+// given sweep starts; the sweeps' ends are the bodies' current c and a.
+__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb, const SweepSnap &sn) {
+    Sweep sA = load_sweep(W, W.fbody[fa]), sB = load_sweep(W, W.fbody[fb]);
+    sA.c0 = v2(sn.ax, sn.ay);
+    sA.a0 = sn.aa;
+    sA.alpha0 = sn.aalpha;
+    sB.c0 = v2(sn.bx, sn.by);
+    sB.a0 = sn.ba;
+    sB.alpha0 = sn.balpha;
     const float alpha0 = sA.alpha0;
     const DProxy pA = make_proxy(W.shp + fa), pB = make_proxy(W.shp + fb);
     float beta;
     const int state = time_of_impact(beta, pA, pB, sA, sB, 1.0f);
     return state == kToiTouching ? fminr(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
 }
+__device__ __forceinline__ SweepSnap snap_of(const ToiWorld &W, int j) {
+    const Contact &c = W.cont[j];
+    const int ba = W.fbody[c.fa], bb = W.fbody[c.fb];
+    const float4 a = W.bpos0[ba], b = W.bpos0[bb];
+    SweepSnap sn;
+    sn.contact = j;
+    sn.ax = a.x;
+    sn.ay = a.y;
+    sn.aa = a.z;
+    sn.aalpha = __int_as_float(W.tb[ba].x);
+    sn.bx = b.x;
+    sn.by = b.y;
+    sn.ba = b.z;
+    sn.balpha = __int_as_float(W.tb[bb].x);
+    return sn;
+}
 
 // One TOI event (b2world.d:1270-1405) on one lane: advance both bodies to the
 // TOI, update the contact, build the island and solve it (b2island.d:282-416).
 // Returns the island body count, 0 when the contact was not touching at the TOI.
-__device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float minAlpha, const StepParams &sp,
+__device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, float minAlpha, const StepParams &sp,
                                       int *ibod, VelCon *vcs, PosCon *pcs, float4 *imp) {
     Contact *cont = W.cont;
     const int ia = W.fbody[cont[ci].fa], ib = W.fbody[cont[ci].fb];
@@ -2227,10 +2307,10 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
     body_advance(W, ia, minAlpha);
     body_advance(W, ib, minAlpha);
     Contact mc = cont[ci];
-    toi_update(W, mc);
+    toi_update(W, L, mc);
     cont[ci] = mc;
-    const int2 t0 = W.tc[ci];
-    W.tc[ci].y = (((t0.y & kTcCount) + 1) & kTcCount) | (t0.y & kTcIsland);
+    const int t0 = W.tc[ci].y;
+    W.tc[ci].y = (((t0 & kTcCount) + 1) & kTcCount) | (t0 & (kTcIsland | kTcDirty));
     if (!(mc.flags & kContactEnabled) || !(mc.flags & kContactTouching)) {
         cont[ci].flags = mc.flags & ~kContactEnabled;
         store_sweep(W, ia, backup1);
@@ -2239,8 +2319,8 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
         sync_transform(W, ib);
         return 0;
     }
-    wake_body(W.bfl, W.bpos, ia);
-    wake_body(W.bfl, W.bpos, ib);
+    toi_wake(W, L, ia);
+    toi_wake(W, L, ib);
     int icon[kMaxToiContacts];
     int nbod = 0, ncon = 0;
     ibod[nbod++] = ia;
@@ -2250,47 +2330,45 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
     W.tb[ib].y = 1;
     W.tc[ci].y |= kTcIsland;
     // contacts of the two bodies with static, kinematic or bullet bodies that
-    // touch at the TOI, each body's list newest first
+    // touch at the TOI
     for (int side = 0; side < 2; ++side) {
         const int body = side ? ib : ia;
         const uint32_t bf = W.bfl[body];
         if (body_type(bf) != kDynamic) continue;
-        for (int j = n - 1; j >= 0; --j) {
-            const int ja = W.fbody[cont[j].fa], jb = W.fbody[cont[j].fb];
-            if (ja != body && jb != body) continue;
-            if (nbod == 2 * kMaxToiContacts || ncon == kMaxToiContacts) break;
-            if (W.tc[j].y & kTcIsland) continue;
-            const int other = ja == body ? jb : ja;
+        body_contacts(W, body, [&](int j, int other) {
+            if (nbod == 2 * kMaxToiContacts || ncon == kMaxToiContacts) return false;
+            if (W.tc[j].y & kTcIsland) return true;
             const uint32_t of = W.bfl[other];
-            if (body_type(of) == kDynamic && !(bf & kBodyBullet) && !(of & kBodyBullet)) continue;
+            if (body_type(of) == kDynamic && !(bf & kBodyBullet) && !(of & kBodyBullet)) return true;
             const Sweep backup = load_sweep(W, other);
             if (W.tb[other].y == 0) body_advance(W, other, minAlpha);
             Contact cu = cont[j];
-            toi_update(W, cu);
+            toi_update(W, L, cu);
             cont[j] = cu;
             if (!(cu.flags & kContactEnabled) || !(cu.flags & kContactTouching)) {
                 store_sweep(W, other, backup);
                 sync_transform(W, other);
-                continue;
+                return true;
             }
             W.tc[j].y |= kTcIsland;
             icon[ncon++] = j;
-            if (W.tb[other].y) continue;
+            if (W.tb[other].y) return true;
             W.tb[other].y = 1;
-            if (body_type(W.bfl[other]) != kStatic) wake_body(W.bfl, W.bpos, other);
+            if (body_type(W.bfl[other]) != kStatic) toi_wake(W, L, other);
             ibod[nbod++] = other;
-        }
+            return true;
+        });
     }
     // the sub-step: the rest of the step, no warm starting (b2world.d:1391-1398)
-    StepParams sub = sp;
-    sub.dt = (1.0f - minAlpha) * sp.dt;
-    sub.inv_dt = 1.0f / sub.dt;
-    sub.dtRatio = 1.0f;
-    sub.flags = sp.flags & ~1u; // no warm starting
+    StepParams ss = sp;
+    ss.dt = (1.0f - minAlpha) * sp.dt;
+    ss.inv_dt = 1.0f / ss.dt;
+    ss.dtRatio = 1.0f;
+    ss.flags = sp.flags & ~1u; // no warm starting
     // The solver works on the bodies' own sweep positions, which the island's
     // position array starts from and which nothing else reads until the end.
     for (int k = 0; k < ncon; ++k)
-        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, sub);
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss);
     // SolveTOIPositionConstraints (b2contactsolver.d:152-242): only the two TOI
     // bodies move
     for (int it = 0; it < 20; ++it) {
@@ -2319,7 +2397,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
         W.bpos0[ib] = make_float4(pb.x, pb.y, pb.z, W.bpos0[ib].w);
     }
     for (int k = 0; k < ncon; ++k) {
-        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, sub);
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss);
         imp[k] = make_float4(vcs[k].nImp0, vcs[k].tImp0, vcs[k].nImp1, vcs[k].tImp1);
     }
     for (int it = 0; it < sp.velocityIterations; ++it)
@@ -2333,7 +2411,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
             W.bvel[v.indexB] = vb;
         }
     // integrate positions (b2island.d:370-413)
-    const float h = sub.dt;
+    const float h = ss.dt;
     for (int i = 0; i < nbod; ++i) {
         const int b = ibod[i];
         const float4 p = W.bpos[b], vel = W.bvel[b];
@@ -2355,21 +2433,75 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, int ci, int n, float mi
         W.bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
         sync_transform(W, b);
     }
+    // island flags reset; fixtures of the dynamic bodies synchronised (the
+    // moved ones enter the move buffer); their contacts' TOIs dropped
+    // (b2world.d:1401-1429).  Every contact whose bodies changed is looked at
+    // again by the next search.
+    for (int i = 0; i < nbod; ++i) {
+        const int b = ibod[i];
+        W.tb[b].y = 0;
+        const int type = body_type(W.bfl[b]);
+        if (type == kStatic) continue;
+        if (type == kDynamic) {
+            const float4 p0 = W.bpos0[b], ms = W.bmass[b];
+            const Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
+            const Xf x2 = load_xf(W.bxf, b);
+            const float2 disp = sub(x2.p, x1.p);
+            for (int k = 0; k < W.bfixCnt[b]; ++k) {
+                const int f = W.bfixList[W.bfixOff[b] + k];
+                const float4 a1 = shape_aabb(W.shp + f, x1), a2 = shape_aabb(W.shp + f, x2);
+                const float4 aabb =
+                    make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+                const float4 fa = W.fat[f];
+                if (fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w) continue;
+                float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                         aabb.w + kAabbExtension);
+                const float2 dd = scl(kAabbMultiplier, disp);
+                if (dd.x < 0.0f) nbx.x += dd.x;
+                else nbx.z += dd.x;
+                if (dd.y < 0.0f) nbx.y += dd.y;
+                else nbx.w += dd.y;
+                W.fat[f] = nbx;
+                if (!W.moved[f]) {
+                    W.moved[f] = 1;
+                    if (L.nmoved < kToiListCap) L.moved[L.nmoved++] = f;
+                    else L.overflow |= 2;
+                }
+            }
+        }
+        const bool invalidate = type == kDynamic;
+        body_contacts(W, b, [&](int j, int) {
+            const int y = W.tc[j].y;
+            W.tc[j].y = (invalidate ? (y & ~(kTcToi | kTcIsland)) : y) | kTcDirty;
+            return true;
+        });
+    }
     return nbod;
 }
 
 // b2ContactManager.AddPair filters and b2Contact.Create (b2contactmanager.d:52-176)
-// for a pair that has no contact yet; appends the contact at slot n.
-__device__ bool toi_add_pair(const ToiWorld &W, int fa, int fb, int n, int cap) {
+// for a pair that has no contact yet: appends the contact at slot n and puts it
+// at the head of both bodies' lists.  Returns whether a contact was made.
+__device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int n, int cap, int &error) {
     const int ba = W.fbody[fa], bb = W.fbody[fb];
     if (ba == bb) return false;
+    bool exists = false;
+    body_contacts(W, bb, [&](int j, int) {
+        const Contact &c = W.cont[j];
+        exists = (c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa);
+        return !exists;
+    });
+    if (exists) return false;
     if (!(body_type(W.bfl[ba]) == kDynamic || body_type(W.bfl[bb]) == kDynamic)) return false;
     const int2 A = W.ffilter[fa], B = W.ffilter[fb];
     bool accept;
     if (A.y == B.y && A.y != 0) accept = A.y > 0;
     else accept = ((A.x >> 16) & (B.x & 0xffff)) != 0 && ((A.x & 0xffff) & (B.x >> 16)) != 0;
     if (!accept || !pair_has_type(W.shp[fa].type, W.shp[fb].type)) return false;
-    if (n >= cap) return true; // counted; the caller reports the capacity error
+    if (n >= cap) {
+        error |= 1;
+        return false;
+    }
     if (!pair_is_primary(W.shp[fa].type, W.shp[fb].type)) {
         const int t = fa;
         fa = fb;
@@ -2389,9 +2521,10 @@ __device__ bool toi_add_pair(const ToiWorld &W, int fa, int fb, int n, int cap) 
     c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.cont[n] = c;
-    W.tc[n] = make_int2(__float_as_int(1.0f), 0);
-    wake_body(W.bfl, W.bpos, ba);
-    wake_body(W.bfl, W.bpos, bb);
+    W.tc[n] = make_int4(__float_as_int(1.0f), kTcDirty, -1, -1);
+    link_contact(W, n);
+    toi_wake(W, L, ba);
+    toi_wake(W, L, bb);
     return true;
 }
 
@@ -2402,7 +2535,9 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     __shared__ PosCon s_pc[kMaxToiContacts];
     __shared__ float4 s_imp[kMaxToiContacts];
     __shared__ unsigned long long s_pairs[kToiPairCap];
-    __shared__ int s_nb, s_np, s_n;
+    __shared__ SweepSnap s_snap[kToiListCap];
+    __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap];
+    __shared__ int s_nb, s_np, s_n, s_nsnap, s_all, s_nmoved;
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
     const int lane = threadIdx.x;
@@ -2422,55 +2557,88 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     W.bvel = d.bvel + wd.bodyBase;
     W.bxf = d.bxf + wd.bodyBase;
     W.bmass = d.bmass + wd.bodyBase;
+    W.adjOff = d.adjOff + wd.bodyBase;
+    W.adjCnt = d.adjCnt + wd.bodyBase;
+    W.adj = d.adjList + 2 * wd.contactBase;
+    W.bfixOff = d.bfixOff + wd.bodyBase;
+    W.bfixCnt = d.bfixCnt + wd.bodyBase;
+    W.bfixList = d.bfixList + wd.fixBase;
     const int nb = wd.bodyCount, nf = wd.fixCount;
     int n = d.S[w].contactCount;
+    const int m = min(d.S[w].adjContacts, n);
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
-    for (int b = lane; b < nb; b += 64) W.tb[b] = make_int2(0, 0);
-    for (int i = lane; i < n; i += 64) W.tc[i] = make_int2(__float_as_int(1.0f), 0);
+    for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
+    for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+    __syncthreads();
+    if (lane == 0) // contacts made since the Collide, oldest first, so that each list is newest first
+        for (int j = m; j < n; ++j) link_contact(W, j);
+    __syncthreads();
+    // first search: every sweep is at alpha0 = 0, so no contact advances a body
+    for (int i = lane; i < n; i += 64) {
+        const Contact &c = W.cont[i];
+        if (toi_status(W, c, 0) != 2) continue;
+        const float a = contact_toi(W, c.fa, c.fb, snap_of(W, i));
+        W.tc[i].x = __float_as_int(a);
+        W.tc[i].y = kTcToi;
+    }
     __syncthreads();
     int events = 0, error = 0;
     // each pass of the loop either ends it or raises one contact's toiCount,
     // and a contact stops taking part past b2_maxSubSteps
     const int guard = (kMaxSubSteps + 1) * wd.contactCap + 1;
     for (int pass = 0; pass < guard; ++pass) {
-        // the first TOI over the contact list (b2world.d:1148-1261)
-        int serial = 0;
-        for (int i = lane; i < n; i += 64) {
-            const Contact &c = W.cont[i];
-            if (toi_status(W, c, W.tc[i].y) != 2) continue;
-            const float xa = __int_as_float(W.tb[W.fbody[c.fa]].x), xb = __int_as_float(W.tb[W.fbody[c.fb]].x);
-            if (xa != xb) serial = 1;
-        }
-        serial = __syncthreads_or(serial);
-        if (!serial) {
-            for (int i = lane; i < n; i += 64) {
-                const Contact &c = W.cont[i];
-                const int tcy = W.tc[i].y;
-                if (toi_status(W, c, tcy) != 2) continue;
-                W.tc[i] = make_int2(__float_as_int(contact_toi(W, c.fa, c.fb)), tcy | kTcToi);
-            }
-        } else if (lane == 0) {
-            for (int i = n - 1; i >= 0; --i) { // list order
-                const Contact &c = W.cont[i];
-                const int tcy = W.tc[i].y;
-                if (toi_status(W, c, tcy) != 2) continue;
-                const int ba = W.fbody[c.fa], bb = W.fbody[c.fb];
-                const float xa = __int_as_float(W.tb[ba].x), xb = __int_as_float(W.tb[bb].x);
-                if (xa < xb || xb < xa) { // b2Sweep.Advance of the body at the earlier time
-                    const int b = xa < xb ? ba : bb;
-                    Sweep sw = load_sweep(W, b);
-                    sweep_advance(sw, xa < xb ? xb : xa);
-                    store_sweep(W, b, sw);
+        if (pass > 0) {
+            // the contacts the last event may have changed, in list order: a body
+            // whose sweep starts earlier than the other's is advanced to the
+            // other's time, then the TOI is computed from the sweeps as they stand
+            if (lane == 0) s_nsnap = 0;
+            __syncthreads();
+            for (int p0 = 0; p0 < n; p0 += 64) {
+                const int i = n - 1 - (p0 + lane);
+                const bool dirty = p0 + lane < n && (W.tc[i].y & (s_all ? -1 : kTcDirty)) != 0;
+                unsigned long long dm = __ballot(dirty || (s_all && p0 + lane < n));
+                if (dm && lane == 0) {
+                    int k = s_nsnap;
+                    while (dm) {
+                        const int j = n - 1 - (p0 + __ffsll((long long)dm) - 1);
+                        dm &= dm - 1;
+                        const int y = W.tc[j].y & ~kTcDirty;
+                        W.tc[j].y = y;
+                        const Contact &c = W.cont[j];
+                        if (toi_status(W, c, y) != 2) continue;
+                        const int ba = W.fbody[c.fa], bb = W.fbody[c.fb];
+                        const float xa = __int_as_float(W.tb[ba].x), xb = __int_as_float(W.tb[bb].x);
+                        if (xa < xb || xb < xa) { // b2Sweep.Advance of the body at the earlier time
+                            const int b = xa < xb ? ba : bb;
+                            Sweep sw = load_sweep(W, b);
+                            sweep_advance(sw, xa < xb ? xb : xa);
+                            store_sweep(W, b, sw);
+                        }
+                        s_snap[k++] = snap_of(W, j);
+                    }
+                    s_nsnap = k;
                 }
-                W.tc[i] = make_int2(__float_as_int(contact_toi(W, c.fa, c.fb)), tcy | kTcToi);
+                __syncthreads();
+                const int ns = s_nsnap;
+                if (ns > kToiListCap - 64 || (p0 + 64 >= n && ns > 0)) {
+                    for (int k = lane; k < ns; k += 64) {
+                        const int j = s_snap[k].contact;
+                        const float a = contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k]);
+                        W.tc[j].x = __float_as_int(a);
+                        W.tc[j].y |= kTcToi;
+                    }
+                    __syncthreads();
+                    if (lane == 0) s_nsnap = 0;
+                    __syncthreads();
+                }
             }
         }
-        __syncthreads();
+        // the first TOI (b2world.d:1148-1261): smallest alpha, first in list order
         float bestA = 1.0f;
         int bestPos = INT_MAX;
         for (int i = lane; i < n; i += 64) {
-            const int2 t = W.tc[i];
-            if (!(t.y & kTcToi) || (t.y & kTcCount) > kMaxSubSteps || !(W.cont[i].flags & kContactEnabled)) continue;
+            const int4 t = W.tc[i];
+            if (!(t.y & kTcToi)) continue;
             const float a = __int_as_float(t.x);
             const int pos = n - 1 - i;
             if (a < bestA || (a == bestA && pos < bestPos)) {
@@ -2489,67 +2657,47 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         }
         if (bestPos == INT_MAX || 1.0f - 10.0f * kEps < bestA) break;
         const int ci = n - 1 - bestPos;
-        if (lane == 0) s_nb = toi_event(W, ci, n, bestA, sp, s_ibod, s_vc, s_pc, s_imp);
+        if (lane == 0) {
+            ToiLists L;
+            L.woken = s_woken;
+            L.moved = s_movedList;
+            L.nwoken = L.nmoved = L.overflow = 0;
+            s_nb = toi_event(W, L, ci, bestA, sp, s_ibod, s_vc, s_pc, s_imp);
+            // contacts of bodies the event woke take part again
+            for (int k = 0; k < L.nwoken; ++k)
+                body_contacts(W, L.woken[k], [&](int j, int) {
+                    W.tc[j].y |= kTcDirty;
+                    return true;
+                });
+            s_nmoved = L.nmoved;
+            s_all = L.overflow & 1;
+            if (L.overflow & 2) error |= 4;
+            s_np = 0;
+        }
         __syncthreads();
-        const int nbod = s_nb;
-        if (nbod == 0) continue;
+        if (s_nb == 0) continue;
         ++events;
-        // SynchronizeFixtures of the island's dynamic bodies and their contacts'
-        // cached TOIs dropped (b2world.d:1401-1429)
-        for (int f = lane; f < nf; f += 64) {
-            const int b = W.fbody[f];
-            if (W.tb[b].y == 0 || body_type(W.bfl[b]) != kDynamic) continue;
-            const float4 p0 = W.bpos0[b], ms = W.bmass[b];
-            const Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
-            const Xf x2 = load_xf(W.bxf, b);
-            const float4 a1 = shape_aabb(W.shp + f, x1), a2 = shape_aabb(W.shp + f, x2);
-            const float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
-            const float2 disp = sub(x2.p, x1.p);
-            const float4 fa = W.fat[f];
-            if (fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w) continue;
-            float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
-                                     aabb.w + kAabbExtension);
-            const float2 dd = scl(kAabbMultiplier, disp);
-            if (dd.x < 0.0f) nbx.x += dd.x;
-            else nbx.z += dd.x;
-            if (dd.y < 0.0f) nbx.y += dd.y;
-            else nbx.w += dd.y;
-            W.fat[f] = nbx;
-            W.moved[f] = 1;
-        }
-        for (int i = lane; i < n; i += 64) {
-            const int ba = W.fbody[W.cont[i].fa], bb = W.fbody[W.cont[i].fb];
-            const bool da = W.tb[ba].y && body_type(W.bfl[ba]) == kDynamic;
-            const bool db = W.tb[bb].y && body_type(W.bfl[bb]) == kDynamic;
-            if (da || db) W.tc[i].y &= ~(kTcToi | kTcIsland);
-        }
-        if (lane == 0) s_np = 0;
-        __syncthreads();
-        for (int i = lane; i < nbod; i += 64) W.tb[s_ibod[i]].y = 0;
         // FindNewContacts (b2world.d:1434): every proxy whose fat AABB overlaps a
         // moved one's, as sorted unique (smaller, larger) pairs
-        for (int m0 = 0; m0 < nf; m0 += 64) {
-            unsigned long long mm = __ballot(m0 + lane < nf && W.moved[m0 + lane] != 0);
-            while (mm) {
-                const int m = m0 + __ffsll((long long)mm) - 1;
-                mm &= mm - 1;
-                const float4 fm = W.fat[m];
-                for (int p = lane; p < nf; p += 64) {
-                    if (p == m || !overlap(fm, W.fat[p])) continue;
-                    const int slot = atomicAdd(&s_np, 1);
-                    const unsigned int lo = p < m ? p : m, hi = p < m ? m : p;
-                    if (slot < kToiPairCap) s_pairs[slot] = ((unsigned long long)lo << 32) | hi;
-                }
+        const int nmoved = s_nmoved;
+        for (int k = 0; k < nmoved; ++k) {
+            const int mv = s_movedList[k];
+            const float4 fm = W.fat[mv];
+            for (int p = lane; p < nf; p += 64) {
+                if (p == mv || !overlap(fm, W.fat[p])) continue;
+                const int slot = atomicAdd(&s_np, 1);
+                const unsigned int lo = p < mv ? p : mv, hi = p < mv ? mv : p;
+                if (slot < kToiPairCap) s_pairs[slot] = ((unsigned long long)lo << 32) | hi;
             }
         }
         __syncthreads();
-        int np = s_np;
-        if (np > kToiPairCap) {
-            error |= 4;
-            np = 0;
-        }
-        if (lane == 0) { // sort and drop duplicates
-            for (int i = 1; i < np; ++i) {
+        if (lane == 0) {
+            int np = s_np;
+            if (np > kToiPairCap) {
+                error |= 4;
+                np = 0;
+            }
+            for (int i = 1; i < np; ++i) { // sort, then AddPair each unique pair in order
                 const unsigned long long v = s_pairs[i];
                 int j = i - 1;
                 while (j >= 0 && s_pairs[j] > v) {
@@ -2558,39 +2706,28 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 }
                 s_pairs[j + 1] = v;
             }
-            int u = 0;
-            for (int i = 0; i < np; ++i)
-                if (u == 0 || s_pairs[i] != s_pairs[u - 1]) s_pairs[u++] = s_pairs[i];
-            s_np = u;
+            ToiLists L;
+            L.woken = s_woken;
+            L.moved = s_movedList;
+            L.nwoken = L.nmoved = L.overflow = 0;
+            int nn = n;
+            for (int i = 0; i < np; ++i) {
+                if (i > 0 && s_pairs[i] == s_pairs[i - 1]) continue;
+                if (toi_add_pair(W, L, (int)(s_pairs[i] >> 32), (int)(s_pairs[i] & 0xffffffffu), nn, wd.contactCap,
+                                 error))
+                    ++nn;
+            }
+            for (int k = 0; k < L.nwoken; ++k)
+                body_contacts(W, L.woken[k], [&](int j, int) {
+                    W.tc[j].y |= kTcDirty;
+                    return true;
+                });
+            for (int k = 0; k < nmoved; ++k) W.moved[s_movedList[k]] = 0;
+            if (L.overflow) s_all = 1;
+            s_n = nn;
         }
         __syncthreads();
-        np = np ? s_np : 0;
-        // pairs that already have a contact are marked ~0
-        for (int e = 0; e < np; ++e) {
-            const int fa = (int)(s_pairs[e] >> 32), fb = (int)(s_pairs[e] & 0xffffffffu);
-            int found = 0;
-            for (int i = lane; i < n; i += 64) {
-                const Contact &c = W.cont[i];
-                if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) found = 1;
-            }
-            found = __syncthreads_or(found);
-            if (found && lane == 0) s_pairs[e] = ~0ull;
-        }
-        __syncthreads();
-        if (lane == 0) {
-            int m = n;
-            for (int e = 0; e < np; ++e) {
-                if (s_pairs[e] == ~0ull) continue;
-                if (toi_add_pair(W, (int)(s_pairs[e] >> 32), (int)(s_pairs[e] & 0xffffffffu), m, wd.contactCap)) ++m;
-            }
-            if (m > wd.contactCap) {
-                error |= 1;
-                m = wd.contactCap;
-            }
-            s_n = m;
-        }
-        for (int f = lane; f < nf; f += 64) W.moved[f] = 0;
-        __syncthreads();
+        for (int f = lane; f < nf && (error & 4); f += 64) W.moved[f] = 0; // untracked moved proxies
         n = s_n;
     }
     if (lane == 0) {

@@ -788,6 +788,35 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
     DBX_HIP(h, dalloc(h, &d.toiC, nc));
     DBX_HIP(h, dalloc(h, &d.toiB, nb));
+    // each body's fixtures (b2Body.m_fixtureList as a range), for the continuous pass;
+    // the host arrays live until the stream synchronisation below
+    std::vector<int> off(nb), cnt(nb), list(nf);
+    {
+        for (int w = 0; w < nw; ++w) {
+            const WorldDesc &D = h->W[w];
+            for (int f = 0; f < D.fixCount; ++f) cnt[D.bodyBase + fbody[D.fixBase + f]]++;
+            int run = 0;
+            for (int b = 0; b < D.bodyCount; ++b) {
+                off[D.bodyBase + b] = run;
+                run += cnt[D.bodyBase + b];
+                cnt[D.bodyBase + b] = 0;
+            }
+            for (int f = 0; f < D.fixCount; ++f) {
+                const int b = fbody[D.fixBase + f];
+                list[D.fixBase + off[D.bodyBase + b] + cnt[D.bodyBase + b]++] = f;
+            }
+        }
+        int *p_off, *p_cnt, *p_list;
+        DBX_HIP(h, dalloc(h, &p_off, nb));
+        DBX_HIP(h, dalloc(h, &p_cnt, nb));
+        DBX_HIP(h, dalloc(h, &p_list, nf));
+        DBX_HIP(h, upload(h, p_off, off));
+        DBX_HIP(h, upload(h, p_cnt, cnt));
+        DBX_HIP(h, upload(h, p_list, list));
+        d.bfixOff = p_off;
+        d.bfixCnt = p_cnt;
+        d.bfixList = p_list;
+    }
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
     if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
