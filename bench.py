@@ -4,7 +4,9 @@
 Workload: 4096 independent worlds (lognormal body counts, median 64, sigma 0.8,
 clipped to [8, 1024]) of convex polygons and circles dropped into 40 m x 80 m
 U containers, stepped together at dt = 1/60 with 8 velocity and 3 position
-iterations, warm starting and sleeping on, continuous physics off.
+iterations, with dbox's defaults: warm starting, sleeping and continuous physics
+(b2World.SolveTOI) on, on the GPU and in the CPU baseline alike.  The same steps
+without continuous physics are timed once more and reported beside it.
 
 Prints one JSON line.  value = dynamic bodies (awake or asleep) x steps / wall
 seconds over steps 1..K of freshly built worlds (after a separate warm-up run
@@ -61,7 +63,7 @@ def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int,
     return c * 12 + nbodies * 8
 
 
-def run_cpu_baseline(worlds, steps, threads, sample_every):
+def run_cpu_baseline(worlds, steps, threads, sample_every, continuous):
     """Steps 1..steps of every sample_every-th world on the oracle (C++ restatement
     of dbox), single-threaded and on a pool of threads, one world per task."""
     from oracle import pyoracle as po
@@ -73,12 +75,12 @@ def run_cpu_baseline(worlds, steps, threads, sample_every):
             for w in ws:
                 w.step(DT, VEL, POS)
 
-    single = [po.OracleWorld(s) for s in sample]
+    single = [po.OracleWorld(s, continuous=continuous) for s in sample]
     t0 = time.perf_counter()
     run(single, steps)
     t1 = time.perf_counter() - t0
 
-    ows = [po.OracleWorld(s) for s in sample]
+    ows = [po.OracleWorld(s, continuous=continuous) for s in sample]
     parts = [ows[i::threads] for i in range(threads)]
     ts = [threading.Thread(target=run, args=(p, steps)) for p in parts]
     t0 = time.perf_counter()
@@ -138,6 +140,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain warm-start comparison (iterations to a fixed residual)")
+    ap.add_argument("--no-ccd", action="store_true",
+                    help="continuous physics off on both sides (dbox's default is on)")
+    ap.add_argument("--no-discrete", action="store_true",
+                    help="skip the extra timed run without continuous physics")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the second timed run whose steps are replayed from captured hipGraphs")
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
@@ -177,7 +183,7 @@ def main():
     nbodies = sum(len(s.bodies) for s in worlds)
     nprox = sum(len(s.fixtures) for s in worlds)
     sbytes = sum(shape_bytes(s.fixtures) for s in worlds)
-    flags = native.WARM | native.SLEEP | (0 if args.coloured else native.ORDERED)
+    flags = native.WARM | native.SLEEP | (0 if args.coloured else native.ORDERED) | (0 if args.no_ccd else native.CCD)
     log(f"bodies {nbodies} dynamic {ndyn} proxies {nprox}")
 
     try:
@@ -247,6 +253,8 @@ def main():
     if dom == "solve" and not args.coloured and not args.subset and len(worlds) == 4096 and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes"]
+    toi_ms = prof["solveTOI"] / max(nsteps, 1)
+    toi_events = int(h.toi_events().sum())
     graph = None
     if not args.no_graph:
         # the same steps 1..K once more on fresh worlds, each tier's step replayed
@@ -266,6 +274,23 @@ def main():
         tg = time.perf_counter() - t0
         graph = {"value": ndyn * args.steps / tg, "ms_per_step": 1e3 * tg / args.steps,
                  "steps": f"1-{args.steps}, capture included"}
+    discrete = None
+    if not args.no_ccd and not args.no_discrete:
+        # the same steps 1..K on fresh worlds with continuous physics off
+        h.close()
+        h = build()
+        h.set_tier_blocks(blocks)
+        h.sync()
+        if have_torch:
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.step(DT, VEL, POS, flags & ~native.CCD, n=args.steps)
+        h.sync()
+        if have_torch:
+            torch.cuda.synchronize()
+        td = time.perf_counter() - t0
+        discrete = {"value": ndyn * args.steps / td, "ms_per_step": 1e3 * td / args.steps,
+                    "steps": f"1-{args.steps}, continuous physics off"}
     result = {
         "metric": "body-steps/s at 4096 worlds; velocity iters to fixed residual, learned vs plain",
         "value": value,
@@ -290,7 +315,7 @@ def main():
             "position_iterations": POS,
             "warm_starting": True,
             "sleep": True,
-            "continuous": False,
+            "continuous": not args.no_ccd,
             "solver": "coloured" if args.coloured else "ordered",
             "contacts_end": tot["contacts"],
             "touching_end": tot["touching"],
@@ -310,6 +335,8 @@ def main():
         "lead_tier_phase_ms_per_step": {k: v / max(nsteps, 1) for k, v in kern_ms.items()},
         "tiers": [{"worlds": int(r[1]), "lds": int(r[2]), "block": int(r[3])} for r in tier_rows],
         "graph_replay": graph,
+        "discrete_only": discrete,
+        "continuous_pass": {"lead_tier_ms_per_step": toi_ms, "toi_events_last_step": toi_events},
         "step_algorithmic_gbs": step_bytes / (elapsed / args.steps) / 1e9,
     }
     if not args.no_learned and not args.subset:
@@ -318,15 +345,16 @@ def main():
         result["learned_vs_plain"] = learned_vs_plain(256, 150)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.steps, threads, args.cpu_sample_every)
+        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.steps, threads, args.cpu_sample_every, not args.no_ccd)
         result["cpu_baseline"] = {
             "value": dyn * args.steps / tn,
             "unit": "body-steps/s",
             "cores": threads,
             "kind": "port",
             "sample": f"every {args.cpu_sample_every}th world ({nsample} worlds, {dyn} dynamic bodies), the same "
-                      f"steps 1-{args.steps}, C++ restatement of dbox -O2 -ffp-contract=off, one world per task on "
-                      "a thread pool",
+                      f"steps 1-{args.steps} with the same flags (continuous physics "
+                      f"{'off' if args.no_ccd else 'on'}), C++ restatement of dbox -O2 -ffp-contract=off, one world "
+                      "per task on a thread pool",
             "single_thread_value": dyn * args.steps / t1,
         }
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]

@@ -22,9 +22,10 @@
  *   b2TimeOfImpact             closed-form impact time of a translating box over an edge;
  *                              brute-force sampling of the swept distance (no earlier
  *                              crossing of the target, separation at the returned time)
- *   SolveTOI                   a fast box over a thin edge stops on it instead of tunnelling;
- *                              the event's end state against the closed form; the sub-step
- *                              island order is pinned only to this restatement.
+ *   SolveTOI                   a fast box over a thin edge stops on it instead of tunnelling and
+ *                              rests at the height the target separation fixes; the sub-step
+ *                              island order and the b2_toiBaugarte factor are pinned only to
+ *                              this restatement (parity unpinned).
  */
 #include "b2oracle.h"
 
