@@ -2140,6 +2140,7 @@ constexpr int kMaxToiContacts = 32;     // b2settings.d:110
 constexpr float kToiBaumgarte = 0.75f;  // b2settings.d:138
 constexpr int kToiPairCap = 512;        // pairs one event's broadphase query may find
 constexpr int kToiListCap = 128;        // woken bodies / moved proxies per event, sweep snapshots per batch
+constexpr int kToiDirtyCap = 512;       // contacts to look at again after an event
 constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty = 1 << 18;
 
 struct ToiWorld {
@@ -2158,13 +2159,24 @@ struct ToiWorld {
     const int2 *adj;
     const int *bfixOff, *bfixCnt, *bfixList;
 };
-// Per-event lists kept in LDS; overflow bit 0: more woken bodies than the list
-// holds (the next search then looks at every contact), bit 1: more moved
-// proxies than it holds (reported as a pair capacity error).
+// Lists kept in LDS by the lane that runs the events: the bodies an event woke,
+// the proxies it moved, and the contacts the next search must look at again
+// (each once, flagged kTcDirty).  overflow bit 0: more woken bodies or dirty
+// contacts than the lists hold (the next search then looks at every contact),
+// bit 1: more moved proxies than the list holds (reported as a pair capacity
+// error).
 struct ToiLists {
-    int *woken, *moved;
-    int nwoken, nmoved, overflow;
+    int *woken, *moved, *dirty;
+    int nwoken, nmoved, ndirty, overflow;
 };
+__device__ __forceinline__ void mark_dirty(const ToiWorld &W, ToiLists &L, int j, int clear) {
+    const int y = W.tc[j].y;
+    if (!(y & kTcDirty)) {
+        if (L.ndirty < kToiDirtyCap) L.dirty[L.ndirty++] = j;
+        else L.overflow |= 1;
+    }
+    W.tc[j].y = (y & ~clear) | kTcDirty;
+}
 
 __device__ __forceinline__ Sweep load_sweep(const ToiWorld &W, int b) {
     Sweep s;
@@ -2469,10 +2481,9 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
                 }
             }
         }
-        const bool invalidate = type == kDynamic;
+        const int clear = type == kDynamic ? (kTcToi | kTcIsland) : 0;
         body_contacts(W, b, [&](int j, int) {
-            const int y = W.tc[j].y;
-            W.tc[j].y = (invalidate ? (y & ~(kTcToi | kTcIsland)) : y) | kTcDirty;
+            mark_dirty(W, L, j, clear);
             return true;
         });
     }
@@ -2521,7 +2532,8 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.cont[n] = c;
-    W.tc[n] = make_int4(__float_as_int(1.0f), kTcDirty, -1, -1);
+    W.tc[n] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+    mark_dirty(W, L, n, 0);
     link_contact(W, n);
     toi_wake(W, L, ba);
     toi_wake(W, L, bb);
@@ -2536,8 +2548,9 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     __shared__ float4 s_imp[kMaxToiContacts];
     __shared__ unsigned long long s_pairs[kToiPairCap];
     __shared__ SweepSnap s_snap[kToiListCap];
-    __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap];
-    __shared__ int s_nb, s_np, s_n, s_nsnap, s_all, s_nmoved;
+    __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
+    __shared__ ToiLists s_L;
+    __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved;
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
     const int lane = threadIdx.x;
@@ -2569,6 +2582,12 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
     for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
     for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+    if (lane == 0) {
+        s_L.woken = s_woken;
+        s_L.moved = s_movedList;
+        s_L.dirty = s_dirty;
+        s_L.nwoken = s_L.nmoved = s_L.ndirty = s_L.overflow = 0;
+    }
     __syncthreads();
     if (lane == 0) // contacts made since the Collide, oldest first, so that each list is newest first
         for (int j = m; j < n; ++j) link_contact(W, j);
@@ -2591,17 +2610,24 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             // the contacts the last event may have changed, in list order: a body
             // whose sweep starts earlier than the other's is advanced to the
             // other's time, then the TOI is computed from the sweeps as they stand
-            if (lane == 0) s_nsnap = 0;
-            __syncthreads();
-            for (int p0 = 0; p0 < n; p0 += 64) {
-                const int i = n - 1 - (p0 + lane);
-                const bool dirty = p0 + lane < n && (W.tc[i].y & (s_all ? -1 : kTcDirty)) != 0;
-                unsigned long long dm = __ballot(dirty || (s_all && p0 + lane < n));
-                if (dm && lane == 0) {
-                    int k = s_nsnap;
-                    while (dm) {
-                        const int j = n - 1 - (p0 + __ffsll((long long)dm) - 1);
-                        dm &= dm - 1;
+            const bool all = (s_L.overflow & 1) != 0;
+            const int nd = all ? n : s_L.ndirty;
+            if (lane == 0 && !all) // list order is descending slot order
+                for (int i = 1; i < nd; ++i) {
+                    const int v = s_dirty[i];
+                    int j = i - 1;
+                    while (j >= 0 && s_dirty[j] < v) {
+                        s_dirty[j + 1] = s_dirty[j];
+                        --j;
+                    }
+                    s_dirty[j + 1] = v;
+                }
+            for (int c0 = 0; c0 < nd; c0 += 64) {
+                if (lane == 0) {
+                    int k = 0;
+                    const int c1 = c0 + 64 < nd ? c0 + 64 : nd;
+                    for (int q = c0; q < c1; ++q) {
+                        const int j = all ? n - 1 - q : s_dirty[q];
                         const int y = W.tc[j].y & ~kTcDirty;
                         W.tc[j].y = y;
                         const Contact &c = W.cont[j];
@@ -2620,18 +2646,19 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 }
                 __syncthreads();
                 const int ns = s_nsnap;
-                if (ns > kToiListCap - 64 || (p0 + 64 >= n && ns > 0)) {
-                    for (int k = lane; k < ns; k += 64) {
-                        const int j = s_snap[k].contact;
-                        const float a = contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k]);
-                        W.tc[j].x = __float_as_int(a);
-                        W.tc[j].y |= kTcToi;
-                    }
-                    __syncthreads();
-                    if (lane == 0) s_nsnap = 0;
-                    __syncthreads();
+                for (int k = lane; k < ns; k += 64) {
+                    const int j = s_snap[k].contact;
+                    const float a = contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k]);
+                    W.tc[j].x = __float_as_int(a);
+                    W.tc[j].y |= kTcToi;
                 }
+                __syncthreads();
             }
+            if (lane == 0) {
+                s_L.ndirty = 0;
+                s_L.overflow &= ~1;
+            }
+            __syncthreads();
         }
         // the first TOI (b2world.d:1148-1261): smallest alpha, first in list order
         float bestA = 1.0f;
@@ -2658,20 +2685,18 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         if (bestPos == INT_MAX || 1.0f - 10.0f * kEps < bestA) break;
         const int ci = n - 1 - bestPos;
         if (lane == 0) {
-            ToiLists L;
-            L.woken = s_woken;
-            L.moved = s_movedList;
-            L.nwoken = L.nmoved = L.overflow = 0;
+            ToiLists &L = s_L;
+            L.nwoken = L.nmoved = 0;
             s_nb = toi_event(W, L, ci, bestA, sp, s_ibod, s_vc, s_pc, s_imp);
             // contacts of bodies the event woke take part again
             for (int k = 0; k < L.nwoken; ++k)
                 body_contacts(W, L.woken[k], [&](int j, int) {
-                    W.tc[j].y |= kTcDirty;
+                    mark_dirty(W, L, j, 0);
                     return true;
                 });
             s_nmoved = L.nmoved;
-            s_all = L.overflow & 1;
             if (L.overflow & 2) error |= 4;
+            L.overflow &= ~2;
             s_np = 0;
         }
         __syncthreads();
@@ -2706,10 +2731,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 }
                 s_pairs[j + 1] = v;
             }
-            ToiLists L;
-            L.woken = s_woken;
-            L.moved = s_movedList;
-            L.nwoken = L.nmoved = L.overflow = 0;
+            ToiLists &L = s_L;
+            L.nwoken = 0;
             int nn = n;
             for (int i = 0; i < np; ++i) {
                 if (i > 0 && s_pairs[i] == s_pairs[i - 1]) continue;
@@ -2719,11 +2742,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             }
             for (int k = 0; k < L.nwoken; ++k)
                 body_contacts(W, L.woken[k], [&](int j, int) {
-                    W.tc[j].y |= kTcDirty;
+                    mark_dirty(W, L, j, 0);
                     return true;
                 });
             for (int k = 0; k < nmoved; ++k) W.moved[s_movedList[k]] = 0;
-            if (L.overflow) s_all = 1;
             s_n = nn;
         }
         __syncthreads();
