@@ -2276,8 +2276,9 @@ struct SweepSnap {
     float ax, ay, aa, aalpha, bx, by, ba, balpha;
 };
 // The contact's TOI as a fraction of the step (b2world.d:1221-1238) from the
-// given sweep starts; the sweeps' ends are the bodies' current c and a.
-__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb, const SweepSnap &sn) {
+// given sweep starts; the sweeps' ends are the bodies' current c and a.  vbuf:
+// this lane's LDS space for the two proxies' vertices.
+__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb, const SweepSnap &sn, float2 *vbuf) {
     Sweep sA = load_sweep(W, W.fbody[fa]), sB = load_sweep(W, W.fbody[fb]);
     sA.c0 = v2(sn.ax, sn.ay);
     sA.a0 = sn.aa;
@@ -2286,7 +2287,7 @@ __device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb, con
     sB.a0 = sn.ba;
     sB.alpha0 = sn.balpha;
     const float alpha0 = sA.alpha0;
-    const DProxy pA = make_proxy(W.shp + fa), pB = make_proxy(W.shp + fb);
+    const DProxy pA = make_proxy_in(W.shp + fa, vbuf), pB = make_proxy_in(W.shp + fb, vbuf + kMaxPolygonVertices);
     float beta;
     const int state = time_of_impact(beta, pA, pB, sA, sB, 1.0f);
     return state == kToiTouching ? fminr(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
@@ -2550,6 +2551,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     __shared__ SweepSnap s_snap[kToiListCap];
     __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
     __shared__ ToiLists s_L;
+    __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices]; // per lane: the two proxies of one TOI
     __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved;
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
@@ -2579,6 +2581,9 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     const int nb = wd.bodyCount, nf = wd.fixCount;
     int n = d.S[w].contactCount;
     const int m = min(d.S[w].adjContacts, n);
+    const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && lane == 0;
+    long long tWalk = 0, tEvent = 0, tMark = 0;
+    if (timed) d.ktime[25] = __builtin_amdgcn_s_memtime();
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
     for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
     for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
@@ -2592,20 +2597,44 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     if (lane == 0) // contacts made since the Collide, oldest first, so that each list is newest first
         for (int j = m; j < n; ++j) link_contact(W, j);
     __syncthreads();
-    // first search: every sweep is at alpha0 = 0, so no contact advances a body
-    for (int i = lane; i < n; i += 64) {
-        const Contact &c = W.cont[i];
-        if (toi_status(W, c, 0) != 2) continue;
-        const float a = contact_toi(W, c.fa, c.fb, snap_of(W, i));
-        W.tc[i].x = __float_as_int(a);
-        W.tc[i].y = kTcToi;
-    }
+    // first search: every sweep is at alpha0 = 0, so no contact advances a body.
+    // The contacts that need a TOI are gathered 256 at a time (four independent
+    // load chains per lane) and their TOIs dealt out evenly over the lanes.
+    if (lane == 0) s_nsnap = 0;
     __syncthreads();
+    for (int base = 0; base < n; base += 256) {
+        int need[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = base + lane + 64 * u;
+            need[u] = 0;
+            if (i < n) need[u] = toi_status(W, W.cont[i], 0) == 2;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (need[u]) s_dirty[atomicAdd(&s_nsnap, 1)] = base + lane + 64 * u;
+        __syncthreads();
+        const int ne = s_nsnap;
+        if (ne > kToiDirtyCap - 256 || (base + 256 >= n && ne > 0)) {
+            for (int k = lane; k < ne; k += 64) {
+                const int i = s_dirty[k];
+                const float a = contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
+                                            s_verts + 2 * kMaxPolygonVertices * lane);
+                W.tc[i].x = __float_as_int(a);
+                W.tc[i].y = kTcToi;
+            }
+            __syncthreads();
+            if (lane == 0) s_nsnap = 0;
+            __syncthreads();
+        }
+    }
+    if (timed) d.ktime[26] = __builtin_amdgcn_s_memtime();
     int events = 0, error = 0;
     // each pass of the loop either ends it or raises one contact's toiCount,
     // and a contact stops taking part past b2_maxSubSteps
     const int guard = (kMaxSubSteps + 1) * wd.contactCap + 1;
     for (int pass = 0; pass < guard; ++pass) {
+        if (timed) tMark = __builtin_amdgcn_s_memtime();
         if (pass > 0) {
             // the contacts the last event may have changed, in list order: a body
             // whose sweep starts earlier than the other's is advanced to the
@@ -2648,7 +2677,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 const int ns = s_nsnap;
                 for (int k = lane; k < ns; k += 64) {
                     const int j = s_snap[k].contact;
-                    const float a = contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k]);
+                    const float a =
+                        contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k], s_verts + 2 * kMaxPolygonVertices * lane);
                     W.tc[j].x = __float_as_int(a);
                     W.tc[j].y |= kTcToi;
                 }
@@ -2663,14 +2693,22 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         // the first TOI (b2world.d:1148-1261): smallest alpha, first in list order
         float bestA = 1.0f;
         int bestPos = INT_MAX;
-        for (int i = lane; i < n; i += 64) {
-            const int4 t = W.tc[i];
-            if (!(t.y & kTcToi)) continue;
-            const float a = __int_as_float(t.x);
-            const int pos = n - 1 - i;
-            if (a < bestA || (a == bestA && pos < bestPos)) {
-                bestA = a;
-                bestPos = pos;
+        for (int i0 = lane; i0 < n; i0 += 256) { // four independent loads in flight per lane
+            int2 t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + 64 * u;
+                t[u] = i < n ? make_int2(W.tc[i].x, W.tc[i].y) : make_int2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!(t[u].y & kTcToi)) continue;
+                const float a = __int_as_float(t[u].x);
+                const int pos = n - 1 - (i0 + 64 * u);
+                if (a < bestA || (a == bestA && pos < bestPos)) {
+                    bestA = a;
+                    bestPos = pos;
+                }
             }
         }
 #pragma unroll
@@ -2682,8 +2720,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 bestPos = op;
             }
         }
+        if (timed) tWalk += __builtin_amdgcn_s_memtime() - tMark;
         if (bestPos == INT_MAX || 1.0f - 10.0f * kEps < bestA) break;
         const int ci = n - 1 - bestPos;
+        if (timed) tMark = __builtin_amdgcn_s_memtime();
         if (lane == 0) {
             ToiLists &L = s_L;
             L.nwoken = L.nmoved = 0;
@@ -2700,7 +2740,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             s_np = 0;
         }
         __syncthreads();
-        if (s_nb == 0) continue;
+        if (s_nb == 0) {
+            if (timed) tEvent += __builtin_amdgcn_s_memtime() - tMark;
+            continue;
+        }
         ++events;
         // FindNewContacts (b2world.d:1434): every proxy whose fat AABB overlaps a
         // moved one's, as sorted unique (smaller, larger) pairs
@@ -2708,11 +2751,21 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         for (int k = 0; k < nmoved; ++k) {
             const int mv = s_movedList[k];
             const float4 fm = W.fat[mv];
-            for (int p = lane; p < nf; p += 64) {
-                if (p == mv || !overlap(fm, W.fat[p])) continue;
-                const int slot = atomicAdd(&s_np, 1);
-                const unsigned int lo = p < mv ? p : mv, hi = p < mv ? mv : p;
-                if (slot < kToiPairCap) s_pairs[slot] = ((unsigned long long)lo << 32) | hi;
+            for (int p0 = lane; p0 < nf; p0 += 256) { // four independent loads in flight per lane
+                float4 fp[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int p = p0 + 64 * u;
+                    fp[u] = p < nf ? W.fat[p] : make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int p = p0 + 64 * u;
+                    if (p >= nf || p == mv || !overlap(fm, fp[u])) continue;
+                    const int slot = atomicAdd(&s_np, 1);
+                    const unsigned int lo = p < mv ? p : mv, hi = p < mv ? mv : p;
+                    if (slot < kToiPairCap) s_pairs[slot] = ((unsigned long long)lo << 32) | hi;
+                }
             }
         }
         __syncthreads();
@@ -2751,6 +2804,14 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         __syncthreads();
         for (int f = lane; f < nf && (error & 4); f += 64) W.moved[f] = 0; // untracked moved proxies
         n = s_n;
+        if (timed) tEvent += __builtin_amdgcn_s_memtime() - tMark;
+    }
+    if (timed) {
+        d.ktime[27] = __builtin_amdgcn_s_memtime();
+        d.ktime[28] = events;
+        d.ktime[29] = tWalk;
+        d.ktime[30] = tEvent;
+        d.ktime[31] = n;
     }
     if (lane == 0) {
         d.S[w].toiEvents = events;

@@ -1124,6 +1124,9 @@ dbx_status dbx_sync(dbx_handle *h) {
                             "adj-sort %lld | broadphase: pairs %lld addpair %lld\n",
                     t[17] - t[16], t[18] - t[17], t[19] - t[18], t[20] - t[19], t[21] - t[20], t[23] - t[22],
                     t[24] - t[23]);
+            fprintf(stderr, "dbx ktime toi: init+first search %lld, then %lld events: searches %lld events %lld "
+                            "(total %lld, %lld contacts)\n",
+                    t[26] - t[25], t[28], t[29], t[30], t[27] - t[25], t[31]);
         }
     }
     return DBX_OK;

@@ -29,6 +29,17 @@ __device__ __forceinline__ DProxy make_proxy(const Shape *s) {
     p.radius = s->radius;
     return p;
 }
+// The same proxy with its vertices copied to buf (kMaxPolygonVertices entries,
+// LDS), so that the many support and evaluate calls of one TOI read them from
+// LDS instead of global memory; the copies are independent loads.
+__device__ __forceinline__ DProxy make_proxy_in(const Shape *s, float2 *buf) {
+    DProxy p = make_proxy(s);
+#pragma unroll
+    for (int i = 0; i < kMaxPolygonVertices; ++i)
+        if (i < p.count) buf[i] = p.v[i];
+    p.v = buf;
+    return p;
+}
 // GetSupport (b2distance.d:94-111): first vertex of largest dot product.
 __device__ __forceinline__ int support(const DProxy &p, float2 d) {
     int best = 0;
