@@ -149,6 +149,8 @@ struct Dev {
     int *fcount;            // per fixture scratch (pair counts)
     // contacts, double buffered (collide compacts one into the other)
     Contact *cont[2];
+    float *ctan[2];         // b2Contact.m_tangentSpeed of each contact slot, moving with the contacts
+    int useTan;             // some contact has had a tangent speed set (until then every slot holds 0)
     // per body scratch
     int *adjOff, *adjCnt;   // contact adjacency of each body (CSR)
     int *bisl;              // island of a body this step, -1 when not simulated

@@ -174,6 +174,8 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
     const int n = d.S[w].contactCount;
     Contact *src = d.cont[cur] + wd.contactBase;
     Contact *dst = d.cont[cur ^ 1] + wd.contactBase;
+    const float *stan = d.ctan[cur] + wd.contactBase;
+    float *dtan = d.ctan[cur ^ 1] + wd.contactBase;
     const int *fbody = d.fbody + wd.fixBase;
     const Shape *shp = d.fshape + wd.fixBase;
     const float4 *fat = d.ffat + wd.fixBase;
@@ -249,6 +251,7 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
         if (keep) {
             c.flags &= ~kContactDormant;
             dst[m + off] = c;
+            if (d.useTan) dtan[m + off] = stan[i];
         }
         m += tot;
         int t2;
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int
 __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, int island, const int *__restrict__ fbody,
                                 const float4 *__restrict__ fmat, const uint32_t *__restrict__ bfl,
                                 const float4 *__restrict__ bmass, const float4 *__restrict__ bpos,
-                                const float4 *__restrict__ bvel, const StepParams &sp) {
+                                const float4 *__restrict__ bvel, const StepParams &sp, float tangentSpeed) {
     int ba = fbody[c.fa], bb = fbody[c.fb];
     float4 ma = bmass[ba], mb = bmass[bb];
     float radiusA = fmat[c.fa].w, radiusB = fmat[c.fb].w;
@@ -642,7 +645,7 @@ __device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, 
     v.nImp1 = warm ? sp.dtRatio * c.pt1.z : 0.0f;
     v.tImp1 = warm ? sp.dtRatio * c.pt1.w : 0.0f;
     v.friction = c.friction;
-    v.tangentSpeed = 0.0f;
+    v.tangentSpeed = tangentSpeed;
     v.indexA = ba;
     v.indexB = bb;
     v.invMassA = ma.x;
@@ -1140,6 +1143,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     const int nbatch = d.S[w].batchCount;
     const int nisl = d.S[w].islandCount;
     Contact *cont = d.cont[cur] + wd.contactBase;
+    const float *ctan = d.ctan[cur] + wd.contactBase;
     uint32_t *bfl = d.bflags + wd.bodyBase;
     float4 *bxf = d.bxf + wd.bodyBase;
     float4 *bpos = d.bpos + wd.bodyBase;
@@ -1191,7 +1195,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         int ci = ordC[k];
         VelCon v;
         PosCon p;
-        init_constraint(v, p, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp);
+        init_constraint(v, p, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp, d.useTan ? ctan[ci] : 0.0f);
         vc[k] = v;
         pc[k] = p;
     }
@@ -1571,6 +1575,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     int *Lsol = (int *)p;
 
     Contact *cont = d.cont[cur] + wd.contactBase;
+    const float *ctan = d.ctan[cur] + wd.contactBase;
     uint32_t *bfl = d.bflags + wd.bodyBase;
     float4 *bxf = d.bxf + wd.bodyBase;
     float4 *bpos = d.bpos + wd.bodyBase;
@@ -1628,7 +1633,8 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         const int ci = ordC[k];
         VelCon v;
         PosCon q;
-        init_constraint(v, q, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, Lpos, Lvel, sp);
+        init_constraint(v, q, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, Lpos, Lvel, sp,
+                        d.useTan ? ctan[ci] : 0.0f);
         vc[j] = v;
         pc[j] = q;
         Limp[j] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
@@ -2094,6 +2100,7 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
                 c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 cont[slot] = c;
+                d.ctan[cur][wd.contactBase + slot] = 0.0f;
                 wake_body(bfl, bpos, fbody[fa]);
                 wake_body(bfl, bpos, fbody[fb]);
             }
@@ -2145,6 +2152,7 @@ constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty
 
 struct ToiWorld {
     Contact *cont;
+    float *ctan;
     int4 *tc, *tb;
     const int *fbody;
     const Shape *shp;
@@ -2381,7 +2389,8 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
     // The solver works on the bodies' own sweep positions, which the island's
     // position array starts from and which nothing else reads until the end.
     for (int k = 0; k < ncon; ++k)
-        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss);
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss,
+                        W.ctan[icon[k]]);
     // SolveTOIPositionConstraints (b2contactsolver.d:152-242): only the two TOI
     // bodies move
     for (int it = 0; it < 20; ++it) {
@@ -2410,7 +2419,8 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
         W.bpos0[ib] = make_float4(pb.x, pb.y, pb.z, W.bpos0[ib].w);
     }
     for (int k = 0; k < ncon; ++k) {
-        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss);
+        init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss,
+                        W.ctan[icon[k]]);
         imp[k] = make_float4(vcs[k].nImp0, vcs[k].tImp0, vcs[k].nImp1, vcs[k].tImp1);
     }
     for (int it = 0; it < sp.velocityIterations; ++it)
@@ -2533,6 +2543,7 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.cont[n] = c;
+    W.ctan[n] = 0.0f;
     W.tc[n] = make_int4(__float_as_int(1.0f), 0, -1, -1);
     mark_dirty(W, L, n, 0);
     link_contact(W, n);
@@ -2558,6 +2569,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     const int lane = threadIdx.x;
     ToiWorld W;
     W.cont = d.cont[cur] + wd.contactBase;
+    W.ctan = d.ctan[cur] + wd.contactBase;
     W.tc = d.toiC + wd.contactBase;
     W.tb = d.toiB + wd.bodyBase;
     W.fbody = d.fbody + wd.fixBase;

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/dbox_hip.h"
@@ -427,6 +428,12 @@ struct dbx_handle {
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
     std::vector<Tier> tiers;      // tier 0 = the largest worlds, on `stream`
     std::vector<WideBp> wide;     // worlds whose broadphase scratch does not fit in LDS, by tier
+    // DBX_STEP_PRESOLVE: a step stopped after Collide, finished by dbx_step_finish
+    bool splitWaiting = false;    // the first half has run, the second has not
+    bool lastSplit = false;       // the last (or pending) step was split: PreSolve events are kept
+    StepParams pendSp{};
+    uint32_t pendFlags = 0;
+    Contact *contPre = nullptr;   // the list Collide walked, as it was before (old manifolds)
     std::vector<void *> allocs;
     hipEvent_t ev[6] = {};
     std::vector<hipEvent_t> ring;   // kStepEvents events per step since the last sync
@@ -756,6 +763,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
     // contacts and scratch
     DBX_HIP(h, dalloc(h, &d.cont[0], nc));
     DBX_HIP(h, dalloc(h, &d.cont[1], nc));
+    DBX_HIP(h, dalloc(h, &d.ctan[0], nc));
+    DBX_HIP(h, dalloc(h, &d.ctan[1], nc));
     DBX_HIP(h, dalloc(h, &d.adjOff, nb));
     DBX_HIP(h, dalloc(h, &d.adjCnt, nb));
     DBX_HIP(h, dalloc(h, &d.bisl, nb));
@@ -839,8 +848,10 @@ static dbx_status stage_mlp(dbx_handle *h) {
 }
 
 // The step chain of one tier (b2World.Step, b2world.d:367-434, for its worlds).
+// part bits: 1 = FindNewContacts (when fixtures were added), 2 = Collide, 4 = the
+// island search and everything after it.
 static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t flags, bool findNew,
-                            const hipEvent_t *ev) {
+                            const hipEvent_t *ev, int part = 7) {
     const Tier &T = h->tiers[t];
     bool lead = t == 0;
     const int cin = h->cur, cout = cin ^ 1;
@@ -848,9 +859,10 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     hipStream_t s = T.stream;
     const Dev &d = h->d;
     const WideBp *wide = T.wideCount ? &h->wide[T.wideFirst] : nullptr;
-    if (findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
+    if ((part & 1) && findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
         DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
-    DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s));
+    if (part & 2) DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s));
+    if (!(part & 4)) return DBX_OK;
     const bool timed = lead && ev; // phase events: lead tier, eager steps only
     if (timed) (void)hipEventRecord(ev[1], s);
     if (sp.dt > 0.0f) {
@@ -989,6 +1001,7 @@ static dbx_status tune_account(dbx_handle *h) {
 dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t flags) {
     if (!h) return DBX_E_INVALID_ARG;
     if (!h->finalized) return fail(h, DBX_E_STATE, "dbx_finalize has not been called");
+    if (h->splitWaiting) return fail(h, DBX_E_STATE, "a PRESOLVE step is waiting for dbx_step_finish");
     if (!std::isfinite(dt) || vel < 0 || pos < 0) return fail(h, DBX_E_INVALID_ARG, "bad step parameters");
     if ((flags & DBX_STEP_LEARNED) && !h->mlp.ready) return fail(h, DBX_E_STATE, "no warm-start network loaded");
     if ((flags & DBX_STEP_LEARNED) && !(flags & DBX_STEP_WARM))
@@ -1025,6 +1038,35 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     h->d.logCount = h->logCount;
     h->d.logCap = h->logCap;
     const bool findNew = h->newFixture;
+    if (flags & DBX_STEP_PRESOLVE) {
+        // b2World.Step up to the end of Collide, every tier in turn on the handle's
+        // stream: FindNewContacts, a copy of the list Collide is about to walk (the
+        // old manifolds that PreSolve receives), Collide.  dbx_step_finish runs the rest.
+        if (!h->contPre) DBX_HIP(h, dalloc(h, &h->contPre, h->totalContacts));
+        for (size_t t = 0; t < h->tiers.size(); ++t) {
+            hipStream_t ts = h->tiers[t].stream;
+            h->tiers[t].stream = h->stream;
+            dbx_status st = step_tier(h, (int)t, sp, flags, findNew, nullptr, 1);
+            h->tiers[t].stream = ts;
+            if (st != DBX_OK) return st;
+        }
+        DBX_HIP(h, hipMemcpyAsync(h->contPre, h->d.cont[h->cur], h->totalContacts * sizeof(Contact),
+                                  hipMemcpyDeviceToDevice, h->stream));
+        for (size_t t = 0; t < h->tiers.size(); ++t) {
+            hipStream_t ts = h->tiers[t].stream;
+            h->tiers[t].stream = h->stream;
+            dbx_status st = step_tier(h, (int)t, sp, flags, findNew, nullptr, 2);
+            h->tiers[t].stream = ts;
+            if (st != DBX_OK) return st;
+        }
+        h->newFixture = false;
+        h->splitWaiting = true;
+        h->lastSplit = true;
+        h->pendSp = sp;
+        h->pendFlags = flags;
+        return DBX_OK;
+    }
+    h->lastSplit = false;
     const bool tuning = h->tuneDone < h->tuneSteps;
     if (tuning) {
         const int c = h->tuneDone / h->tunePer;
@@ -1063,6 +1105,70 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     h->cur ^= 1;
     if (dt > 0.0f) h->inv_dt0 = sp.inv_dt;
     if (tuning) return tune_account(h);
+    return DBX_OK;
+}
+
+static dbx_status check_world(dbx_handle *h, int32_t world);
+
+dbx_status dbx_step_finish(dbx_handle *h) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (!h->splitWaiting) return fail(h, DBX_E_STATE, "no PRESOLVE step is waiting");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    for (size_t t = 0; t < h->tiers.size(); ++t) {
+        hipStream_t ts = h->tiers[t].stream;
+        h->tiers[t].stream = h->stream;
+        dbx_status st = step_tier(h, (int)t, h->pendSp, h->pendFlags, false, nullptr, 4);
+        h->tiers[t].stream = ts;
+        if (st != DBX_OK) return st;
+    }
+    h->splitWaiting = false;
+    h->cur ^= 1;
+    if (h->pendSp.dt > 0.0f) h->inv_dt0 = h->pendSp.inv_dt;
+    return DBX_OK;
+}
+
+dbx_status dbx_write_contact_settings(dbx_handle *h, int32_t world, const dbx_contact_setting *s, int32_t n) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    if (!h->splitWaiting) return fail(h, DBX_E_STATE, "contact settings are written between dbx_step(PRESOLVE) and dbx_step_finish");
+    if (n < 0 || (n > 0 && !s)) return fail(h, DBX_E_INVALID_ARG, "bad settings array");
+    const WorldDesc &D = h->W[world];
+    WorldState S;
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
+    const int m = S.contactCount, buf = h->cur ^ 1; // the list Collide left
+    std::vector<Contact> c(m);
+    std::vector<float> tan(m);
+    if (m) {
+        DBX_HIP(h, hipMemcpy(c.data(), h->d.cont[buf] + D.contactBase, m * sizeof(Contact), hipMemcpyDeviceToHost));
+        DBX_HIP(h, hipMemcpy(tan.data(), h->d.ctan[buf] + D.contactBase, m * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    std::unordered_map<long long, int> at;
+    at.reserve(2 * (size_t)m);
+    for (int i = 0; i < m; ++i) {
+        at[((long long)c[i].fa << 32) | (unsigned)c[i].fb] = i;
+        at[((long long)c[i].fb << 32) | (unsigned)c[i].fa] = i;
+    }
+    for (int k = 0; k < n; ++k) {
+        auto it = at.find(((long long)s[k].fixture_a << 32) | (unsigned)s[k].fixture_b);
+        if (it == at.end()) return fail(h, DBX_E_INVALID_ARG, "no contact between these fixtures");
+        if (!std::isfinite(s[k].friction) || !std::isfinite(s[k].restitution) || !std::isfinite(s[k].tangent_speed))
+            return fail(h, DBX_E_INVALID_ARG, "contact setting is not finite");
+        Contact &x = c[it->second];
+        const int mk = s[k].mask;
+        // b2Contact.SetEnabled / SetFriction / SetRestitution / SetTangentSpeed (b2contact.d:108-198)
+        if (mk & DBX_SET_ENABLED) x.flags = s[k].enabled ? (x.flags | kContactEnabled) : (x.flags & ~kContactEnabled);
+        if (mk & DBX_SET_FRICTION) x.friction = s[k].friction;
+        if (mk & DBX_SET_RESTITUTION) x.restitution = s[k].restitution;
+        if (mk & DBX_SET_TANGENT_SPEED) {
+            tan[it->second] = s[k].tangent_speed;
+            if (s[k].tangent_speed != 0.0f) h->d.useTan = 1; // from now on the slots' speeds move with the contacts
+        }
+    }
+    if (m) {
+        DBX_HIP(h, hipMemcpy(h->d.cont[buf] + D.contactBase, c.data(), m * sizeof(Contact), hipMemcpyHostToDevice));
+        DBX_HIP(h, hipMemcpy(h->d.ctan[buf] + D.contactBase, tan.data(), m * sizeof(float), hipMemcpyHostToDevice));
+    }
     return DBX_OK;
 }
 
@@ -1211,6 +1317,7 @@ dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, in
 
 dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *in, const float *force3, int32_t n) {
     dbx_status st = check_world(h, world);
+    if (h->splitWaiting) return fail(h, DBX_E_STATE, "a PRESOLVE step is waiting for dbx_step_finish");
     if (st) return st;
     const WorldDesc &D = h->W[world];
     if (!in || n != D.bodyCount) return fail(h, DBX_E_INVALID_ARG, "one state per body is required");
@@ -1259,13 +1366,21 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
     DBX_HIP(h, hipMemcpy(&S, h->d.S + world, sizeof S, hipMemcpyDeviceToHost));
     // The last step's Collide walked the list held in the other buffer, where its
     // flags stay; the solved contacts are in the current list, in island order.
-    const int n0 = S.collideCount, m = S.contactCount, nc = S.constraints;
-    std::vector<Contact> walked(n0), now(m);
+    // While a PRESOLVE step waits, Collide's walked list is still the current one
+    // and nothing has been solved yet.
+    const int walkedBuf = h->splitWaiting ? h->cur : h->cur ^ 1, nowBuf = walkedBuf ^ 1;
+    const int n0 = S.collideCount, m = S.contactCount, nc = h->splitWaiting ? 0 : S.constraints;
+    std::vector<Contact> walked(n0), now(m), pre;
     std::vector<int32_t> ord(nc);
-    if (n0) DBX_HIP(h, hipMemcpy(walked.data(), h->d.cont[h->cur ^ 1] + D.contactBase, n0 * sizeof(Contact),
+    if (n0) DBX_HIP(h, hipMemcpy(walked.data(), h->d.cont[walkedBuf] + D.contactBase, n0 * sizeof(Contact),
                                  hipMemcpyDeviceToHost));
-    if (m) DBX_HIP(h, hipMemcpy(now.data(), h->d.cont[h->cur] + D.contactBase, m * sizeof(Contact), hipMemcpyDeviceToHost));
+    if (m) DBX_HIP(h, hipMemcpy(now.data(), h->d.cont[nowBuf] + D.contactBase, m * sizeof(Contact), hipMemcpyDeviceToHost));
     if (nc) DBX_HIP(h, hipMemcpy(ord.data(), h->d.ordC + D.contactBase, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const bool preSolve = h->lastSplit && h->contPre;
+    if (preSolve && n0) {
+        pre.resize(n0);
+        DBX_HIP(h, hipMemcpy(pre.data(), h->contPre + D.contactBase, n0 * sizeof(Contact), hipMemcpyDeviceToHost));
+    }
     std::vector<dbx_contact_event> ev;
     ev.reserve(nc + 16);
     for (int i = n0 - 1; i >= 0; --i) { // list order: newest first
@@ -1279,6 +1394,25 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
         } else if (c.flags & kContactEnded) {
             e.type = DBX_EVENT_END;
             ev.push_back(e);
+        }
+        // PreSolve after the update of a contact that touches (b2contact.d:352-355),
+        // with the manifold as it was before the update
+        if (preSolve && (c.flags & kContactTouching) && !(c.flags & (kContactDormant | kContactDestroyed))) {
+            const Contact &o = pre[i];
+            dbx_contact_event p{};
+            p.type = DBX_EVENT_PRE_SOLVE;
+            p.fixture_a = c.fa;
+            p.fixture_b = c.fb;
+            p.point_count = (o.meta >> 4) & 0xf;
+            if (p.point_count > 0) {
+                p.normal_impulse[0] = o.pt0.z;
+                p.tangent_impulse[0] = o.pt0.w;
+            }
+            if (p.point_count > 1) {
+                p.normal_impulse[1] = o.pt1.z;
+                p.tangent_impulse[1] = o.pt1.w;
+            }
+            ev.push_back(p);
         }
     }
     for (int k = 0; k < nc; ++k) {
@@ -1315,7 +1449,8 @@ dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int
     if (count) *count = n;
     if (!out || cap <= 0) return DBX_OK;
     std::vector<Contact> c(n);
-    if (n) DBX_HIP(h, hipMemcpy(c.data(), h->d.cont[h->cur] + D.contactBase, n * sizeof(Contact), hipMemcpyDeviceToHost));
+    const int buf = h->splitWaiting ? h->cur ^ 1 : h->cur; // while a PRESOLVE step waits: the list Collide left
+    if (n) DBX_HIP(h, hipMemcpy(c.data(), h->d.cont[buf] + D.contactBase, n * sizeof(Contact), hipMemcpyDeviceToHost));
     int m = std::min(n, cap);
     for (int k = 0; k < m; ++k) {
         const Contact &s = c[n - 1 - k]; // list order: newest first

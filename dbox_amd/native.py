@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdbox_hip.so")
 
 DBX_OK, DBX_E_INVALID_ARG, DBX_E_CAPACITY, DBX_E_UNSUPPORTED, DBX_E_HIP, DBX_E_LOCKED, DBX_E_STATE = 0, -1, -2, -3, -4, -5, -6
-WARM, SLEEP, CCD, ORDERED, LEARNED, RESIDUAL, LOG = 1, 2, 4, 8, 16, 32, 64
+WARM, SLEEP, CCD, ORDERED, LEARNED, RESIDUAL, LOG, PRESOLVE = 1, 2, 4, 8, 16, 32, 64, 128
 DEFAULT_FLAGS = WARM | SLEEP
 
 BODY_STATE_DTYPE = np.dtype([
@@ -34,7 +34,12 @@ CONTACT_DTYPE = np.dtype([
     ("friction", "<f4"), ("restitution", "<f4"),
 ])
 
-EVENT_BEGIN, EVENT_END, EVENT_POST_SOLVE = 1, 2, 3
+EVENT_BEGIN, EVENT_END, EVENT_POST_SOLVE, EVENT_PRE_SOLVE = 1, 2, 3, 4
+SET_ENABLED, SET_FRICTION, SET_RESTITUTION, SET_TANGENT_SPEED = 1, 2, 4, 8
+SETTING_DTYPE = np.dtype([  # dbx_contact_setting
+    ("fixture_a", "<i4"), ("fixture_b", "<i4"), ("mask", "<i4"), ("enabled", "<i4"),
+    ("friction", "<f4"), ("restitution", "<f4"), ("tangent_speed", "<f4"),
+])
 EVENT_DTYPE = np.dtype([
     ("type", "<i4"), ("fixture_a", "<i4"), ("fixture_b", "<i4"), ("point_count", "<i4"),
     ("normal_impulse", "<f4", (2,)), ("tangent_impulse", "<f4", (2,)),
@@ -65,7 +70,7 @@ EXPORTS = [
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
-    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events",
+    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events", "dbx_write_contact_settings", "dbx_step_finish",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
@@ -115,6 +120,8 @@ def lib():
         L.dbx_set_graph.argtypes = [vp, i32]
         L.dbx_read_residuals.argtypes = [vp, vp, i32]
         L.dbx_read_toi_events.argtypes = [vp, vp, i32]
+        L.dbx_write_contact_settings.argtypes = [vp, i32, vp, i32]
+        L.dbx_step_finish.argtypes = [vp]
         L.dbx_mlp_init.argtypes = [vp, u64]
         L.dbx_mlp_set.argtypes = [vp, vp, vp]
         L.dbx_mlp_get.argtypes = [vp, vp, vp]
@@ -329,6 +336,14 @@ class StepWorlds:
     def set_tier_blocks(self, blocks):
         b = np.ascontiguousarray(blocks, np.int32)
         self._check(self.L.dbx_set_tier_blocks(self.h, _p(b), len(b)))
+
+    def step_finish(self):
+        """The rest of a step started with the PRESOLVE flag."""
+        self._check(self.L.dbx_step_finish(self.h))
+
+    def write_contact_settings(self, world, settings):
+        s = np.ascontiguousarray(settings, SETTING_DTYPE)
+        self._check(self.L.dbx_write_contact_settings(self.h, world, _p(s), len(s)))
 
     def toi_events(self):
         """Per world: TOI events solved by the last step's continuous pass."""

@@ -48,7 +48,10 @@ enum {
     DBX_STEP_ORDERED = 1u << 3, /* solve in the reference's exact island order (dependency levels) */
     DBX_STEP_LEARNED = 1u << 4, /* MLP warm start for new contact points */
     DBX_STEP_RESIDUAL = 1u << 5, /* iterate velocities to a residual instead of a fixed count */
-    DBX_STEP_LOG = 1u << 6       /* append new points' features and solved impulses to the training log */
+    DBX_STEP_LOG = 1u << 6,      /* append new points' features and solved impulses to the training log */
+    DBX_STEP_PRESOLVE = 1u << 7  /* stop after b2ContactManager.Collide: the host runs its PreSolve
+                                    callbacks, may change contacts with dbx_write_contact_settings,
+                                    and dbx_step_finish runs the rest of the step */
 };
 
 /* Body definition, same fields as b2BodyDef (src/dbox/dynamics/b2body.d:51-104). */
@@ -108,13 +111,25 @@ typedef struct {
  * b2contactmanager.d:190-193); POST_SOLVE from b2Island.Report
  * (src/dbox/dynamics/b2island.d:438-461), with b2ContactImpulse's count and
  * impulses. */
-enum { DBX_EVENT_BEGIN = 1, DBX_EVENT_END = 2, DBX_EVENT_POST_SOLVE = 3 };
+enum { DBX_EVENT_BEGIN = 1, DBX_EVENT_END = 2, DBX_EVENT_POST_SOLVE = 3, DBX_EVENT_PRE_SOLVE = 4 };
 typedef struct {
     int32_t type;                 /* DBX_EVENT_* */
     int32_t fixture_a, fixture_b; /* as in dbx_contact */
-    int32_t point_count;          /* POST_SOLVE: b2ContactImpulse.count (1 for a downgraded two-point manifold) */
-    float normal_impulse[2], tangent_impulse[2];
+    int32_t point_count;          /* POST_SOLVE: b2ContactImpulse.count (1 for a downgraded two-point manifold);
+                                     PRE_SOLVE: the old manifold's pointCount */
+    float normal_impulse[2], tangent_impulse[2]; /* PRE_SOLVE: the old manifold's impulses */
 } dbx_contact_event;
+
+/* What a PreSolve callback may change on a contact (b2Contact.SetEnabled,
+ * SetFriction, SetRestitution, SetTangentSpeed: b2contact.d:108-198), for the
+ * contact between two fixtures. */
+enum { DBX_SET_ENABLED = 1, DBX_SET_FRICTION = 2, DBX_SET_RESTITUTION = 4, DBX_SET_TANGENT_SPEED = 8 };
+typedef struct {
+    int32_t fixture_a, fixture_b;
+    int32_t mask;     /* DBX_SET_*: the fields below that are applied */
+    int32_t enabled;
+    float friction, restitution, tangent_speed;
+} dbx_contact_setting; /* 28 bytes */
 
 /* b2Profile (src/dbox/dynamics/b2timestep.d:37-46), milliseconds of the last step,
  * measured with HIP events around each phase. */
@@ -172,11 +187,24 @@ dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int
  * contact list (b2contactmanager.d:251-317; a contact destroyed while touching
  * gives END at its place in the list), then POST_SOLVE for every solved contact
  * in island order (b2world.d:1085, b2island.d:239).  With more than one step per
- * dbx_step call only the last step's callbacks are kept.  PreSolve is not
- * delivered (it would need the step to stop between Collide and Solve).
+ * dbx_step call only the last step's callbacks are kept.  After a DBX_STEP_PRESOLVE
+ * step each touching contact Collide updated also gives PRE_SOLVE right after its
+ * BEGIN / END (b2contact.d:342-355); while that step waits for dbx_step_finish only
+ * the Collide callbacks exist.  The continuous pass's callbacks are not recorded.
  * *count = number of events; DBX_E_CAPACITY when cap is smaller (nothing else
  * written). */
 dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out, int32_t cap, int32_t *count);
+/* Between dbx_step(..., DBX_STEP_PRESOLVE) and dbx_step_finish: apply what the
+ * PreSolve callbacks changed, for n contacts of one world named by their fixture
+ * pair (either order).  Friction, restitution and tangent speed stay with the
+ * contact for later steps, as in b2Contact; enabled lasts for this step
+ * (b2Contact.Update re-enables every contact it updates).  DBX_E_STATE outside
+ * that window, DBX_E_INVALID_ARG for a pair without a contact. */
+dbx_status dbx_write_contact_settings(dbx_handle *h, int32_t world, const dbx_contact_setting *s, int32_t n);
+/* The rest of a DBX_STEP_PRESOLVE step (island search, solve, broadphase,
+ * continuous pass) with its parameters; asynchronous like dbx_step.  Split steps
+ * run each tier in turn on the handle's stream and are not timed in dbx_profile. */
+dbx_status dbx_step_finish(dbx_handle *h);
 /* Fat AABBs of each fixture's proxy (lower x, lower y, upper x, upper y). */
 dbx_status dbx_read_fat_aabbs(dbx_handle *h, int32_t world, float *out, int32_t n);
 /* Pairs the last broadphase pass handed to AddPair, as (fixture, fixture) in the

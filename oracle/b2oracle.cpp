@@ -1888,6 +1888,9 @@ struct World {
     bool newFixture = false;
     bool stepComplete = true;
     int toiEvents = 0; /* test hook: TOI events solved by the last step */
+    /* test hook, standing in for a b2ContactListener.PreSolve: 0 none, 1 record the
+     * calls, 2 record them and apply presolvePolicy */
+    int presolveMode = 0;
     float inv_dt0 = 0.0f;
     int solveOrder = 0;
     int islandCount = 0; /* test hook: 1 = solve island contacts in greedy-colour order */
@@ -2163,9 +2166,33 @@ struct World {
         }
         if (touching) c.flags |= cf_touching;
         else c.flags &= ~cf_touching;
-        /* listener calls b2contact.d:342-350 (PreSolve is not recorded) */
+        /* listener calls b2contact.d:342-355 */
         if (!wasTouching && touching) recordEvent(1, c, nullptr);
         if (wasTouching && !touching) recordEvent(2, c, nullptr);
+        if (touching && presolveMode) {
+            orc_event e = {};
+            e.type = 4;
+            e.fixture_a = c.fixtureA;
+            e.fixture_b = c.fixtureB;
+            e.point_count = oldManifold.pointCount;
+            for (int j = 0; j < oldManifold.pointCount; ++j) {
+                e.normal_impulse[j] = oldManifold.points[j].normalImpulse;
+                e.tangent_impulse[j] = oldManifold.points[j].tangentImpulse;
+            }
+            events.push_back(e);
+            if (presolveMode == 2) presolvePolicy(c);
+        }
+    }
+    /* The test listener's PreSolve: by k = fixtureA + fixtureB, k % 7 == 0 disables
+     * the contact, 1 sets friction 0.05, 2 restitution 0.6, 3 tangent speed 0.8
+     * (b2Contact.SetEnabled / SetFriction / SetRestitution / SetTangentSpeed,
+     * b2contact.d:108-198). */
+    static void presolvePolicy(Contact &c) {
+        const int k = (c.fixtureA + c.fixtureB) % 7;
+        if (k == 0) c.flags &= ~cf_enabled;
+        else if (k == 1) c.friction = 0.05f;
+        else if (k == 2) c.restitution = 0.6f;
+        else if (k == 3) c.tangentSpeed = 0.8f;
     }
     /* b2ContactManager.Collide b2contactmanager.d:251-317 */
     void collide() {
@@ -3281,6 +3308,11 @@ int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const or
     return timeOfImpact(*t_out, pa, pb, sweepIn(sweep_a9), sweepIn(sweep_b9), t_max);
 }
 int orc_toi_events(void *w) { return ((World *)w)->toiEvents; }
+int orc_world_set_presolve(void *w, int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    ((World *)w)->presolveMode = mode;
+    return 0;
+}
 int orc_shape_geometry(const orc_fixture_def *def, float *o, float *mass_out4) {
     Shape s;
     World::buildShape(*def, s);

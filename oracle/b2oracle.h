@@ -160,6 +160,12 @@ int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const or
                        float t_max, float *t_out);
 /* Test hook: TOI events (sub-step island solves) of the last orc_step. */
 int orc_toi_events(void *w);
+/* Test hook standing in for b2ContactListener.PreSolve (b2contact.d:352-355):
+ * mode 1 records each call as an event of type 4 (fixtures, and the old
+ * manifold's point count and impulses); mode 2 also applies a fixed policy by
+ * k = (fixtureA + fixtureB) % 7: 0 disable, 1 friction 0.05, 2 restitution 0.6,
+ * 3 tangent speed 0.8. */
+int orc_world_set_presolve(void *w, int mode);
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ def lib():
         L.orc_shape_distance.argtypes = [vp, vp, vp, vp, i32, vp]
         L.orc_time_of_impact.argtypes = [vp, vp, vp, vp, f32, vp]
         L.orc_toi_events.argtypes = [vp]
+        L.orc_world_set_presolve.argtypes = [vp, i32]
         _lib = L
     return _lib
 
@@ -157,6 +158,11 @@ class OracleWorld:
 
     def max_penetration(self) -> float:
         return float(self.L.orc_max_penetration(self.h))
+
+    def set_presolve(self, mode: int):
+        """PreSolve test listener: 1 record, 2 record and apply the fixed policy."""
+        if self.L.orc_world_set_presolve(self.h, mode) != 0:
+            raise ValueError(mode)
 
     def toi_events(self) -> int:
         """TOI events (sub-step island solves) of the last step."""

@@ -40,9 +40,9 @@ def test_struct_layouts_match_numpy_records(tmp_path):
         #include <stddef.h>
         #include "{HEADER}"
         int main(void) {{
-            printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(dbx_body_def), sizeof(dbx_fixture_def),
+            printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(dbx_body_def), sizeof(dbx_fixture_def),
                    sizeof(dbx_body_state), sizeof(dbx_contact), sizeof(dbx_profile), sizeof(dbx_config),
-                   offsetof(dbx_fixture_def, p), sizeof(dbx_contact_event));
+                   offsetof(dbx_fixture_def, p), sizeof(dbx_contact_event), sizeof(dbx_contact_setting));
             return 0;
         }}"""))
     exe = tmp_path / "sizes"
@@ -56,6 +56,7 @@ def test_struct_layouts_match_numpy_records(tmp_path):
     assert sizes[5] == C.sizeof(native.DbxConfig)
     assert sizes[6] == scenes.FIXTURE_DTYPE.fields["p"][1]
     assert sizes[7] == native.EVENT_DTYPE.itemsize
+    assert sizes[8] == native.SETTING_DTYPE.itemsize
 
 
 def test_create_without_device_fails_cleanly_or_succeeds():

@@ -88,3 +88,62 @@ def test_resting_box_post_solve_is_one_step_of_gravity():
     total = float(post["normal_impulse"][0, : post["point_count"][0]].sum())
     assert abs(total - m * 10.0 / 60.0) < 1e-3 * m * 10.0 / 60.0
     assert np.all(np.abs(post["tangent_impulse"][0]) < 1e-6)
+
+
+PRE = 4
+
+
+def test_presolve_record_is_passive_and_covers_the_touching_contacts():
+    # recording PreSolve calls changes nothing; with sleeping off every contact
+    # that touches after Collide gets exactly one call, after its Begin
+    sc = scenes.mixed_container(120, 40.0, 80.0, seed=31)
+    quiet = po.OracleWorld(sc, allow_sleep=False)
+    rec = po.OracleWorld(sc, allow_sleep=False)
+    rec.set_presolve(1)
+    calls = 0
+    for _ in range(80):
+        quiet.step()
+        rec.step()
+        assert rec.bodies().tobytes() == quiet.bodies().tobytes()
+        e = rec.events()
+        pre = e[e["type"] == PRE]
+        keys = [(int(a), int(b)) for a, b in zip(pre["fixture_a"], pre["fixture_b"])]
+        assert len(keys) == len(set(keys))
+        assert set(keys) == _touching_pairs(rec)
+        # a Begin comes before the PreSolve of the same contact
+        order = {k: i for i, k in enumerate((int(a), int(b), int(t)) for a, b, t in
+                                            zip(e["fixture_a"], e["fixture_b"], e["type"]))}
+        for a, b in keys:
+            if (a, b, BEGIN) in order:
+                assert order[(a, b, BEGIN)] < order[(a, b, PRE)]
+        assert np.all(pre["point_count"] <= 2)
+        calls += len(keys)
+    assert calls > 800
+
+
+def test_presolve_policy_disables_and_changes_friction():
+    # the test policy: (fixtureA + fixtureB) % 7 == 0 disables the contact for the
+    # step (no PostSolve), == 1 sets friction 0.05: tangent impulses stay within
+    # 0.1 x normal (the solver clamps against the previous iteration's normal
+    # impulse, so a final ratio slightly above 0.05 occurs; the mixed default
+    # frictions here are >= 0.2), and some contacts slide at that cone
+    sc = scenes.mixed_container(150, 40.0, 80.0, seed=32)
+    o = po.OracleWorld(sc)
+    o.set_presolve(2)
+    low_friction = disabled_seen = sliding = 0
+    for _ in range(120):
+        o.step()
+        e = o.events()
+        post = e[e["type"] == POST]
+        k = (post["fixture_a"] + post["fixture_b"]) % 7
+        assert not np.any(k == 0)
+        pre = e[e["type"] == PRE]
+        disabled_seen += int(np.count_nonzero((pre["fixture_a"] + pre["fixture_b"]) % 7 == 0))
+        lf = post[k == 1]
+        for ev in lf:
+            n = int(ev["point_count"])
+            t, nn = np.abs(ev["tangent_impulse"][:n]), ev["normal_impulse"][:n]
+            assert np.all(t <= 0.1 * nn + 1e-6)
+            sliding += int(np.count_nonzero((t > 0.04 * nn) & (nn > 1e-4)))
+        low_friction += len(lf)
+    assert disabled_seen > 50 and low_friction > 50 and sliding > 20
