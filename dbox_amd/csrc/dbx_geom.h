@@ -41,7 +41,8 @@ enum : int { kFeatVertex = 0, kFeatFace = 1 };
 // One shape, 160 bytes, read-only on the device.
 //  polygon: v[0..count) vertices, n[0..count) normals, c = centroid
 //  circle : c = local centre
-//  edge   : v[0] = vertex1, v[1] = vertex2, v[2] = vertex0, v[3] = vertex3, flags bit0/bit1 = has v0/v3
+//  edge   : v[0] = vertex1, v[1] = vertex2, v[2] = vertex0, v[3] = vertex3, flags bit0/bit1 = has v0/v3;
+//           flags bit2: a child edge of a chain (b2ChainShape.GetChildEdge), whose AABB has no radius
 struct alignas(16) Shape {
     int type;
     int count;
@@ -245,7 +246,9 @@ __device__ inline float4 shape_aabb(const Shape *__restrict__ s, const Xf &xf) {
     float2 a = mul(xf, s->v[0]);
     float2 b = mul(xf, s->v[1]);
     float2 lo = vminr(a, b), hi = vmaxr(a, b);
-    float r = s->radius;
+    // b2EdgeShape.ComputeAABB adds the radius; b2ChainShape.ComputeAABB does not
+    // (b2edgeshape.d:163-176, b2chainshape.d:214-226); x - 0 is x exactly
+    float r = (s->flags & 4) ? 0.0f : s->radius;
     return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
 }
 

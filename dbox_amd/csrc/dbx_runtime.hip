@@ -210,7 +210,70 @@ float4 shape_aabb_host(const Shape &s, const HXf &xf) { // ComputeAABB at creati
     }
     HV a = hmul(xf, hv(s.v[0].x, s.v[0].y)), b = hmul(xf, hv(s.v[1].x, s.v[1].y));
     HV lo = hmin(a, b), hi = hmax(a, b);
-    return make_float4(lo.x - s.radius, lo.y - s.radius, hi.x + s.radius, hi.y + s.radius);
+    const float r = (s.flags & 4) ? 0.0f : s.radius; // a chain's child: no radius (b2chainshape.d:214-226)
+    return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
+}
+
+// b2ChainShape (b2chainshape.d:24-263) as its child edges (GetChildEdge,
+// b2chainshape.d:134-170), one proxy each.  Definition: count vertices in
+// p[0 .. 2 count), flags bit1 / bit2 a previous / next ghost vertex in p[16..17] /
+// p[18..19] (SetPrevVertex / SetNextVertex), bit4 a loop (CreateLoop).
+bool chain_children(const dbx_fixture_def &d, std::vector<Shape> &out, std::string &err) {
+    const bool loop = (d.flags & 16) != 0;
+    const int cnt = d.count;
+    if (cnt < (loop ? 3 : 2) || cnt > 8) {
+        err = "a chain needs 2 to 8 vertices (3 to 8 for a loop)";
+        return false;
+    }
+    HV v[9];
+    for (int i = 0; i < cnt; ++i) {
+        if (!finite(d.p[2 * i]) || !finite(d.p[2 * i + 1])) {
+            err = "chain vertex is not finite";
+            return false;
+        }
+        v[i] = hv(d.p[2 * i], d.p[2 * i + 1]);
+        if (i > 0) { // CreateChain / CreateLoop assert b2DistanceSquared > b2_linearSlop^2
+            const HV e = v[i] - v[i - 1];
+            if (!(hdot(e, e) > kLinearSlop * kLinearSlop)) {
+                err = "chain vertices are too close";
+                return false;
+            }
+        }
+    }
+    int m = cnt;
+    HV prev = hv(0.0f, 0.0f), next = hv(0.0f, 0.0f);
+    bool hasPrev = false, hasNext = false;
+    if (loop) {
+        m = cnt + 1;
+        v[cnt] = v[0];
+        prev = v[m - 2];
+        next = v[1];
+        hasPrev = hasNext = true;
+    } else {
+        if (d.flags & 2) {
+            prev = hv(d.p[16], d.p[17]);
+            hasPrev = true;
+        }
+        if (d.flags & 4) {
+            next = hv(d.p[18], d.p[19]);
+            hasNext = true;
+        }
+    }
+    for (int i = 0; i + 1 < m; ++i) {
+        Shape s;
+        memset(&s, 0, sizeof(s));
+        s.type = kEdge;
+        s.radius = kPolygonRadius;
+        s.v[0] = make_float2(v[i].x, v[i].y);
+        s.v[1] = make_float2(v[i + 1].x, v[i + 1].y);
+        const bool has0 = i > 0 ? true : hasPrev, has3 = i < m - 2 ? true : hasNext;
+        const HV v0 = i > 0 ? v[i - 1] : prev, v3 = i < m - 2 ? v[i + 2] : next;
+        s.v[2] = make_float2(v0.x, v0.y);
+        s.v[3] = make_float2(v3.x, v3.y);
+        s.flags = (has0 ? 1 : 0) | (has3 ? 2 : 0) | 4;
+        out.push_back(s);
+    }
+    return true;
 }
 
 bool build_shape(const dbx_fixture_def &d, Shape &s, std::string &err) {
@@ -259,6 +322,7 @@ struct WorldHost {
     std::vector<uint32_t> bflags;
     std::vector<float4> bxf, bpos, bpos0, bvel, bforce, bmass, bparam;
     std::vector<int> fbody;
+    std::vector<int> pfix, pchild; // per proxy: its fixture (definition index) and child
     std::vector<Shape> fshape;
     std::vector<float4> fmat, ffat;
     std::vector<int2> ffilter;
@@ -304,11 +368,10 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
         W.bparam[i] = make_float4(d.linear_damping, d.angular_damping, d.gravity_scale, 0.0f);
         if (d.type == kDynamic) W.dynamicCount++;
     }
-    W.fbody.resize(nf);
-    W.fshape.resize(nf);
-    W.fmat.resize(nf);
-    W.ffat.resize(nf);
-    W.ffilter.resize(nf);
+    // One proxy per fixture child (b2Fixture.CreateProxies, b2fixture.d:414-431), in
+    // creation order: a chain contributes one edge proxy per segment, every other
+    // shape one; the device's "fixture" arrays are these proxies.
+    std::vector<int> firstProxy(nf);
     for (int i = 0; i < nf; ++i) {
         const dbx_fixture_def &d = fx[i];
         if (d.body < 0 || d.body >= nb) { err = "fixture body out of range"; return false; }
@@ -317,15 +380,27 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
             err = "fixture definition is not finite";
             return false;
         }
-        Shape s;
-        if (!build_shape(d, s, err)) return false;
-        W.fbody[i] = d.body;
-        W.fshape[i] = s;
-        W.fmat[i] = make_float4(d.friction, d.restitution, d.density, s.radius);
-        W.ffilter[i] = make_int2((int)d.category | ((int)d.mask << 16), (int)d.group);
-        // b2Fixture.CreateProxies -> b2DynamicTree.CreateProxy: fatten by the extension.
-        float4 a = shape_aabb_host(s, xf[d.body]);
-        W.ffat[i] = make_float4(a.x - kAabbExtension, a.y - kAabbExtension, a.z + kAabbExtension, a.w + kAabbExtension);
+        std::vector<Shape> kids;
+        if (d.shape == 4) {
+            if (!chain_children(d, kids, err)) return false;
+        } else {
+            Shape s;
+            if (!build_shape(d, s, err)) return false;
+            kids.push_back(s);
+        }
+        firstProxy[i] = (int)W.fshape.size();
+        for (const Shape &s : kids) {
+            W.fbody.push_back(d.body);
+            W.fshape.push_back(s);
+            W.fmat.push_back(make_float4(d.friction, d.restitution, d.density, s.radius));
+            W.ffilter.push_back(make_int2((int)d.category | ((int)d.mask << 16), (int)d.group));
+            // b2Fixture.CreateProxies -> b2DynamicTree.CreateProxy: fatten by the extension.
+            float4 a = shape_aabb_host(s, xf[d.body]);
+            W.ffat.push_back(
+                make_float4(a.x - kAabbExtension, a.y - kAabbExtension, a.z + kAabbExtension, a.w + kAabbExtension));
+            W.pfix.push_back(i);
+            W.pchild.push_back((int)(&s - &kids[0]));
+        }
         fixturesOf[d.body].insert(fixturesOf[d.body].begin(), i); // m_fixtureList: newest first
         // b2Body.ResetMassData (b2body.d:555-625) after each fixture with density
         if (d.density > 0.0f) {
@@ -340,7 +415,8 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
             HV center = hv(0.0f, 0.0f);
             for (int fi : fixturesOf[b]) {
                 if (fx[fi].density == 0.0f) continue;
-                HMass md = shape_mass(W.fshape[fi], fx[fi].density);
+                // a chain has no mass (b2chainshape.d:228-234), as its first child edge
+                HMass md = shape_mass(W.fshape[firstProxy[fi]], fx[fi].density);
                 mass[b] += md.mass;
                 center = center + md.mass * md.center;
                 I[b] += md.I;
@@ -428,6 +504,7 @@ struct dbx_handle {
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
     std::vector<Tier> tiers;      // tier 0 = the largest worlds, on `stream`
     std::vector<WideBp> wide;     // worlds whose broadphase scratch does not fit in LDS, by tier
+    std::vector<int> proxyFix, proxyChild; // per proxy (from each world's fixBase): fixture and child
     // DBX_STEP_PRESOLVE: a step stopped after Collide, finished by dbx_step_finish
     bool splitWaiting = false;    // the first half has run, the second has not
     bool lastSplit = false;       // the last (or pending) step was split: PreSolve events are kept
@@ -715,6 +792,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
         bmass.insert(bmass.end(), H.bmass.begin(), H.bmass.end());
         bparam.insert(bparam.end(), H.bparam.begin(), H.bparam.end());
         fbody.insert(fbody.end(), H.fbody.begin(), H.fbody.end());
+        h->proxyFix.insert(h->proxyFix.end(), H.pfix.begin(), H.pfix.end());
+        h->proxyChild.insert(h->proxyChild.end(), H.pchild.begin(), H.pchild.end());
         fshape.insert(fshape.end(), H.fshape.begin(), H.fshape.end());
         fmat.insert(fmat.end(), H.fmat.begin(), H.fmat.end());
         ffat.insert(ffat.end(), H.ffat.begin(), H.ffat.end());
@@ -1435,6 +1514,21 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
     if (!out) return DBX_OK;
     if (cap < (int32_t)ev.size()) return fail(h, DBX_E_CAPACITY, "event buffer too small");
     std::copy(ev.begin(), ev.end(), out);
+    return DBX_OK;
+}
+
+dbx_status dbx_read_proxies(dbx_handle *h, int32_t world, int32_t *fixture, int32_t *child, int32_t cap,
+                            int32_t *count) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    if (!count) return fail(h, DBX_E_INVALID_ARG, "count is required");
+    const WorldDesc &D = h->W[world];
+    *count = D.fixCount;
+    if (cap < D.fixCount || !fixture || !child) return DBX_OK;
+    for (int i = 0; i < D.fixCount; ++i) {
+        fixture[i] = h->proxyFix[D.fixBase + i];
+        child[i] = h->proxyChild[D.fixBase + i];
+    }
     return DBX_OK;
 }
 

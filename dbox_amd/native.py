@@ -12,6 +12,8 @@ import os
 
 import numpy as np
 
+from . import scenes
+
 from .scenes import BODY_DTYPE, FIXTURE_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -70,7 +72,7 @@ EXPORTS = [
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
-    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events", "dbx_write_contact_settings", "dbx_step_finish",
+    "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events", "dbx_write_contact_settings", "dbx_step_finish", "dbx_read_proxies",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
 ]
@@ -122,6 +124,7 @@ def lib():
         L.dbx_read_toi_events.argtypes = [vp, vp, i32]
         L.dbx_write_contact_settings.argtypes = [vp, i32, vp, i32]
         L.dbx_step_finish.argtypes = [vp]
+        L.dbx_read_proxies.argtypes = [vp, i32, vp, vp, i32, C.POINTER(i32)]
         L.dbx_mlp_init.argtypes = [vp, u64]
         L.dbx_mlp_set.argtypes = [vp, vp, vp]
         L.dbx_mlp_get.argtypes = [vp, vp, vp]
@@ -180,7 +183,7 @@ class StepWorlds:
         self._check(self.L.dbx_add_world(self.h, _p(b), len(b), _p(f), len(f), float(scene.gravity[0]),
                                          float(scene.gravity[1]), C.byref(wid)))
         self.body_counts.append(len(b))
-        self.fixture_counts.append(len(f))
+        self.fixture_counts.append(scenes.proxy_count(f))  # read-backs are per proxy
         self.dynamic_counts.append(int(np.count_nonzero(b["type"] == 2)))
         return wid.value
 
@@ -336,6 +339,15 @@ class StepWorlds:
     def set_tier_blocks(self, blocks):
         b = np.ascontiguousarray(blocks, np.int32)
         self._check(self.L.dbx_set_tier_blocks(self.h, _p(b), len(b)))
+
+    def proxies(self, world=0):
+        """(fixture, child) of each proxy index the read-backs use."""
+        n = C.c_int32()
+        self._check(self.L.dbx_read_proxies(self.h, world, None, None, 0, C.byref(n)))
+        f = np.zeros(n.value, np.int32)
+        c = np.zeros(n.value, np.int32)
+        self._check(self.L.dbx_read_proxies(self.h, world, _p(f), _p(c), n.value, C.byref(n)))
+        return f, c
 
     def step_finish(self):
         """The rest of a step started with the PRESOLVE flag."""

@@ -33,7 +33,18 @@ FIXTURE_DTYPE = np.dtype([
 assert BODY_DTYPE.itemsize == 56 and FIXTURE_DTYPE.itemsize == 120
 
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
-SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX = 0, 1, 2, 3
+SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX, SHAPE_CHAIN = 0, 1, 2, 3, 4
+CHAIN_PREV, CHAIN_NEXT, CHAIN_LOOP = 2, 4, 16  # chain fixture flags
+
+
+def proxy_count(fixtures: np.ndarray) -> int:
+    """Broadphase proxies of a fixture list: one per fixture, one per chain segment
+    (b2ChainShape.GetChildCount, b2chainshape.d:129-132)."""
+    f = np.asarray(fixtures)
+    chain = f["shape"] == SHAPE_CHAIN
+    loop = (f["flags"] & CHAIN_LOOP) != 0
+    kids = np.where(chain, np.where(loop, f["count"], f["count"] - 1), 1)
+    return int(kids.sum())
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -361,3 +372,53 @@ def projectile_room(n: int = 40, seed: int | None = None, size: float = 20.0, sp
     pts[kind == 0] = 0.0
     g["p"] = pts
     return Scene(b, f, name=f"projectiles{n}", meta={"size": size})
+
+
+def chain_terrain(n: int = 60, seed: int | None = None, pieces: int = 4) -> Scene:
+    """Bodies of the container workloads' shapes dropped onto a bumpy ground made of
+    `pieces` chain fixtures (b2ChainShape, b2chainshape.d:24-263) of 8 vertices,
+    each joined to its neighbours by ghost vertices (SetPrevVertex / SetNextVertex),
+    inside a closed loop chain (CreateLoop) that bounds the world."""
+    rng = Stream(config_seed(7) if seed is None else seed)
+    seg = 7 * pieces
+    xs = np.linspace(-20.0, 20.0, seg + 1)
+    ys = rng.uniform(seg + 1, 0.0, 1.2)
+    b = body_defs(n + 1)
+    b[0]["type"] = STATIC
+    f = fixture_defs(pieces + 1 + n)
+    for k in range(pieces):
+        c = f[k]
+        c["body"] = 0
+        c["shape"] = SHAPE_CHAIN
+        c["count"] = 8
+        c["friction"] = 0.6
+        i0 = 7 * k
+        pts = np.stack([xs[i0:i0 + 8], ys[i0:i0 + 8]], 1).astype(np.float32)
+        p = np.zeros(20, np.float32)
+        p[:16] = pts.ravel()
+        flags = 0
+        if k > 0:
+            p[16:18] = (xs[i0 - 1], ys[i0 - 1])
+            flags |= CHAIN_PREV
+        if k < pieces - 1:
+            p[18:20] = (xs[i0 + 8], ys[i0 + 8])
+            flags |= CHAIN_NEXT
+        c["p"] = p
+        c["flags"] = flags
+    box = f[pieces]
+    box["body"] = 0
+    box["shape"] = SHAPE_CHAIN
+    box["count"] = 4
+    box["flags"] = CHAIN_LOOP
+    box["friction"] = 0.6
+    p = np.zeros(20, np.float32)
+    p[:8] = (-22.0, -2.0, 22.0, -2.0, 22.0, 40.0, -22.0, 40.0)
+    box["p"] = p
+    # the dynamic bodies: the mixed container's shape and material draws, above the ground
+    mix = mixed_container(n, 36.0, 30.0, rng.next_u64(1)[0].item())
+    b[1:] = mix.bodies[1:]
+    b[1:]["py"] += 3.0
+    d = f[pieces + 1:]
+    d[:] = mix.fixtures[3:]
+    d["body"] = np.arange(1, n + 1)
+    return Scene(b, f, name=f"chains{n}")

@@ -71,7 +71,18 @@ typedef struct {
  *   shape 2 polygon: count points in p[0 .. 2*count-1]          (b2PolygonShape.Set, hulled)
  *   shape 3 box    : p[0..1] = hx, hy; flags bit3 = centre p[2..3], angle p[4]
  *                                                               (b2PolygonShape.SetAsBox)
- * flags bit0 = sensor (unsupported). */
+ *   shape 4 chain  : count (2..8) points in p[0 .. 2*count-1]; flags bit4 = loop
+ *                    (b2ChainShape.CreateLoop, count >= 3), else CreateChain with
+ *                    flags bit1 / bit2 = previous / next ghost vertex in p[16..17] /
+ *                    p[18..19] (SetPrevVertex / SetNextVertex).  A longer chain is
+ *                    several chain fixtures joined by ghost vertices.
+ * flags bit0 = sensor (unsupported).
+ *
+ * Proxy indices.  Every fixture child has a broadphase proxy (b2Fixture.m_proxies):
+ * one per fixture, one per segment of a chain.  The read-backs below name the
+ * fixtures of pairs, contacts and events by proxy index, counting the children of
+ * all fixtures in creation order; without chains that is the fixture index.
+ * dbx_read_proxies maps proxy indices to (fixture, child). */
 typedef struct {
     int32_t body;
     int32_t shape;
@@ -180,6 +191,10 @@ dbx_status dbx_read_bodies(dbx_handle *h, int32_t world, dbx_body_state *out, in
  * fat AABB no longer contains the new pose are re-fattened and moved, as
  * b2Fixture.Synchronize(xf, xf) does after SetTransform. */
 dbx_status dbx_write_bodies(dbx_handle *h, int32_t world, const dbx_body_state *in, const float *force3, int32_t n);
+/* For each proxy of one world (see "Proxy indices" above): its fixture's creation
+ * index and its child index.  *count = proxies; nothing is written when cap is smaller. */
+dbx_status dbx_read_proxies(dbx_handle *h, int32_t world, int32_t *fixture, int32_t *child, int32_t cap,
+                            int32_t *count);
 /* Contacts of one world in b2World contact-list order. */
 dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int32_t cap, int32_t *count);
 /* The contact-listener callbacks dbox would have made during the last dbx_step of

@@ -268,7 +268,7 @@ static int clipSegmentToLine(ClipVertex vOut[2], const ClipVertex vIn[2], V2 nor
 }
 
 /* ---- shapes (src/dbox/collision/shapes) ---- */
-enum { s_circle = 0, s_edge = 1, s_polygon = 2 };
+enum { s_circle = 0, s_edge = 1, s_polygon = 2, s_chain = 3 };
 struct Shape {
     int type = s_circle;
     float radius = 0;
@@ -279,7 +279,34 @@ struct Shape {
     int count = 0;
     V2 v0, v1, v2, v3;           /* edge */
     bool has0 = false, has3 = false;
+    V2 cv[kMaxPolygonVertices + 1]; /* chain: m_vertices (a loop repeats its first vertex) */
+    int ccount = 0;              /* chain: m_count */
+    V2 cprev, cnext;             /* chain: m_prevVertex, m_nextVertex */
+    bool chasPrev = false, chasNext = false;
 };
+/* b2ChainShape.GetChildCount / GetChildEdge (b2chainshape.d:129-170) */
+static int childCount(const Shape &s) { return s.type == s_chain ? s.ccount - 1 : 1; }
+static void childEdge(const Shape &chain, int index, Shape &edge) {
+    edge = Shape();
+    edge.type = s_edge;
+    edge.radius = chain.radius;
+    edge.v1 = chain.cv[index + 0];
+    edge.v2 = chain.cv[index + 1];
+    if (index > 0) {
+        edge.v0 = chain.cv[index - 1];
+        edge.has0 = true;
+    } else {
+        edge.v0 = chain.cprev;
+        edge.has0 = chain.chasPrev;
+    }
+    if (index < chain.ccount - 2) {
+        edge.v3 = chain.cv[index + 2];
+        edge.has3 = true;
+    } else {
+        edge.v3 = chain.cnext;
+        edge.has3 = chain.chasNext;
+    }
+}
 struct MassData {
     float mass = 0;
     V2 center;
@@ -426,10 +453,20 @@ static void circleMass(const Shape &sh, MassData &md, float density) {
 static void computeMass(const Shape &sh, MassData &md, float density) {
     if (sh.type == s_polygon) polyMass(sh, md, density);
     else if (sh.type == s_circle) circleMass(sh, md, density);
+    else if (sh.type == s_chain) { md.mass = 0.0f; md.center = mk(0.0f, 0.0f); md.I = 0.0f; } /* b2chainshape.d:228-234 */
     else { md.mass = 0.0f; md.center = 0.5f * (sh.v1 + sh.v2); md.I = 0.0f; } /* b2edgeshape.d:179-186 */
 }
 /* ComputeAABB: polygon b2polygonshape.d:356-373, circle b2circleshape.d:110-117, edge b2edgeshape.d:163-176 */
-static void computeAABB(const Shape &sh, AABB &aabb, const Xf &xf) {
+static void computeAABB(const Shape &sh, AABB &aabb, const Xf &xf, int childIndex = 0) {
+    if (sh.type == s_chain) { /* b2chainshape.d:214-226: no radius */
+        int i1 = childIndex, i2 = childIndex + 1;
+        if (i2 == sh.ccount) i2 = 0;
+        V2 v1 = mul(xf, sh.cv[i1]);
+        V2 v2 = mul(xf, sh.cv[i2]);
+        aabb.lo = vmin(v1, v2);
+        aabb.hi = vmax(v1, v2);
+        return;
+    }
     if (sh.type == s_polygon) {
         V2 lower = mul(xf, sh.v[0]);
         V2 upper = lower;
@@ -919,17 +956,30 @@ static void collideEdgeAndPolygon(Manifold &m, const Shape &edgeA, const Xf &xfA
 
 /* Contact type registry b2contact.d:430-436: primary orders are
  * circle-circle, polygon-circle, polygon-polygon, edge-circle, edge-polygon. */
+/* b2Contact.InitializeRegisters (b2contact.d:425-435): edge and chain pair only
+ * with circles and polygons, and are then shape A */
 static bool hasContactType(int ta, int tb) {
-    return !(ta == s_edge && tb == s_edge);
+    const bool la = ta == s_edge || ta == s_chain, lb = tb == s_edge || tb == s_chain;
+    return !(la && lb);
 }
 static bool isPrimary(int ta, int tb) {
     if (ta == tb) return true;
     if (ta == s_polygon && tb == s_circle) return true;
-    if (ta == s_edge) return true;
+    if (ta == s_edge || ta == s_chain) return true;
     return false;
 }
-/* The Evaluate override of each contact class (b2polygoncontact.d:56 etc.) */
-static void evaluate(Manifold &m, const Shape &a, const Xf &xfA, const Shape &b, const Xf &xfB) {
+/* The Evaluate override of each contact class (b2polygoncontact.d:56 etc.); a
+ * chain is evaluated as its child edge indexA (b2chainandpolygoncontact.d,
+ * b2chainandcirclecontact.d: GetChildEdge then the edge collide). */
+static void evaluate(Manifold &m, const Shape &a, const Xf &xfA, const Shape &b, const Xf &xfB, int indexA = 0) {
+    if (a.type == s_chain) {
+        Shape edge;
+        childEdge(a, indexA, edge);
+        if (b.type == s_circle) collideEdgeAndCircle(m, edge, xfA, b, xfB);
+        else if (b.type == s_polygon) collideEdgeAndPolygon(m, edge, xfA, b, xfB);
+        else m.pointCount = 0;
+        return;
+    }
     if (a.type == s_polygon && b.type == s_polygon) collidePolygons(m, a, xfA, b, xfB);
     else if (a.type == s_circle && b.type == s_circle) collideCircles(m, a, xfA, b, xfB);
     else if (a.type == s_polygon && b.type == s_circle) collidePolygonAndCircle(m, a, xfA, b, xfB);
@@ -1298,10 +1348,15 @@ struct DistanceProxy {
     DistanceProxy() = default;
     DistanceProxy(const DistanceProxy &) = delete; /* vertices may point into buf */
     /* b2DistanceProxy.Set b2distance.d:34-91 (no chain shapes) */
-    void set(const Shape &s) {
+    void set(const Shape &s, int index = 0) {
         if (s.type == s_circle) {
             vertices = &s.p;
             count = 1;
+        } else if (s.type == s_chain) { /* b2distance.d:56-74 */
+            buf[0] = s.cv[index];
+            buf[1] = index + 1 < s.ccount ? s.cv[index + 1] : s.cv[0];
+            vertices = buf;
+            count = 2;
         } else if (s.type == s_polygon) {
             vertices = s.v;
             count = s.count;
@@ -1774,8 +1829,9 @@ struct Fixture {
     uint16_t category = 1, mask = 0xFFFF;
     int16_t group = 0;
     bool sensor = false;
-    int proxyId = kNull;
-    AABB proxyAabb;
+    std::vector<int> proxyIds;  /* m_proxies: one tree proxy per child */
+    std::vector<AABB> proxyAabbs;
+    int proxyBase = 0;          /* canonical index of child 0: children counted in creation order */
 };
 struct Body {
     int type = b_static;
@@ -1829,6 +1885,7 @@ struct Contact {
     int prev = kNull, next = kNull;
     ContactEdge node[2];
     int fixtureA = 0, fixtureB = 0;
+    int indexA = 0, indexB = 0; /* child indices (chains) */
     Manifold manifold;
     float friction = 0, restitution = 0, tangentSpeed = 0;
     bool alive = false;
@@ -1897,7 +1954,8 @@ struct World {
     float profile[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<Pair> lastPairs; /* fixture-index pairs handed to AddPair */
     std::vector<int> solveLog;   /* test hook: per solved contact (island, bodyA, bodyB, dynamic mask) */
-    std::vector<int> proxyToFixture;
+    std::vector<int> canonFix, canonChild; /* per canonical proxy index: fixture, child */
+    int proxyOf(int fixture, int child) const { return fixtures[fixture].proxyBase + child; }
 
     /* island scratch */
     std::vector<int> islandBodies, islandContacts, stack;
@@ -2004,10 +2062,18 @@ struct World {
         f.sensor = (d.flags & 1) != 0;
         buildShape(d, f.shape);
         Body &b = bodies[d.body];
-        computeAABB(f.shape, f.proxyAabb, b.xf);
-        f.proxyId = bp.createProxy(f.proxyAabb, f.index);
-        if ((int)proxyToFixture.size() <= f.proxyId) proxyToFixture.resize(f.proxyId + 1, -1);
-        proxyToFixture[f.proxyId] = f.index;
+        /* b2Fixture.CreateProxies (b2fixture.d:414-431); a proxy's user data is its
+         * canonical index, which orders like the tree's ids (allocated in the same order) */
+        f.proxyBase = (int)canonFix.size();
+        const int kids = childCount(f.shape);
+        for (int i = 0; i < kids; ++i) {
+            AABB a;
+            computeAABB(f.shape, a, b.xf, i);
+            f.proxyAabbs.push_back(a);
+            f.proxyIds.push_back(bp.createProxy(a, f.proxyBase + i));
+            canonFix.push_back(f.index);
+            canonChild.push_back(i);
+        }
         fixtures.push_back(f);
         b.fixtures.insert(b.fixtures.begin(), f.index);
         if (f.density > 0.0f) resetMassData(b);
@@ -2035,6 +2101,22 @@ struct World {
             int n = d.count < kMaxPolygonVertices ? d.count : kMaxPolygonVertices;
             for (int i = 0; i < n; ++i) pts[i] = mk(d.p[2 * i], d.p[2 * i + 1]);
             polySet(s, pts, n);
+        } else if (d.shape == 4) { /* b2ChainShape.CreateLoop / CreateChain (b2chainshape.d:46-100) */
+            s.type = s_chain;
+            s.radius = kPolygonRadius;
+            const int n = d.count;
+            for (int i = 0; i < n; ++i) s.cv[i] = mk(d.p[2 * i], d.p[2 * i + 1]);
+            if (d.flags & 16) {
+                s.ccount = n + 1;
+                s.cv[n] = s.cv[0];
+                s.cprev = s.cv[s.ccount - 2];
+                s.cnext = s.cv[1];
+                s.chasPrev = s.chasNext = true;
+            } else {
+                s.ccount = n;
+                if (d.flags & 2) { s.cprev = mk(d.p[16], d.p[17]); s.chasPrev = true; } /* SetPrevVertex */
+                if (d.flags & 4) { s.cnext = mk(d.p[18], d.p[19]); s.chasNext = true; } /* SetNextVertex */
+            }
         } else {
             if (d.flags & 8) polySetAsBoxOriented(s, d.p[0], d.p[1], mk(d.p[2], d.p[3]), d.p[4]);
             else polySetAsBox(s, d.p[0], d.p[1]);
@@ -2053,7 +2135,10 @@ struct World {
     }
 
     /* b2ContactManager.AddPair b2contactmanager.d:52-176 */
-    void addPair(int fixA, int fixB) {
+    /* b2ContactManager.AddPair (b2contactmanager.d:52-176) for two canonical proxies */
+    void addPair(int proxyA, int proxyB) {
+        int fixA = canonFix[proxyA], fixB = canonFix[proxyB];
+        int indexA = canonChild[proxyA], indexB = canonChild[proxyB];
         Fixture *fixtureA = &fixtures[fixA];
         Fixture *fixtureB = &fixtures[fixB];
         int bodyA = fixtureA->body, bodyB = fixtureB->body;
@@ -2061,15 +2146,19 @@ struct World {
         for (int e = bodies[bodyB].contactList; e != kNull; e = edge(e).next) {
             if (edge(e).other == bodyA) {
                 const Contact &c = contacts[edgeContact(e)];
-                if (c.fixtureA == fixA && c.fixtureB == fixB) return;
-                if (c.fixtureA == fixB && c.fixtureB == fixA) return;
+                if (c.fixtureA == fixA && c.fixtureB == fixB && c.indexA == indexA && c.indexB == indexB) return;
+                if (c.fixtureA == fixB && c.fixtureB == fixA && c.indexA == indexB && c.indexB == indexA) return;
             }
         }
         if (!bodyShouldCollide(bodies[bodyB], bodies[bodyA])) return;
         if (!filterShouldCollide(*fixtureA, *fixtureB)) return;
         int ta = fixtureA->shape.type, tb = fixtureB->shape.type;
         if (!hasContactType(ta, tb)) return;
-        if (!isPrimary(ta, tb)) { std::swap(fixA, fixB); std::swap(fixtureA, fixtureB); }
+        if (!isPrimary(ta, tb)) {
+            std::swap(fixA, fixB);
+            std::swap(fixtureA, fixtureB);
+            std::swap(indexA, indexB);
+        }
         int ci;
         if (!freeContacts.empty()) { ci = freeContacts.back(); freeContacts.pop_back(); }
         else { ci = (int)contacts.size(); contacts.emplace_back(); }
@@ -2079,6 +2168,8 @@ struct World {
         c.flags = cf_enabled;
         c.fixtureA = fixA;
         c.fixtureB = fixB;
+        c.indexA = indexA;
+        c.indexB = indexB;
         c.manifold.pointCount = 0;
         c.friction = sqrtf(fixtureA->friction * fixtureB->friction);   /* b2MixFriction */
         c.restitution = fixtureA->restitution > fixtureB->restitution ? fixtureA->restitution : fixtureB->restitution;
@@ -2146,7 +2237,7 @@ struct World {
         bool wasTouching = (c.flags & cf_touching) == cf_touching;
         Body &bodyA = bodies[fixtures[c.fixtureA].body];
         Body &bodyB = bodies[fixtures[c.fixtureB].body];
-        evaluate(c.manifold, fixtures[c.fixtureA].shape, bodyA.xf, fixtures[c.fixtureB].shape, bodyB.xf);
+        evaluate(c.manifold, fixtures[c.fixtureA].shape, bodyA.xf, fixtures[c.fixtureB].shape, bodyB.xf, c.indexA);
         bool touching = c.manifold.pointCount > 0;
         for (int i = 0; i < c.manifold.pointCount; ++i) {
             ManifoldPoint &mp2 = c.manifold.points[i];
@@ -2172,8 +2263,8 @@ struct World {
         if (touching && presolveMode) {
             orc_event e = {};
             e.type = 4;
-            e.fixture_a = c.fixtureA;
-            e.fixture_b = c.fixtureB;
+            e.fixture_a = proxyOf(c.fixtureA, c.indexA);
+            e.fixture_b = proxyOf(c.fixtureB, c.indexB);
             e.point_count = oldManifold.pointCount;
             for (int j = 0; j < oldManifold.pointCount; ++j) {
                 e.normal_impulse[j] = oldManifold.points[j].normalImpulse;
@@ -2204,7 +2295,8 @@ struct World {
             bool activeA = bodyA.awake() && bodyA.type != b_static;
             bool activeB = bodyB.awake() && bodyB.type != b_static;
             if (!activeA && !activeB) { ci = c.next; continue; }
-            bool overlap = bp.testOverlapProxies(fixtures[c.fixtureA].proxyId, fixtures[c.fixtureB].proxyId);
+            bool overlap = bp.testOverlapProxies(fixtures[c.fixtureA].proxyIds[c.indexA],
+                                                 fixtures[c.fixtureB].proxyIds[c.indexB]);
             if (!overlap) {
                 int nuke = ci;
                 ci = c.next;
@@ -2222,12 +2314,14 @@ struct World {
         xf1.p = b.sweep.c0 - mul(xf1.q, b.sweep.localCenter);
         for (int fi : b.fixtures) {
             Fixture &f = fixtures[fi];
-            AABB aabb1, aabb2;
-            computeAABB(f.shape, aabb1, xf1);
-            computeAABB(f.shape, aabb2, b.xf);
-            f.proxyAabb.combine(aabb1, aabb2);
-            V2 displacement = b.xf.p - xf1.p;
-            bp.moveProxy(f.proxyId, f.proxyAabb, displacement);
+            for (size_t i = 0; i < f.proxyIds.size(); ++i) {
+                AABB aabb1, aabb2;
+                computeAABB(f.shape, aabb1, xf1, (int)i);
+                computeAABB(f.shape, aabb2, b.xf, (int)i);
+                f.proxyAabbs[i].combine(aabb1, aabb2);
+                V2 displacement = b.xf.p - xf1.p;
+                bp.moveProxy(f.proxyIds[i], f.proxyAabbs[i], displacement);
+            }
         }
     }
 
@@ -2484,8 +2578,8 @@ struct World {
     void recordEvent(int type, const Contact &c, const VelocityConstraint *vc) {
         orc_event e = {};
         e.type = type;
-        e.fixture_a = c.fixtureA;
-        e.fixture_b = c.fixtureB;
+        e.fixture_a = proxyOf(c.fixtureA, c.indexA);
+        e.fixture_b = proxyOf(c.fixtureB, c.indexB);
         if (vc) {
             e.point_count = vc->pointCount;
             for (int j = 0; j < vc->pointCount; ++j) {
@@ -2932,8 +3026,8 @@ struct World {
                         bB.sweep.advance(alpha0);
                     }
                     DistanceProxy pA, pB;
-                    pA.set(fA.shape);
-                    pB.set(fB.shape);
+                    pA.set(fA.shape, c.indexA);
+                    pB.set(fB.shape, c.indexB);
                     float beta = 0.0f;
                     int state = timeOfImpact(beta, pA, pB, bA.sweep, bB.sweep, 1.0f);
                     if (state == toi_touching) alpha = fmin_(alpha0 + (1.0f - alpha0) * beta, 1.0f);
@@ -3166,8 +3260,8 @@ int orc_get_contacts(void *w, orc_contact *out, int cap) {
     for (int ci = W->contactList; ci != kNull && n < cap; ci = W->contacts[ci].next, ++n) {
         const Contact &c = W->contacts[ci];
         orc_contact &o = out[n];
-        o.fixture_a = c.fixtureA;
-        o.fixture_b = c.fixtureB;
+        o.fixture_a = W->proxyOf(c.fixtureA, c.indexA);
+        o.fixture_b = W->proxyOf(c.fixtureB, c.indexB);
         o.flags = (int)c.flags;
         o.type = c.manifold.type;
         o.point_count = c.manifold.pointCount;
@@ -3216,10 +3310,11 @@ float orc_max_penetration(void *w) {
     }
     return -minSep;
 }
+int orc_proxy_count(void *w) { return (int)((World *)w)->canonFix.size(); }
 void orc_get_fat_aabbs(void *w, float *out) {
     World *W = (World *)w;
-    for (size_t i = 0; i < W->fixtures.size(); ++i) {
-        const AABB &a = W->bp.tree.nodes[W->fixtures[i].proxyId].aabb;
+    for (size_t i = 0; i < W->canonFix.size(); ++i) {
+        const AABB &a = W->bp.tree.nodes[W->fixtures[W->canonFix[i]].proxyIds[W->canonChild[i]]].aabb;
         out[4 * i + 0] = a.lo.x;
         out[4 * i + 1] = a.lo.y;
         out[4 * i + 2] = a.hi.x;
@@ -3311,6 +3406,27 @@ int orc_toi_events(void *w) { return ((World *)w)->toiEvents; }
 int orc_world_set_presolve(void *w, int mode) {
     if (mode < 0 || mode > 2) return -1;
     ((World *)w)->presolveMode = mode;
+    return 0;
+}
+int orc_collide_child(const orc_fixture_def *chain, int child, const float *xfa3, const orc_fixture_def *b,
+                      const float *xfb3, orc_manifold *out) {
+    Shape sa, sb;
+    World::buildShape(*chain, sa);
+    World::buildShape(*b, sb);
+    if (sa.type != s_chain || child < 0 || child >= childCount(sa)) return -1;
+    if (sb.type != s_circle && sb.type != s_polygon) return -1;
+    Xf xfa, xfb;
+    xfa.p = mk(xfa3[0], xfa3[1]);
+    xfa.q.set(xfa3[2]);
+    xfb.p = mk(xfb3[0], xfb3[1]);
+    xfb.q.set(xfb3[2]);
+    memset(out, 0, sizeof(*out));
+    Manifold m;
+    evaluate(m, sa, xfa, sb, xfb, child);
+    WorldManifold wm;
+    wm.initialize(m, xfa, sa.radius, xfb, sb.radius);
+    manifoldOut(m, out);
+    if (m.pointCount > 0) worldManifoldOut(wm, out);
     return 0;
 }
 int orc_shape_geometry(const orc_fixture_def *def, float *o, float *mass_out4) {

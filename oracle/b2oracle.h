@@ -38,6 +38,9 @@ typedef struct {
  *   shape 2 polygon: count points in p[0..2*count-1], hulled by b2PolygonShape.Set
  *   shape 3 box    : p[0..1] = hx, hy (b2PolygonShape.SetAsBox);
  *                    flags bit3 = oriented box with centre p[2..3] and angle p[4]
+ *   shape 4 chain  : count (2..8) points in p[0..2*count-1]; flags bit4 = loop
+ *                    (b2ChainShape.CreateLoop), else CreateChain with flags bit1 / bit2 =
+ *                    previous / next ghost vertex in p[16..17] / p[18..19]
  */
 typedef struct {
     int32_t body;
@@ -118,8 +121,12 @@ int orc_get_contacts(void *w, orc_contact *out, int cap);
 int orc_events(void *w, orc_event *out, int cap);
 /* World manifold of contact-list entry i at the bodies' current transforms. */
 int orc_contact_world_manifold(void *w, int list_index, orc_manifold *out);
-/* Fat AABB of each fixture's proxy, 4 floats each (lower.x, lower.y, upper.x, upper.y). */
+/* Fat AABB of each proxy, 4 floats each (lower.x, lower.y, upper.x, upper.y).
+ * Proxies are the fixtures' children (one per fixture, one per chain segment)
+ * counted in creation order; contacts, events and pairs name fixtures by this
+ * proxy index, which is the fixture index when there are no chains. */
 void orc_get_fat_aabbs(void *w, float *out);
+int orc_proxy_count(void *w);
 /* The unique pairs that the most recent UpdatePairs handed to AddPair, as
  * (fixture index, fixture index) with the smaller proxy id first, in the
  * sorted order of b2PairLessThan. Returns the count (<= cap pairs written). */
@@ -139,6 +146,12 @@ float orc_max_penetration(void *w);
 int orc_collide(const orc_fixture_def *a, const float *xfa3, const orc_fixture_def *b,
                 const float *xfb3, orc_manifold *out);
 
+/* The chain-and-circle / chain-and-polygon collide of one chain segment
+ * (b2ChainShape.GetChildEdge then the edge collide, b2chainandpolygoncontact.d,
+ * b2chainandcirclecontact.d); the chain is shape A.  Returns -1 for a bad child
+ * index or a shape B that is not a circle or polygon. */
+int orc_collide_child(const orc_fixture_def *chain, int child, const float *xfa3, const orc_fixture_def *b,
+                      const float *xfb3, orc_manifold *out);
 /* Shape geometry exactly as the reference builds it (hull, normals, centroid),
  * for comparing the GPU library's own builder. out: count, radius, centroid(2),
  * vertices(16), normals(16) = 36 floats; mass_out: mass, cx, cy, I. */

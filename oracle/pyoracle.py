@@ -60,7 +60,8 @@ def lib():
         L.orc_world_set_solve_order.argtypes = [vp, i32]
         L.orc_solve_log.argtypes = [vp, vp, i32]
         L.orc_step.argtypes = [vp, f32, i32, i32]
-        for name in ("orc_body_count", "orc_fixture_count", "orc_contact_count", "orc_dynamic_body_count"):
+        for name in ("orc_body_count", "orc_fixture_count", "orc_contact_count", "orc_dynamic_body_count",
+                     "orc_proxy_count"):
             getattr(L, name).argtypes = [vp]
         L.orc_get_bodies.argtypes = [vp, vp]
         L.orc_get_contacts.argtypes = [vp, vp, i32]
@@ -72,6 +73,7 @@ def lib():
         L.orc_max_penetration.argtypes = [vp]
         L.orc_max_penetration.restype = f32
         L.orc_collide.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_collide_child.argtypes = [vp, i32, vp, vp, vp, vp]
         L.orc_shape_geometry.argtypes = [vp, vp, vp]
         L.orc_tree_pairs.argtypes = [vp, vp, i32, vp, i32]
         L.orc_shape_distance.argtypes = [vp, vp, vp, vp, i32, vp]
@@ -135,7 +137,7 @@ class OracleWorld:
         return out[0]
 
     def fat_aabbs(self) -> np.ndarray:
-        out = np.zeros((self.L.orc_fixture_count(self.h), 4), np.float32)
+        out = np.zeros((self.L.orc_proxy_count(self.h), 4), np.float32)  # one per proxy
         self.L.orc_get_fat_aabbs(self.h, _p(out))
         return out
 
@@ -182,6 +184,19 @@ def collide(fix_a, xfa, fix_b, xfb) -> np.ndarray | None:
     xb = np.asarray(xfb, np.float32)
     out = np.zeros(1, MANIFOLD_DTYPE)
     if L.orc_collide(_p(a), _p(xa), _p(b), _p(xb), _p(out)) != 0:
+        return None
+    return out[0]
+
+
+def collide_child(chain, child, xfa, fix_b, xfb):
+    """One chain segment collided with a circle or polygon (the chain is shape A)."""
+    L = lib()
+    a = np.ascontiguousarray(np.array(chain).reshape(1))
+    b = np.ascontiguousarray(np.array(fix_b).reshape(1))
+    xa = np.asarray(xfa, np.float32)
+    xb = np.asarray(xfb, np.float32)
+    out = np.zeros(1, MANIFOLD_DTYPE)
+    if L.orc_collide_child(_p(a), int(child), _p(xa), _p(b), _p(xb), _p(out)) != 0:
         return None
     return out[0]
 
