@@ -72,7 +72,19 @@ struct WorldState {
     int widePairs;    // pairs found by the device-wide broadphase pass (worlds too big for one workgroup)
     int toiEvents;    // TOI events (sub-step solves) of the last step's continuous pass
     int adjContacts;  // contacts the last Collide kept, which its adjacency lists cover
+    int toiCallbacks; // listener callbacks the last continuous pass made (may exceed its log's capacity)
 };
+
+// One listener callback of the continuous pass, laid out as dbx_contact_event:
+// BEGIN / END / PRE_SOLVE from b2Contact.Update, POST_SOLVE from the sub-step
+// island's b2Island.Report.
+struct ToiCallback {
+    int type, fa, fb, pointCount;
+    float n0, n1, t0, t1;
+};
+// Log record that is no callback: the discrete solve's PostSolve count and
+// impulses of contact slot fa, saved before the pass first updates it.
+constexpr int kLogSnapshot = 0x100;
 
 // A contact with its manifold, 80 bytes (b2Contact + b2Manifold, b2contact.d:437-465).
 struct alignas(16) Contact {
@@ -190,6 +202,8 @@ struct Dev {
     int4 *toiB;             // per body: (sweep alpha0 as float bits, island flag, head edge of that list, 0)
     const int *bfixOff, *bfixCnt; // per body: its fixtures' range in bfixList
     const int *bfixList;    // per world from fixBase: fixture indices grouped by body
+    ToiCallback *toiLog;    // per world from toiLogOff[w]: the continuous pass's callbacks in order
+    const int *toiLogOff;   // nworlds + 1 offsets into toiLog
 };
 
 // Order-preserving int image of a float (for atomicMin on floats).

@@ -2148,7 +2148,7 @@ constexpr float kToiBaumgarte = 0.75f;  // b2settings.d:138
 constexpr int kToiPairCap = 512;        // pairs one event's broadphase query may find
 constexpr int kToiListCap = 128;        // woken bodies / moved proxies per event, sweep snapshots per batch
 constexpr int kToiDirtyCap = 512;       // contacts to look at again after an event
-constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty = 1 << 18;
+constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty = 1 << 18, kTcSnap = 1 << 19;
 
 struct ToiWorld {
     Contact *cont;
@@ -2166,6 +2166,8 @@ struct ToiWorld {
     const int *adjOff, *adjCnt;
     const int2 *adj;
     const int *bfixOff, *bfixCnt, *bfixList;
+    ToiCallback *log; // the world's listener callbacks of this pass, logCap of them
+    int logCap;
 };
 // Lists kept in LDS by the lane that runs the events: the bodies an event woke,
 // the proxies it moved, and the contacts the next search must look at again
@@ -2176,7 +2178,24 @@ struct ToiWorld {
 struct ToiLists {
     int *woken, *moved, *dirty;
     int nwoken, nmoved, ndirty, overflow;
+    int nlog; // callbacks made so far (those beyond W.logCap are counted, not kept)
 };
+__device__ __forceinline__ void toi_log(const ToiWorld &W, ToiLists &L, int type, const Contact &c, int pointCount,
+                                       float4 imp) {
+    if (L.nlog < W.logCap) {
+        ToiCallback r;
+        r.type = type;
+        r.fa = c.fa;
+        r.fb = c.fb;
+        r.pointCount = pointCount;
+        r.n0 = imp.x;
+        r.t0 = imp.y;
+        r.n1 = pointCount > 1 ? imp.z : 0.0f;
+        r.t1 = pointCount > 1 ? imp.w : 0.0f;
+        W.log[L.nlog] = r;
+    }
+    ++L.nlog;
+}
 __device__ __forceinline__ void mark_dirty(const ToiWorld &W, ToiLists &L, int j, int clear) {
     const int y = W.tc[j].y;
     if (!(y & kTcDirty)) {
@@ -2228,12 +2247,29 @@ __device__ __forceinline__ void toi_wake(const ToiWorld &W, ToiLists &L, int b) 
     if (L.nwoken < kToiListCap) L.woken[L.nwoken++] = b;
     else L.overflow |= 1;
 }
-// b2Contact.Update with the wake on a change of touching.
-__device__ __forceinline__ void toi_update(const ToiWorld &W, ToiLists &L, Contact &c) {
+// b2Contact.Update with the wake on a change of touching and its listener
+// callbacks (b2contact.d:342-355): BEGIN / END on the change, then PRE_SOLVE
+// with the old manifold's count and impulses while the contact touches.  The
+// first update of a slot in the pass also logs what the discrete solve's
+// PostSolve reported for it (kLogSnapshot: slot, count, impulses), since the
+// update replaces the manifold that report is read from.
+__device__ __forceinline__ void toi_update(const ToiWorld &W, ToiLists &L, Contact &c, int slot) {
+    if (!(W.tc[slot].y & kTcSnap)) {
+        W.tc[slot].y |= kTcSnap;
+        Contact key = c;
+        key.fa = slot;
+        key.fb = 0;
+        toi_log(W, L, kLogSnapshot, key, (c.flags & kContactSolvedOne) ? 1 : man_count(c.meta),
+                make_float4(c.pt0.z, c.pt0.w, c.pt1.z, c.pt1.w));
+    }
+    const int oldCount = man_count(c.meta);
+    const float4 oldImp = make_float4(c.pt0.z, c.pt0.w, c.pt1.z, c.pt1.w);
     if (contact_update(c, W.fbody, W.shp, W.bxf)) {
         toi_wake(W, L, W.fbody[c.fa]);
         toi_wake(W, L, W.fbody[c.fb]);
+        toi_log(W, L, (c.flags & kContactTouching) ? 1 : 2, c, 0, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     }
+    if (c.flags & kContactTouching) toi_log(W, L, 4, c, oldCount, oldImp);
 }
 // Body b's contacts in its list order (newest first): f(contact, other body)
 // returns false to stop the walk.
@@ -2328,10 +2364,10 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
     body_advance(W, ia, minAlpha);
     body_advance(W, ib, minAlpha);
     Contact mc = cont[ci];
-    toi_update(W, L, mc);
+    toi_update(W, L, mc, ci);
     cont[ci] = mc;
     const int t0 = W.tc[ci].y;
-    W.tc[ci].y = (((t0 & kTcCount) + 1) & kTcCount) | (t0 & (kTcIsland | kTcDirty));
+    W.tc[ci].y = (((t0 & kTcCount) + 1) & kTcCount) | (t0 & (kTcIsland | kTcDirty | kTcSnap));
     if (!(mc.flags & kContactEnabled) || !(mc.flags & kContactTouching)) {
         cont[ci].flags = mc.flags & ~kContactEnabled;
         store_sweep(W, ia, backup1);
@@ -2364,7 +2400,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
             const Sweep backup = load_sweep(W, other);
             if (W.tb[other].y == 0) body_advance(W, other, minAlpha);
             Contact cu = cont[j];
-            toi_update(W, L, cu);
+            toi_update(W, L, cu, j);
             cont[j] = cu;
             if (!(cu.flags & kContactEnabled) || !(cu.flags & kContactTouching)) {
                 store_sweep(W, other, backup);
@@ -2433,6 +2469,9 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
             W.bvel[v.indexA] = va;
             W.bvel[v.indexB] = vb;
         }
+    // b2Island.Report of the sub-step island (b2island.d:415,438-461): the
+    // solver's impulses, which are not stored into the manifolds
+    for (int k = 0; k < ncon; ++k) toi_log(W, L, 3, cont[icon[k]], vcs[k].pointCount, imp[k]);
     // integrate positions (b2island.d:370-413)
     const float h = ss.dt;
     for (int i = 0; i < nbod; ++i) {
@@ -2590,6 +2629,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     W.bfixOff = d.bfixOff + wd.bodyBase;
     W.bfixCnt = d.bfixCnt + wd.bodyBase;
     W.bfixList = d.bfixList + wd.fixBase;
+    W.log = d.toiLog ? d.toiLog + d.toiLogOff[w] : nullptr;
+    W.logCap = d.toiLog ? d.toiLogOff[w + 1] - d.toiLogOff[w] : 0;
     const int nb = wd.bodyCount, nf = wd.fixCount;
     int n = d.S[w].contactCount;
     const int m = min(d.S[w].adjContacts, n);
@@ -2603,7 +2644,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         s_L.woken = s_woken;
         s_L.moved = s_movedList;
         s_L.dirty = s_dirty;
-        s_L.nwoken = s_L.nmoved = s_L.ndirty = s_L.overflow = 0;
+        s_L.nwoken = s_L.nmoved = s_L.ndirty = s_L.overflow = s_L.nlog = 0;
     }
     __syncthreads();
     if (lane == 0) // contacts made since the Collide, oldest first, so that each list is newest first
@@ -2827,6 +2868,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     }
     if (lane == 0) {
         d.S[w].toiEvents = events;
+        d.S[w].toiCallbacks = s_L.nlog;
         d.S[w].contactCount = n;
         if (error) d.S[w].error |= error;
     }

@@ -508,6 +508,8 @@ struct dbx_handle {
     // DBX_STEP_PRESOLVE: a step stopped after Collide, finished by dbx_step_finish
     bool splitWaiting = false;    // the first half has run, the second has not
     bool lastSplit = false;       // the last (or pending) step was split: PreSolve events are kept
+    bool lastCcd = false;         // the last step ran the continuous pass (its callbacks are in toiLog)
+    std::vector<int> toiLogOff;   // host copy of Dev::toiLogOff
     StepParams pendSp{};
     uint32_t pendFlags = 0;
     Contact *contPre = nullptr;   // the list Collide walked, as it was before (old manifolds)
@@ -905,6 +907,17 @@ dbx_status dbx_finalize(dbx_handle *h) {
         d.bfixCnt = p_cnt;
         d.bfixList = p_list;
     }
+    // the continuous pass's listener callbacks: 128 + 4 per body for each world
+    std::vector<int> logOff(nw + 1, 0);
+    {
+        for (int w = 0; w < nw; ++w) logOff[w + 1] = logOff[w] + 128 + 4 * h->W[w].bodyCount;
+        int *p_off;
+        DBX_HIP(h, dalloc(h, &p_off, nw + 1));
+        DBX_HIP(h, upload(h, p_off, logOff));
+        DBX_HIP(h, dalloc(h, &d.toiLog, logOff[nw]));
+        d.toiLogOff = p_off;
+        h->toiLogOff = logOff;
+    }
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
     if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
@@ -1117,6 +1130,7 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     h->d.logCount = h->logCount;
     h->d.logCap = h->logCap;
     const bool findNew = h->newFixture;
+    h->lastCcd = (flags & DBX_STEP_CCD) != 0;
     if (flags & DBX_STEP_PRESOLVE) {
         // b2World.Step up to the end of Collide, every tier in turn on the handle's
         // stream: FindNewContacts, a copy of the list Collide is about to walk (the
@@ -1460,6 +1474,21 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
         pre.resize(n0);
         DBX_HIP(h, hipMemcpy(pre.data(), h->contPre + D.contactBase, n0 * sizeof(Contact), hipMemcpyDeviceToHost));
     }
+    // the continuous pass's log: its callbacks, and the discrete PostSolve data of
+    // the contacts it updated (kLogSnapshot), which their slots no longer hold
+    std::vector<ToiCallback> log;
+    if (h->lastCcd && !h->splitWaiting) {
+        const int cap = h->toiLogOff[world + 1] - h->toiLogOff[world];
+        if (S.toiCallbacks > cap)
+            return fail(h, DBX_E_CAPACITY, "the continuous pass made more listener callbacks than its log holds");
+        log.resize(S.toiCallbacks);
+        if (!log.empty())
+            DBX_HIP(h, hipMemcpy(log.data(), h->d.toiLog + h->toiLogOff[world], log.size() * sizeof(ToiCallback),
+                                 hipMemcpyDeviceToHost));
+    }
+    std::unordered_map<int, const ToiCallback *> solvedBefore;
+    for (const ToiCallback &r : log)
+        if (r.type == kLogSnapshot) solvedBefore.emplace(r.fa, &r);
     std::vector<dbx_contact_event> ev;
     ev.reserve(nc + 16);
     for (int i = n0 - 1; i >= 0; --i) { // list order: newest first
@@ -1501,6 +1530,17 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
         e.type = DBX_EVENT_POST_SOLVE;
         e.fixture_a = c.fa;
         e.fixture_b = c.fb;
+        auto snap = solvedBefore.find(ord[k]);
+        if (snap != solvedBefore.end()) {
+            const ToiCallback &r = *snap->second;
+            e.point_count = r.pointCount;
+            e.normal_impulse[0] = r.n0;
+            e.tangent_impulse[0] = r.t0;
+            e.normal_impulse[1] = r.n1;
+            e.tangent_impulse[1] = r.t1;
+            ev.push_back(e);
+            continue;
+        }
         e.point_count = (c.flags & kContactSolvedOne) ? 1 : ((c.meta >> 4) & 0xf);
         e.normal_impulse[0] = c.pt0.z;
         e.tangent_impulse[0] = c.pt0.w;
@@ -1508,6 +1548,15 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
             e.normal_impulse[1] = c.pt1.z;
             e.tangent_impulse[1] = c.pt1.w;
         }
+        ev.push_back(e);
+    }
+    // the continuous pass's callbacks, which come after the discrete solve's
+    // (b2world.d:425-428); PRE_SOLVE only for a split step, as above
+    static_assert(sizeof(ToiCallback) == sizeof(dbx_contact_event), "callback records share one layout");
+    for (const ToiCallback &r : log) {
+        if (r.type == kLogSnapshot || (r.type == DBX_EVENT_PRE_SOLVE && !h->lastSplit)) continue;
+        dbx_contact_event e;
+        std::memcpy(&e, &r, sizeof e);
         ev.push_back(e);
     }
     *count = (int32_t)ev.size();

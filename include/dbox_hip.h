@@ -205,7 +205,15 @@ dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int
  * dbx_step call only the last step's callbacks are kept.  After a DBX_STEP_PRESOLVE
  * step each touching contact Collide updated also gives PRE_SOLVE right after its
  * BEGIN / END (b2contact.d:342-355); while that step waits for dbx_step_finish only
- * the Collide callbacks exist.  The continuous pass's callbacks are not recorded.
+ * the Collide callbacks exist.  With DBX_STEP_CCD the continuous pass's
+ * callbacks follow, in its order (b2world.d:1127-1452): BEGIN / END (and, after a
+ * split step, PRE_SOLVE) from each b2Contact.Update of a TOI event, then POST_SOLVE
+ * for every contact of the event's sub-step island with the sub-step solver's
+ * impulses (b2island.d:415).  PreSolve changes cannot reach the continuous pass:
+ * its PRE_SOLVE events are reports only.  Each world's pass logs up to 128 + 4 x
+ * its body count of records per step (its callbacks, and one per contact it
+ * updates, which keeps that contact's discrete POST_SOLVE data); DBX_E_CAPACITY
+ * when its last pass needed more.
  * *count = number of events; DBX_E_CAPACITY when cap is smaller (nothing else
  * written). */
 dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out, int32_t cap, int32_t *count);

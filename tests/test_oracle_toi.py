@@ -236,3 +236,24 @@ def test_solve_toi_stops_a_fast_box_on_a_thin_edge():
         on.step()
     y = float(on.bodies()["cy"][1])
     assert 0.25 <= y <= 0.25 + 2 * POLY_RADIUS + 0.01, y
+
+
+def test_solve_toi_callbacks_begin_then_post_solve_with_the_stopping_impulse():
+    # One TOI event: its b2Contact.Update begins the contact (b2contact.d:342-345),
+    # then b2Island.Report gives the sub-step solver's impulses (b2island.d:415).
+    # The box lands flat with no restitution, so the sub-step's normal impulses
+    # stop it: together m |v|, where |v| = 120 m/s after the translation clamp
+    # (b2island.d:176-183), split evenly over the two points.
+    o = po.OracleWorld(_fast_box_scene(), continuous=True)
+    o.step()
+    assert o.toi_events() == 1
+    ev = o.events()
+    assert list(ev["type"]) == [1, 3]
+    assert (int(ev["fixture_a"][0]), int(ev["fixture_b"][0])) == (0, 1)
+    post = ev[1]
+    assert post["point_count"] == 2
+    m = 0.25 * 1.0
+    total = float(post["normal_impulse"].sum())
+    assert abs(total - m * 120.0) < 1e-3 * m * 120.0, total
+    assert abs(float(post["normal_impulse"][0]) - float(post["normal_impulse"][1])) < 1e-3 * total
+    assert abs(float(o.bodies()["vy"][1])) < 1e-2
