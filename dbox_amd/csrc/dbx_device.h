@@ -44,6 +44,7 @@ enum : uint32_t {
     kContactNewPt0 = 0x1000,   // manifold point 0 has no match in the previous manifold
     kContactNewPt1 = 0x2000,
     kContactSolvedOne = 0x4000, // a two-point manifold solved as one point (block-solver downgrade)
+    kContactSensor = 0x8000,    // a fixture is a sensor: overlap only, never solved (b2contact.d:289-298)
 };
 
 struct WorldDesc {

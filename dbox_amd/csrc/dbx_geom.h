@@ -43,6 +43,8 @@ enum : int { kFeatVertex = 0, kFeatFace = 1 };
 //  circle : c = local centre
 //  edge   : v[0] = vertex1, v[1] = vertex2, v[2] = vertex0, v[3] = vertex3, flags bit0/bit1 = has v0/v3;
 //           flags bit2: a child edge of a chain (b2ChainShape.GetChildEdge), whose AABB has no radius
+//  any    : flags bit3 (kShapeSensor): the fixture is a sensor (b2Fixture.m_isSensor)
+constexpr int kShapeSensor = 8;
 struct alignas(16) Shape {
     int type;
     int count;

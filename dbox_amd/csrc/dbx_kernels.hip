@@ -90,8 +90,9 @@ __device__ __forceinline__ void wake_body(uint32_t *bfl, float4 *bpos, int b) {
 // ---------------------------------------------------------------------------
 // b2Contact.Update (b2contact.d:270-356) at the bodies' current transforms:
 // narrowphase, impulses carried over to points with a matching feature key,
-// touching and begin/end flags.  Returns whether touching changed (the
-// reference then wakes both bodies).
+// touching and begin/end flags.  Returns whether touching changed and the
+// reference therefore wakes both bodies; a sensor contact is touching while the
+// shapes overlap, has no manifold and wakes nothing (b2contact.d:289-298).
 __device__ __forceinline__ bool contact_update(Contact &c, const int *__restrict__ fbody, const Shape *__restrict__ shp,
                                                const float4 *__restrict__ bxf) {
     int ba = fbody[c.fa], bb = fbody[c.fb];
@@ -99,6 +100,16 @@ __device__ __forceinline__ bool contact_update(Contact &c, const int *__restrict
     c.flags |= kContactEnabled;
     bool wasTouching = (c.flags & kContactTouching) != 0;
     Xf xfA = load_xf(bxf, ba), xfB = load_xf(bxf, bb);
+    if (c.flags & kContactSensor) {
+        const bool touching = test_overlap(shp + c.fa, xfA, shp + c.fb, xfB);
+        uint32_t f = c.flags & ~(kContactTouching | kContactNewPt0 | kContactNewPt1);
+        if (touching) f |= kContactTouching;
+        if (!wasTouching && touching) f |= kContactBegan;
+        if (wasTouching && !touching) f |= kContactEnded;
+        c.flags = f;
+        c.meta &= 0xf; // pointCount = 0
+        return false;
+    }
     Manifold m;
     m.type = man_type(c.meta);
     m.count = count;
@@ -424,7 +435,7 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
     for (int e = lane; e < 2 * m; e += 64) {
         int2 ent = gadj[e];
         const uint32_t cf = cont[ent.x >> 1].flags;
-        if (!((cf & kContactTouching) && (cf & kContactEnabled))) ent.x = -1 - ent.x;
+        if (!((cf & kContactTouching) && (cf & kContactEnabled)) || (cf & kContactSensor)) ent.x = -1 - ent.x;
         M.adj[e] = ent;
     }
     __syncthreads();
@@ -2089,7 +2100,7 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
                 Contact c;
                 c.fa = fa;
                 c.fb = fb;
-                c.flags = kContactEnabled;
+                c.flags = kContactEnabled | ((shp[fa].flags | shp[fb].flags) & kShapeSensor ? kContactSensor : 0);
                 c.meta = 0;
                 c.friction = sqrtf(fmat[fa].x * fmat[fb].x);        // b2MixFriction
                 float ra = fmat[fa].y, rb = fmat[fb].y;
@@ -2101,8 +2112,10 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
                 c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 cont[slot] = c;
                 d.ctan[cur][wd.contactBase + slot] = 0.0f;
-                wake_body(bfl, bpos, fbody[fa]);
-                wake_body(bfl, bpos, fbody[fb]);
+                if (!(c.flags & kContactSensor)) { // b2contactmanager.d:169-173
+                    wake_body(bfl, bpos, fbody[fa]);
+                    wake_body(bfl, bpos, fbody[fb]);
+                }
             }
         }
         created += tot;
@@ -2304,6 +2317,7 @@ __device__ __forceinline__ int toi_status(const ToiWorld &W, const Contact &c, i
     if ((c.flags & kContactEnabled) == 0) return 0;
     if ((tcy & kTcCount) > kMaxSubSteps) return 0;
     if (tcy & kTcToi) return 1;
+    if (c.flags & kContactSensor) return 0;
     const uint32_t fa = W.bfl[W.fbody[c.fa]], fb = W.bfl[W.fbody[c.fb]];
     const int ta = body_type(fa), tb = body_type(fb);
     const bool activeA = (fa & kBodyAwake) && ta != kStatic;
@@ -2395,6 +2409,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
         body_contacts(W, body, [&](int j, int other) {
             if (nbod == 2 * kMaxToiContacts || ncon == kMaxToiContacts) return false;
             if (W.tc[j].y & kTcIsland) return true;
+            if (W.cont[j].flags & kContactSensor) return true;
             const uint32_t of = W.bfl[other];
             if (body_type(of) == kDynamic && !(bf & kBodyBullet) && !(of & kBodyBullet)) return true;
             const Sweep backup = load_sweep(W, other);
@@ -2571,7 +2586,7 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     Contact c;
     c.fa = fa;
     c.fb = fb;
-    c.flags = kContactEnabled;
+    c.flags = kContactEnabled | ((W.shp[fa].flags | W.shp[fb].flags) & kShapeSensor ? kContactSensor : 0);
     c.meta = 0;
     c.friction = sqrtf(W.fmat[fa].x * W.fmat[fb].x);
     const float ra = W.fmat[fa].y, rb = W.fmat[fb].y;
@@ -2586,8 +2601,10 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     W.tc[n] = make_int4(__float_as_int(1.0f), 0, -1, -1);
     mark_dirty(W, L, n, 0);
     link_contact(W, n);
-    toi_wake(W, L, ba);
-    toi_wake(W, L, bb);
+    if (!(c.flags & kContactSensor)) {
+        toi_wake(W, L, ba);
+        toi_wake(W, L, bb);
+    }
     return true;
 }
 

@@ -375,7 +375,6 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
     for (int i = 0; i < nf; ++i) {
         const dbx_fixture_def &d = fx[i];
         if (d.body < 0 || d.body >= nb) { err = "fixture body out of range"; return false; }
-        if (d.flags & 1) { err = "sensors are not supported"; return false; }
         if (!finite(d.density) || d.density < 0.0f || !finite(d.friction) || !finite(d.restitution)) {
             err = "fixture definition is not finite";
             return false;
@@ -388,6 +387,8 @@ bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, 
             if (!build_shape(d, s, err)) return false;
             kids.push_back(s);
         }
+        if (d.flags & 1) // b2FixtureDef.isSensor
+            for (Shape &s : kids) s.flags |= kShapeSensor;
         firstProxy[i] = (int)W.fshape.size();
         for (const Shape &s : kids) {
             W.fbody.push_back(d.body);
@@ -1505,7 +1506,7 @@ dbx_status dbx_read_events(dbx_handle *h, int32_t world, dbx_contact_event *out,
         }
         // PreSolve after the update of a contact that touches (b2contact.d:352-355),
         // with the manifold as it was before the update
-        if (preSolve && (c.flags & kContactTouching) && !(c.flags & (kContactDormant | kContactDestroyed))) {
+        if (preSolve && (c.flags & kContactTouching) && !(c.flags & (kContactDormant | kContactDestroyed | kContactSensor))) {
             const Contact &o = pre[i];
             dbx_contact_event p{};
             p.type = DBX_EVENT_PRE_SOLVE;

@@ -287,6 +287,18 @@ __device__ float gjk_distance(SimplexCache &cache, const DProxy &pA, const Xf &x
     return sqrtf(dd.x * dd.x + dd.y * dd.y);
 }
 
+// b2TestOverlap of two shapes (b2collision.d:449-468): b2Distance with radii from
+// an empty cache (b2distance.d:327-344); overlapping below 10 epsilon.
+__device__ __noinline__ bool test_overlap(const Shape *a, const Xf &xfA, const Shape *b, const Xf &xfB) {
+    SimplexCache cache;
+    cache.count = 0;
+    const DProxy pA = make_proxy(a), pB = make_proxy(b);
+    const float d = gjk_distance(cache, pA, xfA, pB, xfB);
+    const float r = pA.radius + pB.radius;
+    const float dist = (d > r && d > kEps) ? d - r : 0.0f;
+    return dist < 10.0f * kEps;
+}
+
 // b2SeparationFunction (b2timeofimpact.d:297-530)
 struct SepFn {
     Sweep sA, sB;

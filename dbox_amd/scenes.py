@@ -422,3 +422,25 @@ def chain_terrain(n: int = 60, seed: int | None = None, pieces: int = 4) -> Scen
     d[:] = mix.fixtures[3:]
     d["body"] = np.arange(1, n + 1)
     return Scene(b, f, name=f"chains{n}")
+
+
+SENSOR = 1  # fixture flags bit0: b2FixtureDef.isSensor
+
+
+def sensor_container(n: int = 150, seed: int = 61, every: int = 9) -> Scene:
+    """A mixed container (40 m x 80 m) with sensors: a static 16 m x 10 m sensor box
+    across the middle that the falling bodies pass through, and every `every`-th
+    body's fixture made a sensor (it falls through everything, reporting the
+    overlaps it passes)."""
+    s = mixed_container(n, 40.0, 80.0, seed)
+    b = np.concatenate([s.bodies, body_defs(1)])
+    b[-1]["type"] = STATIC
+    b[-1]["py"] = 30.0
+    f = np.concatenate([s.fixtures, fixture_defs(1)])
+    z = f[-1]
+    z["body"] = len(b) - 1
+    z["shape"] = SHAPE_BOX
+    z["p"][:2] = (8.0, 5.0)
+    z["flags"] = SENSOR
+    f["flags"][3::every] |= SENSOR
+    return Scene(b, f, name=f"sensors{n}", meta=dict(s.meta))

@@ -33,7 +33,7 @@ enum {
     DBX_OK = 0,
     DBX_E_INVALID_ARG = -1,
     DBX_E_CAPACITY = -2,
-    DBX_E_UNSUPPORTED = -3, /* joints, chains, sensors, continuous physics */
+    DBX_E_UNSUPPORTED = -3, /* a feature this library does not implement */
     DBX_E_HIP = -4,
     DBX_E_LOCKED = -5,      /* world mutation after dbx_finalize */
     DBX_E_STATE = -6
@@ -76,7 +76,9 @@ typedef struct {
  *                    flags bit1 / bit2 = previous / next ghost vertex in p[16..17] /
  *                    p[18..19] (SetPrevVertex / SetNextVertex).  A longer chain is
  *                    several chain fixtures joined by ghost vertices.
- * flags bit0 = sensor (unsupported).
+ * flags bit0 = sensor (b2FixtureDef.isSensor): its contacts touch while the
+ * shapes overlap (b2TestOverlap), carry no manifold, wake no body, give
+ * BEGIN / END but no PRE_SOLVE / POST_SOLVE and are never solved.
  *
  * Proxy indices.  Every fixture child has a broadphase proxy (b2Fixture.m_proxies):
  * one per fixture, one per segment of a chain.  The read-backs below name the

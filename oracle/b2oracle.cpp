@@ -1612,6 +1612,19 @@ static void gjkDistance(DistanceOutput &output, SimplexCache &cache, const Dista
     }
 }
 
+/* b2TestOverlap of two shapes b2collision.d:449-468: b2Distance with radii from an
+ * empty cache, overlapping below 10 epsilon */
+static bool testOverlap(const Shape &a, int indexA, const Xf &xfA, const Shape &b, int indexB, const Xf &xfB) {
+    DistanceProxy pA, pB;
+    pA.set(a, indexA);
+    pB.set(b, indexB);
+    SimplexCache cache;
+    cache.count = 0;
+    DistanceOutput out;
+    gjkDistance(out, cache, pA, xfA, pB, xfB, true);
+    return out.distance < 10.0f * kEps;
+}
+
 /* ---- b2TimeOfImpact: conservative advancement on separating axes (b2timeofimpact.d:67-530) ---- */
 enum { toi_unknown = 0, toi_failed = 1, toi_overlapped = 2, toi_touching = 3, toi_separated = 4 };
 struct SeparationFunction {
@@ -2230,13 +2243,25 @@ struct World {
             addPair(fa, fb);
         });
     }
-    /* b2Contact.Update b2contact.d:270-356 (no listener, no sensors) */
+    /* b2Contact.Update b2contact.d:270-356 with the test listener */
     void updateContact(Contact &c) {
         Manifold oldManifold = c.manifold;
         c.flags |= cf_enabled;
         bool wasTouching = (c.flags & cf_touching) == cf_touching;
         Body &bodyA = bodies[fixtures[c.fixtureA].body];
         Body &bodyB = bodies[fixtures[c.fixtureB].body];
+        if (fixtures[c.fixtureA].sensor || fixtures[c.fixtureB].sensor) {
+            /* a sensor: shape overlap by b2TestOverlap, no manifold, no wake, no
+             * PreSolve (b2contact.d:289-298,352) */
+            const bool touching = testOverlap(fixtures[c.fixtureA].shape, c.indexA, bodyA.xf,
+                                              fixtures[c.fixtureB].shape, c.indexB, bodyB.xf);
+            c.manifold.pointCount = 0;
+            if (touching) c.flags |= cf_touching;
+            else c.flags &= ~cf_touching;
+            if (!wasTouching && touching) recordEvent(1, c, nullptr);
+            if (wasTouching && !touching) recordEvent(2, c, nullptr);
+            return;
+        }
         evaluate(c.manifold, fixtures[c.fixtureA].shape, bodyA.xf, fixtures[c.fixtureB].shape, bodyB.xf, c.indexA);
         bool touching = c.manifold.pointCount > 0;
         for (int i = 0; i < c.manifold.pointCount; ++i) {

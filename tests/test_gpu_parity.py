@@ -328,11 +328,13 @@ def test_invalid_inputs_are_rejected():
     with pytest.raises(native.DbxError) as e:
         g.add_scene(bad)
     assert e.value.status == native.DBX_E_INVALID_ARG
-    sensor = scenes.Scene(s.bodies.copy(), s.fixtures.copy())
-    sensor.fixtures["flags"][5] = 1
+    loop = scenes.Scene(s.bodies.copy(), s.fixtures.copy())
+    loop.fixtures["shape"][5] = scenes.SHAPE_CHAIN  # a loop needs at least 3 points
+    loop.fixtures["count"][5] = 2
+    loop.fixtures["flags"][5] = scenes.CHAIN_LOOP
     with pytest.raises(native.DbxError) as e:
-        g.add_scene(sensor)
-    assert e.value.status == native.DBX_E_UNSUPPORTED
+        g.add_scene(loop)
+    assert e.value.status == native.DBX_E_INVALID_ARG
     g.add_scene(s)
     g.finalize()
     with pytest.raises(native.DbxError) as e:
