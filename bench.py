@@ -124,6 +124,54 @@ def learned_vs_plain(n_eval: int, steps: int) -> dict:
     }
 
 
+def single_world(which: int, steps: int, warmup: int, no_cpu: bool) -> dict:
+    """One big world on its own: config 2 (20,000 bodies in a 200 m x 400 m
+    container) or config 4 (the 50,000-box pyramid), dbox's defaults (warm
+    starting, sleeping, continuous physics), exact order.  Steps warmup+1 ..
+    warmup+steps are timed on the GPU with the per-phase split, and the same
+    steps on the oracle (one thread) as the CPU baseline."""
+    sc = scenes.big_mixed_world() if which == 2 else scenes.big_pyramid()
+    flags = native.WARM | native.SLEEP | native.ORDERED | native.CCD
+    h = native.StepWorlds()
+    h.add_scene(sc)
+    h.finalize()
+    if warmup:
+        h.step(DT, VEL, POS, flags, n=warmup)
+    h.sync()
+    h.profile_totals(reset=True)
+    t0 = time.perf_counter()
+    h.step(DT, VEL, POS, flags, n=steps)
+    h.sync()
+    t = time.perf_counter() - t0
+    prof, k = h.profile_totals(reset=True)
+    ndyn = sc.dynamic_count
+    out = {
+        "metric": f"body-steps/s, single world (config {which})", "value": ndyn * steps / t, "unit": "body-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * t / steps, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("20,000 bodies dropped into a 200 m x 400 m U container" if which == 2 else
+                                "pyramid of 50,000 1 m boxes on a ground edge"),
+                   "dynamic_bodies": ndyn, "continuous": True, "solver": "ordered"},
+        "phase_ms_per_step": {kk: v / max(k, 1) for kk, v in prof.items()},
+        "toi_events_last_step": int(h.toi_events()[0]),
+    }
+    h.close()
+    if not no_cpu:
+        from oracle import pyoracle as po
+        o = po.OracleWorld(sc, continuous=True)
+        if warmup:
+            o.step(DT, VEL, POS, n=warmup)
+        t0 = time.perf_counter()
+        o.step(DT, VEL, POS, n=steps)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ndyn * steps / tc, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                               "ms_per_step": 1e3 * tc / steps,
+                               "sample": f"the same world and steps {warmup + 1}-{warmup + steps} on the C++ "
+                                         "restatement of dbox (-O2 -ffp-contract=off), one thread"}
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,7 +197,16 @@ def main():
     ap.add_argument("--subset", default="", help="largest:N or smallest:N of the generated worlds (timing studies)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 20 device-to-device copies of this many MiB (HBM counter calibration) and exit")
+    ap.add_argument("--single-world", type=int, choices=(2, 4), default=0,
+                    help="time one big world instead (config 2: 20,000-body container; config 4: 50,000-box "
+                         "pyramid) with --steps timed steps after --warmup, beside the oracle on one thread")
     args = ap.parse_args()
+    if args.single_world:
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(single_world(args.single_world, args.steps, args.warmup, args.no_cpu)))
+        return
     if args.copy_mib:
         import torch
         x = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")

@@ -475,9 +475,17 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
                 int lastb = sbst.x;
                 unsigned long long maskb = ordered ? 0ull : M.mask[b];
                 // contacts of b are added now: an edge to a body popped earlier
-                // was added when that body was popped (b2world.d:1006-1046)
-                if (lane == 0) M.st[b].y = sbst.y | kIslPopped;
-                wave_order();
+                // was added when that body was popped (b2world.d:1006-1046).  No
+                // edge of b leads back to b and nothing else writes b's state
+                // during its pop, so b's final state (popped flag, last level,
+                // colours) is stored once, with the last chunk's stores.
+                if (deg == 0) {
+                    if (lane == 0) {
+                        M.st[b] = make_int2(lastb, sbst.y | kIslPopped);
+                        if (!ordered) M.mask[b] = maskb;
+                    }
+                    wave_order();
+                }
                 for (int c0 = 0; c0 < deg; c0 += 64) {
                     const int e = c0 + lane;
                     const int2 ent = e < deg ? M.adj[o + e] : make_int2(-1, 0);
@@ -561,13 +569,12 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
                         top = __builtin_amdgcn_readlane(other, 63 - __clzll((long long)pmask));
                         topKnown = true;
                     }
+                    if (c0 + 64 >= deg && lane == 0) {
+                        M.st[b] = make_int2(lastb, sbst.y | kIslPopped);
+                        if (!ordered) M.mask[b] = maskb;
+                    }
                     wave_order();
                 }
-                if (lane == 0) {
-                    M.st[b].x = lastb;
-                    if (!ordered) M.mask[b] = maskb;
-                }
-                wave_order();
             }
             ++nisl;
         }

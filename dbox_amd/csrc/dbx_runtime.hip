@@ -685,13 +685,14 @@ dbx_status dbx_finalize(dbx_handle *h) {
         D.gx = H.gx;
         D.gy = H.gy;
         D.islandBase = D.bodyBase;
-        // bit 0: some body has more than one fixture, so two contacts of one body
-        // can lead to the same other body (the island search must de-duplicate pushes)
+        // bit 0: some non-static body has more than one proxy, so two contacts of
+        // one body can lead to the same other non-static body (the island search
+        // must de-duplicate pushes; static bodies are never pushed)
         D.pad = 0;
         {
             std::vector<int> nfix(D.bodyCount, 0);
             for (int b : H.fbody)
-                if (++nfix[b] > 1) D.pad |= 1;
+                if (body_type(H.bflags[b]) != kStatic && ++nfix[b] > 1) D.pad |= 1;
         }
         D.islScratchBase = islBytes;
         islBytes += (long long)islands_scratch_bytes(D.bodyCount, D.contactCap);
