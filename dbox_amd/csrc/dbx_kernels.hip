@@ -456,6 +456,30 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
             }
             unsigned long long cm = __ballot(cand);
             if (!cm) break;
+            // Seeds with no touching edge are islands of one body that no search
+            // can reach: the run of them before the first other seed is numbered
+            // at once, in seed order, as the search would number them one by one.
+            bool iso = false;
+            if (cand) {
+                const int2 ab = M.ao[sb];
+                iso = true;
+                for (int k = 0; k < ab.y; ++k)
+                    if (M.adj[ab.x + k].x >= 0) {
+                        iso = false;
+                        break;
+                    }
+            }
+            const unsigned long long im = __ballot(iso), rest = cm & ~im;
+            const unsigned long long run = rest ? im & ((rest & (~rest + 1ull)) - 1ull) : im;
+            if (run) {
+                if ((run >> lane) & 1ull) {
+                    const int r = __popcll(run & ltmask);
+                    M.st[sb].y = M.st[sb].y | ((nisl + r + 1) << 4) | kIslPopped;
+                }
+                nisl += __popcll(run);
+                wave_order();
+                continue;
+            }
             const int seed = top0 - (__ffsll((long long)cm) - 1); // highest body index first
             if (lane == 0) M.st[seed].y |= (nisl + 1) << 4;
             wave_order();
