@@ -11,8 +11,11 @@ constexpr int kMaxBlock = 1024;
 
 // Step kernels over the worlds order[worldOffset .. worldOffset + nworlds); lead
 // selects the instantiation that profiles report for the lead (largest) tier.
+struct WideBp;
+// wide: the tier's worlds too big for one workgroup, whose first Collide pass
+// runs over the whole device first
 hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
-                          hipStream_t s);
+                          hipStream_t s, const WideBp *wide, int nwide);
 size_t islands_scratch_bytes(int bodies, int contacts);
 size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands);
 hipError_t configure_kernels(int maxDynamicLds);

@@ -199,13 +199,19 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && tid == 0;
     if (timed) d.ktime[16] = __builtin_amdgcn_s_memtime();
 
-    for (int b = tid; b < nb; b += BS) wake[b] = INT_MAX;
+    // a world too big for one workgroup (pad bit 1) had its first pass done by
+    // k_collide_wide over the whole device
+    const bool pre = (wd.pad & 2) != 0;
+    if (!pre)
+        for (int b = tid; b < nb; b += BS) wake[b] = INT_MAX;
     __syncthreads();
 
     const uint32_t transient = kContactDestroyed | kContactDormant | kContactBegan | kContactEnded | kContactIsland |
                                kContactNewPt0 | kContactNewPt1 | kContactSolvedOne;
     int dormant = 0;
-    for (int i = tid; i < n; i += BS) {
+    if (pre)
+        for (int i = tid; i < n; i += BS) dormant |= (src[i].flags & kContactDormant) ? 1 : 0;
+    for (int i = pre ? n : tid; i < n; i += BS) {
         Contact c = src[i];
         c.flags &= ~transient;
         uint32_t fa = bfl[fbody[c.fa]], fb = bfl[fbody[c.fb]];
@@ -3328,9 +3334,51 @@ __global__ void k_max_penetration(Dev d, int cur, float *out) {
         else hipLaunchKernelGGL(kernel<false>, grid, block, lds, s, __VA_ARGS__);                                     \
     } while (0)
 
+// The first pass of k_collide for one big world, one contact per thread over
+// the whole device: wake marks cleared, then every contact's transient flags
+// cleared and, unless both bodies are inactive (dormant), its update.
+__global__ __launch_bounds__(256) void k_collide_wide_init(Dev d, int w) {
+    const WorldDesc wd = d.W[w];
+    int *wake = d.wake + wd.bodyBase;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) wake[b] = INT_MAX;
+}
+__global__ __launch_bounds__(256) void k_collide_wide(Dev d, int cur, int w) {
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount;
+    Contact *src = d.cont[cur] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    const float4 *fat = d.ffat + wd.fixBase;
+    const uint32_t *bfl = d.bflags + wd.bodyBase;
+    const float4 *bxf = d.bxf + wd.bodyBase;
+    int *wake = d.wake + wd.bodyBase;
+    const uint32_t transient = kContactDestroyed | kContactDormant | kContactBegan | kContactEnded | kContactIsland |
+                               kContactNewPt0 | kContactNewPt1 | kContactSolvedOne;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        Contact c = src[i];
+        c.flags &= ~transient;
+        const uint32_t fa = bfl[fbody[c.fa]], fb = bfl[fbody[c.fb]];
+        const bool activeA = (fa & kBodyAwake) && body_type(fa) != kStatic;
+        const bool activeB = (fb & kBodyAwake) && body_type(fb) != kStatic;
+        if (!activeA && !activeB) {
+            src[i].flags = c.flags | kContactDormant;
+            continue;
+        }
+        process_contact(c, n - 1 - i, fbody, shp, fat, bxf, wake);
+        src[i] = c;
+    }
+}
+
 hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
-                          hipStream_t s) {
+                          hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
+    for (int k = 0; k < nwide; ++k) {
+        const WorldDesc &D = wide[k].desc;
+        const int nbg = (D.bodyCount + 255) / 256, ncg = (D.contactCap + 255) / 256;
+        const int gb = nbg < 1024 ? nbg : 1024, gc = ncg < 4096 ? ncg : 4096;
+        hipLaunchKernelGGL(k_collide_wide_init, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, wide[k].world);
+        hipLaunchKernelGGL(k_collide_wide, dim3(gc > 0 ? gc : 1), dim3(256), 0, s, d, cur, wide[k].world);
+    }
     DBX_LAUNCH(k_collide, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
 }

@@ -955,7 +955,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     const WideBp *wide = T.wideCount ? &h->wide[T.wideFirst] : nullptr;
     if ((part & 1) && findNew) // FindNewContacts when fixtures were added (b2world.d:372-376)
         DBX_HIP(h, launch_broadphase(d, cin, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
-    if (part & 2) DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s));
+    if (part & 2) DBX_HIP(h, launch_collide(d, cin, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
     if (!(part & 4)) return DBX_OK;
     const bool timed = lead && ev; // phase events: lead tier, eager steps only
     if (timed) (void)hipEventRecord(ev[1], s);
