@@ -52,7 +52,7 @@ hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds
 hipError_t launch_refit(const Dev &d, int world, hipStream_t s);
 // b2World.SolveTOI (continuous physics) after the step's broadphase.
 hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, bool lead,
-                      hipStream_t s);
+                      hipStream_t s, const WideBp *wide, int nwide);
 hipError_t launch_bp_single(const float4 *fat, const int *moved, int n, char *scratch, int2 *out, int cap, int *total,
                             hipStream_t s);
 hipError_t launch_collide_pairs(const Shape *sa, const float *xa, const Shape *sb, const float *xb, int n, float *out,

@@ -2645,21 +2645,8 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     return true;
 }
 
-template <bool kLead>
-__global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
-    __shared__ int s_ibod[2 * kMaxToiContacts];
-    __shared__ VelCon s_vc[kMaxToiContacts];
-    __shared__ PosCon s_pc[kMaxToiContacts];
-    __shared__ float4 s_imp[kMaxToiContacts];
-    __shared__ unsigned long long s_pairs[kToiPairCap];
-    __shared__ SweepSnap s_snap[kToiListCap];
-    __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
-    __shared__ ToiLists s_L;
-    __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices]; // per lane: the two proxies of one TOI
-    __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved;
-    const int w = d.order[worldOffset + blockIdx.x];
-    const WorldDesc wd = d.W[w];
-    const int lane = threadIdx.x;
+// One world's arrays for the continuous pass.
+__device__ __forceinline__ ToiWorld toi_world(const Dev &d, int cur, int w, const WorldDesc &wd) {
     ToiWorld W;
     W.cont = d.cont[cur] + wd.contactBase;
     W.ctan = d.ctan[cur] + wd.contactBase;
@@ -2685,6 +2672,51 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     W.bfixList = d.bfixList + wd.fixBase;
     W.log = d.toiLog ? d.toiLog + d.toiLogOff[w] : nullptr;
     W.logCap = d.toiLog ? d.toiLogOff[w + 1] - d.toiLogOff[w] : 0;
+    return W;
+}
+
+// The start of the continuous pass for a world too big for one workgroup (pad
+// bit 1), over the whole device: every sweep at alpha0 = 0 (k_toi_wide_bodies),
+// then every contact's cached TOI reset and, where the first search needs one,
+// computed one contact per lane (b2world.d:1131-1146,1176-1238); k_toi then
+// starts from these.
+__global__ __launch_bounds__(256) void k_toi_wide_bodies(Dev d, int w) {
+    const WorldDesc wd = d.W[w];
+    int4 *tb = d.toiB + wd.bodyBase;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) tb[b] = make_int4(0, 0, -1, 0);
+}
+__global__ __launch_bounds__(64) void k_toi_wide_first(Dev d, int cur, int w) {
+    __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices];
+    const WorldDesc wd = d.W[w];
+    const ToiWorld W = toi_world(d, cur, w, wd);
+    const int n = d.S[w].contactCount;
+    for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+        int4 t = make_int4(__float_as_int(1.0f), 0, -1, -1);
+        if (toi_status(W, W.cont[i], 0) == 2) {
+            t.x = __float_as_int(contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
+                                             s_verts + 2 * kMaxPolygonVertices * threadIdx.x));
+            t.y = kTcToi;
+        }
+        W.tc[i] = t;
+    }
+}
+
+template <bool kLead>
+__global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
+    __shared__ int s_ibod[2 * kMaxToiContacts];
+    __shared__ VelCon s_vc[kMaxToiContacts];
+    __shared__ PosCon s_pc[kMaxToiContacts];
+    __shared__ float4 s_imp[kMaxToiContacts];
+    __shared__ unsigned long long s_pairs[kToiPairCap];
+    __shared__ SweepSnap s_snap[kToiListCap];
+    __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
+    __shared__ ToiLists s_L;
+    __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices]; // per lane: the two proxies of one TOI
+    __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved;
+    const int w = d.order[worldOffset + blockIdx.x];
+    const WorldDesc wd = d.W[w];
+    const int lane = threadIdx.x;
+    const ToiWorld W = toi_world(d, cur, w, wd);
     const int nb = wd.bodyCount, nf = wd.fixCount;
     int n = d.S[w].contactCount;
     const int m = min(d.S[w].adjContacts, n);
@@ -2692,8 +2724,11 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     long long tWalk = 0, tEvent = 0, tMark = 0;
     if (timed) d.ktime[25] = __builtin_amdgcn_s_memtime();
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
-    for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
-    for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+    const bool pre = (wd.pad & 2) != 0; // k_toi_wide_* did the resets and the first search
+    if (!pre) {
+        for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
+        for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+    }
     if (lane == 0) {
         s_L.woken = s_woken;
         s_L.moved = s_movedList;
@@ -2709,7 +2744,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     // load chains per lane) and their TOIs dealt out evenly over the lanes.
     if (lane == 0) s_nsnap = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 256) {
+    for (int base = 0; base < (pre ? 0 : n); base += 256) {
         int need[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -3429,8 +3464,15 @@ hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int wo
 }
 size_t broadphase_scratch_bytes(int proxies) { return bp_bytes(proxies); }
 hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, bool lead,
-                      hipStream_t s) {
+                      hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
+    for (int k = 0; k < nwide; ++k) {
+        const WorldDesc &D = wide[k].desc;
+        const int nbg = (D.bodyCount + 255) / 256, ncg = (D.contactCap + 63) / 64;
+        const int gb = nbg < 1024 ? nbg : 1024, gc = ncg < 8192 ? ncg : 8192;
+        hipLaunchKernelGGL(k_toi_wide_bodies, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, wide[k].world);
+        hipLaunchKernelGGL(k_toi_wide_first, dim3(gc > 0 ? gc : 1), dim3(64), 0, s, d, cur, wide[k].world);
+    }
     DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(64), 0, s, d, cur, sp, worldOffset);
     return hipGetLastError();
 }

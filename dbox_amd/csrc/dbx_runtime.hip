@@ -969,7 +969,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
         if (timed) (void)hipEventRecord(ev[4], s);
         // b2World.SolveTOI when continuous physics is on (b2world.d:414-419)
-        if (flags & DBX_STEP_CCD) DBX_HIP(h, launch_toi(d, cout, sp, T.first, T.count, lead, s));
+        if (flags & DBX_STEP_CCD) DBX_HIP(h, launch_toi(d, cout, sp, T.first, T.count, lead, s, wide, T.wideCount));
     } else if (timed) {
         (void)hipEventRecord(ev[2], s);
         (void)hipEventRecord(ev[3], s);
