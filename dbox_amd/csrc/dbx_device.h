@@ -191,12 +191,17 @@ struct Dev {
     float *mlpFeat;         // staged feature rows: 2 per contact slot from 2 * contactBase
     int2 *mlpList;          // per staged row: (constraint * 2 + point, log row or -1)
     int *mlpNew;            // per world: rows staged in the current step
-    float2 *mlpBody;        // per body: sum and count of matched warm-start normal impulses
+    longlong2 *mlpBody;     // per body: sum (fixed point, 2^-32 N s) and count of matched warm-start normal
+                            // impulses; integer sums do not depend on the order of the atomic adds
     float *logX;            // log rows of kMlpFeatStride features
     float *logT;            // 2 labels (normal, tangent impulse) per log row
     unsigned long long *logCount;
     long long logCap;
     long long *ktime;       // optional: phase timestamps of the largest world (s_memtime)
+    // b2Profile phases of the largest world's island solve (b2island.d:118-237): a ring of
+    // kProfSlots slots of kProfStamps wall-clock stamps (wall_clock64) after a counter in
+    // prof[0..kProfStamps); slot s starts at prof[kProfStamps * (s + 1)]
+    unsigned long long *prof;
     // continuous pass (b2World.SolveTOI) state, valid during that pass only
     int4 *toiC;             // per contact slot: (cached TOI as float bits, toiCount | flags, next edge of
                             // body A's and of body B's list of contacts made since the last Collide)
@@ -205,6 +210,28 @@ struct Dev {
     const int *bfixList;    // per world from fixBase: fixture indices grouped by body
     ToiCallback *toiLog;    // per world from toiLogOff[w]: the continuous pass's callbacks in order
     const int *toiLogOff;   // nworlds + 1 offsets into toiLog
+};
+
+constexpr int kProfSlots = 256;
+constexpr int kProfStamps = 8;
+// Stamp indices of one slot: solveInit from kProfInit0 to kProfVel0, solveVelocity to
+// kProfVel1, solvePosition from kProfPos0 to kProfPos1.
+enum : int { kProfInit0 = 0, kProfVel0 = 1, kProfVel1 = 2, kProfPos0 = 3, kProfPos1 = 4 };
+
+// Phase stamps of the island solve of the largest world (the lead tier's first
+// workgroup): the first stamp takes a slot, the others write into it.
+struct ProfStamp {
+    unsigned long long *slot = nullptr;
+    __device__ __forceinline__ void begin(const Dev &d, bool lead) {
+        if (lead && d.prof) {
+            const unsigned long long k = atomicAdd(d.prof, 1ull) % (unsigned long long)kProfSlots;
+            slot = d.prof + kProfStamps * (k + 1);
+            slot[kProfInit0] = wall_clock64();
+        }
+    }
+    __device__ __forceinline__ void at(int i) const {
+        if (slot) slot[i] = wall_clock64();
+    }
 };
 
 // Order-preserving int image of a float (for atomicMin on floats).

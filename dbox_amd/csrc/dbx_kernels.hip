@@ -1071,23 +1071,27 @@ __device__ __forceinline__ void mlp_new_points(const Dev &d, int w, int nc, uint
     const WorldDesc wd = d.W[w];
     const bool learned = (flags & (1u << 4)) != 0, logging = (flags & (1u << 6)) != 0;
     const int tid = threadIdx.x, BS = blockDim.x;
-    float2 *nbs = d.mlpBody + wd.bodyBase;
+    longlong2 *nbs = d.mlpBody + wd.bodyBase;
     float *feat = d.mlpFeat + (size_t)2 * wd.contactBase * kMlpFeatStride;
     int2 *list = d.mlpList + (size_t)2 * wd.contactBase;
-    for (int b = tid; b < wd.bodyCount; b += BS) nbs[b] = make_float2(0.0f, 0.0f);
+    for (int b = tid; b < wd.bodyCount; b += BS) nbs[b] = make_longlong2(0, 0);
     if (tid == 0) d.mlpNew[w] = 0;
     __syncthreads();
+    // Sums of the matched points' impulses per body in 32.32 fixed point: integer adds
+    // commute, so the features do not depend on which lane adds first, and a world gives
+    // the same features alone or in any batch.
     for (int k = tid; k < nc; k += BS) {
         const VelCon &v = vc[k];
         const uint32_t fl = cont[v.contact].flags;
         const float4 im = imp[k];
         for (int j = 0; j < v.pointCount; ++j) {
             if (fl & (j ? kContactNewPt1 : kContactNewPt0)) continue;
-            const float li = j ? im.z : im.x;
-            atomicAdd(&nbs[v.indexA].x, li);
-            atomicAdd(&nbs[v.indexA].y, 1.0f);
-            atomicAdd(&nbs[v.indexB].x, li);
-            atomicAdd(&nbs[v.indexB].y, 1.0f);
+            const float li = fminr(fmaxr(j ? im.z : im.x, -1.0e6f), 1.0e6f);
+            const unsigned long long q = (unsigned long long)__double2ll_rn((double)li * 4294967296.0);
+            atomicAdd((unsigned long long *)&nbs[v.indexA].x, q);
+            atomicAdd((unsigned long long *)&nbs[v.indexA].y, 1ull);
+            atomicAdd((unsigned long long *)&nbs[v.indexB].x, q);
+            atomicAdd((unsigned long long *)&nbs[v.indexB].y, 1ull);
         }
     }
     __syncthreads();
@@ -1098,8 +1102,10 @@ __device__ __forceinline__ void mlp_new_points(const Dev &d, int w, int nc, uint
             if (!(fl & (j ? kContactNewPt1 : kContactNewPt0))) continue;
             const int row = atomicAdd(&d.mlpNew[w], 1); // at most two rows per constraint
             float f[kMlpFeatStride];
-            mlp_features(f, v, pc[k], j, vel[v.indexA], vel[v.indexB], pos[pc[k].indexA], pos[pc[k].indexB], nbs[v.indexA],
-                         nbs[v.indexB]);
+            const longlong2 sa = nbs[v.indexA], sb = nbs[v.indexB];
+            const float2 nbA = make_float2((float)((double)sa.x * 2.3283064365386963e-10), (float)sa.y);
+            const float2 nbB = make_float2((float)((double)sb.x * 2.3283064365386963e-10), (float)sb.y);
+            mlp_features(f, v, pc[k], j, vel[v.indexA], vel[v.indexB], pos[pc[k].indexA], pos[pc[k].indexB], nbA, nbB);
             float *dst = feat + (size_t)row * kMlpFeatStride;
             for (int i = 0; i < kMlpFeatStride; ++i) dst[i] = f[i];
             int logRow = -1;
@@ -1183,7 +1189,7 @@ __device__ float solve_position(const PosCon &p, int indexA, int indexB, int dyn
     return ms;
 }
 
-__device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w) {
+__device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead) {
     __shared__ float s_res;
     const WorldDesc wd = d.W[w];
     const int nb = wd.bodyCount, nf = wd.fixCount;
@@ -1237,6 +1243,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
     }
     __syncthreads();
+    ProfStamp prof;
+    if (tid == 0) prof.begin(d, lead);
 
     // Constraint set-up.
     for (int k = tid; k < nc; k += BS) {
@@ -1258,6 +1266,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
             __syncthreads();
         }
     }
+    __syncthreads();
+    prof.at(kProfVel0);
 
     // Velocity iterations (b2island.d:153-161); with the residual flag the
     // loop runs until the velocity residual falls below the tolerance.
@@ -1309,6 +1319,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
             c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
         }
     }
+    __syncthreads();
+    prof.at(kProfVel1);
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
     for (int b = tid; b < nb; b += BS) {
         if (bisl[b] < 0) continue;
@@ -1330,6 +1342,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     }
     for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
     __syncthreads();
+    prof.at(kProfPos0);
 
     // Position iterations with the per-island early exit (b2island.d:206-224).
     for (int it = 0; it < sp.positionIterations; ++it) {
@@ -1365,6 +1378,8 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         Xf x = xf_from(v2(pos.x, pos.y), pos.z, v2(ms.z, ms.w));
         bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
     }
+    __syncthreads();
+    prof.at(kProfPos1);
     // Sleep (b2island.d:241-279).
     if (sp.flags & 2u) {
         for (int i = tid; i < nisl; i += BS) islMin[i] = __float_as_int(kMaxFloat);
@@ -1431,7 +1446,7 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
 
 template <bool kLead>
 __global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, int worldOffset) {
-    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x]);
+    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x], worldOffset + blockIdx.x == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1611,7 +1626,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     const int nbatch = d.S[w].batchCount;
     const int nisl = d.S[w].islandCount;
     if (solve_lds_bytes(nb, nc, nbatch, nisl) > (size_t)ldsBytes) {
-        solve_world_global(d, cur, sp, w);
+        solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0);
         return;
     }
     char *p = lds;
@@ -1674,6 +1689,8 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         Lvel[b] = make_float4(vx, vy, wv, vw);
     }
     __syncthreads();
+    ProfStamp prof;
+    if (lane == 0) prof.begin(d, worldOffset + blockIdx.x == 0);
     // Constraints are stored in schedule order (slot j holds constraint sched[j]),
     // so a level's constraints are a contiguous range of slots.
     for (int j = lane; j < nc; j += 64) {
@@ -1694,6 +1711,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
     if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lbo, nbatch, vc, Lvel, Limp, lane);
     if (timed && lane == 0) d.ktime[6] = __builtin_amdgcn_s_memtime();
+    prof.at(kProfVel0);
     {
         const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
         int used = 0;
@@ -1731,6 +1749,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
             c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
         }
     }
+    prof.at(kProfVel1);
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
     for (int b = lane; b < nb; b += 64) {
         if (bisl[b] < 0) continue;
@@ -1751,6 +1770,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         Lvel[b] = make_float4(v.x, v.y, wv, 0.0f);
     }
     for (int i = lane; i < nisl; i += 64) Lsol[i] = 0;
+    prof.at(kProfPos0);
     // Position iterations with the per-island early exit (b2island.d:206-224).
     for (int it = 0; it < sp.positionIterations; ++it) {
         for (int i = lane; i < nisl; i += 64)
@@ -1802,6 +1822,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         bvel[b] = vel;
     }
     __syncthreads();
+    prof.at(kProfPos1);
     // SynchronizeFixtures + fat-AABB rule (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
     for (int f = lane; f < nf; f += 64) {
         int b = fbody[f];

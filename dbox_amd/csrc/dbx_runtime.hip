@@ -526,6 +526,9 @@ struct dbx_handle {
     int tierSteps = 0;
     int ringUsed = 0;
     dbx_profile prof{};             // last step
+    std::vector<char> ringSolved;   // per ring entry: the step ran the island solve (dt > 0)
+    unsigned long long profSeen = 0; // phase-stamp slots of the solve kernels already read
+    double wallKHz = 100000.0;      // rate of the kernels' wall_clock64 stamps
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
     int sumSteps = 0;
     long long totalBodies = 0, totalFix = 0, totalContacts = 0, totalPairs = 0;
@@ -609,6 +612,9 @@ dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
         h->ldsMax = 65536;
         (void)configure_kernels(h->ldsMax);
     }
+    int wallKHz = 0;
+    if (hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, h->cfg.device) == hipSuccess && wallKHz > 0)
+        h->wallKHz = wallKHz;
     (void)hipGetLastError(); // clear any error state left by the attribute calls
     if (getenv("DBX_VERBOSE"))
         fprintf(stderr, "dbx: device %d, %d CUs, LDS per workgroup %d bytes\n", h->cfg.device,
@@ -923,6 +929,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
     if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
+    DBX_HIP(h, dalloc(h, &d.prof, (size_t)kProfStamps * (kProfSlots + 1)));
     DBX_HIP(h, hipStreamSynchronize(h->stream));
     h->pending.clear();
     h->pending.shrink_to_fit();
@@ -1124,6 +1131,7 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
             h->ring.push_back(e);
         }
         h->ringGraph.push_back(0);
+        h->ringSolved.push_back(0);
     }
     hipEvent_t *ev = &h->ring[kStepEvents * h->ringUsed];
     h->d.mlp = mlp_dev(h->mlp);
@@ -1185,6 +1193,7 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
         h->ringGraph[h->ringUsed] = 0;
     }
     if (st != DBX_OK) return st;
+    h->ringSolved[h->ringUsed] = dt > 0.0f;
     if (timeTiers && !tuning) {
         DBX_HIP(h, hipStreamSynchronize(h->stream));
         h->tierMs.resize(h->tiers.size(), 0.0);
@@ -1276,23 +1285,53 @@ dbx_status dbx_sync(dbx_handle *h) {
     DBX_HIP(h, hipMemcpy(S.data(), h->d.S, h->nworlds * sizeof(WorldState), hipMemcpyDeviceToHost));
     for (int w = 0; w < h->nworlds; ++w) {
         if (S[w].error) {
-            char buf[128];
-            snprintf(buf, sizeof buf, "world %d exceeded its %s capacity", w, (S[w].error & 1) ? "contact" : "pair");
+            // bit 1: contact slots; bit 2: the step's broadphase pair list; bit 4: the
+            // continuous pass's per-event lists (pairs of the moved proxies, moved proxies)
+            char buf[160];
+            const int e = S[w].error;
+            snprintf(buf, sizeof buf, "world %d exceeded its %s capacity", w,
+                     (e & 1) ? "contact" : (e & 2) ? "broadphase pair" : "continuous-pass pair/moved-proxy list");
             return fail(h, DBX_E_CAPACITY, buf);
         }
     }
+    // b2Profile's island-solve phases (b2island.d:118-237) from the stamps the largest
+    // world's solve left: one slot per step that ran the solve, in step order.
+    std::vector<unsigned long long> stamps((size_t)kProfStamps * (kProfSlots + 1));
+    DBX_HIP(h, hipMemcpy(stamps.data(), h->d.prof, stamps.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    const unsigned long long profCount = stamps[0];
+    int solvedSteps = 0;
+    for (int k = 0; k < h->ringUsed; ++k) solvedSteps += h->ringSolved[k] ? 1 : 0;
+    const bool stampsMatch = profCount - h->profSeen == (unsigned long long)solvedSteps;
+    unsigned long long nextSlot = h->profSeen;
+    h->profSeen = profCount;
     for (int k = 0; k < h->ringUsed; ++k) {
         hipEvent_t *ev = &h->ring[kStepEvents * k];
         float ms[5] = {0, 0, 0, 0, 0}, total = 0.0f;
+        float phase[3] = {0, 0, 0};
+        if (h->ringSolved[k]) {
+            const unsigned long long slot = nextSlot++;
+            if (stampsMatch && profCount - slot <= (unsigned long long)kProfSlots) {
+                const unsigned long long *t = &stamps[(size_t)kProfStamps * (slot % kProfSlots + 1)];
+                const double perMs = h->wallKHz;
+                phase[0] = (float)((double)(t[kProfVel0] - t[kProfInit0]) / perMs);
+                phase[1] = (float)((double)(t[kProfVel1] - t[kProfVel0]) / perMs);
+                phase[2] = (float)((double)(t[kProfPos1] - t[kProfPos0]) / perMs);
+            }
+        }
         if (!h->ringGraph[k]) // a replayed graph records only the step's ends
             for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
         (void)hipEventElapsedTime(&total, ev[0], ev[6]);
+        // b2World.Step's fields (b2world.d:397-431): collide; solve = all of b2World.Solve,
+        // i.e. the island search, the island solves, SynchronizeFixtures and FindNewContacts
+        // (whose share is also broadphase, b2world.d:1102-1123); the island-solve phases
+        // accumulate over islands (b2world.d:1079-1081), here the phases of the largest
+        // world, which the other worlds run beside.
         dbx_profile p{};
         p.collide = ms[0];
-        p.solveInit = ms[1];
-        p.solveVelocity = ms[2];
-        p.solvePosition = 0.0f;
         p.solve = ms[1] + ms[2] + ms[3];
+        p.solveInit = phase[0];
+        p.solveVelocity = phase[1];
+        p.solvePosition = phase[2];
         p.broadphase = ms[3];
         p.solveTOI = ms[4];
         p.step = total;
@@ -1302,6 +1341,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->sum.solve += p.solve;
         h->sum.solveInit += p.solveInit;
         h->sum.solveVelocity += p.solveVelocity;
+        h->sum.solvePosition += p.solvePosition;
         h->sum.broadphase += p.broadphase;
         h->sum.solveTOI += p.solveTOI;
         h->sumSteps++;

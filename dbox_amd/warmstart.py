@@ -63,6 +63,8 @@ class IterationStats:
     max_penetration: float      # over all worlds and steps
     world_steps: int
     mean_residual: float        # velocity residual after the last iteration, mean over those pairs
+    per_step: np.ndarray = dataclasses.field(repr=False, default=None)  # [step, world] iterations, 0 = no solve
+    awake_body_steps: int = 0   # sum over steps of the awake dynamic bodies after the step
 
 
 def iterations(scene_list, steps: int, params=None, norm=None, tolerance: float = 1e-3,
@@ -75,21 +77,41 @@ def iterations(scene_list, steps: int, params=None, norm=None, tolerance: float 
     if params is not None:
         h.mlp_set(params, norm)
         flags |= native.LEARNED
-    its, res, pen = [], [], 0.0
+    its, res, pen, awake = [], [], 0.0, 0
     for _ in range(steps):
         h.step(DT, VEL, POS, flags)
         h.sync()
         it, p = h.solver_stats()
         r = h.residuals()
-        its.append(it[it > 0])
+        its.append(it.copy())
         res.append(r[it > 0])
         pen = max(pen, float(p.max()))
+        awake += h.totals()["awake"]
     h.close()
-    a = np.concatenate(its) if its else np.zeros(0)
+    per = np.stack(its) if its else np.zeros((0, len(scene_list)), np.int32)
+    a = per[per > 0]
     if len(a) == 0:
-        return IterationStats(0.0, 0.0, 0.0, pen, 0, 0.0)
+        return IterationStats(0.0, 0.0, 0.0, pen, 0, 0.0, per, awake)
     return IterationStats(float(a.mean()), float(np.percentile(a, 95)), float(np.mean(a >= max_iterations)), pen,
-                          int(len(a)), float(np.concatenate(res).mean()))
+                          int(len(a)), float(np.concatenate(res).mean()), per, awake)
+
+
+def compare(plain: IterationStats, learned: IterationStats) -> dict:
+    """Learned against plain over the same (world, step) pairs: those where both runs
+    solved at least one constraint (a run that keeps a world awake where the other let
+    it sleep would otherwise be averaged over a different set).  Also the world-steps
+    and awake body-steps of each run, which show any change in sleeping."""
+    both = (plain.per_step > 0) & (learned.per_step > 0)
+    p = plain.per_step[both].astype(np.float64)
+    q = learned.per_step[both].astype(np.float64)
+    out = {"common_world_steps": int(both.sum()),
+           "plain_mean_iterations": float(p.mean()) if p.size else 0.0,
+           "learned_mean_iterations": float(q.mean()) if q.size else 0.0,
+           "plain_world_steps": plain.world_steps, "learned_world_steps": learned.world_steps,
+           "plain_awake_body_steps": plain.awake_body_steps, "learned_awake_body_steps": learned.awake_body_steps,
+           "plain_max_penetration_m": plain.max_penetration, "learned_max_penetration_m": learned.max_penetration}
+    out["iteration_reduction"] = 1.0 - out["learned_mean_iterations"] / out["plain_mean_iterations"] if p.size else 0.0
+    return out
 
 
 def training_scenes(seed: int = 11, pyramids: int = 8, stacks: int = 48, mixed: int = 8):

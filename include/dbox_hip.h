@@ -145,7 +145,22 @@ typedef struct {
 } dbx_contact_setting; /* 28 bytes */
 
 /* b2Profile (src/dbox/dynamics/b2timestep.d:37-46), milliseconds of the last step,
- * measured with HIP events around each phase. */
+ * with b2World.Step's meaning of each field (b2world.d:367-434, 930-1124):
+ *   step       the whole step (HIP events on the handle's stream)
+ *   collide    b2ContactManager.Collide
+ *   solve      all of b2World.Solve: island search, island solves, SynchronizeFixtures
+ *              and FindNewContacts (so it includes broadphase)
+ *   solveInit / solveVelocity / solvePosition
+ *              the phases of b2Island.Solve (b2island.d:118-148: constraint set-up and
+ *              warm start; 151-165: velocity iterations and StoreImpulses; 203-237:
+ *              position iterations and the copy back), timed inside the solve kernel
+ *              of the largest world with the device's constant-rate wall clock; dbox
+ *              sums them over islands (b2world.d:1079-1081), here the islands of the
+ *              largest world run in one workgroup while the other worlds run beside it
+ *   broadphase SynchronizeFixtures + FindNewContacts (b2world.d:1102-1123)
+ *   solveTOI   b2World.SolveTOI
+ * A step replayed from a hipGraph records step and the three island-solve phases only;
+ * split (DBX_STEP_PRESOLVE) steps are not timed. */
 typedef struct {
     float step, collide, solve, solveInit, solveVelocity, solvePosition, broadphase, solveTOI;
 } dbx_profile;

@@ -96,9 +96,58 @@ static inline V2 vmax(V2 a, V2 b) { return mk(fmax_(a.x, b.x), fmax_(a.y, b.y));
 static inline float fabs_(float a) { return a > 0.0f ? a : -a; }
 static inline float clampf(float a, float lo, float hi) { return fmax_(lo, fmin_(a, hi)); }
 
+/* Test switch (not reference behaviour): 0 = b2Rot.Set with glibc sinf/cosf as dbox
+ * calls them; 1 = sin and cos evaluated in double and rounded once to float, by the
+ * textbook method below (Cody-Waite reduction by pi/2 split into three doubles, Taylor
+ * series of sin r and cos r up to r^15 / r^16 in Horner form with fused multiply-adds).
+ * Mode 1 is the same evaluation, operation for operation, as the device library's, so
+ * GPU results can be compared with this restatement bit for bit. */
+static int g_rotMode = 0;
+static void rotExact(float angle, float &s, float &c) {
+    const double x = (double)angle;
+    if (!(std::fabs(x) < 1.0e6)) {
+        s = (float)std::sin(x);
+        c = (float)std::cos(x);
+        return;
+    }
+    const double P1 = 0x1.921fb54400000p+0, P2 = 0x1.0b4611a600000p-34, P3 = 0x1.3198a2e037073p-69;
+    const double k = std::rint(x * 0.6366197723675814);
+    double r = std::fma(-k, P1, x);
+    r = std::fma(-k, P2, r);
+    r = std::fma(-k, P3, r);
+    const double z = r * r;
+    /* 1/n! with alternating signs: sin r = r + r z (s6 + z (s5 + ...)), cos r = 1 + z (-1/2 + z (c6 + ...)) */
+    static const double S[7] = {-7.647163731819816e-13, 1.6059043836821613e-10, -2.505210838544172e-08,
+                                2.7557319223985893e-06, -0.0001984126984126984, 0.008333333333333333,
+                                -0.16666666666666666};
+    static const double Cc[7] = {4.779477332387385e-14, -1.1470745597729725e-11, 2.08767569878681e-09,
+                                 -2.755731922398589e-07, 2.48015873015873e-05, -0.001388888888888889,
+                                 0.041666666666666664};
+    double ps = std::fma(S[0], z, S[1]);
+    double pc = std::fma(Cc[0], z, Cc[1]);
+    for (int i = 2; i < 7; ++i) {
+        ps = std::fma(ps, z, S[i]);
+        pc = std::fma(pc, z, Cc[i]);
+    }
+    pc = std::fma(pc, z, -0.5);
+    pc = std::fma(pc, z, 1.0);
+    ps = std::fma(ps * z, r, r);
+    const unsigned q = (unsigned)(int)k;
+    const float fs = (float)ps, fc = (float)pc;
+    const bool odd = (q & 1u) != 0u;
+    float so = odd ? fc : fs, co = odd ? fs : fc;
+    if (q & 2u) so = -so;
+    if ((q + 1u) & 2u) co = -co;
+    s = so;
+    c = co;
+}
+
 struct Rot {
     float s = 0, c = 0;
-    void set(float angle) { s = sinf(angle); c = cosf(angle); } /* b2math.d:483-488 */
+    void set(float angle) { /* b2math.d:483-488 */
+        if (g_rotMode) rotExact(angle, s, c);
+        else { s = sinf(angle); c = cosf(angle); }
+    }
 };
 struct Xf {
     V2 p;
@@ -1961,6 +2010,7 @@ struct World {
     /* test hook, standing in for a b2ContactListener.PreSolve: 0 none, 1 record the
      * calls, 2 record them and apply presolvePolicy */
     int presolveMode = 0;
+    int rotMode = -1; /* test switch: -1 = leave g_rotMode as it is, else g_rotMode during step() */
     float inv_dt0 = 0.0f;
     int solveOrder = 0;
     int islandCount = 0; /* test hook: 1 = solve island contacts in greedy-colour order */
@@ -2411,7 +2461,6 @@ struct World {
             float radiusA = pc.radiusA, radiusB = pc.radiusB;
             const Manifold &manifold = contacts[islandContacts[vc.contactIndex]].manifold;
             int indexA = vc.indexA, indexB = vc.indexB;
-            float mA = vc.invMassA, mB = vc.invMassB, iA = vc.invIA, iB = vc.invIB;
             V2 localCenterA = pc.localCenterA, localCenterB = pc.localCenterB;
             V2 cA = positions[indexA].c;
             float aA = positions[indexA].a;
@@ -2434,38 +2483,47 @@ struct World {
                 VelocityConstraintPoint &vcp = vc.points[j];
                 vcp.rA = wm.points[j] - cA;
                 vcp.rB = wm.points[j] - cB;
-                float rnA = cross(vcp.rA, vc.normal);
-                float rnB = cross(vcp.rB, vc.normal);
-                float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
-                vcp.normalMass = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
-                V2 tangent = cross(vc.normal, 1.0f);
-                float rtA = cross(vcp.rA, tangent);
-                float rtB = cross(vcp.rB, tangent);
-                float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
-                vcp.tangentMass = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+                pointMasses(vc, vcp);
                 vcp.velocityBias = 0.0f;
                 float vRel = dot(vc.normal, vB + cross(wB, vcp.rB) - vA - cross(wA, vcp.rA));
                 if (vRel < -kVelocityThreshold) vcp.velocityBias = -vc.restitution * vRel;
             }
-            if (vc.pointCount == 2) { /* g_blockSolve == true (b2contactsolver.d:799) */
-                VelocityConstraintPoint &vcp1 = vc.points[0];
-                VelocityConstraintPoint &vcp2 = vc.points[1];
-                float rn1A = cross(vcp1.rA, vc.normal);
-                float rn1B = cross(vcp1.rB, vc.normal);
-                float rn2A = cross(vcp2.rA, vc.normal);
-                float rn2B = cross(vcp2.rB, vc.normal);
-                float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
-                float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
-                float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
-                const float k_maxConditionNumber = 1000.0f;
-                if (k11 * k11 < k_maxConditionNumber * (k11 * k22 - k12 * k12)) {
-                    vc.K.ex = mk(k11, k12);
-                    vc.K.ey = mk(k12, k22);
-                    vc.normalMass = vc.K.inverse();
-                } else {
-                    vc.pointCount = 1;
-                }
-            }
+            blockMasses(vc);
+        }
+    }
+    /* normal and tangent mass of one point (b2contactsolver.d:384-402) */
+    static void pointMasses(const VelocityConstraint &vc, VelocityConstraintPoint &vcp) {
+        float mA = vc.invMassA, mB = vc.invMassB, iA = vc.invIA, iB = vc.invIB;
+        float rnA = cross(vcp.rA, vc.normal);
+        float rnB = cross(vcp.rB, vc.normal);
+        float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+        vcp.normalMass = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+        V2 tangent = cross(vc.normal, 1.0f);
+        float rtA = cross(vcp.rA, tangent);
+        float rtB = cross(vcp.rB, tangent);
+        float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+        vcp.tangentMass = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+    }
+    /* the two-point block K, its inverse, or the downgrade to one point (b2contactsolver.d:418-447) */
+    static void blockMasses(VelocityConstraint &vc) {
+        if (vc.pointCount != 2) return; /* g_blockSolve == true (b2contactsolver.d:799) */
+        float mA = vc.invMassA, mB = vc.invMassB, iA = vc.invIA, iB = vc.invIB;
+        VelocityConstraintPoint &vcp1 = vc.points[0];
+        VelocityConstraintPoint &vcp2 = vc.points[1];
+        float rn1A = cross(vcp1.rA, vc.normal);
+        float rn1B = cross(vcp1.rB, vc.normal);
+        float rn2A = cross(vcp2.rA, vc.normal);
+        float rn2B = cross(vcp2.rB, vc.normal);
+        float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
+        float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
+        float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
+        const float k_maxConditionNumber = 1000.0f;
+        if (k11 * k11 < k_maxConditionNumber * (k11 * k22 - k12 * k12)) {
+            vc.K.ex = mk(k11, k12);
+            vc.K.ey = mk(k12, k22);
+            vc.normalMass = vc.K.inverse();
+        } else {
+            vc.pointCount = 1;
         }
     }
     void warmStart() {
@@ -2892,6 +2950,9 @@ struct World {
 
     /* b2ContactSolver.SolveTOIPositionConstraints b2contactsolver.d:152-242 */
     bool solveTOIPositionConstraints(int toiIndexA, int toiIndexB) {
+        return solveTOIPositionConstraintsMin(toiIndexA, toiIndexB) >= -1.5f * kLinearSlop;
+    }
+    float solveTOIPositionConstraintsMin(int toiIndexA, int toiIndexB) {
         float minSeparation = 0.0f;
         for (size_t i = 0; i < pcs.size(); ++i) {
             const PositionConstraint &pc = pcs[i];
@@ -2940,7 +3001,7 @@ struct World {
             positions[indexB].c = cB;
             positions[indexB].a = aB;
         }
-        return minSeparation >= -1.5f * kLinearSlop;
+        return minSeparation;
     }
 
     /* b2Island.SolveTOI b2island.d:282-416 */
@@ -3151,7 +3212,93 @@ struct World {
     }
 
     /* b2World.Step b2world.d:367-434 */
+    /* Test hook: b2ContactSolver.SolveVelocityConstraints (b2contactsolver.d:492-772) on one
+     * constructed constraint between island bodies 0 and 1, after the masses of
+     * InitializeVelocityConstraints (b2contactsolver.d:384-447). */
+    void solveOneConstraint(const float *in, float *out) {
+        velocities.assign(2, Velocity());
+        velocities[0].v = mk(in[0], in[1]);
+        velocities[0].w = in[2];
+        velocities[1].v = mk(in[3], in[4]);
+        velocities[1].w = in[5];
+        VelocityConstraint vc;
+        vc.indexA = 0;
+        vc.indexB = 1;
+        vc.invMassA = in[6];
+        vc.invIA = in[7];
+        vc.invMassB = in[8];
+        vc.invIB = in[9];
+        vc.normal = mk(in[10], in[11]);
+        vc.pointCount = (int)in[28];
+        for (int j = 0; j < vc.pointCount; ++j) {
+            VelocityConstraintPoint &vcp = vc.points[j];
+            vcp.rA = mk(in[12 + 4 * j], in[13 + 4 * j]);
+            vcp.rB = mk(in[14 + 4 * j], in[15 + 4 * j]);
+            vcp.normalImpulse = in[20 + 2 * j];
+            vcp.tangentImpulse = in[21 + 2 * j];
+            vcp.velocityBias = in[24 + j];
+            pointMasses(vc, vcp);
+        }
+        blockMasses(vc);
+        vc.friction = in[26];
+        vc.tangentSpeed = in[27];
+        vcs.assign(1, vc);
+        const int iterations = (int)in[29];
+        for (int i = 0; i < iterations; ++i) solveVelocityConstraints();
+        const VelocityConstraint &r = vcs[0];
+        out[0] = velocities[0].v.x;
+        out[1] = velocities[0].v.y;
+        out[2] = velocities[0].w;
+        out[3] = velocities[1].v.x;
+        out[4] = velocities[1].v.y;
+        out[5] = velocities[1].w;
+        out[6] = r.points[0].normalImpulse;
+        out[7] = r.points[0].tangentImpulse;
+        out[8] = r.points[1].normalImpulse;
+        out[9] = r.points[1].tangentImpulse;
+        out[10] = (float)r.pointCount;
+        vcs.clear();
+    }
+    /* Test hook: one b2ContactSolver.SolveTOIPositionConstraints pass (b2contactsolver.d:152-242)
+     * over the contact at list index listIndex, with the bodies of fixture toiFixture's body as
+     * the TOI body (the other body keeps zero mass in the pass).  out: the TOI body's sweep c
+     * and a after the pass, and the pass's minimum separation. */
+    int toiPositionOnce(int listIndex, int toiBody, float *out) {
+        int ci = contactList;
+        for (int i = 0; i < listIndex && ci != kNull; ++i) ci = contacts[ci].next;
+        if (ci == kNull) return -1;
+        const Contact &c = contacts[ci];
+        const int ba = fixtures[c.fixtureA].body, bb = fixtures[c.fixtureB].body;
+        if (toiBody != ba && toiBody != bb) return -1;
+        islandBodies = {ba, bb};
+        bodies[ba].islandIndex = 0;
+        bodies[bb].islandIndex = 1;
+        islandContacts = {ci};
+        positions.assign(2, Position());
+        velocities.assign(2, Velocity());
+        for (int i = 0; i < 2; ++i) {
+            const Body &b = bodies[islandBodies[i]];
+            positions[i].c = b.sweep.c;
+            positions[i].a = b.sweep.a;
+        }
+        TimeStep sub;
+        sub.dtRatio = 1.0f;
+        sub.warmStarting = false;
+        solverCreate(sub);
+        const int t = toiBody == ba ? 0 : 1;
+        out[3] = solveTOIPositionConstraintsMin(t, t);
+        out[0] = positions[t].c.x;
+        out[1] = positions[t].c.y;
+        out[2] = positions[t].a;
+        return 0;
+    }
     void step(float dt, int velocityIterations, int positionIterations) {
+        const int savedRotMode = g_rotMode;
+        if (rotMode >= 0) g_rotMode = rotMode;
+        stepImpl(dt, velocityIterations, positionIterations);
+        g_rotMode = savedRotMode;
+    }
+    void stepImpl(float dt, int velocityIterations, int positionIterations) {
         auto t0 = Clock::now();
         events.clear();
         if (newFixture) {
@@ -3428,6 +3575,23 @@ int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const or
     return timeOfImpact(*t_out, pa, pb, sweepIn(sweep_a9), sweepIn(sweep_b9), t_max);
 }
 int orc_toi_events(void *w) { return ((World *)w)->toiEvents; }
+int orc_set_rot_mode(int mode) {
+    const int prev = g_rotMode;
+    g_rotMode = mode ? 1 : 0;
+    return prev;
+}
+int orc_world_set_rot_mode(void *w, int mode) {
+    if (mode < -1 || mode > 1) return -1;
+    ((World *)w)->rotMode = mode;
+    return 0;
+}
+void orc_solve_one_constraint(const float *in30, float *out11) {
+    World w;
+    w.solveOneConstraint(in30, out11);
+}
+int orc_toi_position_once(void *w, int list_index, int toi_body, float *out4) {
+    return ((World *)w)->toiPositionOnce(list_index, toi_body, out4);
+}
 int orc_world_set_presolve(void *w, int mode) {
     if (mode < 0 || mode > 2) return -1;
     ((World *)w)->presolveMode = mode;

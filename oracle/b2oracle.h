@@ -179,6 +179,28 @@ int orc_toi_events(void *w);
  * k = (fixtureA + fixtureB) % 7: 0 disable, 1 friction 0.05, 2 restitution 0.6,
  * 3 tangent speed 0.8. */
 int orc_world_set_presolve(void *w, int mode);
+/* Test switch for b2Rot.Set (not reference behaviour).  Mode 0 (default) calls
+ * glibc sinf/cosf as dbox does; mode 1 evaluates sin and cos in double and rounds
+ * once (Cody-Waite reduction, Taylor series in Horner form with fused
+ * multiply-adds), the device library's evaluation, so that GPU and oracle results
+ * can be compared bit for bit.  orc_set_rot_mode sets the process-wide mode (for
+ * the stand-alone calls) and returns the previous one; orc_world_set_rot_mode
+ * makes orc_step of one world run in mode 0 or 1 (-1: the process-wide mode). */
+int orc_set_rot_mode(int mode);
+int orc_world_set_rot_mode(void *w, int mode);
+/* Test hook: b2ContactSolver.SolveVelocityConstraints (b2contactsolver.d:492-772)
+ * on one constructed constraint, its masses computed as InitializeVelocityConstraints
+ * does (b2contactsolver.d:384-447).  in30: vA(3) vB(3) invMassA invIA invMassB invIB
+ * normal(2) rA1(2) rB1(2) rA2(2) rB2(2) normalImpulse1 tangentImpulse1 normalImpulse2
+ * tangentImpulse2 velocityBias1 velocityBias2 friction tangentSpeed pointCount
+ * iterations.  out11: vA(3) vB(3) the four impulses, pointCount after the block
+ * solver's condition check. */
+void orc_solve_one_constraint(const float *in30, float *out11);
+/* Test hook: one b2ContactSolver.SolveTOIPositionConstraints pass
+ * (b2contactsolver.d:152-242) over contact-list entry list_index, with body toi_body
+ * as the TOI body.  out4: its sweep c (2) and a after the pass, minimum separation.
+ * Returns -1 when the entry or the body does not exist. */
+int orc_toi_position_once(void *w, int list_index, int toi_body, float *out4);
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,11 @@ def lib():
         L.orc_time_of_impact.argtypes = [vp, vp, vp, vp, f32, vp]
         L.orc_toi_events.argtypes = [vp]
         L.orc_world_set_presolve.argtypes = [vp, i32]
+        L.orc_set_rot_mode.argtypes = [i32]
+        L.orc_world_set_rot_mode.argtypes = [vp, i32]
+        L.orc_solve_one_constraint.argtypes = [vp, vp]
+        L.orc_solve_one_constraint.restype = None
+        L.orc_toi_position_once.argtypes = [vp, i32, i32, vp]
         _lib = L
     return _lib
 
@@ -91,7 +96,12 @@ def _p(a: np.ndarray):
 class OracleWorld:
     """b2World restated on the CPU, built from a scene's definitions."""
 
-    def __init__(self, scene, warm_starting=True, allow_sleep=True, colour_order=False, continuous=False):
+    def __init__(self, scene, warm_starting=True, allow_sleep=True, colour_order=False, continuous=False,
+                 exact_rot=False):
+        """exact_rot: step with b2Rot.Set evaluated as the device library evaluates it
+        (double-precision Cody-Waite + Taylor, rounded once) instead of glibc sinf/cosf,
+        so that GPU results can be compared bit for bit.  Bodies are still created with
+        glibc, as the library's host-side builder creates them."""
         L = lib()
         self.L = L
         self.h = L.orc_world_create(float(scene.gravity[0]), float(scene.gravity[1]))
@@ -102,6 +112,7 @@ class OracleWorld:
         self.fixtures_def = np.ascontiguousarray(scene.fixtures)
         if L.orc_build(self.h, _p(self.bodies_def), len(self.bodies_def), _p(self.fixtures_def), len(self.fixtures_def)) != 0:
             raise ValueError("oracle rejected the scene")
+        L.orc_world_set_rot_mode(self.h, 1 if exact_rot else 0)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -170,9 +181,58 @@ class OracleWorld:
         """TOI events (sub-step island solves) of the last step."""
         return int(self.L.orc_toi_events(self.h))
 
+    def toi_position_once(self, list_index: int, toi_body: int):
+        """One TOI position pass over one contact with toi_body as the TOI body:
+        (c.x, c.y, a, minimum separation)."""
+        out = np.zeros(4, np.float32)
+        if self.L.orc_toi_position_once(self.h, int(list_index), int(toi_body), _p(out)) != 0:
+            raise ValueError("no such contact or body")
+        return out
+
     @property
     def dynamic_count(self) -> int:
         return self.L.orc_dynamic_body_count(self.h)
+
+
+class rot_mode:
+    """Context manager for the process-wide b2Rot.Set mode of the stand-alone calls
+    (collide, distance, time of impact): 1 = the device library's evaluation."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = lib().orc_set_rot_mode(int(self.mode))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_rot_mode(self.prev)
+        return False
+
+
+def solve_one_constraint(va, vb, masses, normal, ra, rb, impulses, bias, friction=0.0, tangent_speed=0.0,
+                         point_count=2, iterations=1):
+    """SolveVelocityConstraints on one constructed constraint between two bodies.
+    va, vb: (vx, vy, w); masses: (invMassA, invIA, invMassB, invIB); ra, rb: 2x2 (point, xy);
+    impulses: (n1, t1, n2, t2); bias: velocity biases of the two points.
+    Returns (vA(3), vB(3), impulses(4), point count after the condition check)."""
+    x = np.zeros(30, np.float32)
+    x[0:3] = va
+    x[3:6] = vb
+    x[6:10] = masses
+    x[10:12] = normal
+    ra = np.asarray(ra, np.float32).reshape(2, 2)
+    rb = np.asarray(rb, np.float32).reshape(2, 2)
+    x[12:14], x[14:16], x[16:18], x[18:20] = ra[0], rb[0], ra[1], rb[1]
+    x[20:24] = impulses
+    x[24:26] = bias
+    x[26] = friction
+    x[27] = tangent_speed
+    x[28] = point_count
+    x[29] = iterations
+    out = np.zeros(11, np.float32)
+    lib().orc_solve_one_constraint(_p(x), _p(out))
+    return out[0:3].copy(), out[3:6].copy(), out[6:10].copy(), int(out[10])
 
 
 def collide(fix_a, xfa, fix_b, xfb) -> np.ndarray | None:

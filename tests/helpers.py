@@ -83,3 +83,78 @@ def scene_of(bodies: list[tuple], fixtures: list) -> scenes.Scene:
             b[i]["vx"], b[i]["vy"], b[i]["w"] = t[4:7]
     f = np.array(fixtures, scenes.FIXTURE_DTYPE)
     return scenes.Scene(b, f)
+
+
+# ---------------------------------------------------------------------------
+# bit-exact comparisons of GPU read-backs with the oracle's
+# ---------------------------------------------------------------------------
+BODY_FLOATS = ("px", "py", "qs", "qc", "cx", "cy", "a", "sleep_time", "vx", "vy", "w", "c0x", "c0y", "a0", "inv_mass")
+
+
+def body_mismatch(bg: np.ndarray, bo: np.ndarray) -> str | None:
+    """None when every float field of every body has the same bits and the awake
+    flags agree; otherwise a description of the first difference."""
+    if len(bg) != len(bo):
+        return f"body count {len(bg)} != {len(bo)}"
+    for f in BODY_FLOATS:
+        a, b = bg[f].view(np.uint32), bo[f].view(np.uint32)
+        bad = np.nonzero(a != b)[0]
+        if bad.size:
+            i = int(bad[0])
+            d = np.abs(bg[f].astype(np.float64) - bo[f].astype(np.float64))
+            return f"{bad.size} bodies differ in {f}, first {i}: {bg[f][i]!r} vs {bo[f][i]!r}, max |d| {d.max():.3e}"
+    awake = (bg["flags"] & 2) != (bo["flags"] & 2)
+    if awake.any():
+        return f"{int(awake.sum())} bodies differ in the awake flag, first {int(np.nonzero(awake)[0][0])}"
+    return None
+
+
+def contact_mismatch(cg: np.ndarray, co: np.ndarray) -> str | None:
+    """None when the contact lists are equal in order, fixtures, touching/enabled flags,
+    manifold (type, points, normal, local points, feature ids), impulses of the
+    manifold's points and mixed material, all bit for bit."""
+    if len(cg) != len(co):
+        return f"contact count {len(cg)} != {len(co)}"
+    for f in ("fixture_a", "fixture_b", "type", "point_count"):
+        if not np.array_equal(cg[f], co[f]):
+            return f"contacts differ in {f} at {int(np.nonzero(cg[f] != co[f])[0][0])}"
+    if not np.array_equal(cg["flags"] & 6, co["flags"] & 6):
+        return "contacts differ in touching/enabled"
+    for f in ("local_normal", "local_point", "friction", "restitution"):
+        if cg[f].tobytes() != co[f].tobytes():
+            return f"contacts differ in {f}"
+    for k in range(len(co)):
+        n = int(co["point_count"][k])
+        for f, w in (("point_local", 2 * n), ("normal_impulse", n), ("tangent_impulse", n), ("id", n)):
+            if cg[f][k][:w].tobytes() != co[f][k][:w].tobytes():
+                return f"contact {k} differs in {f}: {cg[f][k][:w]} vs {co[f][k][:w]}"
+    return None
+
+
+def varied_container(n: int, seed: int, width: float = 30.0, height: float = 40.0) -> scenes.Scene:
+    """The mixed container of the batched workload with every per-body and per-fixture
+    option the default scenes leave at its default: seeded collision categories,
+    masks and groups (b2Filter), restitution U[0, 0.8], linear and angular damping
+    U[0, 0.6], gravity scale U[0.5, 1.5], initial velocities, fixed rotation on some
+    bodies and a few bullets."""
+    s = scenes.mixed_container(n, width, height, seed)
+    rng = scenes.Stream(seed ^ 0x5EED)
+    b, f = s.bodies, s.fixtures
+    dyn = b["type"] == scenes.DYNAMIC
+    m = int(dyn.sum())
+    b["linear_damping"][dyn] = rng.uniform(m, 0.0, 0.6)
+    b["angular_damping"][dyn] = rng.uniform(m, 0.0, 0.6)
+    b["gravity_scale"][dyn] = rng.uniform(m, 0.5, 1.5)
+    b["vx"][dyn] = rng.uniform(m, -3.0, 3.0)
+    b["vy"][dyn] = rng.uniform(m, -3.0, 3.0)
+    b["w"][dyn] = rng.uniform(m, -2.0, 2.0)
+    b["fixed_rotation"][dyn] = (rng.uniform(m) < 0.1).astype(np.int32)
+    b["bullet"][dyn] = (rng.uniform(m) < 0.05).astype(np.int32)
+    body_dyn = dyn[f["body"]]
+    k = int(body_dyn.sum())
+    f["restitution"][body_dyn] = rng.uniform(k, 0.0, 0.8)
+    f["category"][body_dyn] = np.array([1, 2, 4], np.uint16)[rng.integers(k, 0, 2)]
+    f["mask"][body_dyn] = np.array([0xFFFF, 0xFFFF, 0xFFFF ^ 2, 0xFFFF ^ 4, 1 | 4], np.uint16)[rng.integers(k, 0, 4)]
+    f["group"][body_dyn] = np.array([0, 0, 0, 0, -1, -2, 1], np.int16)[rng.integers(k, 0, 6)]
+    s.name = f"varied{n}"
+    return s

@@ -12,7 +12,7 @@ import pytest
 
 from dbox_amd import native, scenes
 from oracle import pyoracle as po
-from tests.helpers import box, circle, edge, random_shape, scene_of
+from tests.helpers import body_mismatch, box, circle, contact_mismatch, edge, random_shape, scene_of
 
 pytestmark = pytest.mark.gpu
 DT = 1.0 / 60.0
@@ -74,6 +74,40 @@ def test_narrowphase_random_pairs_match_oracle():
         exact += int(np.array_equal(g[20:20 + cnt], ref["separations"][:cnt]))
     assert touching > 500
     assert exact > 0.95 * touching  # identical except where a libm sin/cos rounds differently
+
+
+def test_narrowphase_random_pairs_bit_identical_with_exact_sincos():
+    # with the oracle's b2Rot.Set evaluated as the kernels evaluate it, every manifold
+    # of every collide function is the oracle's bit for bit
+    print("case: narrowphase random pairs, exact sin/cos, seed 12, n 4000", flush=True)
+    rng = np.random.default_rng(12)
+    n = 4000
+    fa = np.array([random_shape(rng) for _ in range(n)], scenes.FIXTURE_DTYPE)
+    fb = np.array([random_shape(rng) for _ in range(n)], scenes.FIXTURE_DTYPE)
+    xa = np.stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-3.2, 3.2, n)], 1).astype(np.float32)
+    xb = np.stack([xa[:, 0] + rng.uniform(-1.6, 1.6, n), xa[:, 1] + rng.uniform(-1.6, 1.6, n),
+                   rng.uniform(-3.2, 3.2, n)], 1).astype(np.float32)
+    h = native.StepWorlds()
+    out = native.collide_pairs(h, fa, xa, fb, xb)
+    touching = 0
+    with po.rot_mode(1):
+        for i in range(n):
+            ref = po.collide(fa[i], xa[i], fb[i], xb[i])
+            g = out[i]
+            if ref is None:
+                assert g[0] == -1.0
+                continue
+            cnt = int(ref["point_count"])
+            assert int(g[1]) == cnt and int(g[2]) == ref["swapped"], i
+            if cnt == 0:
+                continue
+            touching += 1
+            assert int(g[0]) == ref["type"]
+            assert g[12:14].view(np.uint32)[:cnt].tobytes() == ref["id"][:cnt].tobytes(), i
+            for lo, hi, key, w in ((4, 6, "local_normal", 2), (6, 8, "local_point", 2), (8, 12, "point_local", 2 * cnt),
+                                   (14, 16, "normal", 2), (16, 20, "points", 2 * cnt), (20, 22, "separations", cnt)):
+                assert g[lo:lo + w].tobytes() == ref[key][:w].tobytes(), (i, key)
+    assert touching > 700
 
 
 # ---------------------------------------------------------------- broadphase
@@ -178,8 +212,10 @@ def test_island_order_and_levels_match_reference(scene_name):
     # b2World.Solve's depth-first search, and its batches must be the
     # dependency levels of that order.
     print(f"case: island order {scene_name}", flush=True)
+    # The oracle evaluates b2Rot.Set as the kernels do, so the two runs stay bit-identical
+    # and every checkpoint compares the same island contents.
     s = scenes.pyramid() if scene_name == "pyramid" else scenes.mixed_container(400, 40.0, 80.0, seed=77)
-    o = po.OracleWorld(s)
+    o = po.OracleWorld(s, exact_rot=True)
     g = gpu_world([s])
     dyn = s.bodies["type"] == 2
     checked = 0
@@ -187,11 +223,7 @@ def test_island_order_and_levels_match_reference(scene_name):
         o.step()
         g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
         if i % 30 == 29:
-            bo, bg = o.bodies(), g.bodies()
-            if np.abs(bo["cx"] - bg["cx"]).max() > 1e-4 or np.abs(bo["cy"] - bg["cy"]).max() > 1e-4:
-                # trajectories have drifted apart (libm rounding); the contact sets no longer agree
-                print(f"states diverged at step {i + 1}")
-                break
+            assert body_mismatch(g.bodies(), o.bodies()) is None, i
             ref = o.solve_log()
             got = g.schedule()
             assert got.shape[0] == ref.shape[0]
@@ -201,7 +233,7 @@ def test_island_order_and_levels_match_reference(scene_name):
             assert np.array_equal(got[:, 0], ref[:, 0])
             assert np.array_equal(got[:, 3], _levels(ref[:, :3], dyn))
             checked += 1
-    assert checked >= 1
+    assert checked == 8
 
 
 def test_pyramid_60_steps_within_tolerance():
@@ -295,16 +327,26 @@ def test_repeated_runs_are_identical():
     assert outs[0] == outs[1]
 
 
-def test_mixed_world_tracks_oracle_ordered():
-    print("case: mixed 300-body container world, 60 ordered steps", flush=True)
+def test_mixed_world_distance_to_reference_is_the_libm_rounding():
+    # The GPU run equals the oracle run with the kernels' sin/cos bit for bit, so all of
+    # its distance from the reference-faithful run (glibc sinf/cosf) is the libm's
+    # last-bit rounding amplified by the pile; that distance is reported, not bounded.
+    print("case: mixed 300-body container world, 60 ordered steps, glibc vs exact sin/cos", flush=True)
     s = scenes.mixed_container(300, 40.0, 80.0, seed=1234)
-    o, g = _lockstep(s, 60, native.WARM | native.SLEEP | native.ORDERED)
-    bo, bg = o.bodies(), g.bodies()
+    o = po.OracleWorld(s)
+    x = po.OracleWorld(s, exact_rot=True)
+    g = gpu_world([s])
+    for _ in range(60):
+        o.step()
+        x.step()
+        g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
+    bg, bx, bo = g.bodies(), x.bodies(), o.bodies()
+    assert body_mismatch(bg, bx) is None
+    assert contact_mismatch(g.contacts(), x.contacts()) is None
     dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"])
-    print(f"mixed world: median |dx| {np.median(dp):.2e}, max {dp.max():.2e}")
-    assert np.median(dp) < 1e-4
-    co, cg = o.contacts(), g.contacts()
-    assert abs(len(co) - len(cg)) <= max(2, len(co) // 50)
+    da = angle_diff(bg["a"], bo["a"])
+    print(f"mixed world vs glibc run: median |dx| {np.median(dp):.2e} m, max {dp.max():.2e} m, "
+          f"max |da| {da.max():.2e} rad; {int((dp > 1e-3).sum())} of {len(dp)} bodies beyond 1e-3 m")
 
 
 def test_capacity_error_is_reported_and_handle_survives():
@@ -433,38 +475,27 @@ def test_listener_events_match_oracle(scene_name):
     # dbx_read_events: the BeginContact / EndContact / PostSolve callbacks dbox makes
     # during b2World.Step (b2contact.d:342-350, b2contactmanager.d:190-193,
     # b2island.d:438-461), in its order, against the oracle's record of the same
-    # callbacks.  Kinds, pairs and counts must be identical; PostSolve impulses agree
-    # to 2e-3 N s absolute + 1e-3 relative (the trajectories differ by sin/cos rounding).
-    # The pyramid is compared for 80 steps: from about step 85 its resting two-point
-    # contacts shift load between their points with the last bits of the positions
-    # (per-contact totals still agree to 0.06 N s), so single points stop being
-    # comparable while the kinds and order stay identical.
+    # callbacks.
+    # With the oracle's b2Rot.Set evaluated as the kernels do, the callback streams are
+    # identical, PostSolve impulses included, at every step of the run.
     print(f"case: listener events {scene_name}", flush=True)
     s = scenes.pyramid() if scene_name == "pyramid" else scenes.mixed_container(300, 40.0, 80.0, seed=5)
-    o = po.OracleWorld(s)
+    o = po.OracleWorld(s, exact_rot=True)
     g = gpu_world([s])
     compared = kinds = 0
-    worst = 0.0
-    for i in range(80 if scene_name == "pyramid" else 120):
+    for i in range(200 if scene_name == "pyramid" else 120):
         o.step()
         g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
-        bo, bg = o.bodies(), g.bodies()
-        if np.abs(bo["cx"] - bg["cx"]).max() > 1e-4 or np.abs(bo["cy"] - bg["cy"]).max() > 1e-4:
-            print(f"states diverged at step {i + 1}")
-            break
         eo, eg = o.events(), g.events()
         assert len(eg) == len(eo), (i, len(eg), len(eo))
         for f in ("type", "fixture_a", "fixture_b", "point_count"):
             assert np.array_equal(eg[f], eo[f]), (i, f)
         post = eo["type"] == native.EVENT_POST_SOLVE
         for f in ("normal_impulse", "tangent_impulse"):
-            d = np.abs(eg[f][post] - eo[f][post])
-            if d.size:
-                worst = max(worst, float(d.max()))
-            np.testing.assert_allclose(eg[f][post], eo[f][post], rtol=1e-3, atol=2e-3)
+            assert eg[f][post].tobytes() == eo[f][post].tobytes(), (i, f)
         compared += len(eo)
         kinds |= int(np.bitwise_or.reduce(1 << eo["type"])) if len(eo) else 0
-    print(f"events compared {compared}, largest impulse difference {worst:.2e}")
+    print(f"events compared {compared}, all bit-identical")
     assert compared > 500
     assert kinds & (1 << native.EVENT_BEGIN) and kinds & (1 << native.EVENT_POST_SOLVE)
     # a buffer smaller than the event count is refused and left untouched
