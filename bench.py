@@ -9,8 +9,13 @@ iterations, with dbox's defaults: warm starting, sleeping and continuous physics
 without continuous physics are timed once more and reported beside it.
 
 Prints one JSON line.  value = dynamic bodies (awake or asleep) x steps / wall
-seconds over steps 1..K of freshly built worlds (after a separate warm-up run
-that also picks the workgroup sizes), with the worlds already resident in HBM.
+seconds over K timed steps, with the worlds already resident in HBM, after a
+separate warm-up run that also picks the workgroup sizes.  With K >= 600 the
+timed steps are steps 1..K of freshly built worlds (the measurement protocol:
+steps 1-600); with fewer, the worlds are first stepped untimed to step 300, the
+middle of that protocol, and steps 301..300+K are timed, so a short run measures
+piles rather than the first free-fall steps (--start overrides the step the timed
+window starts after).
 """
 from __future__ import annotations
 
@@ -63,33 +68,84 @@ def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int,
     return c * 12 + nbodies * 8
 
 
-def run_cpu_baseline(worlds, steps, threads, sample_every, continuous):
-    """Steps 1..steps of every sample_every-th world on the oracle (C++ restatement
-    of dbox), single-threaded and on a pool of threads, one world per task."""
+def host_cpu() -> dict:
+    """The host the CPU baseline runs on: CPU model, logical CPUs, the CPUs this
+    process may use (affinity, cgroup quota, OMP_NUM_THREADS), oracle compiler and flags."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        info["model"] = ""
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = info["nproc"]
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            info["cgroup_cpus"] = None if q == "max" else float(q) / float(per)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    info["omp_num_threads"] = os.environ.get("OMP_NUM_THREADS")
+    try:
+        import subprocess
+        info["compiler"] = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        info["compiler"] = ""
+    info["flags"] = "-std=c++17 -O3 -ffp-contract=off -fno-fast-math (oracle/Makefile)"
+    return info
+
+
+def cpu_threads(info: dict) -> int:
+    """Every CPU this process is granted: affinity, capped by a cgroup quota and by
+    OMP_NUM_THREADS, which the GPU hosts set to the CPU share of one GPU."""
+    n = info["affinity"] or 1
+    if info["cgroup_cpus"]:
+        n = min(n, max(1, int(info["cgroup_cpus"])))
+    if info["omp_num_threads"] and info["omp_num_threads"].isdigit():
+        n = min(n, int(info["omp_num_threads"]))
+    return max(1, n)
+
+
+def run_cpu_baseline(worlds, start, steps, threads, sample_every, single_every, continuous):
+    """Steps start+1..start+steps of every sample_every-th world on the oracle (C++
+    restatement of dbox) on a pool of threads, one world per task, after stepping
+    them untimed to `start`; and the same window of every single_every-th world on
+    one thread."""
     from oracle import pyoracle as po
-    sample = worlds[::sample_every]
-    dyn = sum(s.dynamic_count for s in sample)
 
     def run(ws, n):
         for _ in range(n):
             for w in ws:
                 w.step(DT, VEL, POS)
 
-    single = [po.OracleWorld(s, continuous=continuous) for s in sample]
-    t0 = time.perf_counter()
-    run(single, steps)
-    t1 = time.perf_counter() - t0
+    def pool(ws, n):
+        parts = [ws[i::threads] for i in range(threads)]
+        ts = [threading.Thread(target=run, args=(p, n)) for p in parts]
+        t0 = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return time.perf_counter() - t0
 
+    sample = worlds[::sample_every]
+    dyn = sum(s.dynamic_count for s in sample)
     ows = [po.OracleWorld(s, continuous=continuous) for s in sample]
-    parts = [ows[i::threads] for i in range(threads)]
-    ts = [threading.Thread(target=run, args=(p, steps)) for p in parts]
+    if start:
+        pool(ows, start)
+    tn = pool(ows, steps)
+
+    one = worlds[::single_every]
+    dyn1 = sum(s.dynamic_count for s in one)
+    ows1 = [po.OracleWorld(s, continuous=continuous) for s in one]
+    if start:
+        run(ows1, start)
     t0 = time.perf_counter()
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    tn = time.perf_counter() - t0
-    return dyn, t1, tn, len(sample)
+    run(ows1, steps)
+    t1 = time.perf_counter() - t0
+    return {"dyn": dyn, "t_pool": tn, "n": len(sample), "dyn1": dyn1, "t_one": t1, "n1": len(one)}
 
 
 def learned_vs_plain(n_eval: int, steps: int) -> dict:
@@ -107,17 +163,19 @@ def learned_vs_plain(n_eval: int, steps: int) -> dict:
     test = scenes.stack_worlds(n_eval)
     plain = warmstart.iterations(test, steps)
     learned = warmstart.iterations(test, steps, params=params, norm=norm)
+    cmp = warmstart.compare(plain, learned)
 
     def row(st):
         return {"mean_iterations": st.mean_iterations, "p95_iterations": st.p95_iterations,
                 "capped_fraction": st.capped_fraction, "max_penetration_m": st.max_penetration,
-                "world_steps": st.world_steps}
+                "world_steps": st.world_steps, "awake_body_steps": st.awake_body_steps}
 
     return {
         "workload": f"first {n_eval} worlds of the 16,384 stack-world set, steps 1-{steps}",
         "residual_tolerance": 1e-3, "max_iterations": 100,
         "plain": row(plain), "learned": row(learned),
-        "iteration_reduction": 1.0 - learned.mean_iterations / max(plain.mean_iterations, 1e-9),
+        "same_world_steps": cmp,
+        "iteration_reduction": cmp["iteration_reduction"],
         "training": {"samples": int(samples), "adam_steps": 2000, "batch": 4096,
                      "loss_first_20": float(np.mean(loss[:20])), "loss_last_20": float(np.mean(loss[-20:]))},
         "seconds": time.perf_counter() - t0,
@@ -167,7 +225,7 @@ def single_world(which: int, steps: int, warmup: int, no_cpu: bool) -> dict:
         out["cpu_baseline"] = {"value": ndyn * steps / tc, "unit": "body-steps/s", "cores": 1, "kind": "port",
                                "ms_per_step": 1e3 * tc / steps,
                                "sample": f"the same world and steps {warmup + 1}-{warmup + steps} on the C++ "
-                                         "restatement of dbox (-O2 -ffp-contract=off), one thread"}
+                                         "restatement of dbox (-O3 -ffp-contract=off), one thread"}
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
 
@@ -176,13 +234,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600,
-                    help="timed steps, from step 1 of freshly built worlds")
+                    help="timed steps (steps 1..K of freshly built worlds for K >= 600, else steps 301..300+K)")
+    ap.add_argument("--start", type=int, default=-1,
+                    help="untimed steps of the timed run's worlds before the window (default 0 for >= 600 timed "
+                         "steps, else 300)")
     ap.add_argument("--warmup", type=int, default=30,
                     help="steps of the separate warm-up run (which also autotunes the workgroup sizes)")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--coloured", action="store_true",
                     help="time the greedy-coloured solver instead of the reference's exact Gauss-Seidel order")
-    ap.add_argument("--cpu-sample-every", type=int, default=32)
+    ap.add_argument("--cpu-sample-every", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -229,6 +290,7 @@ def main():
         by_size = sorted(range(len(worlds)), key=lambda i: len(worlds[i].bodies), reverse=(kind == "largest"))
         worlds = [worlds[i] for i in sorted(by_size[: int(n)])]
         log(f"subset {args.subset}: bodies per world {[len(w.bodies) for w in worlds][:8]}...")
+
     def build():
         handle = native.StepWorlds()
         for sc in worlds:
@@ -236,12 +298,14 @@ def main():
         handle.finalize()
         return handle
 
+    start = args.start if args.start >= 0 else (0 if args.steps >= 600 else 300)
+    window = f"{start + 1}-{start + args.steps}"
     ndyn = sum(s.dynamic_count for s in worlds)
     nbodies = sum(len(s.bodies) for s in worlds)
     nprox = sum(len(s.fixtures) for s in worlds)
     sbytes = sum(shape_bytes(s.fixtures) for s in worlds)
     flags = native.WARM | native.SLEEP | (0 if args.coloured else native.ORDERED) | (0 if args.no_ccd else native.CCD)
-    log(f"bodies {nbodies} dynamic {ndyn} proxies {nprox}")
+    log(f"bodies {nbodies} dynamic {ndyn} proxies {nprox}; timed steps {window}")
 
     try:
         import torch
@@ -249,8 +313,13 @@ def main():
     except Exception:  # torch is only used for an extra device barrier
         have_torch = False
 
+    def barrier(h):
+        h.sync()
+        if have_torch:
+            torch.cuda.synchronize()
+
     # Warm-up run on its own copy of the worlds; its first 5 x 2 steps let the
-    # autotuner pick each tier's workgroup size, which the timed run reuses.
+    # autotuner pick each tier's workgroup size, which the timed runs reuse.
     h = build()
     if not args.no_autotune and args.warmup >= 10:
         h.autotune(2)
@@ -260,32 +329,37 @@ def main():
     blocks = [int(r[3]) for r in h.tiers()[0]]
     h.close()
 
-    # Timed run: steps 1..K of freshly built worlds.
-    h = build()
-    h.set_tier_blocks(blocks)
-    h.sync()
-    if have_torch:
-        torch.cuda.synchronize()
-    h.profile_totals(reset=True)
+    def fresh(graph=False):
+        """Freshly built worlds stepped untimed to the start of the window."""
+        handle = build()
+        handle.set_tier_blocks(blocks)
+        if start:
+            handle.step(DT, VEL, POS, flags, n=start)
+        handle.set_graph(graph)
+        barrier(handle)
+        handle.profile_totals(reset=True)
+        handle.phase_totals(reset=True)
+        return handle
+
+    # Timed run
+    h = fresh()
     t0 = time.perf_counter()
     for i in range(args.steps):
         h.step(DT, VEL, POS, flags)
         if (i + 1) % 100 == 0:
-            log(f"step {i + 1}/{args.steps}")
-    h.sync()
-    if have_torch:
-        torch.cuda.synchronize()
+            log(f"step {start + i + 1}")
+    barrier(h)
     elapsed = time.perf_counter() - t0
     prof, nsteps = h.profile_totals(reset=True)
+    kern, ksteps = h.phase_totals(reset=True)
     tot = h.totals()
     value = ndyn * args.steps / elapsed
 
-    # Per-phase times are those of the lead tier (the largest worlds, launched
-    # first on the handle's own stream); every other tier runs concurrently.
-    kern_ms = {"collide": prof["collide"], "islands": prof["solveInit"], "solve": prof["solveVelocity"],
-               "broadphase": prof["broadphase"]}
+    # Per-kernel times of the lead tier (the largest worlds, launched first on the
+    # handle's own stream); every other tier runs concurrently.
+    kern_ms = {k: kern[k] for k in ("collide", "islands", "solve", "broadphase")}
     dom = max(kern_ms, key=kern_ms.get)
-    avg_ms = kern_ms[dom] / max(nsteps, 1)
+    avg_ms = kern_ms[dom] / max(ksteps, 1)
     tier_rows, order = h.tiers()
     lead = [int(w) for w in order[tier_rows[0][0]:tier_rows[0][0] + tier_rows[0][1]]]
     lead_t = {"contacts": 0, "touching": 0}
@@ -300,54 +374,56 @@ def main():
     achieved = abytes / (avg_ms * 1e-3) / 1e9
     npairs = sum(h.pairs(w).shape[0] for w in range(len(worlds)))
     step_bytes = sum(algorithmic_bytes(k, tot, nbodies, ndyn, nprox, sbytes, npairs) for k in kern_ms)
-    log("lead-tier ms/step: " + ", ".join(f"{k} {v / max(nsteps, 1):.3f}" for k, v in kern_ms.items()) +
+    log("lead-tier ms/step: " + ", ".join(f"{k} {v / max(ksteps, 1):.3f}" for k, v in kern.items()) +
         f"; tiers {tier_rows.tolist()}")
-
-    # HBM bytes per launch of the lead-tier solver from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    # summarised in profiles/ (counters cannot be read from inside the timed run)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_solve_wave_lead_4096worlds_traffic.json")
-    if dom == "solve" and not args.coloured and not args.subset and len(worlds) == 4096 and os.path.exists(tpath):
+    # Latency model of the largest world's island solve: its dependency levels (the
+    # exact-order schedule's batches) and the time per level of each phase, from the
+    # phase stamps the solve kernel takes (b2Profile's solveInit / solveVelocity /
+    # solvePosition of that world).
+    sched = h.schedule(lead[0])
+    levels = int(sched[:, 3].max()) + 1 if len(sched) else 0
+    ph = {k: prof[k] / max(nsteps, 1) for k in ("solveInit", "solveVelocity", "solvePosition")}
+    latency = {
+        "world": lead[0], "constraints_last_step": int(len(sched)), "levels_per_sweep_last_step": levels,
+        "phase_ms_per_step": ph,
+        "us_per_velocity_level": 1e3 * ph["solveVelocity"] / max(levels * VEL, 1),
+        "note": "one wave walks the levels of the largest world in order; a level costs its instruction "
+                "chain (about 1,100 shader cycles per velocity level), not memory traffic",
+    } if not args.coloured else None
+    # HBM bytes per launch of the named kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # summarised in profiles/ for this workload and window (counters cannot be read inside the timed run)
+    traffic, traffic_src = None, None
+    kname = KERNEL_NAMES[dom]
+    tpath = os.path.join(ROOT, "profiles", f"{kname}_lead_4096worlds_steps{window}_traffic.json")
+    if not args.coloured and not args.subset and len(worlds) == 4096 and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes"]
-    toi_ms = prof["solveTOI"] / max(nsteps, 1)
+        traffic_src = os.path.relpath(tpath, ROOT)
+    toi_ms = kern["toi"] / max(ksteps, 1)
     toi_events = int(h.toi_events().sum())
     graph = None
     if not args.no_graph:
-        # the same steps 1..K once more on fresh worlds, each tier's step replayed
-        # from a captured hipGraph (captured during steps 1 and 2)
+        # the same window once more on fresh worlds, each tier's step replayed from a
+        # captured hipGraph (captured during the first two timed steps)
         h.close()
-        h = build()
-        h.set_tier_blocks(blocks)
-        h.set_graph(True)
-        h.sync()
-        if have_torch:
-            torch.cuda.synchronize()
+        h = fresh(graph=True)
         t0 = time.perf_counter()
         h.step(DT, VEL, POS, flags, n=args.steps)
-        h.sync()
-        if have_torch:
-            torch.cuda.synchronize()
+        barrier(h)
         tg = time.perf_counter() - t0
         graph = {"value": ndyn * args.steps / tg, "ms_per_step": 1e3 * tg / args.steps,
-                 "steps": f"1-{args.steps}, capture included"}
+                 "steps": f"{window}, capture included"}
     discrete = None
     if not args.no_ccd and not args.no_discrete:
-        # the same steps 1..K on fresh worlds with continuous physics off
+        # the same window on fresh worlds with continuous physics off in the window
         h.close()
-        h = build()
-        h.set_tier_blocks(blocks)
-        h.sync()
-        if have_torch:
-            torch.cuda.synchronize()
+        h = fresh()
         t0 = time.perf_counter()
         h.step(DT, VEL, POS, flags & ~native.CCD, n=args.steps)
-        h.sync()
-        if have_torch:
-            torch.cuda.synchronize()
+        barrier(h)
         td = time.perf_counter() - t0
         discrete = {"value": ndyn * args.steps / td, "ms_per_step": 1e3 * td / args.steps,
-                    "steps": f"1-{args.steps}, continuous physics off"}
+                    "steps": f"{window}, continuous physics off in the timed steps"}
     result = {
         "metric": "body-steps/s at 4096 worlds; velocity iters to fixed residual, learned vs plain",
         "value": value,
@@ -365,6 +441,7 @@ def main():
             "workload": f"{len(worlds)} independent 40x80 m container worlds, lognormal body counts (median 64, "
                         "sigma 0.8, clip 8-1024), 60% convex polygons 3-8 vertices / 40% circles",
             "worlds": len(worlds),
+            "timed_steps": window,
             "bodies": nbodies,
             "dynamic_bodies": ndyn,
             "dt": DT,
@@ -379,40 +456,48 @@ def main():
             "awake_end": tot["awake"],
         },
         "roofline": {
-            "kernel": ("k_solve" if args.coloured and dom == "solve" else KERNEL_NAMES[dom]) + "<true> (lead tier: %d largest worlds)" % len(lead),
+            "kernel": ("k_solve" if args.coloured and dom == "solve" else kname) + "<true> (lead tier: %d largest worlds)" % len(lead),
             "bound": "hbm",
+            "limiter": "latency: the dependency chain of the largest world" if dom in ("solve", "islands") else "latency",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": abytes,
             "avg_kernel_ms": avg_ms,
         },
-        "lead_tier_phase_ms_per_step": {k: v / max(nsteps, 1) for k, v in kern_ms.items()},
+        "latency_model": latency,
+        "lead_tier_kernel_ms_per_step": {k: v / max(ksteps, 1) for k, v in kern.items()},
+        "profile_ms_per_step": {k: v / max(nsteps, 1) for k, v in prof.items()},
         "tiers": [{"worlds": int(r[1]), "lds": int(r[2]), "block": int(r[3])} for r in tier_rows],
         "graph_replay": graph,
         "discrete_only": discrete,
         "continuous_pass": {"lead_tier_ms_per_step": toi_ms, "toi_events_last_step": toi_events},
         "step_algorithmic_gbs": step_bytes / (elapsed / args.steps) / 1e9,
     }
+    h.close()
     if not args.no_learned and not args.subset:
-        h.close()
         log("learned vs plain warm start")
         result["learned_vs_plain"] = learned_vs_plain(256, 150)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        dyn, t1, tn, nsample = run_cpu_baseline(worlds, args.steps, threads, args.cpu_sample_every, not args.no_ccd)
+        info = host_cpu()
+        threads = args.cpu_threads or cpu_threads(info)
+        log(f"cpu baseline: {threads} threads on {info['model']}")
+        c = run_cpu_baseline(worlds, start, args.steps, threads, args.cpu_sample_every, 4 * args.cpu_sample_every,
+                             not args.no_ccd)
         result["cpu_baseline"] = {
-            "value": dyn * args.steps / tn,
+            "value": c["dyn"] * args.steps / c["t_pool"],
             "unit": "body-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"every {args.cpu_sample_every}th world ({nsample} worlds, {dyn} dynamic bodies), the same "
-                      f"steps 1-{args.steps} with the same flags (continuous physics "
-                      f"{'off' if args.no_ccd else 'on'}), C++ restatement of dbox -O2 -ffp-contract=off, one world "
-                      "per task on a thread pool",
-            "single_thread_value": dyn * args.steps / t1,
+            "sample": f"every {args.cpu_sample_every}th world ({c['n']} worlds, {c['dyn']} dynamic bodies), the same "
+                      f"steps {window} with the same flags (continuous physics {'off' if args.no_ccd else 'on'}), C++ "
+                      "restatement of dbox, one world per task on a pool of every CPU this process is granted",
+            "single_thread_value": c["dyn1"] * args.steps / c["t_one"],
+            "single_thread_sample": f"every {4 * args.cpu_sample_every}th world ({c['n1']} worlds), same steps, one thread",
+            "host": info,
         }
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)

@@ -57,6 +57,7 @@ struct WorldDesc {
                               // bit 1: pairs come from the device-wide broadphase pass
     long long islScratchBase; // byte offset of the world's island-search scratch
     long long bpScratchBase;  // byte offset of the world's broadphase scratch
+    int jointBase, jointCount; // the world's joints [jointBase, jointBase + jointCount)
 };
 
 struct WorldState {
@@ -74,6 +75,7 @@ struct WorldState {
     int toiEvents;    // TOI events (sub-step solves) of the last step's continuous pass
     int adjContacts;  // contacts the last Collide kept, which its adjacency lists cover
     int toiCallbacks; // listener callbacks the last continuous pass made (may exceed its log's capacity)
+    int joints;       // joints the last island search added to islands
 };
 
 // One listener callback of the continuous pass, laid out as dbx_contact_event:
@@ -136,6 +138,8 @@ struct StepParams {
     float residualTol;
     int maxIterations;
 };
+
+struct JointRec; // dbx_joints.h
 
 // Flat device arrays of all worlds.
 struct Dev {
@@ -208,6 +212,13 @@ struct Dev {
     int4 *toiB;             // per body: (sweep alpha0 as float bits, island flag, head edge of that list, 0)
     const int *bfixOff, *bfixCnt; // per body: its fixtures' range in bfixList
     const int *bfixList;    // per world from fixBase: fixture indices grouped by body
+    // joints (b2Joint) of worlds that have them, from each world's jointBase
+    JointRec *joints;       // definitions, kept impulses and solver temporaries
+    int *ordJ;              // joints in island order (from jointBase)
+    int *islCoff, *islJoff; // per world from bodyBase + world: each island's first constraint and
+                            // first joint in island order, one past the last island at the end
+    const int *jadjOff, *jadjCnt; // per body: its joint edges' range in jadjList (from 2 * jointBase)
+    const int2 *jadjList;   // (2 * joint + side, other body) in b2Body.m_jointList order, newest first
     ToiCallback *toiLog;    // per world from toiLogOff[w]: the continuous pass's callbacks in order
     const int *toiLogOff;   // nworlds + 1 offsets into toiLog
 };

@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "dbx_device.h"
+#include "dbx_joints.h"
 #include "dbx_kernels.h"
 #include "dbx_toi.h"
 
@@ -662,6 +663,118 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
     }
 }
 
+// ---------------------------------------------------------------------------
+// Islands of a world with joints: the same depth-first search, with each popped
+// body's joint edges followed after its contact edges (b2world.d:1048-1075), on
+// one lane in global memory (joint worlds are small structures next to the
+// contact piles the wave-wide search is built for).  Outputs the constraints in
+// island order (ordC, cisl), the joints in island order (ordJ) and each island's
+// first constraint and first joint (islCoff, islJoff); the solve then runs each
+// island on a lane of its own in the reference's sequence.
+// ---------------------------------------------------------------------------
+__device__ void island_search_joints(const Dev &d, int cur, int w, const WorldDesc &wd) {
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const int nb = wd.bodyCount, m = d.S[w].contactCount;
+    const Contact *cont = d.cont[cur] + wd.contactBase;
+    uint32_t *gfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    int *bisl = d.bisl + wd.bodyBase;
+    int *ordC = d.ordC + wd.contactBase, *cisl = d.cisl + wd.contactBase;
+    int *cflag = d.cbatch + wd.contactBase; // contact already in an island (b2Contact.e_islandFlag)
+    int *stack = d.bscratch + wd.bodyBase;
+    JointRec *J = d.joints + wd.jointBase;
+    int *ordJ = d.ordJ + wd.jointBase;
+    int *coff = d.islCoff + wd.bodyBase + w, *joff = d.islJoff + wd.bodyBase + w;
+    const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
+    const int2 *adj = d.adjList + 2 * wd.contactBase;
+    const int *jOff = d.jadjOff + wd.bodyBase, *jCnt = d.jadjCnt + wd.bodyBase;
+    const int2 *jadj = d.jadjList + 2 * wd.jointBase;
+    // flags reset (b2world.d:943-958)
+    for (int b = tid; b < nb; b += BS) bisl[b] = -1;
+    for (int c = tid; c < m; c += BS) cflag[c] = 0;
+    for (int j = tid; j < wd.jointCount; j += BS) ordJ[j] = -1;
+    __syncthreads();
+    if (tid == 0) {
+        int nc = 0, nj = 0, nisl = 0;
+        for (int seed = nb - 1; seed >= 0; --seed) { // body list order: newest body first
+            if (bisl[seed] >= 0) continue;
+            const uint32_t f = gfl[seed];
+            if (!(f & kBodyAwake) || body_type(f) == kStatic) continue;
+            coff[nisl] = nc;
+            joff[nisl] = nj;
+            int sp = 0;
+            stack[sp++] = seed;
+            bisl[seed] = nisl;
+            while (sp > 0) {
+                const int b = stack[--sp];
+                if (body_type(gfl[b]) == kStatic) continue;
+                const int o = adjOff[b], n = adjCnt[b];
+                for (int e = 0; e < n; ++e) {
+                    const int2 ent = adj[o + e];
+                    const int k = ent.x >> 1, other = ent.y;
+                    if (cflag[k]) continue;
+                    const uint32_t cf = cont[k].flags;
+                    if (!((cf & kContactTouching) && (cf & kContactEnabled)) || (cf & kContactSensor)) continue;
+                    ordC[nc] = k;
+                    cisl[nc] = nisl;
+                    ++nc;
+                    cflag[k] = 1;
+                    if (bisl[other] >= 0 || body_type(gfl[other]) == kStatic) continue;
+                    stack[sp++] = other;
+                    bisl[other] = nisl;
+                }
+                const int jo = jOff[b], jn = jCnt[b];
+                for (int e = 0; e < jn; ++e) {
+                    const int2 ent = jadj[jo + e];
+                    const int ji = ent.x >> 1, other = ent.y;
+                    if (J[ji].flags & kJointInIsland) continue;
+                    ordJ[nj++] = ji;
+                    J[ji].flags |= kJointInIsland;
+                    if (bisl[other] >= 0 || body_type(gfl[other]) == kStatic) continue;
+                    stack[sp++] = other;
+                    bisl[other] = nisl;
+                }
+            }
+            ++nisl;
+        }
+        coff[nisl] = nc;
+        joff[nisl] = nj;
+        d.S[w].islandCount = nisl;
+        d.S[w].batchCount = 0;
+        d.S[w].constraints = nc;
+        d.S[w].joints = nj;
+    }
+    __syncthreads();
+    // b2Body.SetAwake(true) for every body the search reached; island flags of joints cleared
+    for (int b = tid; b < nb; b += BS)
+        if (bisl[b] >= 0) wake_body(gfl, bpos, b);
+    for (int j = tid; j < wd.jointCount; j += BS) J[j].flags &= ~kJointInIsland;
+    const int nc = d.S[w].constraints;
+    for (int i = tid; i < nc; i += BS) cflag[i] = 0; // batch 0 for every constraint
+}
+
+// b2Body.ShouldCollide's joint test (b2body.d:1157-1167): a joint between the two
+// bodies whose collideConnected is false keeps them from colliding.
+struct JointAdj {
+    const int *off, *cnt;
+    const int2 *list;
+    const JointRec *rec;
+    int count;
+};
+__device__ __forceinline__ JointAdj joint_adj(const Dev &d, const WorldDesc &wd) {
+    return JointAdj{d.jadjOff + wd.bodyBase, d.jadjCnt + wd.bodyBase, d.jadjList + 2 * wd.jointBase,
+                    d.joints + wd.jointBase, wd.jointCount};
+}
+__device__ __forceinline__ bool joint_blocks(const JointAdj &A, int ba, int bb) {
+    if (A.count == 0) return false;
+    const int o = A.off[ba], n = A.cnt[ba];
+    for (int k = 0; k < n; ++k) {
+        const int2 e = A.list[o + k];
+        if (e.y == bb && !(A.rec[e.x >> 1].flags & kJointCollide)) return true;
+    }
+    return false;
+}
+
 template <bool kLead>
 __global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
@@ -669,7 +782,9 @@ __global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int
     const WorldDesc wd = d.W[w];
     const int nb = wd.bodyCount;
     const int m = d.S[w].contactCount;
-    if (isl_bytes(nb, m) <= (size_t)ldsBytes)
+    if (wd.jointCount > 0)
+        island_search_joints(d, cur, w, wd);
+    else if (isl_bytes(nb, m) <= (size_t)ldsBytes)
         island_search(d, cur, ordered, worldOffset, w, wd, nb, m, isl_carve(lds, nb, m));
     else
         island_search(d, cur, ordered, worldOffset, w, wd, nb, m, isl_carve(d.islScratch + wd.islScratchBase, nb, m));
@@ -1259,7 +1374,22 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     const ImpRef gimp{reinterpret_cast<float4 *>(&vc[0].nImp0), (int)(sizeof(VelCon) / sizeof(float4))};
     if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, gimp, bvel, bpos);
 
-    if (sp.flags & 1u) {
+    // A world with joints solves each island on a lane of its own, in the sequence of
+    // b2Island.Solve (b2island.d:137-224): contact warm start, joint init, then per
+    // velocity iteration the joints before the contacts, per position iteration the
+    // contacts before the joints.
+    const bool jointWorld = wd.jointCount > 0;
+    const int *icoff = d.islCoff + wd.bodyBase + w, *ijoff = d.islJoff + wd.bodyBase + w;
+    JointRec *J = d.joints + wd.jointBase;
+    const int *ordJ = d.ordJ + wd.jointBase;
+    const JointStep jst{sp.dt, sp.inv_dt, sp.dtRatio, (sp.flags & 1u) != 0};
+    if (jointWorld) {
+        for (int isl = tid; isl < nisl; isl += BS) {
+            if (sp.flags & 1u)
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) warm_start(vc[k], bvel);
+            for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_init(J[ordJ[q]], bpos, bvel, bmass, bparam, jst);
+        }
+    } else if (sp.flags & 1u) {
         for (int bt = 0; bt < nbatch; ++bt) {
             int e = boff[bt + 1];
             for (int j = boff[bt] + tid; j < e; j += BS) warm_start(vc[sched[j]], bvel);
@@ -1279,6 +1409,18 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
         if (tid == 0) s_res = 0.0f;
         __syncthreads();
         float lres = 0.0f;
+        if (jointWorld) {
+            for (int isl = tid; isl < nisl; isl += BS) {
+                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_solve_velocity(J[ordJ[q]], bvel, jst);
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
+                    VelCon v = vc[k];
+                    float r = solve_velocity(v, bvel, residualMode);
+                    lres = fmaxr(lres, r);
+                    *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+                }
+            }
+            __syncthreads();
+        }
         for (int bt = 0; bt < nbatch; ++bt) {
             int e = boff[bt + 1];
             for (int j = boff[bt] + tid; j < e; j += BS) {
@@ -1345,7 +1487,31 @@ __device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, 
     prof.at(kProfPos0);
 
     // Position iterations with the per-island early exit (b2island.d:206-224).
-    for (int it = 0; it < sp.positionIterations; ++it) {
+    if (jointWorld) {
+        for (int isl = tid; isl < nisl; isl += BS) {
+            int solved = 0;
+            for (int it = 0; it < sp.positionIterations; ++it) {
+                float minSep = 0.0f;
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
+                    const PosCon p = pc[k];
+                    minSep = fminr(minSep, solve_position(p, p.indexA, p.indexB, p.dynMask, bpos));
+                }
+                const bool contactsOkay = minSep >= -3.0f * kLinearSlop;
+                bool jointsOkay = true;
+                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) {
+                    const bool jointOkay = joint_solve_position(J[ordJ[q]], bpos);
+                    jointsOkay = jointsOkay && jointOkay;
+                }
+                if (contactsOkay && jointsOkay) {
+                    solved = 1;
+                    break;
+                }
+            }
+            islSolved[isl] = solved;
+        }
+        __syncthreads();
+    }
+    for (int it = 0; it < (jointWorld ? 0 : sp.positionIterations); ++it) {
         for (int i = tid; i < nisl; i += BS) islMin[i] = f2ord(0.0f);
         __syncthreads();
         for (int bt = 0; bt < nbatch; ++bt) {
@@ -1625,7 +1791,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     const int nc = d.S[w].constraints;
     const int nbatch = d.S[w].batchCount;
     const int nisl = d.S[w].islandCount;
-    if (solve_lds_bytes(nb, nc, nbatch, nisl) > (size_t)ldsBytes) {
+    if (wd.jointCount > 0 || solve_lds_bytes(nb, nc, nbatch, nisl) > (size_t)ldsBytes) {
         solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0);
         return;
     }
@@ -2080,6 +2246,7 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
     const Shape *shp = d.fshape + wd.fixBase;
     const float4 *fmat = d.fmat + wd.fixBase;
     const int2 *ffilter = d.ffilter + wd.fixBase;
+    const JointAdj jadj = joint_adj(d, wd);
     uint32_t *bfl = d.bflags + wd.bodyBase;
     float4 *bpos = d.bpos + wd.bodyBase;
     Contact *cont = d.cont[cur] + wd.contactBase;
@@ -2134,8 +2301,9 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
                     }
                 }
             }
-            if (accept) { // b2Body.ShouldCollide: at least one body dynamic
-                accept = body_type(bfl[ba]) == kDynamic || body_type(bfl[bb]) == kDynamic;
+            if (accept) { // b2Body.ShouldCollide: at least one body dynamic, no joint that forbids it
+                accept = (body_type(bfl[ba]) == kDynamic || body_type(bfl[bb]) == kDynamic) &&
+                         !joint_blocks(jadj, bb, ba);
             }
             if (accept) { // b2ContactFilter.ShouldCollide (b2worldcallbacks.d:52-65)
                 int2 A = ffilter[fa], B = ffilter[fb];
@@ -2239,6 +2407,7 @@ struct ToiWorld {
     const int *bfixOff, *bfixCnt, *bfixList;
     ToiCallback *log; // the world's listener callbacks of this pass, logCap of them
     int logCap;
+    JointAdj ja;      // joints that may forbid a pair (b2Body.ShouldCollide)
 };
 // Lists kept in LDS by the lane that runs the events: the bodies an event woke,
 // the proxies it moved, and the contacts the next search must look at again
@@ -2627,6 +2796,7 @@ __device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int
     });
     if (exists) return false;
     if (!(body_type(W.bfl[ba]) == kDynamic || body_type(W.bfl[bb]) == kDynamic)) return false;
+    if (joint_blocks(W.ja, bb, ba)) return false;
     const int2 A = W.ffilter[fa], B = W.ffilter[fb];
     bool accept;
     if (A.y == B.y && A.y != 0) accept = A.y > 0;
@@ -2693,6 +2863,7 @@ __device__ __forceinline__ ToiWorld toi_world(const Dev &d, int cur, int w, cons
     W.bfixList = d.bfixList + wd.fixBase;
     W.log = d.toiLog ? d.toiLog + d.toiLogOff[w] : nullptr;
     W.logCap = d.toiLog ? d.toiLogOff[w + 1] - d.toiLogOff[w] : 0;
+    W.ja = joint_adj(d, wd);
     return W;
 }
 

@@ -18,6 +18,7 @@
 
 #include "../../include/dbox_hip.h"
 #include "dbx_device.h"
+#include "dbx_joints.h"
 #include "dbx_kernels.h"
 #include "dbx_mlp.h"
 
@@ -328,7 +329,82 @@ struct WorldHost {
     std::vector<int2> ffilter;
     int dynamicCount = 0;
     float gx = 0.0f, gy = -10.0f;
+    std::vector<JointRec> joints; // b2World.CreateJoint order
 };
+
+// b2World.CreateJoint and the constructor of each b2Joint subclass: the members each
+// sets from its definition (b2*joint.d), with body B's transform for the mouse
+// joint's anchor.
+bool build_joint(const dbx_joint_def &d, const WorldHost &W, JointRec &j, std::string &err) {
+    const int nb = (int)W.bflags.size();
+    if (d.type == 6) {
+        err = "the gear joint is not supported";
+        return false;
+    }
+    if (d.type < DBX_JOINT_REVOLUTE || d.type > DBX_JOINT_MOTOR) {
+        err = "bad joint type";
+        return false;
+    }
+    if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) {
+        err = "bad joint bodies";
+        return false;
+    }
+    const float *fl = &d.local_anchor_a[0];
+    for (int i = 0; i < 29; ++i)
+        if (!finite(fl[i])) {
+            err = "joint definition is not finite";
+            return false;
+        }
+    if (d.type == DBX_JOINT_PULLEY && d.ratio == 0.0f) {
+        err = "pulley ratio is zero";
+        return false;
+    }
+    j = JointRec{};
+    j.type = d.type;
+    j.bodyA = d.body_a;
+    j.bodyB = d.body_b;
+    j.flags = (d.collide_connected ? kJointCollide : 0) | (d.enable_limit ? kJointLimit : 0) |
+              (d.enable_motor ? kJointMotorOn : 0);
+    j.localAnchorA = make_float2(d.local_anchor_a[0], d.local_anchor_a[1]);
+    j.localAnchorB = make_float2(d.local_anchor_b[0], d.local_anchor_b[1]);
+    j.referenceAngle = d.reference_angle;
+    j.length = d.length;
+    j.frequencyHz = d.frequency_hz;
+    j.dampingRatio = d.damping_ratio;
+    j.lower = d.lower;
+    j.upper = d.upper;
+    j.motorSpeed = d.motor_speed;
+    j.maxMotor = d.max_motor;
+    j.maxForce = d.max_force;
+    j.maxTorque = d.max_torque;
+    if (d.type == DBX_JOINT_PRISMATIC || d.type == DBX_JOINT_WHEEL) {
+        HV x = hv(d.local_axis_a[0], d.local_axis_a[1]);
+        if (d.type == DBX_JOINT_PRISMATIC) hnormalize(x); // b2prismaticjoint.d:174; the wheel keeps it as given
+        j.localXAxisA = make_float2(x.x, x.y);
+        j.localYAxisA = make_float2(-1.0f * x.y, 1.0f * x.x); // b2Cross(1.0f, axis)
+    }
+    if (d.type == DBX_JOINT_MOTOR) {
+        j.linearOffset = make_float2(d.linear_offset[0], d.linear_offset[1]);
+        j.angularOffset = d.angular_offset;
+        j.correctionFactor = d.correction_factor;
+    }
+    if (d.type == DBX_JOINT_MOUSE) { // m_localAnchorB = b2MulT(bodyB.GetTransform(), target) (b2mousejoint.d:124)
+        j.targetA = make_float2(d.target[0], d.target[1]);
+        const float4 x = W.bxf[d.body_b];
+        const float px = d.target[0] - x.x, py = d.target[1] - x.y;
+        j.localAnchorB = make_float2(x.w * px + x.z * py, -x.z * px + x.w * py);
+    }
+    if (d.type == DBX_JOINT_PULLEY) {
+        j.groundAnchorA = make_float2(d.ground_anchor_a[0], d.ground_anchor_a[1]);
+        j.groundAnchorB = make_float2(d.ground_anchor_b[0], d.ground_anchor_b[1]);
+        j.lengthA = d.length_a;
+        j.lengthB = d.length_b;
+        j.ratio = d.ratio;
+        j.constant = d.length_a + j.ratio * d.length_b; // b2pulleyjoint.d:151
+    }
+    j.limitState = kLimitInactive;
+    return true;
+}
 
 // b2World.CreateBody + b2Body.CreateFixture for a whole definition list.
 bool build_world(const dbx_body_def *bodies, int nb, const dbx_fixture_def *fx, int nf, WorldHost &W, std::string &err) {
@@ -530,6 +606,8 @@ struct dbx_handle {
     unsigned long long profSeen = 0; // phase-stamp slots of the solve kernels already read
     double wallKHz = 100000.0;      // rate of the kernels' wall_clock64 stamps
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
+    double phaseSum[5] = {0, 0, 0, 0, 0}; // lead tier: collide, islands, solve, broadphase, TOI (eager steps)
+    int phaseSteps = 0;
     int sumSteps = 0;
     long long totalBodies = 0, totalFix = 0, totalContacts = 0, totalPairs = 0;
     float *dPen = nullptr;
@@ -667,6 +745,21 @@ dbx_status dbx_add_world(dbx_handle *h, const dbx_body_def *bodies, int32_t nb, 
     return DBX_OK;
 }
 
+dbx_status dbx_add_joints(dbx_handle *h, int32_t world, const dbx_joint_def *joints, int32_t n) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (h->finalized) return fail(h, DBX_E_LOCKED, "joints are added before dbx_finalize");
+    if (world < 0 || world >= (int)h->pending.size()) return fail(h, DBX_E_INVALID_ARG, "bad world index");
+    if (n < 0 || (n > 0 && !joints)) return fail(h, DBX_E_INVALID_ARG, "bad joint array");
+    WorldHost &W = h->pending[world];
+    std::vector<JointRec> add((size_t)n);
+    std::string err;
+    for (int i = 0; i < n; ++i)
+        if (!build_joint(joints[i], W, add[i], err))
+            return fail(h, joints[i].type == 6 ? DBX_E_UNSUPPORTED : DBX_E_INVALID_ARG, err);
+    W.joints.insert(W.joints.end(), add.begin(), add.end());
+    return DBX_OK;
+}
+
 dbx_status dbx_finalize(dbx_handle *h) {
     if (!h) return DBX_E_INVALID_ARG;
     if (h->finalized) return fail(h, DBX_E_STATE, "already finalized");
@@ -700,6 +793,8 @@ dbx_status dbx_finalize(dbx_handle *h) {
             for (int b : H.fbody)
                 if (body_type(H.bflags[b]) != kStatic && ++nfix[b] > 1) D.pad |= 1;
         }
+        D.jointBase = w == 0 ? 0 : h->W[w - 1].jointBase + h->W[w - 1].jointCount;
+        D.jointCount = (int)H.joints.size();
         D.islScratchBase = islBytes;
         islBytes += (long long)islands_scratch_bytes(D.bodyCount, D.contactCap);
         D.bpScratchBase = bpBytes;
@@ -925,6 +1020,52 @@ dbx_status dbx_finalize(dbx_handle *h) {
         DBX_HIP(h, dalloc(h, &d.toiLog, logOff[nw]));
         d.toiLogOff = p_off;
         h->toiLogOff = logOff;
+    }
+    // Joints: records (definitions and kept state) from each world's jointBase, and
+    // each body's joint edges in b2Body.m_jointList order (CreateJoint prepends, so the
+    // newest joint comes first), offsets relative to the world's 2 * jointBase.
+    {
+        const long long nj = (long long)h->W[nw - 1].jointBase + h->W[nw - 1].jointCount;
+        std::vector<JointRec> jr;
+        jr.reserve((size_t)nj);
+        std::vector<int> joff((size_t)nb, 0), jcnt((size_t)nb, 0);
+        std::vector<int2> jlist((size_t)std::max<long long>(2 * nj, 1));
+        for (int w = 0; w < nw; ++w) {
+            const WorldDesc &D = h->W[w];
+            const std::vector<JointRec> &J = h->pending[w].joints;
+            jr.insert(jr.end(), J.begin(), J.end());
+            for (const JointRec &j : J) {
+                jcnt[D.bodyBase + j.bodyA]++;
+                jcnt[D.bodyBase + j.bodyB]++;
+            }
+            int run = 0;
+            for (int b = 0; b < D.bodyCount; ++b) {
+                joff[D.bodyBase + b] = run;
+                run += jcnt[D.bodyBase + b];
+            }
+            std::vector<int> fill(D.bodyCount, 0);
+            for (int k = (int)J.size() - 1; k >= 0; --k) {
+                const int a = J[k].bodyA, b = J[k].bodyB;
+                jlist[2 * (size_t)D.jointBase + joff[D.bodyBase + a] + fill[a]++] = make_int2(2 * k, b);
+                jlist[2 * (size_t)D.jointBase + joff[D.bodyBase + b] + fill[b]++] = make_int2(2 * k + 1, a);
+            }
+        }
+        int *p_joff, *p_jcnt;
+        int2 *p_jlist;
+        DBX_HIP(h, dalloc(h, &d.joints, (size_t)std::max<long long>(nj, 1)));
+        if (nj) DBX_HIP(h, upload(h, d.joints, jr));
+        DBX_HIP(h, dalloc(h, &d.ordJ, (size_t)std::max<long long>(nj, 1)));
+        DBX_HIP(h, dalloc(h, &p_joff, nb));
+        DBX_HIP(h, dalloc(h, &p_jcnt, nb));
+        DBX_HIP(h, dalloc(h, &p_jlist, jlist.size()));
+        DBX_HIP(h, upload(h, p_joff, joff));
+        DBX_HIP(h, upload(h, p_jcnt, jcnt));
+        DBX_HIP(h, upload(h, p_jlist, jlist));
+        d.jadjOff = p_joff;
+        d.jadjCnt = p_jcnt;
+        d.jadjList = p_jlist;
+        DBX_HIP(h, dalloc(h, &d.islCoff, nb + nw));
+        DBX_HIP(h, dalloc(h, &d.islJoff, nb + nw));
     }
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
@@ -1345,6 +1486,10 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->sum.broadphase += p.broadphase;
         h->sum.solveTOI += p.solveTOI;
         h->sumSteps++;
+        if (!h->ringGraph[k]) {
+            for (int i = 0; i < 5; ++i) h->phaseSum[i] += ms[i];
+            h->phaseSteps++;
+        }
     }
     h->ringUsed = 0;
     if (h->tierSteps > 0) {
@@ -1665,6 +1810,31 @@ dbx_status dbx_read_contacts(dbx_handle *h, int32_t world, dbx_contact *out, int
     return DBX_OK;
 }
 
+dbx_status dbx_read_joints(dbx_handle *h, int32_t world, float *out, int32_t n) {
+    dbx_status st = check_world(h, world);
+    if (st) return st;
+    const WorldDesc &D = h->W[world];
+    if (n != D.jointCount || (n > 0 && !out)) return fail(h, DBX_E_INVALID_ARG, "n must be the world's joint count");
+    if (n == 0) return DBX_OK;
+    std::vector<JointRec> J(n);
+    DBX_HIP(h, hipStreamSynchronize(h->stream));
+    DBX_HIP(h, hipMemcpy(J.data(), h->d.joints + D.jointBase, n * sizeof(JointRec), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        const JointRec &j = J[i];
+        float *o = out + 8 * i;
+        const bool lin = j.type == DBX_JOINT_FRICTION || j.type == DBX_JOINT_MOTOR || j.type == DBX_JOINT_MOUSE;
+        o[0] = lin ? j.linearImpulse.x : j.ix;
+        o[1] = lin ? j.linearImpulse.y : j.iy;
+        o[2] = lin ? j.angularImpulse : j.iz;
+        o[3] = j.motorImpulse;
+        o[4] = j.springImpulse;
+        o[5] = (float)j.limitState;
+        o[6] = 0.0f;
+        o[7] = 0.0f;
+    }
+    return DBX_OK;
+}
+
 dbx_status dbx_read_fat_aabbs(dbx_handle *h, int32_t world, float *out, int32_t n) {
     dbx_status st = check_world(h, world);
     if (st) return st;
@@ -1701,6 +1871,17 @@ dbx_status dbx_profile_totals(dbx_handle *h, dbx_profile *sum, int32_t *steps, i
     if (reset) {
         h->sum = dbx_profile{};
         h->sumSteps = 0;
+    }
+    return DBX_OK;
+}
+
+dbx_status dbx_phase_totals(dbx_handle *h, float *ms5, int32_t *steps, int32_t reset) {
+    if (!h || !ms5 || !steps) return DBX_E_INVALID_ARG;
+    for (int i = 0; i < 5; ++i) ms5[i] = (float)h->phaseSum[i];
+    *steps = h->phaseSteps;
+    if (reset) {
+        for (double &x : h->phaseSum) x = 0.0;
+        h->phaseSteps = 0;
     }
     return DBX_OK;
 }

@@ -68,9 +68,9 @@ _lib = None
 
 # Every entry point declared in include/dbox_hip.h.
 EXPORTS = [
-    "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_finalize", "dbx_step", "dbx_sync",
+    "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_add_joints", "dbx_read_joints", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
-    "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
+    "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_phase_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
     "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
     "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events", "dbx_write_contact_settings", "dbx_step_finish", "dbx_read_proxies",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
@@ -97,6 +97,8 @@ def lib():
         L.dbx_last_error.restype = C.c_char_p
         L.dbx_add_world.argtypes = [vp, vp, i32, vp, i32, f32, f32, C.POINTER(i32)]
         L.dbx_finalize.argtypes = [vp]
+        L.dbx_add_joints.argtypes = [vp, i32, vp, i32]
+        L.dbx_read_joints.argtypes = [vp, i32, vp, i32]
         L.dbx_step.argtypes = [vp, f32, i32, i32, u32]
         L.dbx_sync.argtypes = [vp]
         L.dbx_world_count.argtypes = [vp, C.POINTER(i32)]
@@ -109,6 +111,7 @@ def lib():
         L.dbx_profile_read.argtypes = [vp, C.POINTER(DbxProfile)]
         L.dbx_profile_totals.argtypes = [vp, C.POINTER(DbxProfile), C.POINTER(i32), i32]
         L.dbx_totals.argtypes = [vp, vp]
+        L.dbx_phase_totals.argtypes = [vp, vp, C.POINTER(i32), i32]
         L.dbx_read_solver_stats.argtypes = [vp, vp, vp, i32]
         L.dbx_broadphase_pairs.argtypes = [vp, vp, vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_collide_pairs.argtypes = [vp, vp, vp, vp, vp, i32, vp]
@@ -163,6 +166,7 @@ class StepWorlds:
         self.body_counts: list[int] = []
         self.fixture_counts: list[int] = []
         self.dynamic_counts: list[int] = []
+        self.joint_counts: list[int] = []
 
     def close(self):
         if getattr(self, "h", None):
@@ -185,7 +189,19 @@ class StepWorlds:
         self.body_counts.append(len(b))
         self.fixture_counts.append(scenes.proxy_count(f))  # read-backs are per proxy
         self.dynamic_counts.append(int(np.count_nonzero(b["type"] == 2)))
+        joints = getattr(scene, "joints", None)
+        self.joint_counts.append(0 if joints is None else len(joints))
+        if joints is not None and len(joints):
+            j = np.ascontiguousarray(joints, scenes.JOINT_DTYPE)
+            self._check(self.L.dbx_add_joints(self.h, wid.value, _p(j), len(j)))
         return wid.value
+
+    def joints(self, world=0) -> np.ndarray:
+        """Per joint: kept impulses (3), motor impulse, spring impulse, limit state, 0, 0."""
+        n = self.joint_counts[world]
+        out = np.zeros((n, 8), np.float32)
+        self._check(self.L.dbx_read_joints(self.h, world, _p(out) if n else None, n))
+        return out
 
     def finalize(self):
         self._check(self.L.dbx_finalize(self.h))
@@ -262,6 +278,14 @@ class StepWorlds:
         n = C.c_int32()
         self._check(self.L.dbx_profile_totals(self.h, C.byref(p), C.byref(n), int(reset)))
         return {k: getattr(p, k) for k, _ in DbxProfile._fields_}, n.value
+
+    def phase_totals(self, reset=True):
+        """Lead-tier kernel milliseconds summed over the eager steps since the last reset:
+        (collide, islands, solve, broadphase, toi), steps."""
+        ms = np.zeros(5, np.float32)
+        n = C.c_int32()
+        self._check(self.L.dbx_phase_totals(self.h, _p(ms), C.byref(n), int(reset)))
+        return dict(zip(("collide", "islands", "solve", "broadphase", "toi"), ms.astype(float))), n.value
 
     def write_bodies(self, world, states, forces=None):
         """Overwrite body poses, velocities, sleep times, awake bits and (optionally) force/torque."""

@@ -30,7 +30,21 @@ FIXTURE_DTYPE = np.dtype([
     ("category", "<u2"), ("mask", "<u2"), ("group", "<i2"), ("pad", "<i2"),
     ("radius", "<f4"), ("p", "<f4", (20,)),
 ])
-assert BODY_DTYPE.itemsize == 56 and FIXTURE_DTYPE.itemsize == 120
+# Joint definition (b2JointDef and the concrete b2*JointDef, src/dbox/dynamics/joints/):
+# the byte layout of dbx_joint_def in include/dbox_hip.h.
+JOINT_DTYPE = np.dtype([
+    ("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
+    ("enable_limit", "<i4"), ("enable_motor", "<i4"), ("pad", "<i4", (2,)),
+    ("local_anchor_a", "<f4", (2,)), ("local_anchor_b", "<f4", (2,)), ("local_axis_a", "<f4", (2,)),
+    ("reference_angle", "<f4"), ("length", "<f4"), ("frequency_hz", "<f4"), ("damping_ratio", "<f4"),
+    ("lower", "<f4"), ("upper", "<f4"), ("motor_speed", "<f4"), ("max_motor", "<f4"),
+    ("max_force", "<f4"), ("max_torque", "<f4"), ("linear_offset", "<f4", (2,)), ("angular_offset", "<f4"),
+    ("correction_factor", "<f4"), ("ground_anchor_a", "<f4", (2,)), ("ground_anchor_b", "<f4", (2,)),
+    ("length_a", "<f4"), ("length_b", "<f4"), ("ratio", "<f4"), ("target", "<f4", (2,)),
+])
+assert BODY_DTYPE.itemsize == 56 and FIXTURE_DTYPE.itemsize == 120 and JOINT_DTYPE.itemsize == 148
+# b2JointType (b2joint.d:28-41); the gear joint (6) is not supported
+REVOLUTE, PRISMATIC, DISTANCE, PULLEY, MOUSE, WHEEL, WELD, FRICTION, ROPE, MOTOR = 1, 2, 3, 4, 5, 7, 8, 9, 10, 11
 
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX, SHAPE_CHAIN = 0, 1, 2, 3, 4
@@ -92,6 +106,7 @@ class Scene:
     gravity: tuple = (0.0, -10.0)
     name: str = ""
     meta: dict = field(default_factory=dict)
+    joints: np.ndarray | None = None  # JOINT_DTYPE records in b2World.CreateJoint order
 
     @property
     def dynamic_count(self) -> int:
@@ -104,6 +119,26 @@ def body_defs(n: int) -> np.ndarray:
     b["allow_sleep"] = 1
     b["awake"] = 1
     return b
+
+
+def joint_defs(n: int, kind: int = 0) -> np.ndarray:
+    """n joint records with the reference's definition defaults: b2DistanceJointDef
+    length 1, b2PrismaticJointDef / b2WheelJointDef axis (1, 0), b2WheelJointDef
+    frequency 2 Hz and damping 0.7, b2PulleyJointDef ratio 1, b2MouseJointDef max
+    force 0, frequency 5 Hz and damping 0.7, b2MotorJointDef max force 1, max
+    torque 1, correction factor 0.3."""
+    j = np.zeros(n, JOINT_DTYPE)
+    j["type"] = kind
+    j["length"] = 1.0
+    j["local_axis_a"] = (1.0, 0.0)
+    j["ratio"] = 1.0
+    if kind == WHEEL:
+        j["frequency_hz"], j["damping_ratio"] = 2.0, 0.7
+    if kind == MOUSE:
+        j["frequency_hz"], j["damping_ratio"] = 5.0, 0.7
+    if kind == MOTOR:
+        j["max_force"], j["max_torque"], j["correction_factor"] = 1.0, 1.0, 0.3
+    return j
 
 
 def fixture_defs(n: int) -> np.ndarray:
@@ -444,3 +479,131 @@ def sensor_container(n: int = 150, seed: int = 61, every: int = 9) -> Scene:
     z["flags"] = SENSOR
     f["flags"][3::every] |= SENSOR
     return Scene(b, f, name=f"sensors{n}", meta=dict(s.meta))
+
+
+# --------------------------------------------------------------------------
+# Joints: a world with every supported joint kind next to contacts.
+# --------------------------------------------------------------------------
+def joint_world(seed: int = 1, debris: int = 12) -> Scene:
+    """A 80 m wide ground edge carrying: a suspension bridge of 12 planks on revolute
+    joints anchored to the ground body at both ends, with debris dropped on it; a car
+    (chassis on two wheel joints with suspension springs, rear wheel motor); a chain
+    of 5 balls on distance joints hanging from an anchor; a box on a rope; an L of
+    three welded boxes; an elevator on a prismatic joint with limits and a motor; two
+    boxes on a pulley; a box slid along the ground against a friction joint; a box
+    driven to an offset by a motor joint; a box held by a mouse joint; and a revolute
+    pendulum with a limit.  Positions, densities and speeds are seeded."""
+    rng = Stream(config_seed(7) ^ seed)
+    bodies, fixtures, joints = [], [], []
+
+    def body(t, x, y, a=0.0, vx=0.0, vy=0.0, w=0.0):
+        b = body_defs(1)[0]
+        b["type"], b["px"], b["py"], b["angle"], b["vx"], b["vy"], b["w"] = t, x, y, a, vx, vy, w
+        bodies.append(b)
+        return len(bodies) - 1
+
+    def box(bi, hx, hy, density=1.0, friction=0.6):
+        f = fixture_defs(1)[0]
+        f["body"], f["shape"], f["density"], f["friction"] = bi, SHAPE_BOX, density, friction
+        f["p"][:2] = (hx, hy)
+        fixtures.append(f)
+
+    def ball(bi, r, density=1.0, friction=0.6):
+        f = fixture_defs(1)[0]
+        f["body"], f["shape"], f["radius"], f["density"], f["friction"] = bi, SHAPE_CIRCLE, r, density, friction
+        fixtures.append(f)
+
+    def joint(kind, a, b, **kw):
+        j = joint_defs(1, kind)[0]
+        j["body_a"], j["body_b"] = a, b
+        for k, v in kw.items():
+            j[k] = v
+        joints.append(j)
+
+    u = lambda lo, hi: float(rng.uniform(1, lo, hi)[0])
+    g = body(STATIC, 0.0, 0.0)
+    f = fixture_defs(1)[0]
+    _edge(f, g, -40.0, 0.0, 40.0, 0.0, 0.6)
+    fixtures.append(f)
+    # bridge: planks 1 m long between x = -30 and -18 at y = 6
+    prev, x0 = g, -30.0
+    for i in range(12):
+        p = body(DYNAMIC, x0 + 0.5 + i, 6.0)
+        box(p, 0.5, 0.125, density=u(15.0, 25.0), friction=0.2)
+        anchor = (x0 + i, 6.0)
+        la = (anchor[0], anchor[1]) if prev == g else (0.5, 0.0)
+        joint(REVOLUTE, prev, p, local_anchor_a=la, local_anchor_b=(-0.5, 0.0))
+        prev = p
+    joint(REVOLUTE, prev, g, local_anchor_a=(0.5, 0.0), local_anchor_b=(x0 + 12.0, 6.0))
+    for i in range(debris):
+        d = body(DYNAMIC, u(-29.0, -19.0), u(8.0, 14.0), u(0.0, 6.28))
+        (box(d, u(0.2, 0.5), u(0.2, 0.5)) if i % 2 else ball(d, u(0.2, 0.5)))
+    # car
+    ch = body(DYNAMIC, -8.0, 1.2)
+    box(ch, 1.5, 0.3, density=1.0)
+    for k, xw in enumerate((-9.0, -7.0)):
+        wh = body(DYNAMIC, xw, 0.45)
+        ball(wh, 0.4, friction=0.9)
+        joint(WHEEL, ch, wh, local_anchor_a=(xw + 8.0, -0.75), local_axis_a=(0.0, 1.0), frequency_hz=4.0,
+              damping_ratio=0.7, enable_motor=1 if k == 0 else 0, motor_speed=-u(4.0, 12.0), max_motor=20.0)
+    # chain of balls on distance joints from an anchor at (0, 12)
+    prev, py = g, 12.0
+    for i in range(5):
+        bb = body(DYNAMIC, u(-0.3, 0.3) + 0.8 * (i + 1), 12.0 - 0.2 * (i + 1))
+        ball(bb, 0.2)
+        if prev == g:
+            joint(DISTANCE, g, bb, local_anchor_a=(0.0, 12.0), length=0.8)
+        else:
+            joint(DISTANCE, prev, bb, length=0.8, frequency_hz=0.0 if i % 2 else 6.0, damping_ratio=0.5)
+        prev = bb
+    # box on a rope from (6, 12)
+    rb = body(DYNAMIC, 8.0, 11.0)
+    box(rb, 0.4, 0.4)
+    joint(ROPE, g, rb, local_anchor_a=(6.0, 12.0), length=3.0)
+    # L of welded boxes falling from height
+    l0 = body(DYNAMIC, 12.0, 8.0, u(-0.3, 0.3))
+    box(l0, 1.0, 0.2)
+    l1 = body(DYNAMIC, 13.2, 8.8, 0.0)
+    box(l1, 0.2, 1.0)
+    l2 = body(DYNAMIC, 10.8, 8.8, 0.0)
+    box(l2, 0.2, 0.6)
+    joint(WELD, l0, l1, local_anchor_a=(1.0, 0.0), local_anchor_b=(-0.2, -0.8), reference_angle=0.0)
+    joint(WELD, l0, l2, local_anchor_a=(-1.0, 0.0), local_anchor_b=(0.2, -0.8), frequency_hz=5.0,
+          damping_ratio=0.3)
+    # elevator on a vertical prismatic joint at x = 18
+    el = body(DYNAMIC, 18.0, 2.0)
+    box(el, 1.0, 0.2, density=2.0)
+    joint(PRISMATIC, g, el, local_anchor_a=(18.0, 2.0), local_axis_a=(0.0, 2.0), enable_limit=1, lower=-1.5,
+          upper=4.0, enable_motor=1, motor_speed=u(0.5, 2.0), max_motor=500.0)
+    # pulley between two boxes
+    pa = body(DYNAMIC, 22.0, 5.0)
+    box(pa, 0.5, 0.5, density=u(1.0, 3.0))
+    pb = body(DYNAMIC, 25.0, 6.0)
+    box(pb, 0.5, 0.5, density=1.0)
+    joint(PULLEY, pa, pb, ground_anchor_a=(22.0, 10.0), ground_anchor_b=(25.0, 10.0), length_a=5.0, length_b=4.0,
+          ratio=u(0.8, 1.5))
+    # box sliding on the ground against a friction joint
+    fr = body(DYNAMIC, 28.0, 0.51, 0.0, u(3.0, 6.0))
+    box(fr, 0.5, 0.5)
+    joint(FRICTION, g, fr, local_anchor_a=(28.0, 0.5), max_force=3.0, max_torque=1.0, collide_connected=1)
+    # box driven to an offset by a motor joint
+    mo = body(DYNAMIC, 32.0, 3.0)
+    box(mo, 0.4, 0.4)
+    joint(MOTOR, g, mo, linear_offset=(33.0, 5.0), angular_offset=0.7, max_force=100.0, max_torque=30.0,
+          correction_factor=0.3)
+    # box held by a mouse joint
+    ms = body(DYNAMIC, 36.0, 4.0, 0.3)
+    box(ms, 0.5, 0.3)
+    j = joint_defs(1, MOUSE)[0]
+    j["body_a"], j["body_b"] = g, ms
+    j["target"] = (36.4, 4.2)
+    j["max_force"] = 500.0
+    joints.append(j)
+    # hinged bar with a limit and a motor
+    hb = body(DYNAMIC, -36.0, 5.0)
+    box(hb, 1.0, 0.1)
+    joint(REVOLUTE, g, hb, local_anchor_a=(-37.0, 5.0), local_anchor_b=(-1.0, 0.0), enable_limit=1, lower=-0.7,
+          upper=0.3, enable_motor=1, motor_speed=1.0, max_motor=5.0)
+    b = np.array(bodies, BODY_DTYPE)
+    fx = np.array(fixtures, FIXTURE_DTYPE)
+    return Scene(b, fx, name=f"joints{seed}", joints=np.array(joints, JOINT_DTYPE))

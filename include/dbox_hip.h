@@ -97,6 +97,34 @@ typedef struct {
     float p[20];
 } dbx_fixture_def;
 
+/* Joint definition: b2JointDef and the concrete joint definitions
+ * (src/dbox/dynamics/joints/b2*joint.d), one record for every kind.  type is
+ * b2JointType (b2joint.d:28-41); the gear joint, which couples four bodies through
+ * two other joints, is not supported.  Fields a kind does not use are ignored. */
+enum {
+    DBX_JOINT_REVOLUTE = 1, DBX_JOINT_PRISMATIC = 2, DBX_JOINT_DISTANCE = 3, DBX_JOINT_PULLEY = 4,
+    DBX_JOINT_MOUSE = 5, DBX_JOINT_WHEEL = 7, DBX_JOINT_WELD = 8, DBX_JOINT_FRICTION = 9, DBX_JOINT_ROPE = 10,
+    DBX_JOINT_MOTOR = 11
+};
+typedef struct {
+    int32_t type;
+    int32_t body_a, body_b;          /* body creation indices */
+    int32_t collide_connected;       /* b2JointDef.collideConnected */
+    int32_t enable_limit, enable_motor; /* revolute, prismatic (limit, motor); wheel (motor) */
+    int32_t pad[2];
+    float local_anchor_a[2], local_anchor_b[2]; /* all but motor and mouse */
+    float local_axis_a[2];           /* prismatic (normalised on creation), wheel (as given) */
+    float reference_angle;           /* revolute, prismatic, weld */
+    float length;                    /* distance: length; rope: maxLength */
+    float frequency_hz, damping_ratio; /* distance, weld, wheel, mouse */
+    float lower, upper;              /* revolute: lower / upper angle; prismatic: translation */
+    float motor_speed, max_motor;    /* revolute, wheel: maxMotorTorque; prismatic: maxMotorForce */
+    float max_force, max_torque;     /* friction, motor; mouse: maxForce */
+    float linear_offset[2], angular_offset, correction_factor; /* motor */
+    float ground_anchor_a[2], ground_anchor_b[2], length_a, length_b, ratio; /* pulley */
+    float target[2];                 /* mouse: the target, and the anchor on body B at creation */
+} dbx_joint_def; /* 148 bytes */
+
 /* Body state read back after a step: b2Body.m_xf, m_sweep, velocities, m_flags. */
 typedef struct {
     float px, py, qs, qc;
@@ -186,6 +214,21 @@ const char *dbx_last_error(const dbx_handle *h);
 dbx_status dbx_add_world(dbx_handle *h, const dbx_body_def *bodies, int32_t body_count,
                          const dbx_fixture_def *fixtures, int32_t fixture_count, float gravity_x,
                          float gravity_y, int32_t *world_id);
+/* b2World.CreateJoint (b2world.d:196-262) for n joints of one world, in creation
+ * order, between dbx_add_world and dbx_finalize (DBX_E_LOCKED after it).  Islands,
+ * sleeping and b2Body.ShouldCollide (collideConnected) then follow the joints as in
+ * the reference, and each island's joints are solved with its contacts in
+ * b2Island.Solve's sequence (b2island.d:137-224).  DBX_E_UNSUPPORTED for the gear
+ * joint, DBX_E_INVALID_ARG for a bad body index, equal bodies or a value that is not
+ * finite. */
+dbx_status dbx_add_joints(dbx_handle *h, int32_t world, const dbx_joint_def *joints, int32_t n);
+/* Per joint of one world in creation order, 8 floats: the impulses the joint keeps
+ * between steps (revolute, prismatic, weld: m_impulse x, y, z; distance, rope,
+ * pulley, wheel: m_impulse in [0]; friction, motor, mouse: linear impulse in [0..1],
+ * angular impulse in [2]), the motor impulse, the wheel's spring impulse, the limit
+ * state (b2LimitState), 0, 0.  b2Joint.GetReactionForce / GetReactionTorque are
+ * these times 1 / dt (b2*joint.d). */
+dbx_status dbx_read_joints(dbx_handle *h, int32_t world, float *out, int32_t n);
 /* Upload every added world to the device.  After this the world set is fixed. */
 dbx_status dbx_finalize(dbx_handle *h);
 
@@ -255,6 +298,12 @@ dbx_status dbx_profile_read(dbx_handle *h, dbx_profile *out);
  * reset, and the number of steps summed (the b2Profile accumulation the demo's
  * Test.Step does, examples/demo/framework/test.d:558-578). */
 dbx_status dbx_profile_totals(dbx_handle *h, dbx_profile *sum, int32_t *steps, int32_t reset);
+/* Launch-level timing (no counterpart in the reference): milliseconds of the lead
+ * tier's kernels, from HIP events on its stream, summed over the eager steps resolved
+ * by dbx_sync since the last reset, in the order Collide, island search, island
+ * solve, broadphase (SynchronizeFixtures + FindNewContacts), continuous pass; *steps
+ * = steps summed.  The other tiers run beside the lead tier. */
+dbx_status dbx_phase_totals(dbx_handle *h, float *ms5, int32_t *steps, int32_t reset);
 /* Totals over all worlds: dynamic bodies, contacts, touching contacts, awake bodies. */
 dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
 /* Per-world velocity iterations used by the last step (RESIDUAL mode) and max penetration. */

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY (see b2oracle.h).  A scalar restatement of dbox
  * (D port of Box2D 2.3.1) that keeps the reference's float operation order.
- * Built with -O2 -ffp-contract=off -fno-fast-math against glibc libm, which is
+ * Built with -O3 -ffp-contract=off -fno-fast-math against glibc libm, which is
  * what dbox reaches through core.stdc.math.  Citations are relative to the
  * reference's src/dbox/.
  *
@@ -169,6 +169,13 @@ struct Mat22 {
     }
 };
 static inline V2 mul(const Mat22 &A, V2 v) { return mk(A.ex.x * v.x + A.ey.x * v.y, A.ex.y * v.x + A.ey.y * v.y); }
+/* b2Mat22.Solve b2math.d:341-352 */
+static inline V2 solve22m(const Mat22 &A, V2 b) {
+    float a11 = A.ex.x, a12 = A.ey.x, a21 = A.ex.y, a22 = A.ey.y;
+    float det = a11 * a22 - a12 * a21;
+    if (det != 0.0f) det = 1.0f / det;
+    return mk(det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x));
+}
 static inline V2 mul(Rot q, V2 v) { return mk(q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y); }
 static inline V2 mulT(Rot q, V2 v) { return mk(q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y); }
 static inline V2 mul(const Xf &T, V2 v) {
@@ -1910,6 +1917,7 @@ struct Body {
     float sleepTime = 0;
     std::vector<int> fixtures; /* m_fixtureList: most recently created first */
     int contactList = kNull;   /* head edge: contact index * 2 + (0 = nodeA, 1 = nodeB) */
+    int jointList = kNull;     /* head joint edge: joint index * 2 + (0 = edgeA, 1 = edgeB) */
     int prev = kNull, next = kNull;
     bool awake() const { return (flags & bf_awake) != 0; }
     /* b2Body.SetAwake b2body.d:827-846 */
@@ -1988,6 +1996,8 @@ struct TimeStep {
     bool warmStarting = true;
 };
 
+#include "b2oracle_joints.h"
+
 using Clock = std::chrono::steady_clock;
 static inline float msSince(Clock::time_point t0) {
     return std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
@@ -2021,7 +2031,36 @@ struct World {
     int proxyOf(int fixture, int child) const { return fixtures[fixture].proxyBase + child; }
 
     /* island scratch */
-    std::vector<int> islandBodies, islandContacts, stack;
+    std::vector<int> islandBodies, islandContacts, islandJoints, stack;
+    std::vector<Joint> joints;
+    int jointList = kNull;
+    JointEdge &jedge(int e) { return joints[e >> 1].node[e & 1]; }
+
+    /* b2World.CreateJoint b2world.d:196-262 (no contact exists to flag for filtering
+     * before the first step) */
+    int createJoint(const orc_joint_def &d) {
+        const int nb = (int)bodies.size();
+        if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return -1;
+        if (d.type < j_revolute || d.type > j_motor || d.type == j_gear) return -1;
+        Joint j;
+        jointFromDef(j, d, bodies[d.body_b].xf);
+        const int ji = (int)joints.size();
+        j.prev = kNull;
+        j.next = jointList;
+        if (jointList != kNull) joints[jointList].prev = ji;
+        jointList = ji;
+        joints.push_back(j);
+        for (int side = 0; side < 2; ++side) {
+            Body &b = bodies[side == 0 ? d.body_a : d.body_b];
+            JointEdge &e = joints[ji].node[side];
+            e.other = side == 0 ? d.body_b : d.body_a;
+            e.prev = kNull;
+            e.next = b.jointList;
+            if (b.jointList != kNull) jedge(b.jointList).prev = 2 * ji + side;
+            b.jointList = 2 * ji + side;
+        }
+        return ji;
+    }
     std::vector<Position> positions;
     std::vector<Velocity> velocities;
     std::vector<VelocityConstraint> vcs;
@@ -2186,9 +2225,12 @@ struct World {
         }
     }
 
-    /* b2Body.ShouldCollide b2body.d:1149-1170 (no joints) */
-    bool bodyShouldCollide(const Body &a, const Body &b) const {
+    /* b2Body.ShouldCollide b2body.d:1149-1170 */
+    bool bodyShouldCollide(int ai, int bi) {
+        const Body &a = bodies[ai], &b = bodies[bi];
         if (a.type != b_dynamic && b.type != b_dynamic) return false;
+        for (int e = a.jointList; e != kNull; e = jedge(e).next)
+            if (jedge(e).other == bi && joints[e >> 1].collideConnected == false) return false;
         return true;
     }
     /* b2ContactFilter.ShouldCollide b2worldcallbacks.d:52-65 */
@@ -2213,7 +2255,7 @@ struct World {
                 if (c.fixtureA == fixB && c.fixtureB == fixA && c.indexA == indexB && c.indexB == indexA) return;
             }
         }
-        if (!bodyShouldCollide(bodies[bodyB], bodies[bodyA])) return;
+        if (!bodyShouldCollide(bodyB, bodyA)) return;
         if (!filterShouldCollide(*fixtureA, *fixtureB)) return;
         int ta = fixtureA->shape.type, tb = fixtureB->shape.type;
         if (!hasContactType(ta, tb)) return;
@@ -2788,9 +2830,14 @@ struct World {
         solverCreate(step);
         initializeVelocityConstraints();
         if (step.warmStarting) warmStart();
+        SolverData data{step, positions.data(), velocities.data()};
+        for (int ji : islandJoints) jointInitVelocity(joints[ji], bodies[joints[ji].bodyA], bodies[joints[ji].bodyB], data);
         profile[3] += msSince(t1);
         auto t2 = Clock::now();
-        for (int i = 0; i < step.velocityIterations; ++i) solveVelocityConstraints();
+        for (int i = 0; i < step.velocityIterations; ++i) {
+            for (int ji : islandJoints) jointSolveVelocity(joints[ji], data);
+            solveVelocityConstraints();
+        }
         storeImpulses();
         profile[4] += msSince(t2);
         for (int i = 0; i < nb; ++i) {
@@ -2819,7 +2866,12 @@ struct World {
         bool positionSolved = false;
         for (int i = 0; i < step.positionIterations; ++i) {
             bool contactsOkay = solvePositionConstraints();
-            if (contactsOkay) { positionSolved = true; break; }
+            bool jointsOkay = true;
+            for (int ji : islandJoints) {
+                bool jointOkay = jointSolvePosition(joints[ji], data);
+                jointsOkay = jointsOkay && jointOkay;
+            }
+            if (contactsOkay && jointsOkay) { positionSolved = true; break; }
         }
         for (int i = 0; i < nb; ++i) {
             Body &b = bodies[islandBodies[i]];
@@ -2890,6 +2942,7 @@ struct World {
         islandCount = 0;
         for (int b = bodyList; b != kNull; b = bodies[b].next) bodies[b].flags &= ~bf_island;
         for (int c = contactList; c != kNull; c = contacts[c].next) contacts[c].flags &= ~cf_island;
+        for (int j = jointList; j != kNull; j = joints[j].next) joints[j].islandFlag = false;
         stack.resize(bodies.size());
         for (int seed = bodyList; seed != kNull; seed = bodies[seed].next) {
             Body &sb = bodies[seed];
@@ -2898,6 +2951,7 @@ struct World {
             if (sb.type == b_static) continue;
             islandBodies.clear();
             islandContacts.clear();
+            islandJoints.clear();
             int stackCount = 0;
             stack[stackCount++] = seed;
             sb.flags |= bf_island;
@@ -2917,6 +2971,17 @@ struct World {
                     islandContacts.push_back(ci);
                     contact.flags |= cf_island;
                     int other = edge(e).other;
+                    if (bodies[other].flags & bf_island) continue;
+                    stack[stackCount++] = other;
+                    bodies[other].flags |= bf_island;
+                }
+                for (int e = b.jointList; e != kNull; e = jedge(e).next) { /* b2world.d:1048-1075 */
+                    Joint &jt = joints[e >> 1];
+                    if (jt.islandFlag) continue;
+                    int other = jedge(e).other;
+                    if ((bodies[other].flags & bf_active) == 0) continue;
+                    islandJoints.push_back(e >> 1);
+                    jt.islandFlag = true;
                     if (bodies[other].flags & bf_island) continue;
                     stack[stackCount++] = other;
                     bodies[other].flags |= bf_island;
@@ -3575,6 +3640,24 @@ int orc_time_of_impact(const orc_fixture_def *a, const float *sweep_a9, const or
     return timeOfImpact(*t_out, pa, pb, sweepIn(sweep_a9), sweepIn(sweep_b9), t_max);
 }
 int orc_toi_events(void *w) { return ((World *)w)->toiEvents; }
+int orc_create_joint(void *w, const orc_joint_def *def) { return ((World *)w)->createJoint(*def); }
+int orc_joint_count(void *w) { return (int)((World *)w)->joints.size(); }
+void orc_get_joints(void *w, float *out) {
+    const World &W = *(const World *)w;
+    for (size_t i = 0; i < W.joints.size(); ++i) {
+        const Joint &j = W.joints[i];
+        float *o = out + 8 * i;
+        const bool lin = j.type == j_friction || j.type == j_motor || j.type == j_mouse;
+        o[0] = lin ? j.linearImpulse.x : j.impulse.x;
+        o[1] = lin ? j.linearImpulse.y : j.impulse.y;
+        o[2] = lin ? j.angularImpulse : j.impulse.z;
+        o[3] = j.motorImpulse;
+        o[4] = j.springImpulse;
+        o[5] = (float)j.limitState;
+        o[6] = 0.0f;
+        o[7] = 0.0f;
+    }
+}
 int orc_set_rot_mode(int mode) {
     const int prev = g_rotMode;
     g_rotMode = mode ? 1 : 0;

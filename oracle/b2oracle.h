@@ -95,6 +95,27 @@ typedef struct {
     float normal[2], points[4], separations[2];
 } orc_manifold;
 
+/* Joint definition: b2JointDef and the concrete joint definitions
+ * (src/dbox/dynamics/joints/b2*joint.d), one record for every kind; the byte layout
+ * of dbx_joint_def in include/dbox_hip.h.  type is b2JointType (1 revolute,
+ * 2 prismatic, 3 distance, 4 pulley, 5 mouse, 7 wheel, 8 weld, 9 friction, 10 rope,
+ * 11 motor; the gear joint, 6, is not supported). */
+typedef struct {
+    int32_t type, body_a, body_b, collide_connected;
+    int32_t enable_limit, enable_motor, pad[2];
+    float local_anchor_a[2], local_anchor_b[2];
+    float local_axis_a[2];
+    float reference_angle;
+    float length;                      /* distance: length; rope: maxLength */
+    float frequency_hz, damping_ratio;
+    float lower, upper;                /* revolute: angles; prismatic: translations */
+    float motor_speed, max_motor;      /* maxMotorTorque (revolute, wheel) / maxMotorForce (prismatic) */
+    float max_force, max_torque;       /* friction, motor; mouse: max_force */
+    float linear_offset[2], angular_offset, correction_factor; /* motor */
+    float ground_anchor_a[2], ground_anchor_b[2], length_a, length_b, ratio; /* pulley */
+    float target[2];                   /* mouse */
+} orc_joint_def; /* 148 bytes */
+
 void *orc_world_create(float gx, float gy);
 void orc_world_destroy(void *w);
 /* b2World.SetWarmStarting / SetAllowSleeping / SetContinuousPhysics
@@ -179,6 +200,15 @@ int orc_toi_events(void *w);
  * k = (fixtureA + fixtureB) % 7: 0 disable, 1 friction 0.05, 2 restitution 0.6,
  * 3 tangent speed 0.8. */
 int orc_world_set_presolve(void *w, int mode);
+/* b2World.CreateJoint (b2world.d:196-262) after the bodies it names exist; returns
+ * the joint index (creation order) or -1 for an unsupported type or bad bodies. */
+int orc_create_joint(void *w, const orc_joint_def *def);
+int orc_joint_count(void *w);
+/* Per joint in creation order, 8 floats: the impulses the joint keeps between steps
+ * (revolute / prismatic / weld: m_impulse x, y, z; distance, rope, pulley, wheel:
+ * m_impulse in [0]; friction, motor, mouse: linear impulse in [0..1], angular in [2]),
+ * motor impulse, spring impulse (wheel), limit state, and 2 zeros. */
+void orc_get_joints(void *w, float *out8);
 /* Test switch for b2Rot.Set (not reference behaviour).  Mode 0 (default) calls
  * glibc sinf/cosf as dbox does; mode 1 evaluates sin and cos in double and rounds
  * once (Cody-Waite reduction, Taylor series in Horner form with fused

@@ -80,6 +80,10 @@ def lib():
         L.orc_time_of_impact.argtypes = [vp, vp, vp, vp, f32, vp]
         L.orc_toi_events.argtypes = [vp]
         L.orc_world_set_presolve.argtypes = [vp, i32]
+        L.orc_create_joint.argtypes = [vp, vp]
+        L.orc_joint_count.argtypes = [vp]
+        L.orc_get_joints.argtypes = [vp, vp]
+        L.orc_get_joints.restype = None
         L.orc_set_rot_mode.argtypes = [i32]
         L.orc_world_set_rot_mode.argtypes = [vp, i32]
         L.orc_solve_one_constraint.argtypes = [vp, vp]
@@ -113,6 +117,12 @@ class OracleWorld:
         if L.orc_build(self.h, _p(self.bodies_def), len(self.bodies_def), _p(self.fixtures_def), len(self.fixtures_def)) != 0:
             raise ValueError("oracle rejected the scene")
         L.orc_world_set_rot_mode(self.h, 1 if exact_rot else 0)
+        joints = getattr(scene, "joints", None)
+        if joints is not None:
+            self.joints_def = np.ascontiguousarray(joints)
+            for k in range(len(self.joints_def)):
+                if L.orc_create_joint(self.h, _p(self.joints_def[k:k + 1])) != k:
+                    raise ValueError(f"oracle rejected joint {k}")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -180,6 +190,13 @@ class OracleWorld:
     def toi_events(self) -> int:
         """TOI events (sub-step island solves) of the last step."""
         return int(self.L.orc_toi_events(self.h))
+
+    def joints(self) -> np.ndarray:
+        """Per joint: kept impulses (3), motor impulse, spring impulse, limit state, 0, 0."""
+        out = np.zeros((self.L.orc_joint_count(self.h), 8), np.float32)
+        if len(out):
+            self.L.orc_get_joints(self.h, _p(out))
+        return out
 
     def toi_position_once(self, list_index: int, toi_body: int):
         """One TOI position pass over one contact with toi_body as the TOI body:

@@ -45,7 +45,7 @@ def _gpu(scene_list):
 
 
 def _check(step, g, o, world=0, contacts=True, pairs=True):
-    m = body_mismatch(g.bodies(world), o.bodies())
+    m = body_mismatch(g.bodies(world), o.bodies(), o.bodies_def["type"])
     assert m is None, f"step {step}: {m}"
     if contacts:
         m = contact_mismatch(g.contacts(world), o.contacts())

@@ -176,7 +176,8 @@ def test_learned_batched_worlds_bit_identical_to_single():
         h.finalize()
         h.mlp_init(5)
         params, norm = h.mlp_get()
-        norm[30:32] = 0.05  # output scale: predictions of a few hundredths of N s
+        norm[28:30] = 0.05  # output mean and scale: positive predictions of a few hundredths of N s
+        norm[30:32] = 0.05
         h.mlp_set(None, norm)
         h.step(flags=flags, n=60)
         h.sync()

@@ -223,7 +223,7 @@ def test_island_order_and_levels_match_reference(scene_name):
         o.step()
         g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
         if i % 30 == 29:
-            assert body_mismatch(g.bodies(), o.bodies()) is None, i
+            assert body_mismatch(g.bodies(), o.bodies(), o.bodies_def["type"]) is None, i
             ref = o.solve_log()
             got = g.schedule()
             assert got.shape[0] == ref.shape[0]
@@ -341,7 +341,7 @@ def test_mixed_world_distance_to_reference_is_the_libm_rounding():
         x.step()
         g.step(flags=native.WARM | native.SLEEP | native.ORDERED)
     bg, bx, bo = g.bodies(), x.bodies(), o.bodies()
-    assert body_mismatch(bg, bx) is None
+    assert body_mismatch(bg, bx, x.bodies_def["type"]) is None
     assert contact_mismatch(g.contacts(), x.contacts()) is None
     dp = np.hypot(bg["px"] - bo["px"], bg["py"] - bo["py"])
     da = angle_diff(bg["a"], bo["a"])

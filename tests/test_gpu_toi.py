@@ -132,52 +132,3 @@ def test_continuous_batched_equals_single_and_graph_equals_eager():
     for k in range(6):
         assert gr.bodies(k).tobytes() == batch.bodies(k).tobytes(), k
 
-
-def plank_over_posts(posts: int = 600):
-    """A 60 m plank thrown down at 300 m/s onto a static edge lined with `posts` small
-    static circles: the plank's first TOI event moves its proxy across all of them, so
-    the event's FindNewContacts query finds posts + 1 pairs."""
-    b = scenes.body_defs(3)
-    b[0]["type"] = b[1]["type"] = scenes.STATIC
-    b[2]["type"] = scenes.DYNAMIC
-    b[2]["py"] = 2.0
-    b[2]["vy"] = -300.0
-    f = scenes.fixture_defs(posts + 2)
-    scenes._edge(f[0], 0, -61.0, 0.0, 61.0, 0.0, 0.6)
-    f[1]["body"] = 2
-    f[1]["shape"] = scenes.SHAPE_BOX
-    f[1]["p"][:2] = (30.0, 0.05)
-    f[1]["density"] = 1.0
-    d = f[2:]
-    d["body"] = 1
-    d["shape"] = scenes.SHAPE_CIRCLE
-    d["radius"] = 0.04
-    d["p"][:, 0] = np.linspace(-29.9, 29.9, posts).astype(np.float32)
-    d["p"][:, 1] = 0.2
-    return scenes.Scene(b, f)
-
-
-def test_continuous_pass_pair_capacity_is_reported():
-    # A TOI event whose new-pair query finds more pairs than the pass keeps per event
-    # (512) cannot add them in AddPair order: the step reports DBX_E_CAPACITY naming
-    # the continuous pass's lists, and the handle stays usable.
-    print("case: plank over 600 posts, TOI pair capacity", flush=True)
-    s = plank_over_posts(600)
-    o = po.OracleWorld(s, continuous=True)
-    o.step()
-    assert o.toi_events() >= 1  # the reference has the event
-    g = _gpu([s])
-    g.step(flags=FLAGS)
-    with pytest.raises(native.DbxError) as e:
-        g.sync()
-    assert e.value.status == native.DBX_E_CAPACITY
-    assert "continuous-pass" in str(e.value)
-    # below the capacity the same scene runs clean and equals the oracle's event count
-    s = plank_over_posts(300)
-    o = po.OracleWorld(s, continuous=True)
-    g = _gpu([s])
-    for _ in range(3):
-        o.step()
-        g.step(flags=FLAGS)
-        g.sync()
-        assert int(g.toi_events()[0]) == o.toi_events()
