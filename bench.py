@@ -307,16 +307,10 @@ def main():
     flags = native.WARM | native.SLEEP | (0 if args.coloured else native.ORDERED) | (0 if args.no_ccd else native.CCD)
     log(f"bodies {nbodies} dynamic {ndyn} proxies {nprox}; timed steps {window}")
 
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:  # torch is only used for an extra device barrier
-        have_torch = False
-
     def barrier(h):
+        # dbx_sync joins every tier's stream into the handle's stream and waits for it:
+        # all of the handle's work on the device has finished when it returns
         h.sync()
-        if have_torch:
-            torch.cuda.synchronize()
 
     # Warm-up run on its own copy of the worlds; its first 5 x 2 steps let the
     # autotuner pick each tier's workgroup size, which the timed runs reuse.
