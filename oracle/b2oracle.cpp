@@ -3491,6 +3491,21 @@ void orc_get_bodies(void *w, orc_body_state *out) {
         s.c0x = b.sweep.c0.x; s.c0y = b.sweep.c0.y; s.a0 = b.sweep.a0; s.inv_mass = b.invMass;
     }
 }
+/* The proxy pairs of a body's contact edges (b2Body.m_contactList, b2body.d, kept by
+ * AddPair / Destroy at b2contactmanager.d:138-166 and 88-127), head first. */
+int orc_body_contact_pairs(void *w, int body, int *out_pairs, int cap) {
+    World *W = (World *)w;
+    if (body < 0 || body >= (int)W->bodies.size()) return -1;
+    int n = 0;
+    for (int e = W->bodies[body].contactList; e != kNull; e = W->edge(e).next, ++n) {
+        if (n >= cap) continue;
+        const Contact &c = W->contacts[e >> 1];
+        out_pairs[2 * n] = W->proxyOf(c.fixtureA, c.indexA);
+        out_pairs[2 * n + 1] = W->proxyOf(c.fixtureB, c.indexB);
+    }
+    return n;
+}
+
 int orc_get_contacts(void *w, orc_contact *out, int cap) {
     World *W = (World *)w;
     int n = 0;

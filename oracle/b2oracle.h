@@ -138,6 +138,9 @@ int orc_contact_count(void *w);
 void orc_get_bodies(void *w, orc_body_state *out);
 /* Contacts in m_contactList order; returns the number written (<= cap). */
 int orc_get_contacts(void *w, orc_contact *out, int cap);
+/* A body's contact edges (b2Body.m_contactList), head first, as proxy pairs; returns
+ * the edge count (pairs beyond cap are not written), -1 for a bad body index. */
+int orc_body_contact_pairs(void *w, int body, int *out_pairs, int cap);
 /* Listener callbacks of the last orc_step in call order; returns the count. */
 int orc_events(void *w, orc_event *out, int cap);
 /* World manifold of contact-list entry i at the bodies' current transforms. */

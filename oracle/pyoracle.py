@@ -65,6 +65,7 @@ def lib():
             getattr(L, name).argtypes = [vp]
         L.orc_get_bodies.argtypes = [vp, vp]
         L.orc_get_contacts.argtypes = [vp, vp, i32]
+        L.orc_body_contact_pairs.argtypes = [vp, i32, vp, i32]
         L.orc_events.argtypes = [vp, vp, i32]
         L.orc_contact_world_manifold.argtypes = [vp, i32, vp]
         L.orc_get_fat_aabbs.argtypes = [vp, vp]
@@ -143,6 +144,13 @@ class OracleWorld:
         out = np.zeros(n, CONTACT_DTYPE)
         got = self.L.orc_get_contacts(self.h, _p(out), n)
         return out[:got]
+
+    def body_contact_pairs(self, body: int) -> np.ndarray:
+        """The proxy pairs of body's contact edge list (b2Body.m_contactList), head first."""
+        n = self.L.orc_body_contact_pairs(self.h, body, None, 0)
+        out = np.zeros((max(n, 1), 2), np.int32)
+        self.L.orc_body_contact_pairs(self.h, body, _p(out), n)
+        return out[:n]
 
     def events(self) -> np.ndarray:
         """b2ContactListener callbacks of the last step (BeginContact 1, EndContact 2,

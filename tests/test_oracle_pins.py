@@ -403,3 +403,29 @@ def test_exact_rot_mode_is_correctly_rounded_nearly_always():
     ulps = np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
     assert diff.sum() <= 10, diff.sum()
     assert ulps.max() <= 1
+
+
+def test_body_edge_lists_are_the_world_list_filtered():
+    # AddPair prepends a new contact to the world list and to both bodies' edge lists
+    # (b2contactmanager.d:138-166); Destroy unlinks it from all three (:88-127), also
+    # when SolveTOI's FindNewContacts adds pairs.  So a body's edge list is the world
+    # list filtered to that body, in the same order: the rule a contact-list mirror
+    # rebuilt from the world list alone relies on.  A prepend turned into an append
+    # on either list, or an unlink that skips one, breaks it.
+    s = scenes.projectile_room(30, seed=5)
+    o = po.OracleWorld(s, continuous=True)
+    fbody = s.fixtures["body"]
+    checked, events = 0, 0
+    for step in range(1, 61):
+        o.step()
+        events += o.toi_events()
+        if step % 6:
+            continue
+        wl = o.contacts()
+        pairs = np.stack([wl["fixture_a"], wl["fixture_b"]], 1) if len(wl) else np.zeros((0, 2), np.int32)
+        for b in range(len(s.bodies)):
+            mine = pairs[(fbody[pairs[:, 0]] == b) | (fbody[pairs[:, 1]] == b)]
+            got = o.body_contact_pairs(b)
+            assert np.array_equal(got, mine), (step, b)
+            checked += len(got)
+    assert checked > 200 and events > 0
