@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -24,7 +25,38 @@
 
 using namespace dbx;
 
+// ---------------------------------------------------------------------------
+// Scratch of dbx_broadphase_pairs_noalloc: a fixed device array (each GPU holds
+// its own copy until the process ends), big enough for the sort-and-sweep of
+// DBX_NOALLOC_MAX_PROXIES proxies (keys 8 B per slot of the power of two above
+// the proxy count, fat AABB copy 16 B, three int arrays 12 B per proxy).  Host
+// code never passes `scratch` itself to a kernel: its device address on the
+// current GPU comes from hipGetSymbolAddress, looked up once per device when a
+// handle is created there.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 36u * (size_t)DBX_NOALLOC_MAX_PROXIES;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchDev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device.
+hipError_t noalloc_scratch(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratchDev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchDev[device] = p;
+    }
+    *out = g_scratchDev[device];
+    return hipSuccess;
+}
 
 // ---------------------------------------------------------------------------
 // host geometry (creation time only)
@@ -602,6 +634,7 @@ struct dbx_handle {
     int tierSteps = 0;
     int ringUsed = 0;
     dbx_profile prof{};             // last step
+    void *noallocScratch = nullptr; // device address of `scratch` on this handle's GPU
     std::vector<char> ringSolved;   // per ring entry: the step ran the island solve (dt > 0)
     unsigned long long profSeen = 0; // phase-stamp slots of the solve kernels already read
     double wallKHz = 100000.0;      // rate of the kernels' wall_clock64 stamps
@@ -693,6 +726,7 @@ dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
     int wallKHz = 0;
     if (hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, h->cfg.device) == hipSuccess && wallKHz > 0)
         h->wallKHz = wallKHz;
+    if (noalloc_scratch(h->cfg.device, &h->noallocScratch) != hipSuccess) h->noallocScratch = nullptr;
     (void)hipGetLastError(); // clear any error state left by the attribute calls
     if (getenv("DBX_VERBOSE"))
         fprintf(stderr, "dbx: device %d, %d CUs, LDS per workgroup %d bytes\n", h->cfg.device,
@@ -2008,6 +2042,24 @@ dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uin
     if (e != hipSuccess) return hip_fail(h, e, "broadphase");
     if (count) *count = total;
     return total > cap ? fail(h, DBX_E_CAPACITY, "pair output too small") : DBX_OK;
+}
+
+dbx_status dbx_broadphase_pairs_noalloc(dbx_handle *h, const float *fat_aabbs, const int32_t *moved, int32_t n,
+                                        int32_t *pairs, int32_t cap, int32_t *count) {
+    if (!h) return DBX_E_INVALID_ARG;
+    if (n < 0 || cap < 0 || !count || (n > 0 && (!fat_aabbs || !moved)) || (cap > 0 && !pairs))
+        return fail(h, DBX_E_INVALID_ARG, "bad broadphase arguments");
+    if (((uintptr_t)fat_aabbs & 15) || ((uintptr_t)pairs & 7) || ((uintptr_t)moved & 3) || ((uintptr_t)count & 3))
+        return fail(h, DBX_E_INVALID_ARG, "misaligned device pointer (fat AABBs need 16 bytes, pairs 8, ints 4)");
+    if (n > DBX_NOALLOC_MAX_PROXIES || broadphase_scratch_bytes(n) > kNoAllocScratchBytes)
+        return fail(h, DBX_E_CAPACITY, "more proxies than the fixed scratch holds (DBX_NOALLOC_MAX_PROXIES)");
+    if (!h->noallocScratch) return fail(h, DBX_E_HIP, "no device address for the fixed scratch on this GPU");
+    DBX_HIP(h, hipSetDevice(h->cfg.device));
+    const hipError_t e = launch_bp_single(reinterpret_cast<const float4 *>(fat_aabbs), moved, n,
+                                          static_cast<char *>(h->noallocScratch), reinterpret_cast<int2 *>(pairs), cap,
+                                          count, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "broadphase");
+    return DBX_OK;
 }
 
 dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const float *xa, const dbx_fixture_def *b,

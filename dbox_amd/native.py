@@ -71,7 +71,7 @@ EXPORTS = [
     "dbx_create", "dbx_destroy", "dbx_last_error", "dbx_add_world", "dbx_add_joints", "dbx_read_joints", "dbx_finalize", "dbx_step", "dbx_sync",
     "dbx_world_count", "dbx_world_info", "dbx_read_bodies", "dbx_read_contacts", "dbx_read_events", "dbx_read_fat_aabbs",
     "dbx_read_pairs", "dbx_profile_read", "dbx_profile_totals", "dbx_phase_totals", "dbx_totals", "dbx_read_solver_stats", "dbx_broadphase_pairs",
-    "dbx_collide_pairs", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
+    "dbx_collide_pairs", "dbx_broadphase_pairs_noalloc", "dbx_read_schedule", "dbx_read_tiers", "dbx_set_residual", "dbx_write_bodies",
     "dbx_autotune", "dbx_set_tier_blocks", "dbx_set_graph", "dbx_read_residuals", "dbx_read_toi_events", "dbx_write_contact_settings", "dbx_step_finish", "dbx_read_proxies",
     "dbx_mlp_init", "dbx_mlp_set", "dbx_mlp_get", "dbx_mlp_forward", "dbx_mlp_gradients", "dbx_mlp_log_enable",
     "dbx_mlp_log_count", "dbx_mlp_log_read", "dbx_mlp_log_clear", "dbx_mlp_fit_normalization", "dbx_mlp_train",
@@ -115,6 +115,7 @@ def lib():
         L.dbx_read_solver_stats.argtypes = [vp, vp, vp, i32]
         L.dbx_broadphase_pairs.argtypes = [vp, vp, vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_collide_pairs.argtypes = [vp, vp, vp, vp, vp, i32, vp]
+        L.dbx_broadphase_pairs_noalloc.argtypes = [vp, vp, vp, i32, vp, i32, vp]
         L.dbx_read_schedule.argtypes = [vp, i32, vp, i32, C.POINTER(i32)]
         L.dbx_read_tiers.argtypes = [vp, vp, i32, vp, C.POINTER(i32)]
         i64, u64 = C.c_int64, C.c_uint64

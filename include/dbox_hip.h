@@ -352,6 +352,21 @@ dbx_status dbx_read_schedule(dbx_handle *h, int32_t world, int32_t *out, int32_t
  * AABBs and moved flags of n proxies; pairs are written sorted as (i, j), i < j. */
 dbx_status dbx_broadphase_pairs(dbx_handle *h, const float *fat_aabbs, const uint8_t *moved, int32_t n,
                                 int32_t *out, int32_t cap, int32_t *count);
+/* Stand-alone broadphase that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The pairs of
+ * dbx_broadphase_pairs, with every buffer in device memory: fat_aabbs (4 floats per
+ * proxy, 16-byte aligned) and moved (one int32 per proxy, nonzero = moved) are read;
+ * pairs (cap pairs of 2 int32, (i, j) with i < j, in b2PairLessThan order) is
+ * written only when the pair count fits in cap; *count (device int32) receives the
+ * pair count.  Asynchronous on the handle's stream (dbx_sync waits).  The working
+ * memory is a fixed array of the library on each GPU, sized for at most
+ * DBX_NOALLOC_MAX_PROXIES proxies (DBX_E_CAPACITY above that); it holds the last
+ * call's data, and calls that overlap on one GPU (from other streams, handles or
+ * threads) race on it. */
+#define DBX_NOALLOC_MAX_PROXIES 16384
+dbx_status dbx_broadphase_pairs_noalloc(dbx_handle *h, const float *fat_aabbs, const int32_t *moved, int32_t n,
+                                        int32_t *pairs, int32_t cap, int32_t *count);
 /* Stand-alone narrowphase: for each of n pairs, shape a[i] at transform xa[i]
  * (x, y, angle) against shape b[i] at xb[i]; the pair is put in b2Contact.Create's
  * type order first.  out holds 32 floats per pair: type, pointCount, swapped, 0,
