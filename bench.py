@@ -51,6 +51,107 @@ def shape_bytes(fixtures: np.ndarray) -> int:
     return int(b.sum())
 
 
+def _rocprof_rows(directory: str, suffix: str) -> list[dict]:
+    """Rows of every rocprofv3 CSV file under directory whose name ends in suffix."""
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True)):
+        with open(path, newline="") as f:
+            rows += list(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    return rows
+
+
+def _per_dispatch(rows: list[dict], counter: str | None = None) -> dict:
+    """{kernel name: {dispatch id: value}} (counter rows summed per dispatch; trace rows: duration in ns)."""
+    out: dict = {}
+    for r in rows:
+        if counter is not None:
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+        else:
+            v = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+        d = out.setdefault(r["Kernel_Name"], {})
+        k = int(r["Dispatch_Id"])
+        d[k] = d.get(k, 0.0) + v
+    return out
+
+
+def _kernel_values(table: dict, kernel: str, last: int) -> list[float]:
+    """Values of the last `last` dispatches of the kernel whose name starts with `kernel`
+    (rocprofv3 writes qualified names: dbx::k_solve_wave<true>(...))."""
+    hits = [n for n in table if n.split("(")[0].endswith(kernel)]
+    if len(hits) != 1:
+        raise SystemExit(f"kernel {kernel!r}: matches {hits}")
+    d = table[hits[0]]
+    return [d[k] for k in sorted(d)][-last:]
+
+
+def summarize_counters(args) -> dict:
+    """HBM traffic per launch of one kernel from four rocprofv3 --pmc runs (FETCH_SIZE and
+    WRITE_SIZE of the profiled bench run, then of a plain device copy of --copy-mib MiB
+    each), written as the JSON file bench.py reads into roofline.traffic."""
+    fetch_dir, write_dir, cfetch_dir, cwrite_dir = args.summarize_pmc
+    mib = args.copy_mib or 256
+    out = {"kernel": args.kernel, "workload": args.workload, "dispatches": args.last}
+    for counter, d, cd in (("FETCH_SIZE", fetch_dir, cfetch_dir), ("WRITE_SIZE", write_dir, cwrite_dir)):
+        vals = _kernel_values(_per_dispatch(_rocprof_rows(d, "counter_collection.csv"), counter), args.kernel,
+                              args.last)
+        # the copy run launches one kernel exactly 20 times: the copies
+        copies = [v for v in _per_dispatch(_rocprof_rows(cd, "counter_collection.csv"), counter).values()
+                  if len(v) == 20]
+        if len(copies) != 1:
+            raise SystemExit(f"{cd}: expected one kernel with 20 dispatches")
+        ckib = float(np.median(list(copies[0].values())))
+        scale = mib * 1024.0 / ckib
+        key = counter.split("_")[0].lower()
+        out[f"{key}_kib_mean"] = float(np.mean(vals))
+        out[f"{key}_copy_kib"] = ckib
+        out[f"{key}_scale"] = scale
+        out[f"{key}_bytes"] = int(round(np.mean(vals) * 1024.0 * scale))
+    out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
+    pa = args.profiled_args
+    out["commands"] = [
+        f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '{args.kernel}' --output-format csv -d {fetch_dir} "
+        f"-o pmc -- python3 bench.py {pa}",
+        f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex '{args.kernel}' --output-format csv -d {write_dir} "
+        f"-o pmc -- python3 bench.py {pa}",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {cfetch_dir} -o copy -- python3 bench.py --copy-mib {mib}",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {cwrite_dir} -o copy -- python3 bench.py --copy-mib {mib}",
+        "python3 bench.py --summarize-pmc " + " ".join(args.summarize_pmc) +
+        f" --kernel '{args.kernel}' --last {args.last} --copy-mib {mib} --workload '{args.workload}' "
+        f"--profiled-args '{pa}' --out {args.out}",
+    ]
+    out["note"] = ("scale = bytes the copy moves each way / what the counter reports for it; "
+                   "bytes = mean counter value of the last dispatches (the timed window) x 1024 x scale")
+    return out
+
+
+def summarize_trace(args) -> str:
+    """Markdown table of the lead tier's kernels from a rocprofv3 --kernel-trace run: mean
+    duration over the last --last dispatches of each, plus rocprofv3's own stats rows."""
+    table = _per_dispatch(_rocprof_rows(args.summarize_trace, "kernel_trace.csv"))
+    lines = [f"| kernel | dispatches averaged | mean µs | min µs | max µs |", "|---|---|---|---|---|"]
+    for name in sorted(table, key=lambda n: -sum(table[n].values())):
+        short = name.split("(")[0].replace("dbx::", "")
+        if not short.endswith("<true>"):
+            continue
+        vals = [table[name][k] for k in sorted(table[name])][-args.last:]
+        lines.append(f"| `{short}` | {len(vals)} | {np.mean(vals) / 1e3:.1f} | {np.min(vals) / 1e3:.1f} | "
+                     f"{np.max(vals) / 1e3:.1f} |")
+    stats = _rocprof_rows(args.summarize_trace, "kernel_stats.csv")
+    lines += ["", "rocprofv3 `--stats` (whole run, every dispatch including warm-up and pre-roll):", "",
+              "| kernel | calls | average µs | total ms | % |", "|---|---|---|---|---|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:16]:
+        lines.append(f"| `{r['Name'].split('(')[0].replace('dbx::', '')}` | {r['Calls']} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+                     f"{float(r['Percentage']):.1f} |")
+    return "\n".join(lines)
+
+
 def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int, sbytes: int, npairs: int) -> float:
     """Compulsory HBM bytes of one launch of each step kernel (per step)."""
     c, touch = t["contacts"], t["touching"]
@@ -261,7 +362,27 @@ def main():
     ap.add_argument("--single-world", type=int, choices=(2, 4), default=0,
                     help="time one big world instead (config 2: 20,000-body container; config 4: 50,000-box "
                          "pyramid) with --steps timed steps after --warmup, beside the oracle on one thread")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="write the traffic JSON of --kernel from four rocprofv3 --pmc output folders and exit")
+    ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
+                    help="print the lead tier's kernel durations from a rocprofv3 --kernel-trace folder and exit")
+    ap.add_argument("--kernel", default="k_solve_wave<true>")
+    ap.add_argument("--last", type=int, default=600, help="dispatches (timed steps) the summaries average over")
+    ap.add_argument("--workload", default="")
+    ap.add_argument("--profiled-args", default="")
+    ap.add_argument("--out", default="")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        res = summarize_counters(args)
+        text = json.dumps(res, indent=1)
+        if args.out:
+            with open(args.out, "w") as f:
+                f.write(text + "\n")
+        print(text)
+        return
+    if args.summarize_trace:
+        print(summarize_trace(args))
+        return
     if args.single_world:
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)

@@ -793,7 +793,10 @@ __global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int
 // ---------------------------------------------------------------------------
 // Solver: b2Island.Solve (b2island.d:75-280) for every island of the world.
 // ---------------------------------------------------------------------------
-__device__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, int island, const int *__restrict__ fbody,
+// Inlined at every call: a call would take the addresses of the constraint records
+// and of the step parameters, moving them to scratch memory.
+__device__ __forceinline__ void init_constraint(VelCon &v, PosCon &p, const Contact &c, int ci, int island,
+                                                const int *__restrict__ fbody,
                                 const float4 *__restrict__ fmat, const uint32_t *__restrict__ bfl,
                                 const float4 *__restrict__ bmass, const float4 *__restrict__ bpos,
                                 const float4 *__restrict__ bvel, const StepParams &sp, float tangentSpeed) {
@@ -1304,7 +1307,10 @@ __device__ float solve_position(const PosCon &p, int indexA, int indexB, int dyn
     return ms;
 }
 
-__device__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead) {
+// Inlined into both solve kernels: as a called function it would take the address
+// of the kernel's Dev argument, which puts a copy of Dev in scratch memory and makes
+// the level loops of the LDS path reload device pointers from there.
+__device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead) {
     __shared__ float s_res;
     const WorldDesc wd = d.W[w];
     const int nb = wd.bodyCount, nf = wd.fixCount;
