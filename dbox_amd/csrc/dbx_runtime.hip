@@ -611,7 +611,7 @@ struct dbx_handle {
     bool newFixture = true;
     int block = 256;
     int ldsMax = 65536;           // dynamic LDS a workgroup may use on this device
-    std::vector<Tier> tiers;      // tier 0 = the largest worlds, on `stream`
+    std::vector<Tier> tiers;      // tier 0 = the largest worlds (on `stream` unless leadCus)
     std::vector<WideBp> wide;     // worlds whose broadphase scratch does not fit in LDS, by tier
     std::vector<int> proxyFix, proxyChild; // per proxy (from each world's fixBase): fixture and child
     // DBX_STEP_PRESOLVE: a step stopped after Collide, finished by dbx_step_finish
@@ -638,6 +638,8 @@ struct dbx_handle {
     std::vector<char> ringSolved;   // per ring entry: the step ran the island solve (dt > 0)
     unsigned long long profSeen = 0; // phase-stamp slots of the solve kernels already read
     double wallKHz = 100000.0;      // rate of the kernels' wall_clock64 stamps
+    int cuCount = 0;                // compute units of the device
+    bool leadCus = false;           // the lead tier runs on CUs of its own (CU-masked streams)
     dbx_profile sum{};              // accumulated since the last dbx_profile_totals reset
     double phaseSum[5] = {0, 0, 0, 0, 0}; // lead tier: collide, islands, solve, broadphase, TOI (eager steps)
     int phaseSteps = 0;
@@ -718,6 +720,7 @@ dbx_status dbx_create(const dbx_config *config, dbx_handle **out) {
         size_t lim = prop.sharedMemPerBlock;
         if (lim > 160 * 1024) lim = 160 * 1024;
         if (lim >= 32768) h->ldsMax = (int)lim;
+        h->cuCount = prop.multiProcessorCount;
     }
     if (configure_kernels(h->ldsMax) != hipSuccess) {
         h->ldsMax = 65536;
@@ -792,6 +795,48 @@ dbx_status dbx_add_joints(dbx_handle *h, int32_t world, const dbx_joint_def *joi
             return fail(h, joints[i].type == 6 ? DBX_E_UNSUPPORTED : DBX_E_INVALID_ARG, err);
     W.joints.insert(W.joints.end(), add.begin(), add.end());
     return DBX_OK;
+}
+
+// The lead tier (the few largest worlds) sets the step time, and its kernels need
+// whole CUs (up to all 160 KB of a CU's LDS).  Sharing the device, its workgroups
+// wait at every dispatch until the thousands of small worlds' workgroups leave a
+// CU free.  With more than one tier, the lead tier therefore gets a stream whose
+// CU mask holds eight CUs of its own, one on each of the eight XCDs, and the other
+// tiers' streams the remaining CUs.  Mask bits 33 k (k = 0..7) name one CU per XCD
+// whether the runtime deals mask bits out to the XCDs in turn or in blocks of 32,
+// so every XCD keeps CUs for both masks.  Needs 256 CUs and at most eight lead
+// worlds with no device-wide pass; DBX_SHARED_CUS turns it off.
+constexpr int kLeadCus = 8;
+static hipError_t create_tier_streams(dbx_handle *h, std::vector<Tier> &tiers) {
+    const bool reserve = tiers.size() > 1 && h->cuCount == 256 && tiers[0].count <= kLeadCus &&
+                         tiers[0].wideCount == 0 && !getenv("DBX_SHARED_CUS");
+    uint32_t lead[8] = {}, rest[8];
+    for (uint32_t &m : rest) m = ~0u;
+    for (int k = 0; k < kLeadCus; ++k) {
+        const int bit = 33 * k;
+        lead[bit >> 5] |= 1u << (bit & 31);
+        rest[bit >> 5] &= ~(1u << (bit & 31));
+    }
+    h->leadCus = reserve;
+    for (size_t t = 0; t < tiers.size(); ++t) {
+        Tier &T = tiers[t];
+        hipError_t e = hipSuccess;
+        if (reserve) e = hipExtStreamCreateWithCUMask(&T.stream, 8, t == 0 ? lead : rest);
+        else if (t == 0) T.stream = h->stream;
+        else e = hipStreamCreate(&T.stream);
+        if (e != hipSuccess) return e;
+        if (T.stream == h->stream) continue;
+        T.own = true;
+        e = hipEventCreateWithFlags(&T.done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    if (reserve && getenv("DBX_VERBOSE")) {
+        uint32_t m[8] = {};
+        if (hipExtStreamGetCUMask(tiers[0].stream, 8, m) == hipSuccess)
+            fprintf(stderr, "dbx: lead tier CU mask %08x %08x %08x %08x %08x %08x %08x %08x\n", m[0], m[1], m[2], m[3],
+                    m[4], m[5], m[6], m[7]);
+    }
+    return hipSuccess;
 }
 
 dbx_status dbx_finalize(dbx_handle *h) {
@@ -870,15 +915,6 @@ dbx_status dbx_finalize(dbx_handle *h) {
             }
             tiers.back().count++;
         }
-        for (size_t t = 0; t < tiers.size(); ++t) {
-            if (t == 0) {
-                tiers[t].stream = h->stream;
-                continue;
-            }
-            DBX_HIP(h, hipStreamCreate(&tiers[t].stream));
-            tiers[t].own = true;
-            DBX_HIP(h, hipEventCreateWithFlags(&tiers[t].done, hipEventDisableTiming));
-        }
         // A world whose broadphase scratch does not fit in its tier's LDS has its
         // pairs found by the device-wide pass.
         for (Tier &T : tiers) {
@@ -896,6 +932,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
             }
             T.wideCount = (int)h->wide.size() - T.wideFirst;
         }
+        DBX_HIP(h, create_tier_streams(h, tiers));
         h->tiers = tiers;
         if (getenv("DBX_VERBOSE"))
             for (const Tier &T : h->tiers)
@@ -1168,7 +1205,8 @@ static dbx_status issue_step(dbx_handle *h, const StepParams &sp, uint32_t flags
     hipStream_t s = h->stream;
     hipEvent_t fork = ev ? ev[0] : h->fork;
     DBX_HIP(h, hipEventRecord(fork, s));
-    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, fork, 0));
+    for (Tier &T : h->tiers)
+        if (T.own) DBX_HIP(h, hipStreamWaitEvent(T.stream, fork, 0));
     for (size_t t = 0; t < h->tiers.size(); ++t) {
         Tier &T = h->tiers[t];
         if (timeTiers) DBX_HIP(h, hipEventRecord(T.t0, T.stream));
@@ -1176,9 +1214,10 @@ static dbx_status issue_step(dbx_handle *h, const StepParams &sp, uint32_t flags
         if (st != DBX_OK) return st;
         if (timeTiers) DBX_HIP(h, hipEventRecord(T.t1, T.stream));
     }
-    for (size_t t = 1; t < h->tiers.size(); ++t) {
-        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
-        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    for (Tier &T : h->tiers) {
+        if (!T.own) continue;
+        DBX_HIP(h, hipEventRecord(T.done, T.stream));
+        DBX_HIP(h, hipStreamWaitEvent(s, T.done, 0));
     }
     if (ev) DBX_HIP(h, hipEventRecord(ev[6], s));
     return DBX_OK;
@@ -1237,11 +1276,13 @@ static dbx_status step_graph(dbx_handle *h, const StepParams &sp, uint32_t flags
     }
     hipStream_t s = h->stream;
     DBX_HIP(h, hipEventRecord(ev[0], s));
-    for (size_t t = 1; t < h->tiers.size(); ++t) DBX_HIP(h, hipStreamWaitEvent(h->tiers[t].stream, ev[0], 0));
+    for (Tier &T : h->tiers)
+        if (T.own) DBX_HIP(h, hipStreamWaitEvent(T.stream, ev[0], 0));
     for (size_t t = 0; t < h->tiers.size(); ++t) DBX_HIP(h, hipGraphLaunch(execs[t], h->tiers[t].stream));
-    for (size_t t = 1; t < h->tiers.size(); ++t) {
-        DBX_HIP(h, hipEventRecord(h->tiers[t].done, h->tiers[t].stream));
-        DBX_HIP(h, hipStreamWaitEvent(s, h->tiers[t].done, 0));
+    for (Tier &T : h->tiers) {
+        if (!T.own) continue;
+        DBX_HIP(h, hipEventRecord(T.done, T.stream));
+        DBX_HIP(h, hipStreamWaitEvent(s, T.done, 0));
     }
     DBX_HIP(h, hipEventRecord(ev[6], s));
     return DBX_OK;
