@@ -249,38 +249,70 @@ def run_cpu_baseline(worlds, start, steps, threads, sample_every, single_every, 
     return {"dyn": dyn, "t_pool": tn, "n": len(sample), "dyn1": dyn1, "t_one": t1, "n1": len(one)}
 
 
-def learned_vs_plain(n_eval: int, steps: int) -> dict:
-    """Velocity iterations to a 1e-3 m/s residual (cap 100), plain warm start vs the
-    learned one, on the first n_eval worlds of the 16,384-world stack set (tall
-    uneven stacks, unseen in training).  The network is trained here, on the GPU,
-    on impulses logged from plain-solver runs of other seeds (pyramids, stacks and
-    containers iterated to a 1e-5 residual)."""
+NETWORK = os.path.join(ROOT, "dbox_amd", "learned", "warmstart_c12.npz")
+
+
+def learned_train(path: str, budget_s: float, target: int) -> dict:
+    """Trains the learned warm start at its specified scale and saves it: new-point
+    samples logged from plain-solver runs of configs 1-2 (pyramids and 20,000-body
+    containers of their own seeds, iterated to a 1e-5 residual) until `target` samples
+    or `budget_s` seconds, then Adam on the GPU with the library's own kernels."""
     from dbox_amd import warmstart
     t0 = time.perf_counter()
-    h = warmstart.collect(warmstart.training_scenes(seed=11, pyramids=4, stacks=48, mixed=16), steps=150)
-    samples = h.mlp_log_count()
-    params, norm, loss = warmstart.train(h, steps=2000, batch=4096, lr=2e-3)
+    h, info = warmstart.collect_spec(target=target, budget_s=budget_s, progress=log)
+    params, norm, loss = warmstart.train_schedule(h)
     h.close()
-    test = scenes.stack_worlds(n_eval)
-    plain = warmstart.iterations(test, steps)
-    learned = warmstart.iterations(test, steps, params=params, norm=norm)
-    cmp = warmstart.compare(plain, learned)
+    meta = {"data": info, "training": {"adam": "4000 steps at 2e-3, 2000 at 1e-3, 2000 at 5e-4", "batch": 8192,
+                                       "loss_first_100": float(np.mean(loss[:100])),
+                                       "loss_last_100": float(np.mean(loss[-100:]))},
+            "command": f"python3 bench.py --learned-train {os.path.relpath(path, ROOT)} --learned-budget {budget_s:g} "
+                       f"--learned-target {target}",
+            "seconds": time.perf_counter() - t0}
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    warmstart.save_network(path, params, norm, meta)
+    return meta
+
+
+def learned_vs_plain(c5_steps: int = 600, c4_steps: int = 60) -> dict:
+    """Velocity iterations to a 1e-3 m/s residual (at most 100 sweeps), the plain warm
+    start against the learned one, over the same world-steps, and the largest
+    penetration: config 5 (16,384 worlds of tall uneven stacks, steps 1-c5_steps) and
+    config 4 (the 50,000-box pyramid, steps 1-c4_steps).  The network is the one
+    trained at the specified scale (bench.py --learned-train), or, if that file is
+    absent, one trained here on a small sample."""
+    from dbox_amd import warmstart
+    t0 = time.perf_counter()
+    if os.path.exists(NETWORK):
+        params, norm, meta = warmstart.load_network(NETWORK)
+        source = {"network": os.path.relpath(NETWORK, ROOT), **meta}
+    else:
+        h = warmstart.collect(warmstart.training_scenes(seed=11, pyramids=4, stacks=48, mixed=16), steps=150)
+        samples = h.mlp_log_count()
+        params, norm, loss = warmstart.train(h, steps=2000, batch=4096, lr=2e-3)
+        h.close()
+        source = {"network": "trained here on a small sample", "samples": int(samples)}
 
     def row(st):
         return {"mean_iterations": st.mean_iterations, "p95_iterations": st.p95_iterations,
                 "capped_fraction": st.capped_fraction, "max_penetration_m": st.max_penetration,
                 "world_steps": st.world_steps, "awake_body_steps": st.awake_body_steps}
 
-    return {
-        "workload": f"first {n_eval} worlds of the 16,384 stack-world set, steps 1-{steps}",
-        "residual_tolerance": 1e-3, "max_iterations": 100,
-        "plain": row(plain), "learned": row(learned),
-        "same_world_steps": cmp,
-        "iteration_reduction": cmp["iteration_reduction"],
-        "training": {"samples": int(samples), "adam_steps": 2000, "batch": 4096,
-                     "loss_first_20": float(np.mean(loss[:20])), "loss_last_20": float(np.mean(loss[-20:]))},
-        "seconds": time.perf_counter() - t0,
-    }
+    out = {"residual_tolerance": 1e-3, "max_iterations": 100, "training": source}
+    for name, worlds, steps in (("config5", scenes.stack_worlds(16384), c5_steps),
+                                ("config4", [scenes.big_pyramid()], c4_steps)):
+        if steps <= 0:
+            continue
+        log(f"learned vs plain warm start: {name}, {steps} steps")
+        plain = warmstart.iterations(worlds, steps)
+        learned = warmstart.iterations(worlds, steps, params=params, norm=norm)
+        cmp = warmstart.compare(plain, learned)
+        out[name] = {"workload": (f"16,384 stack worlds, steps 1-{steps}" if name == "config5" else
+                                  f"pyramid of 50,000 boxes, steps 1-{steps}"),
+                     "plain": row(plain), "learned": row(learned), "same_world_steps": cmp,
+                     "iteration_reduction": cmp["iteration_reduction"]}
+    out["iteration_reduction"] = out["config5"]["iteration_reduction"] if "config5" in out else None
+    out["seconds"] = time.perf_counter() - t0
+    return out
 
 
 def single_world(which: int, steps: int, warmup: int, no_cpu: bool) -> dict:
@@ -362,6 +394,14 @@ def main():
     ap.add_argument("--single-world", type=int, choices=(2, 4), default=0,
                     help="time one big world instead (config 2: 20,000-body container; config 4: 50,000-box "
                          "pyramid) with --steps timed steps after --warmup, beside the oracle on one thread")
+    ap.add_argument("--learned-c5-steps", type=int, default=600,
+                    help="steps of the learned-vs-plain comparison on the 16,384 stack worlds (0: skip)")
+    ap.add_argument("--learned-c4-steps", type=int, default=60,
+                    help="steps of the learned-vs-plain comparison on the 50,000-box pyramid (0: skip)")
+    ap.add_argument("--learned-train", metavar="PATH",
+                    help="train the learned warm start at its specified scale, save it to PATH and exit")
+    ap.add_argument("--learned-budget", type=float, default=420.0, help="seconds of data collection for --learned-train")
+    ap.add_argument("--learned-target", type=int, default=10_000_000, help="samples for --learned-train")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="write the traffic JSON of --kernel from four rocprofv3 --pmc output folders and exit")
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
@@ -382,6 +422,10 @@ def main():
         return
     if args.summarize_trace:
         print(summarize_trace(args))
+        return
+    if args.learned_train:
+        print(json.dumps(learned_train(os.path.abspath(args.learned_train), args.learned_budget,
+                                       args.learned_target)))
         return
     if args.single_world:
         if args.gpus != 1:
@@ -595,7 +639,7 @@ def main():
     h.close()
     if not args.no_learned and not args.subset:
         log("learned vs plain warm start")
-        result["learned_vs_plain"] = learned_vs_plain(256, 150)
+        result["learned_vs_plain"] = learned_vs_plain(args.learned_c5_steps, args.learned_c4_steps)
     if not args.no_cpu:
         info = host_cpu()
         threads = args.cpu_threads or cpu_threads(info)

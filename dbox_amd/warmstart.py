@@ -13,6 +13,8 @@ module only drives it.
 from __future__ import annotations
 
 import dataclasses
+import json
+import time
 
 import numpy as np
 
@@ -121,3 +123,70 @@ def training_scenes(seed: int = 11, pyramids: int = 8, stacks: int = 48, mixed: 
     out += [scenes.stacks_world(seed * 7919 + i) for i in range(stacks)]
     out += [scenes.mixed_container(200, 40.0, 80.0, seed * 104729 + i) for i in range(mixed)]
     return out
+
+
+def spec_sources(pyramids: int = 256, containers: int = 64, seed: int = 7000):
+    """The training worlds at the learned warm start's specified scale: seeded config-1
+    pyramids (20 rows, 210 boxes) and config-2 containers (20,000 bodies in a 200 m x
+    400 m U container), with seeds that no evaluation scene uses."""
+    out = [scenes.pyramid(20, seed=seed + i) for i in range(pyramids)]
+    out += [scenes.big_mixed_world(seed=seed + 500 + i) for i in range(containers)]
+    return out
+
+
+def collect_spec(target: int = 10_000_000, budget_s: float = 420.0, pyramids: int = 256, containers: int = 64,
+                 seed: int = 7000, tolerance: float = 1e-5, max_iterations: int = 100, progress=None):
+    """Plain-solver runs of configs 1-2 iterating to `tolerance` (at most
+    `max_iterations` sweeps), logging every new point's features and the impulses the
+    solver reached for it, until `target` samples or `budget_s` seconds.  Steps 1-120
+    are logged (the pyramids' contacts form then), steps 121-300 run with the plain
+    iteration counts (the containers' bodies are still falling and create few
+    contacts), then every step is logged.  Returns (handle, info)."""
+    t0 = time.perf_counter()
+    h = _handle(spec_sources(pyramids, containers, seed))
+    h.mlp_log_enable(target + (target >> 3))
+    h.set_residual(tolerance, max_iterations)
+    step, logged_steps = 0, 0
+    while True:
+        logging = step < 120 or step >= 300
+        flags = (PLAIN | native.LOG) if logging else (native.WARM | native.SLEEP | native.ORDERED)
+        h.step(DT, VEL, POS, flags)
+        step += 1
+        logged_steps += logging
+        if step % 10 == 0:
+            h.sync()
+            n = h.mlp_log_count()
+            el = time.perf_counter() - t0
+            if progress is not None and step % 50 == 0:
+                progress(f"learned-warm-start data: step {step}, {n} samples, {el:.0f} s")
+            if n >= target or el >= budget_s:
+                break
+    n = min(h.mlp_log_count(), target)
+    info = {"samples": int(n), "steps": step, "logged_steps": logged_steps, "seconds": time.perf_counter() - t0,
+            "sources": f"{pyramids} config-1 pyramids (seeds {seed}..{seed + pyramids - 1}) and {containers} config-2 "
+                       f"containers (seeds {seed + 500}..{seed + 499 + containers}), one batch",
+            "labels": f"impulses after iterating to a {tolerance:g} m/s residual, at most {max_iterations} sweeps"}
+    return h, info
+
+
+def train_schedule(h: native.StepWorlds, schedule=((4000, 2e-3), (2000, 1e-3), (2000, 5e-4)), batch: int = 8192,
+                   init_seed: int = 7, seed: int = 1):
+    """Normalisation fitted to the whole log, then Adam in stages of (steps, learning
+    rate).  Returns (params, norm, per-step loss)."""
+    h.mlp_init(init_seed)
+    h.mlp_fit_normalization()
+    losses = [h.mlp_train(n, batch, lr, seed + i) for i, (n, lr) in enumerate(schedule)]
+    params, norm = h.mlp_get()
+    return params, norm, np.concatenate(losses)
+
+
+def save_network(path: str, params: np.ndarray, norm: np.ndarray, meta: dict) -> None:
+    """The trained network, its feature normalisation and how it was made (JSON), as
+    plain arrays (loadable without pickle)."""
+    np.savez(path, params=np.asarray(params, np.float32), norm=np.asarray(norm, np.float32),
+             meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+
+
+def load_network(path: str):
+    with np.load(path) as z:
+        return z["params"], z["norm"], json.loads(bytes(z["meta"]).decode())
