@@ -171,6 +171,10 @@ struct Dev {
     // per body scratch
     int *adjOff, *adjCnt;   // contact adjacency of each body (CSR)
     int *bisl;              // island of a body this step, -1 when not simulated
+    // island search of worlds too big for one workgroup (allocated when there are any)
+    int *iwParent;          // union-find parent, then the component's root
+    int4 *iwAgg;            // per root: seed (largest awake member), constraints, non-static bodies, island
+    int4 *iwIsl;            // per island: seed, first constraint, first stack slot, constraints
     int *bscratch;          // DFS stack / schedule scratch
     int *wake;              // list position of the first wake in Collide
     unsigned long long *bmask; // colour masks

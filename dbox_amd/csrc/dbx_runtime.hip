@@ -1023,6 +1023,11 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.adjOff, nb));
     DBX_HIP(h, dalloc(h, &d.adjCnt, nb));
     DBX_HIP(h, dalloc(h, &d.bisl, nb));
+    if (!h->wide.empty()) {
+        DBX_HIP(h, dalloc(h, &d.iwParent, nb));
+        DBX_HIP(h, dalloc(h, &d.iwAgg, nb));
+        DBX_HIP(h, dalloc(h, &d.iwIsl, nb));
+    }
     DBX_HIP(h, dalloc(h, &d.bscratch, nb));
     DBX_HIP(h, dalloc(h, &d.wake, nb));
     DBX_HIP(h, dalloc(h, &d.bmask, nb));
@@ -1181,9 +1186,14 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     if (sp.dt > 0.0f) {
         const int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
         DBX_HIP(h, launch_islands(d, cout, ordered, T.first, T.count, T.lds, lead, s));
+        if (ordered && T.wideCount && d.iwParent) DBX_HIP(h, launch_islands_wide(d, cout, wide, T.wideCount, s));
         if (timed) (void)hipEventRecord(ev[2], s);
-        if (ordered) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
-        else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, T.block, lead, s));
+        // Exact order: one wave per world with the state in LDS, except for a tier of
+        // worlds too big for LDS (all with the device-wide broadphase): their levels
+        // (hundreds of constraints wide in a 20,000-body container) are swept from
+        // global memory by a 256-thread workgroup, level batches separated by barriers.
+        if (ordered && T.wideCount < T.count) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
+        else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, ordered ? 256 : T.block, lead, s));
         if (timed) (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
         if (timed) (void)hipEventRecord(ev[4], s);

@@ -236,6 +236,34 @@ def test_island_order_and_levels_match_reference(scene_name):
     assert checked == 8
 
 
+@pytest.mark.parametrize("which,steps,every", [("config 2", 90, 30), ("config 4", 30, 10)])
+def test_big_world_island_order_matches_reference(which, steps, every):
+    # Worlds too big for one workgroup are searched island by island (components by
+    # union-find, islands numbered by descending seed, one wave per island): the
+    # constraint order, island numbers and levels must still be b2World.Solve's.
+    print(f"case: big-world island order, {which}, {steps} steps", flush=True)
+    s = scenes.big_mixed_world() if which == "config 2" else scenes.big_pyramid()
+    flags = native.WARM | native.SLEEP | native.ORDERED
+    o = po.OracleWorld(s, continuous=False, exact_rot=True)
+    g = gpu_world([s])
+    dyn = s.bodies["type"] == 2
+    islands = 0
+    for i in range(steps):
+        o.step()
+        g.step(flags=flags)
+        if (i + 1) % every == 0:
+            assert body_mismatch(g.bodies(), o.bodies(), o.bodies_def["type"]) is None, i
+            ref = o.solve_log()
+            got = g.schedule()
+            assert got.shape[0] == ref.shape[0], i
+            assert np.array_equal(got[:, 0], ref[:, 0]), i
+            assert np.array_equal(got[:, 1:3], ref[:, 1:3]), i
+            assert np.array_equal(got[:, 3], _levels(ref[:, :3], dyn)), i
+            islands = max(islands, int(ref[:, 0].max()) + 1 if len(ref) else 0)
+    print(f"{which}: up to {islands} islands with constraints per step")
+    assert islands > (100 if which == "config 2" else 1)
+
+
 def test_pyramid_60_steps_within_tolerance():
     print("case: pyramid 60 steps in the reference order", flush=True)
     o, g = _lockstep(scenes.pyramid(), 60, native.WARM | native.SLEEP | native.ORDERED)
