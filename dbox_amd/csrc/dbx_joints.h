@@ -48,6 +48,11 @@ struct JointRec {
     float m3[9];   // b2Mat33 (revolute, weld: m_mass; prismatic: m_K): ex.xyz, ey.xyz, ez.xyz
     float pad7[3];
     float m2[4];   // b2Mat22 (friction, motor: m_linearMass; mouse: m_mass): ex.xy, ey.xy
+    // gear (b2gearjoint.d:488-527): joint1 couples C (its body A) to A (its body B),
+    // joint2 D to B; the impulse is ix, the effective mass `mass`
+    int bodyC, bodyD, typeA, typeB;
+    float2 localAnchorC, localAnchorD, localAxisC, localAxisD, lcC, lcD, JvAC, JvBD;
+    float referenceAngleA, referenceAngleB, mC, mD, iC, iD, JwA, JwB, JwC, JwD;
 };
 
 struct J3 {
@@ -159,6 +164,67 @@ __device__ void joint_init(JointRec &j, float4 *bpos, float4 *bvel, const float4
     const bool enableLimit = (j.flags & kJointLimit) != 0, enableMotor = (j.flags & kJointMotorOn) != 0;
     bool write = true;
     switch (j.type) {
+    case kJointGear: { // b2gearjoint.d:245-350: all four velocities read, then written A, B, C, D
+        const float4 mCs = bmass[j.bodyC], mDs = bmass[j.bodyD];
+        j.lcC = v2(mCs.z, mCs.w);
+        j.lcD = v2(mDs.z, mDs.w);
+        j.mC = mCs.x;
+        j.mD = mDs.x;
+        j.iC = mCs.y;
+        j.iD = mDs.y;
+        const float4 pC = bpos[j.bodyC], pD = bpos[j.bodyD], wVC = bvel[j.bodyC], wVD = bvel[j.bodyD];
+        float2 vC = v2(wVC.x, wVC.y), vD = v2(wVD.x, wVD.y);
+        float wC = wVC.z, wD = wVD.z;
+        const Rot qC = rot(pC.z), qD = rot(pD.z);
+        j.mass = 0.0f;
+        if (j.typeA == kJointRevolute) {
+            j.JvAC = v2(0.0f, 0.0f);
+            j.JwA = 1.0f;
+            j.JwC = 1.0f;
+            j.mass += iA + j.iC;
+        } else {
+            const float2 u = mul(qC, j.localAxisC);
+            const float2 rC = mul(qC, sub(j.localAnchorC, j.lcC));
+            const float2 rA = mul(qA, sub(j.localAnchorA, j.localCenterA));
+            j.JvAC = u;
+            j.JwC = crs(rC, u);
+            j.JwA = crs(rA, u);
+            j.mass += j.mC + mA + j.iC * j.JwC * j.JwC + iA * j.JwA * j.JwA;
+        }
+        if (j.typeB == kJointRevolute) {
+            j.JvBD = v2(0.0f, 0.0f);
+            j.JwB = j.ratio;
+            j.JwD = j.ratio;
+            j.mass += j.ratio * j.ratio * (iB + j.iD);
+        } else {
+            const float2 u = mul(qD, j.localAxisD);
+            const float2 rD = mul(qD, sub(j.localAnchorD, j.lcD));
+            const float2 rB = mul(qB, sub(j.localAnchorB, j.localCenterB));
+            j.JvBD = scl(j.ratio, u);
+            j.JwD = j.ratio * crs(rD, u);
+            j.JwB = j.ratio * crs(rB, u);
+            j.mass += j.ratio * j.ratio * (j.mD + mB) + j.iD * j.JwD * j.JwD + iB * j.JwB * j.JwB;
+        }
+        j.mass = j.mass > 0.0f ? 1.0f / j.mass : 0.0f;
+        if (st.warm) {
+            const float P = j.ix;
+            vA = add(vA, scl(mA * P, j.JvAC));
+            wA += iA * P * j.JwA;
+            vB = add(vB, scl(mB * P, j.JvBD));
+            wB += iB * P * j.JwB;
+            vC = sub(vC, scl(j.mC * P, j.JvAC));
+            wC -= j.iC * P * j.JwC;
+            vD = sub(vD, scl(j.mD * P, j.JvBD));
+            wD -= j.iD * P * j.JwD;
+        } else {
+            j.ix = 0.0f;
+        }
+        bvel[j.indexA] = make_float4(vA.x, vA.y, wA, wVA.w);
+        bvel[j.indexB] = make_float4(vB.x, vB.y, wB, wVB.w);
+        bvel[j.bodyC] = make_float4(vC.x, vC.y, wC, wVC.w);
+        bvel[j.bodyD] = make_float4(vD.x, vD.y, wD, wVD.w);
+        return;
+    }
     case kJointDistance: { // b2distancejoint.d:235-310
         j.rA = mul(qA, sub(j.localAnchorA, j.localCenterA));
         j.rB = mul(qB, sub(j.localAnchorB, j.localCenterB));
@@ -559,6 +625,28 @@ __device__ void joint_solve_velocity(JointRec &j, float4 *bvel, const JointStep 
     const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
     const bool enableLimit = (j.flags & kJointLimit) != 0, enableMotor = (j.flags & kJointMotorOn) != 0;
     switch (j.type) {
+    case kJointGear: { // b2gearjoint.d:352-386
+        const float4 wVC = bvel[j.bodyC], wVD = bvel[j.bodyD];
+        float2 vC = v2(wVC.x, wVC.y), vD = v2(wVD.x, wVD.y);
+        float wC = wVC.z, wD = wVD.z;
+        float Cdot = dot(j.JvAC, sub(vA, vC)) + dot(j.JvBD, sub(vB, vD));
+        Cdot += (j.JwA * wA - j.JwC * wC) + (j.JwB * wB - j.JwD * wD);
+        const float impulse = -j.mass * Cdot;
+        j.ix += impulse;
+        vA = add(vA, scl(mA * impulse, j.JvAC));
+        wA += iA * impulse * j.JwA;
+        vB = add(vB, scl(mB * impulse, j.JvBD));
+        wB += iB * impulse * j.JwB;
+        vC = sub(vC, scl(j.mC * impulse, j.JvAC));
+        wC -= j.iC * impulse * j.JwC;
+        vD = sub(vD, scl(j.mD * impulse, j.JvBD));
+        wD -= j.iD * impulse * j.JwD;
+        bvel[j.indexA] = make_float4(vA.x, vA.y, wA, wVA.w);
+        bvel[j.indexB] = make_float4(vB.x, vB.y, wB, wVB.w);
+        bvel[j.bodyC] = make_float4(vC.x, vC.y, wC, wVC.w);
+        bvel[j.bodyD] = make_float4(vD.x, vD.y, wD, wVD.w);
+        return;
+    }
     case kJointDistance: { // b2distancejoint.d:312-336
         const float2 vpA = add(vA, crs(wA, j.rA)), vpB = add(vB, crs(wB, j.rB));
         const float Cdot = dot(j.u, sub(vpB, vpA));
@@ -845,6 +933,66 @@ __device__ bool joint_solve_position(JointRec &j, float4 *bpos) {
     const bool enableLimit = (j.flags & kJointLimit) != 0;
     bool ok = true;
     switch (j.type) {
+    case kJointGear: { // b2gearjoint.d:388-486
+        const float4 pC = bpos[j.bodyC], pD = bpos[j.bodyD];
+        float2 cC = v2(pC.x, pC.y), cD = v2(pD.x, pD.y);
+        float aC = pC.z, aD = pD.z;
+        const Rot qA = rot(aA), qB = rot(aB), qC = rot(aC), qD = rot(aD);
+        float coordinateA = 0.0f, coordinateB = 0.0f;
+        float2 JvAC = v2(0.0f, 0.0f), JvBD = v2(0.0f, 0.0f);
+        float JwA = 0.0f, JwB = 0.0f, JwC = 0.0f, JwD = 0.0f;
+        float mass = 0.0f;
+        if (j.typeA == kJointRevolute) {
+            JwA = 1.0f;
+            JwC = 1.0f;
+            mass += iA + j.iC;
+            coordinateA = aA - aC - j.referenceAngleA;
+        } else {
+            const float2 u = mul(qC, j.localAxisC);
+            const float2 rC = mul(qC, sub(j.localAnchorC, j.lcC));
+            const float2 rA = mul(qA, sub(j.localAnchorA, j.localCenterA));
+            JvAC = u;
+            JwC = crs(rC, u);
+            JwA = crs(rA, u);
+            mass += j.mC + mA + j.iC * JwC * JwC + iA * JwA * JwA;
+            const float2 pC_ = sub(j.localAnchorC, j.lcC);
+            const float2 pA_ = mulT(qC, add(rA, sub(cA, cC)));
+            coordinateA = dot(sub(pA_, pC_), j.localAxisC);
+        }
+        if (j.typeB == kJointRevolute) {
+            JwB = j.ratio;
+            JwD = j.ratio;
+            mass += j.ratio * j.ratio * (iB + j.iD);
+            coordinateB = aB - aD - j.referenceAngleB;
+        } else {
+            const float2 u = mul(qD, j.localAxisD);
+            const float2 rD = mul(qD, sub(j.localAnchorD, j.lcD));
+            const float2 rB = mul(qB, sub(j.localAnchorB, j.localCenterB));
+            JvBD = scl(j.ratio, u);
+            JwD = j.ratio * crs(rD, u);
+            JwB = j.ratio * crs(rB, u);
+            mass += j.ratio * j.ratio * (j.mD + mB) + j.iD * JwD * JwD + iB * JwB * JwB;
+            const float2 pD_ = sub(j.localAnchorD, j.lcD);
+            const float2 pB_ = mulT(qD, add(rB, sub(cB, cD)));
+            coordinateB = dot(sub(pB_, pD_), j.localAxisD);
+        }
+        const float C = (coordinateA + j.ratio * coordinateB) - j.constant;
+        float impulse = 0.0f;
+        if (mass > 0.0f) impulse = -C / mass;
+        cA = add(cA, scl(mA * impulse, JvAC));
+        aA += iA * impulse * JwA;
+        cB = add(cB, scl(mB * impulse, JvBD));
+        aB += iB * impulse * JwB;
+        cC = sub(cC, scl(j.mC * impulse, JvAC));
+        aC -= j.iC * impulse * JwC;
+        cD = sub(cD, scl(j.mD * impulse, JvBD));
+        aD -= j.iD * impulse * JwD;
+        bpos[j.indexA] = make_float4(cA.x, cA.y, aA, pA.w);
+        bpos[j.indexB] = make_float4(cB.x, cB.y, aB, pB.w);
+        bpos[j.bodyC] = make_float4(cC.x, cC.y, aC, pC.w);
+        bpos[j.bodyD] = make_float4(cD.x, cD.y, aD, pD.w);
+        return true; // the linear error is never computed (b2gearjoint.d:484-485)
+    }
     case kJointDistance: { // b2distancejoint.d:338-371
         if (j.frequencyHz > 0.0f) return true;
         const Rot qA = rot(aA), qB = rot(aB);

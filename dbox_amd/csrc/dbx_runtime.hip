@@ -367,11 +367,70 @@ struct WorldHost {
 // b2World.CreateJoint and the constructor of each b2Joint subclass: the members each
 // sets from its definition (b2*joint.d), with body B's transform for the mouse
 // joint's anchor.
+// b2GearJoint's constructor (b2gearjoint.d:83-166): bodies A and B become the two
+// joints' bodies B (the joint's edges follow them), C and D their bodies A; the
+// constant is the two joint coordinates at the bodies' transforms when it is made.
+static bool build_gear(const dbx_joint_def &d, const WorldHost &W, JointRec &j, std::string &err) {
+    const int nj = (int)W.joints.size();
+    if (d.joint1 < 0 || d.joint1 >= nj || d.joint2 < 0 || d.joint2 >= nj) {
+        err = "gear: joint1 and joint2 must be joints of this world made before it";
+        return false;
+    }
+    const JointRec &j1 = W.joints[d.joint1], &j2 = W.joints[d.joint2];
+    auto hinge = [](int t) { return t == DBX_JOINT_REVOLUTE || t == DBX_JOINT_PRISMATIC; };
+    if (!hinge(j1.type) || !hinge(j2.type)) {
+        err = "gear: joint1 and joint2 must be revolute or prismatic joints";
+        return false;
+    }
+    if (!finite(d.ratio)) {
+        err = "joint definition is not finite";
+        return false;
+    }
+    j = JointRec{};
+    j.type = DBX_JOINT_GEAR;
+    j.flags = d.collide_connected ? kJointCollide : 0;
+    j.typeA = j1.type;
+    j.typeB = j2.type;
+    // one joint's coordinate: the relative angle (revolute) or the translation along
+    // the axis (prismatic) of body `b` against body `g`
+    auto coordinate = [&](const JointRec &jt, int g, int b, float refAngle, float2 anchorG, float2 anchorB, float2 axis) {
+        if (jt.type == DBX_JOINT_REVOLUTE) return W.bpos[b].z - W.bpos[g].z - refAngle;
+        const float4 xg = W.bxf[g], xb = W.bxf[b];
+        const float rx = xb.w * anchorB.x - xb.z * anchorB.y, ry = xb.z * anchorB.x + xb.w * anchorB.y;
+        const float vx = rx + (xb.x - xg.x), vy = ry + (xb.y - xg.y);
+        const float px = xg.w * vx + xg.z * vy, py = -xg.z * vx + xg.w * vy; // b2MulT(xfG.q, v)
+        return (px - anchorG.x) * axis.x + (py - anchorG.y) * axis.y;
+    };
+    j.bodyC = j1.bodyA;
+    j.bodyA = j1.bodyB;
+    j.localAnchorC = j1.localAnchorA;
+    j.localAnchorA = j1.localAnchorB;
+    j.referenceAngleA = j1.referenceAngle;
+    j.localAxisC = j1.type == DBX_JOINT_REVOLUTE ? make_float2(0.0f, 0.0f) : j1.localXAxisA;
+    const float coordinateA = coordinate(j1, j.bodyC, j.bodyA, j.referenceAngleA, j.localAnchorC, j.localAnchorA,
+                                         j.localAxisC);
+    j.bodyD = j2.bodyA;
+    j.bodyB = j2.bodyB;
+    j.localAnchorD = j2.localAnchorA;
+    j.localAnchorB = j2.localAnchorB;
+    j.referenceAngleB = j2.referenceAngle;
+    j.localAxisD = j2.type == DBX_JOINT_REVOLUTE ? make_float2(0.0f, 0.0f) : j2.localXAxisA;
+    const float coordinateB = coordinate(j2, j.bodyD, j.bodyB, j.referenceAngleB, j.localAnchorD, j.localAnchorB,
+                                         j.localAxisD);
+    j.ratio = d.ratio;
+    j.constant = coordinateA + j.ratio * coordinateB;
+    j.limitState = kLimitInactive;
+    return true;
+}
+
 bool build_joint(const dbx_joint_def &d, const WorldHost &W, JointRec &j, std::string &err) {
     const int nb = (int)W.bflags.size();
-    if (d.type == 6) {
-        err = "the gear joint is not supported";
-        return false;
+    if (d.type == DBX_JOINT_GEAR) {
+        if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) {
+            err = "bad joint bodies";
+            return false;
+        }
+        return build_gear(d, W, j, err);
     }
     if (d.type < DBX_JOINT_REVOLUTE || d.type > DBX_JOINT_MOTOR) {
         err = "bad joint type";
@@ -788,12 +847,16 @@ dbx_status dbx_add_joints(dbx_handle *h, int32_t world, const dbx_joint_def *joi
     if (world < 0 || world >= (int)h->pending.size()) return fail(h, DBX_E_INVALID_ARG, "bad world index");
     if (n < 0 || (n > 0 && !joints)) return fail(h, DBX_E_INVALID_ARG, "bad joint array");
     WorldHost &W = h->pending[world];
-    std::vector<JointRec> add((size_t)n);
+    const size_t had = W.joints.size();
     std::string err;
-    for (int i = 0; i < n; ++i)
-        if (!build_joint(joints[i], W, add[i], err))
-            return fail(h, joints[i].type == 6 ? DBX_E_UNSUPPORTED : DBX_E_INVALID_ARG, err);
-    W.joints.insert(W.joints.end(), add.begin(), add.end());
+    for (int i = 0; i < n; ++i) { // one at a time: a gear may couple joints made earlier in the same call
+        JointRec j;
+        if (!build_joint(joints[i], W, j, err)) {
+            W.joints.resize(had);
+            return fail(h, DBX_E_INVALID_ARG, err);
+        }
+        W.joints.push_back(j);
+    }
     return DBX_OK;
 }
 

@@ -34,7 +34,7 @@ FIXTURE_DTYPE = np.dtype([
 # the byte layout of dbx_joint_def in include/dbox_hip.h.
 JOINT_DTYPE = np.dtype([
     ("type", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"), ("collide_connected", "<i4"),
-    ("enable_limit", "<i4"), ("enable_motor", "<i4"), ("pad", "<i4", (2,)),
+    ("enable_limit", "<i4"), ("enable_motor", "<i4"), ("joint1", "<i4"), ("joint2", "<i4"),
     ("local_anchor_a", "<f4", (2,)), ("local_anchor_b", "<f4", (2,)), ("local_axis_a", "<f4", (2,)),
     ("reference_angle", "<f4"), ("length", "<f4"), ("frequency_hz", "<f4"), ("damping_ratio", "<f4"),
     ("lower", "<f4"), ("upper", "<f4"), ("motor_speed", "<f4"), ("max_motor", "<f4"),
@@ -43,8 +43,8 @@ JOINT_DTYPE = np.dtype([
     ("length_a", "<f4"), ("length_b", "<f4"), ("ratio", "<f4"), ("target", "<f4", (2,)),
 ])
 assert BODY_DTYPE.itemsize == 56 and FIXTURE_DTYPE.itemsize == 120 and JOINT_DTYPE.itemsize == 148
-# b2JointType (b2joint.d:28-41); the gear joint (6) is not supported
-REVOLUTE, PRISMATIC, DISTANCE, PULLEY, MOUSE, WHEEL, WELD, FRICTION, ROPE, MOTOR = 1, 2, 3, 4, 5, 7, 8, 9, 10, 11
+# b2JointType (b2joint.d:28-41)
+REVOLUTE, PRISMATIC, DISTANCE, PULLEY, MOUSE, GEAR, WHEEL, WELD, FRICTION, ROPE, MOTOR = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 SHAPE_CIRCLE, SHAPE_EDGE, SHAPE_POLYGON, SHAPE_BOX, SHAPE_CHAIN = 0, 1, 2, 3, 4

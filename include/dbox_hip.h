@@ -99,11 +99,14 @@ typedef struct {
 
 /* Joint definition: b2JointDef and the concrete joint definitions
  * (src/dbox/dynamics/joints/b2*joint.d), one record for every kind.  type is
- * b2JointType (b2joint.d:28-41); the gear joint, which couples four bodies through
- * two other joints, is not supported.  Fields a kind does not use are ignored. */
+ * b2JointType (b2joint.d:28-41).  A gear joint (b2GearJointDef) names the two
+ * revolute or prismatic joints it couples in joint1 / joint2 and its ratio in ratio;
+ * its bodies become those joints' bodies B, as in b2GearJoint's constructor, while
+ * body_a / body_b must still be valid, distinct bodies.  Fields a kind does not use
+ * are ignored. */
 enum {
     DBX_JOINT_REVOLUTE = 1, DBX_JOINT_PRISMATIC = 2, DBX_JOINT_DISTANCE = 3, DBX_JOINT_PULLEY = 4,
-    DBX_JOINT_MOUSE = 5, DBX_JOINT_WHEEL = 7, DBX_JOINT_WELD = 8, DBX_JOINT_FRICTION = 9, DBX_JOINT_ROPE = 10,
+    DBX_JOINT_MOUSE = 5, DBX_JOINT_GEAR = 6, DBX_JOINT_WHEEL = 7, DBX_JOINT_WELD = 8, DBX_JOINT_FRICTION = 9, DBX_JOINT_ROPE = 10,
     DBX_JOINT_MOTOR = 11
 };
 typedef struct {
@@ -111,7 +114,8 @@ typedef struct {
     int32_t body_a, body_b;          /* body creation indices */
     int32_t collide_connected;       /* b2JointDef.collideConnected */
     int32_t enable_limit, enable_motor; /* revolute, prismatic (limit, motor); wheel (motor) */
-    int32_t pad[2];
+    int32_t joint1, joint2;          /* gear: the revolute or prismatic joints it couples (creation
+                                      * indices of this world's joints, both created before it) */
     float local_anchor_a[2], local_anchor_b[2]; /* all but motor and mouse */
     float local_axis_a[2];           /* prismatic (normalised on creation), wheel (as given) */
     float reference_angle;           /* revolute, prismatic, weld */
@@ -218,13 +222,14 @@ dbx_status dbx_add_world(dbx_handle *h, const dbx_body_def *bodies, int32_t body
  * order, between dbx_add_world and dbx_finalize (DBX_E_LOCKED after it).  Islands,
  * sleeping and b2Body.ShouldCollide (collideConnected) then follow the joints as in
  * the reference, and each island's joints are solved with its contacts in
- * b2Island.Solve's sequence (b2island.d:137-224).  DBX_E_UNSUPPORTED for the gear
- * joint, DBX_E_INVALID_ARG for a bad body index, equal bodies or a value that is not
- * finite. */
+ * b2Island.Solve's sequence (b2island.d:137-224).  DBX_E_INVALID_ARG (and none of
+ * the n joints added) for a bad body index, equal bodies, a value that is not
+ * finite, or a gear whose joint1 / joint2 is not a revolute or prismatic joint made
+ * before it. */
 dbx_status dbx_add_joints(dbx_handle *h, int32_t world, const dbx_joint_def *joints, int32_t n);
 /* Per joint of one world in creation order, 8 floats: the impulses the joint keeps
  * between steps (revolute, prismatic, weld: m_impulse x, y, z; distance, rope,
- * pulley, wheel: m_impulse in [0]; friction, motor, mouse: linear impulse in [0..1],
+ * pulley, wheel, gear: m_impulse in [0]; friction, motor, mouse: linear impulse in [0..1],
  * angular impulse in [2]), the motor impulse, the wheel's spring impulse, the limit
  * state (b2LimitState), 0, 0.  b2Joint.GetReactionForce / GetReactionTorque are
  * these times 1 / dt (b2*joint.d). */

@@ -2041,19 +2041,29 @@ struct World {
     int createJoint(const orc_joint_def &d) {
         const int nb = (int)bodies.size();
         if (d.body_a < 0 || d.body_a >= nb || d.body_b < 0 || d.body_b >= nb || d.body_a == d.body_b) return -1;
-        if (d.type < j_revolute || d.type > j_motor || d.type == j_gear) return -1;
+        if (d.type < j_revolute || d.type > j_motor) return -1;
         Joint j;
-        jointFromDef(j, d, bodies[d.body_b].xf);
+        if (d.type == j_gear) { /* b2GearJointDef: two revolute or prismatic joints made before it */
+            const int nj = (int)joints.size();
+            if (d.joint1 < 0 || d.joint1 >= nj || d.joint2 < 0 || d.joint2 >= nj) return -1;
+            const Joint &j1 = joints[d.joint1], &j2 = joints[d.joint2];
+            if ((j1.type != j_revolute && j1.type != j_prismatic) || (j2.type != j_revolute && j2.type != j_prismatic))
+                return -1;
+            gearFromDef(j, d, j1, j2, bodies.data());
+        } else {
+            jointFromDef(j, d, bodies[d.body_b].xf);
+        }
         const int ji = (int)joints.size();
         j.prev = kNull;
         j.next = jointList;
         if (jointList != kNull) joints[jointList].prev = ji;
         jointList = ji;
         joints.push_back(j);
+        const int ja = joints[ji].bodyA, jb = joints[ji].bodyB; /* a gear's are its joints' bodies B */
         for (int side = 0; side < 2; ++side) {
-            Body &b = bodies[side == 0 ? d.body_a : d.body_b];
+            Body &b = bodies[side == 0 ? ja : jb];
             JointEdge &e = joints[ji].node[side];
-            e.other = side == 0 ? d.body_b : d.body_a;
+            e.other = side == 0 ? jb : ja;
             e.prev = kNull;
             e.next = b.jointList;
             if (b.jointList != kNull) jedge(b.jointList).prev = 2 * ji + side;
@@ -2831,7 +2841,11 @@ struct World {
         initializeVelocityConstraints();
         if (step.warmStarting) warmStart();
         SolverData data{step, positions.data(), velocities.data()};
-        for (int ji : islandJoints) jointInitVelocity(joints[ji], bodies[joints[ji].bodyA], bodies[joints[ji].bodyB], data);
+        for (int ji : islandJoints) {
+            Joint &jt = joints[ji];
+            if (jt.type == j_gear) gearBodies(jt, bodies[jt.bodyC], bodies[jt.bodyD]);
+            jointInitVelocity(jt, bodies[jt.bodyA], bodies[jt.bodyB], data);
+        }
         profile[3] += msSince(t1);
         auto t2 = Clock::now();
         for (int i = 0; i < step.velocityIterations; ++i) {

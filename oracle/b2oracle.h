@@ -102,7 +102,8 @@ typedef struct {
  * 11 motor; the gear joint, 6, is not supported). */
 typedef struct {
     int32_t type, body_a, body_b, collide_connected;
-    int32_t enable_limit, enable_motor, pad[2];
+    int32_t enable_limit, enable_motor;
+    int32_t joint1, joint2;   /* gear: the two joints it couples, by creation index in the world */
     float local_anchor_a[2], local_anchor_b[2];
     float local_axis_a[2];
     float reference_angle;

@@ -121,6 +121,11 @@ struct Joint {
     float curLength = 0, angularError = 0;
     Mat33 m3;                   /* revolute, weld: m_mass; prismatic: m_K */
     Mat22 m2;                   /* friction, motor: m_linearMass; mouse: m_mass */
+    /* gear (b2gearjoint.d:488-527): joint1 couples C (its body A) to A (its body B), joint2 D to B */
+    int joint1 = -1, joint2 = -1, typeA = 0, typeB = 0, bodyC = 0, bodyD = 0, indexC = 0, indexD = 0;
+    V2 localAnchorC, localAnchorD, localAxisC, localAxisD, lcC, lcD, JvAC, JvBD;
+    float referenceAngleA = 0, referenceAngleB = 0, mC = 0, mD = 0, iC = 0, iD = 0;
+    float JwA = 0, JwB = 0, JwC = 0, JwD = 0;
 };
 
 struct SolverData {
@@ -181,6 +186,68 @@ static void jointFromDef(Joint &j, const orc_joint_def &d, const Xf &xfB) {
     }
 }
 
+/* b2GearJoint's constructor (b2gearjoint.d:83-166): bodies A and B become the two
+ * joints' bodies B (the joint's edges follow them), C and D their bodies A, and the
+ * constant is the two joint coordinates at the bodies' current transforms. */
+static void gearFromDef(Joint &j, const orc_joint_def &d, const Joint &j1, const Joint &j2, const Body *bodies) {
+    j.type = j_gear;
+    j.collideConnected = d.collide_connected != 0;
+    j.joint1 = d.joint1;
+    j.joint2 = d.joint2;
+    j.typeA = j1.type;
+    j.typeB = j2.type;
+    float coordinateA = 0.0f, coordinateB = 0.0f;
+    j.bodyC = j1.bodyA;
+    j.bodyA = j1.bodyB;
+    {
+        const Body &A = bodies[j.bodyA], &C = bodies[j.bodyC];
+        j.localAnchorC = j1.localAnchorA;
+        j.localAnchorA = j1.localAnchorB;
+        j.referenceAngleA = j1.referenceAngle;
+        if (j.typeA == j_revolute) {
+            j.localAxisC = mk(0.0f, 0.0f);
+            coordinateA = A.sweep.a - C.sweep.a - j.referenceAngleA;
+        } else {
+            j.localAxisC = j1.localXAxisA;
+            V2 pC = j.localAnchorC;
+            V2 pA = mulT(C.xf.q, mul(A.xf.q, j.localAnchorA) + (A.xf.p - C.xf.p));
+            coordinateA = dot(pA - pC, j.localAxisC);
+        }
+    }
+    j.bodyD = j2.bodyA;
+    j.bodyB = j2.bodyB;
+    {
+        const Body &B = bodies[j.bodyB], &D = bodies[j.bodyD];
+        j.localAnchorD = j2.localAnchorA;
+        j.localAnchorB = j2.localAnchorB;
+        j.referenceAngleB = j2.referenceAngle;
+        if (j.typeB == j_revolute) {
+            j.localAxisD = mk(0.0f, 0.0f);
+            coordinateB = B.sweep.a - D.sweep.a - j.referenceAngleB;
+        } else {
+            j.localAxisD = j2.localXAxisA;
+            V2 pD = j.localAnchorD;
+            V2 pB = mulT(D.xf.q, mul(B.xf.q, j.localAnchorB) + (B.xf.p - D.xf.p));
+            coordinateB = dot(pB - pD, j.localAxisD);
+        }
+    }
+    j.ratio = d.ratio;
+    j.constant = coordinateA + j.ratio * coordinateB;
+    j.impulse = V3{0.0f, 0.0f, 0.0f};
+}
+
+/* The gear's bodies C and D for the island solve (b2gearjoint.d:247-262) */
+static void gearBodies(Joint &j, const Body &bC, const Body &bD) {
+    j.indexC = bC.islandIndex;
+    j.indexD = bD.islandIndex;
+    j.lcC = bC.sweep.localCenter;
+    j.lcD = bD.sweep.localCenter;
+    j.mC = bC.invMass;
+    j.mD = bD.invMass;
+    j.iC = bC.invI;
+    j.iD = bD.invI;
+}
+
 static void jointBodies(Joint &j, const Body &bA, const Body &bB) {
     j.indexA = bA.islandIndex;
     j.indexB = bB.islandIndex;
@@ -208,6 +275,69 @@ static void jointInitVelocity(Joint &j, const Body &bA, const Body &bB, SolverDa
     qB.set(aB);
     const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
     switch (j.type) {
+    case j_gear: { /* b2gearjoint.d:245-350: all four velocities read, then written A, B, C, D */
+        float aC = data.positions[j.indexC].a;
+        V2 vC = data.velocities[j.indexC].v;
+        float wC = data.velocities[j.indexC].w;
+        float aD = data.positions[j.indexD].a;
+        V2 vD = data.velocities[j.indexD].v;
+        float wD = data.velocities[j.indexD].w;
+        Rot qC, qD;
+        qC.set(aC);
+        qD.set(aD);
+        j.mass = 0.0f;
+        if (j.typeA == j_revolute) {
+            j.JvAC = mk(0.0f, 0.0f);
+            j.JwA = 1.0f;
+            j.JwC = 1.0f;
+            j.mass += iA + j.iC;
+        } else {
+            V2 u = mul(qC, j.localAxisC);
+            V2 rC = mul(qC, j.localAnchorC - j.lcC);
+            V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+            j.JvAC = u;
+            j.JwC = cross(rC, u);
+            j.JwA = cross(rA, u);
+            j.mass += j.mC + mA + j.iC * j.JwC * j.JwC + iA * j.JwA * j.JwA;
+        }
+        if (j.typeB == j_revolute) {
+            j.JvBD = mk(0.0f, 0.0f);
+            j.JwB = j.ratio;
+            j.JwD = j.ratio;
+            j.mass += j.ratio * j.ratio * (iB + j.iD);
+        } else {
+            V2 u = mul(qD, j.localAxisD);
+            V2 rD = mul(qD, j.localAnchorD - j.lcD);
+            V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+            j.JvBD = j.ratio * u;
+            j.JwD = j.ratio * cross(rD, u);
+            j.JwB = j.ratio * cross(rB, u);
+            j.mass += j.ratio * j.ratio * (j.mD + mB) + j.iD * j.JwD * j.JwD + iB * j.JwB * j.JwB;
+        }
+        j.mass = j.mass > 0.0f ? 1.0f / j.mass : 0.0f;
+        if (step.warmStarting) {
+            const float P = j.impulse.x;
+            vA += (mA * P) * j.JvAC;
+            wA += iA * P * j.JwA;
+            vB += (mB * P) * j.JvBD;
+            wB += iB * P * j.JwB;
+            vC -= (j.mC * P) * j.JvAC;
+            wC -= j.iC * P * j.JwC;
+            vD -= (j.mD * P) * j.JvBD;
+            wD -= j.iD * P * j.JwD;
+        } else {
+            j.impulse.x = 0.0f;
+        }
+        data.velocities[j.indexA].v = vA;
+        data.velocities[j.indexA].w = wA;
+        data.velocities[j.indexB].v = vB;
+        data.velocities[j.indexB].w = wB;
+        data.velocities[j.indexC].v = vC;
+        data.velocities[j.indexC].w = wC;
+        data.velocities[j.indexD].v = vD;
+        data.velocities[j.indexD].w = wD;
+        return;
+    }
     case j_distance: { /* b2distancejoint.d:235-310 */
         j.rA = mul(qA, j.localAnchorA - j.localCenterA);
         j.rB = mul(qB, j.localAnchorB - j.localCenterB);
@@ -631,6 +761,33 @@ static void jointSolveVelocity(Joint &j, SolverData &data) {
     float wB = data.velocities[j.indexB].w;
     const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
     switch (j.type) {
+    case j_gear: { /* b2gearjoint.d:352-386 */
+        V2 vC = data.velocities[j.indexC].v;
+        float wC = data.velocities[j.indexC].w;
+        V2 vD = data.velocities[j.indexD].v;
+        float wD = data.velocities[j.indexD].w;
+        float Cdot = dot(j.JvAC, vA - vC) + dot(j.JvBD, vB - vD);
+        Cdot += (j.JwA * wA - j.JwC * wC) + (j.JwB * wB - j.JwD * wD);
+        float impulse = -j.mass * Cdot;
+        j.impulse.x += impulse;
+        vA += (mA * impulse) * j.JvAC;
+        wA += iA * impulse * j.JwA;
+        vB += (mB * impulse) * j.JvBD;
+        wB += iB * impulse * j.JwB;
+        vC -= (j.mC * impulse) * j.JvAC;
+        wC -= j.iC * impulse * j.JwC;
+        vD -= (j.mD * impulse) * j.JvBD;
+        wD -= j.iD * impulse * j.JwD;
+        data.velocities[j.indexA].v = vA;
+        data.velocities[j.indexA].w = wA;
+        data.velocities[j.indexB].v = vB;
+        data.velocities[j.indexB].w = wB;
+        data.velocities[j.indexC].v = vC;
+        data.velocities[j.indexC].w = wC;
+        data.velocities[j.indexD].v = vD;
+        data.velocities[j.indexD].w = wD;
+        return;
+    }
     case j_distance: { /* b2distancejoint.d:312-336 */
         V2 vpA = vA + cross(wA, j.rA);
         V2 vpB = vB + cross(wB, j.rB);
@@ -914,6 +1071,75 @@ static bool jointSolvePosition(Joint &j, SolverData &data) {
     const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
     bool ok = true;
     switch (j.type) {
+    case j_gear: { /* b2gearjoint.d:388-486 */
+        V2 cC = data.positions[j.indexC].c;
+        float aC = data.positions[j.indexC].a;
+        V2 cD = data.positions[j.indexD].c;
+        float aD = data.positions[j.indexD].a;
+        Rot qA, qB, qC, qD;
+        qA.set(aA);
+        qB.set(aB);
+        qC.set(aC);
+        qD.set(aD);
+        float coordinateA = 0.0f, coordinateB = 0.0f;
+        V2 JvAC = mk(0.0f, 0.0f), JvBD = mk(0.0f, 0.0f);
+        float JwA = 0.0f, JwB = 0.0f, JwC = 0.0f, JwD = 0.0f;
+        float mass = 0.0f;
+        if (j.typeA == j_revolute) {
+            JwA = 1.0f;
+            JwC = 1.0f;
+            mass += iA + j.iC;
+            coordinateA = aA - aC - j.referenceAngleA;
+        } else {
+            V2 u = mul(qC, j.localAxisC);
+            V2 rC = mul(qC, j.localAnchorC - j.lcC);
+            V2 rA = mul(qA, j.localAnchorA - j.localCenterA);
+            JvAC = u;
+            JwC = cross(rC, u);
+            JwA = cross(rA, u);
+            mass += j.mC + mA + j.iC * JwC * JwC + iA * JwA * JwA;
+            V2 pC = j.localAnchorC - j.lcC;
+            V2 pA = mulT(qC, rA + (cA - cC));
+            coordinateA = dot(pA - pC, j.localAxisC);
+        }
+        if (j.typeB == j_revolute) {
+            JwB = j.ratio;
+            JwD = j.ratio;
+            mass += j.ratio * j.ratio * (iB + j.iD);
+            coordinateB = aB - aD - j.referenceAngleB;
+        } else {
+            V2 u = mul(qD, j.localAxisD);
+            V2 rD = mul(qD, j.localAnchorD - j.lcD);
+            V2 rB = mul(qB, j.localAnchorB - j.localCenterB);
+            JvBD = j.ratio * u;
+            JwD = j.ratio * cross(rD, u);
+            JwB = j.ratio * cross(rB, u);
+            mass += j.ratio * j.ratio * (j.mD + mB) + j.iD * JwD * JwD + iB * JwB * JwB;
+            V2 pD = j.localAnchorD - j.lcD;
+            V2 pB = mulT(qD, rB + (cB - cD));
+            coordinateB = dot(pB - pD, j.localAxisD);
+        }
+        float C = (coordinateA + j.ratio * coordinateB) - j.constant;
+        float impulse = 0.0f;
+        if (mass > 0.0f) impulse = -C / mass;
+        cA += mA * impulse * JvAC;
+        aA += iA * impulse * JwA;
+        cB += mB * impulse * JvBD;
+        aB += iB * impulse * JwB;
+        cC -= j.mC * impulse * JvAC;
+        aC -= j.iC * impulse * JwC;
+        cD -= j.mD * impulse * JvBD;
+        aD -= j.iD * impulse * JwD;
+        data.positions[j.indexA].c = cA;
+        data.positions[j.indexA].a = aA;
+        data.positions[j.indexB].c = cB;
+        data.positions[j.indexB].a = aB;
+        data.positions[j.indexC].c = cC;
+        data.positions[j.indexC].a = aC;
+        data.positions[j.indexD].c = cD;
+        data.positions[j.indexD].a = aD;
+        return true; /* the linear error is never computed (b2gearjoint.d:484-485) */
+    }
     case j_distance: { /* b2distancejoint.d:338-371 */
         if (j.frequencyHz > 0.0f) return true;
         Rot qA, qB;

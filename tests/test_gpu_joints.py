@@ -81,6 +81,24 @@ def _closed_form_scenes():
     yield "pulley", W([(0, 0, 0, 0), (2, -2, 5, 0), (2, 2, 6, 0)], [b(1, 0.5, 0.5, 2.0), b(2, 0.5, 0.5, 1.0)],
                       [J(S.PULLEY, 1, 2, ground_anchor_a=(-2, 10), ground_anchor_b=(2, 10), length_a=5.0, length_b=4.0,
                          ratio=1.0)])
+    gw = tj._gear_wheels(2.0, 3.0)
+    yield "gear wheels", gw
+    for first in (True, False):
+        prism = J(S.PRISMATIC, 0, 1, local_anchor_a=(0.0, 1.0), local_axis_a=(0.8, 0.6))
+        rev = J(S.REVOLUTE, 0, 2, local_anchor_a=(0.0, 0.0))
+        rp = W([(0, 0, 0, 0), (2, 0, 1, 0.2), (2, 0, 0, 0.7)], [b(1, 2.0, 0.25), c(2, 0.5)],
+               ([prism, rev] if first else [rev, prism]) + [J(S.GEAR, 1, 2, joint1=0, joint2=1, ratio=0.5)])
+        rp.bodies["vx"][1] = 2.0
+        yield f"rack and pinion {'rack' if first else 'pinion'} first", rp
+    # wheels on revolute joints of one falling dynamic frame: the gear's C and D are the
+    # same dynamic body, whose velocity the reference writes twice (C, then D)
+    fr = W([(2, 0, 5, 0), (2, -1, 5, 0), (2, 1, 5, 0), (0, 0, 0, 0)],
+           [b(0, 1.5, 0.2), c(1, 0.4), c(2, 0.6), g],
+           [J(S.REVOLUTE, 0, 1, local_anchor_a=(-1.0, 0.0)), J(S.REVOLUTE, 0, 2, local_anchor_a=(1.0, 0.0)),
+            J(S.GEAR, 1, 2, joint1=0, joint2=1, ratio=-1.5)])
+    fr.fixtures["body"][3] = 3
+    fr.bodies["w"][1] = 6.0
+    yield "gear on a shared frame", fr
 
 
 @pytest.mark.parametrize("name,scene", list(_closed_form_scenes()), ids=[n for n, _ in _closed_form_scenes()])
@@ -136,9 +154,14 @@ def test_joint_definitions_rejected():
     sc = tj.world([(0, 0, 0, 0), (2, 0, 1, 0)], [tj.box(1, 0.5, 0.5)], [])
     h = native.StepWorlds()
     w = h.add_scene(sc)
-    gear = scenes.joint_defs(1, 6)
-    gear["body_a"], gear["body_b"] = 0, 1
-    assert h.L.dbx_add_joints(h.h, w, native._p(gear), 1) == native.DBX_E_UNSUPPORTED
+    # a gear must couple revolute or prismatic joints made before it; a failed batch adds nothing
+    pair = np.concatenate([scenes.joint_defs(1, scenes.DISTANCE), scenes.joint_defs(1, scenes.GEAR)])
+    pair["body_a"], pair["body_b"] = 0, 1
+    pair["joint1"][1], pair["joint2"][1] = 0, 0  # joint 0 is a distance joint
+    assert h.L.dbx_add_joints(h.h, w, native._p(pair), 2) == native.DBX_E_INVALID_ARG
+    gear = scenes.joint_defs(1, scenes.GEAR)
+    gear["body_a"], gear["body_b"], gear["joint1"], gear["joint2"] = 0, 1, 0, 1  # no joints yet
+    assert h.L.dbx_add_joints(h.h, w, native._p(gear), 1) == native.DBX_E_INVALID_ARG
     bad = scenes.joint_defs(1, scenes.REVOLUTE)
     bad["body_a"], bad["body_b"] = 1, 1
     assert h.L.dbx_add_joints(h.h, w, native._p(bad), 1) == native.DBX_E_INVALID_ARG
@@ -146,6 +169,7 @@ def test_joint_definitions_rejected():
     nan["body_a"], nan["body_b"], nan["length"] = 0, 1, np.nan
     assert h.L.dbx_add_joints(h.h, w, native._p(nan), 1) == native.DBX_E_INVALID_ARG
     h.finalize()
+    assert h.L.dbx_read_joints(h.h, w, None, 0) == native.DBX_OK  # none of the rejected joints was added
     ok = scenes.joint_defs(1, scenes.DISTANCE)
     ok["body_a"], ok["body_b"] = 0, 1
     assert h.L.dbx_add_joints(h.h, w, native._p(ok), 1) == native.DBX_E_LOCKED

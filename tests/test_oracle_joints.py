@@ -321,3 +321,116 @@ def test_jointed_bodies_share_an_island():
             if first[k - 1] is None and not (s["flags"][k] & 2):
                 first[k - 1] = n
     assert first[0] < first[1]
+
+
+# ---------------------------------------------------------------- gear
+def _gear_wheels(ratio, w0, ra=0.5, rb=0.8):
+    # wheel A (body 1) and wheel B (body 2) on revolute joints at their centres to the
+    # static ground; joint 2 gears them (b2GearJoint: coordinate A + ratio * coordinate B)
+    sc = world([(0, 0, 0, 0), (2, 0, 0, 0), (2, 3, 0, 0)], [circle(1, ra), circle(2, rb)],
+               [joint(scenes.REVOLUTE, 0, 1), joint(scenes.REVOLUTE, 0, 2, local_anchor_a=(3.0, 0.0)),
+                joint(scenes.GEAR, 1, 2, joint1=0, joint2=1, ratio=ratio)], gravity=(0, 0))
+    sc.bodies["w"][1] = w0
+    return sc
+
+
+@pytest.mark.parametrize("ratio", [2.0, -0.5])
+def test_gear_wheels_closed_form(ratio):
+    # one velocity constraint wA + ratio * wB = 0: the first step's impulse is
+    # P = -w0 / (iA + ratio^2 iB), leaving wA = w0 ratio^2 iB / (iA + ratio^2 iB) and
+    # wB = -ratio iB w0 / (iA + ratio^2 iB), after which nothing changes; the angles keep
+    # aA + ratio * aB = 0 (its value when the gear was made)
+    ra, rb, w0 = 0.5, 0.8, 3.0
+    iA = 1.0 / (0.5 * math.pi * ra ** 4)
+    iB = 1.0 / (0.5 * math.pi * rb ** 4)
+    den = iA + ratio * ratio * iB
+    # one velocity iteration: the impulse is -Cdot / mass exactly, which pins the effective mass
+    o1 = po.OracleWorld(_gear_wheels(ratio, w0, ra, rb), allow_sleep=False)
+    o1.step(DT, 1, 3)
+    assert o1.bodies()["w"][1] == pytest.approx(w0 * ratio * ratio * iB / den, rel=2e-5)
+    o, st = run(_gear_wheels(ratio, w0, ra, rb), 120)
+    for s in st:
+        assert s["w"][1] == pytest.approx(w0 * ratio * ratio * iB / den, rel=2e-5)
+        assert s["w"][2] == pytest.approx(-ratio * iB * w0 / den, rel=2e-5)
+        assert abs(s["a"][1] + ratio * s["a"][2]) < 1e-4
+    assert abs(st[-1]["a"][1]) > 0.2  # it turned
+    # the wheels stay on their pivots
+    assert np.abs(st[-1]["cx"][1:] - (0.0, 3.0)).max() < 1e-4
+
+
+@pytest.mark.parametrize("rack_first", [True, False])
+def test_gear_rack_and_pinion_closed_form(rack_first):
+    # a rack (body 1) on a prismatic joint along x and a pinion (body 2) on a revolute
+    # joint.  Rack as joint1, ratio r: vx + r w = 0 after the first impulse
+    # P = -v0 / (m + r^2 i), vx = v0 r^2 i / (m + r^2 i), w = -r i v0 / (m + r^2 i).
+    # Pinion as joint1: w + r vx = 0, P = -v0 r / (i + r^2 m) for the same push,
+    # vx = v0 + r m P, w = i P (the prismatic Jacobian then carries the ratio)
+    r, v0, rb = 0.5, 2.0, 0.5
+    prism = joint(scenes.PRISMATIC, 0, 1, local_anchor_a=(0.0, 1.0), local_axis_a=(1.0, 0.0))
+    rev = joint(scenes.REVOLUTE, 0, 2)
+    js = [prism, rev] if rack_first else [rev, prism]
+    gear = joint(scenes.GEAR, 1, 2, joint1=0, joint2=1, ratio=r) if rack_first else \
+        joint(scenes.GEAR, 2, 1, joint1=0, joint2=1, ratio=r)
+    sc = world([(0, 0, 0, 0), (2, 0, 1, 0), (2, 0, 0, 0)], [box(1, 2.0, 0.25), circle(2, rb)], js + [gear],
+               gravity=(0, 0))
+    sc.bodies["vx"][1] = v0
+    m = 1.0 / (4.0 * 0.5)
+    i = 1.0 / (0.5 * math.pi * rb ** 4)
+    if rack_first:
+        vx, w = v0 * r * r * i / (m + r * r * i), -r * i * v0 / (m + r * r * i)
+    else:
+        P = -v0 * r / (i + r * r * m)
+        vx, w = v0 + r * m * P, i * P
+    o1 = po.OracleWorld(sc, allow_sleep=False)
+    o1.step(DT, 1, 3)  # one iteration: pins the effective mass
+    assert o1.bodies()["vx"][1] == pytest.approx(vx, rel=2e-5)
+    o, st = run(sc, 90)
+    for s in st:
+        assert s["vx"][1] == pytest.approx(vx, rel=2e-5)
+        assert s["w"][2] == pytest.approx(w, rel=2e-5)
+        lin, ang = s["cx"][1], s["a"][2]
+        assert abs((lin + r * ang) if rack_first else (ang + r * lin)) < 1e-4
+        assert abs(s["vy"][1]) < 1e-6 and abs(s["w"][1]) < 1e-6
+
+
+def test_gear_reacts_on_a_dynamic_carrier():
+    # wheel A turns on a revolute joint of a dynamic carrier disk C, coaxial with it,
+    # and C turns on a revolute joint of the ground at the same point; wheel B turns on
+    # the ground (D static).  Every body only rotates about its pin, so the first
+    # impulse of the velocity constraint (wA - wC) + r wB = 0 is
+    # P = -w0 / (iA + iC + r^2 iB): wA = w0 + iA P, wC = -iC P (the carrier takes the
+    # reaction), wB = r iB P; a sign or inertia error on C moves all three
+    r, w0, ra, rc, rb = 1.5, 3.0, 0.4, 0.7, 0.6
+    sc = world([(0, 0, 0, 0), (2, 0, 0, 0), (2, 0, 0, 0), (2, 3, 0, 0)],
+               [circle(1, ra), circle(2, rc), circle(3, rb)],
+               [joint(scenes.REVOLUTE, 0, 2), joint(scenes.REVOLUTE, 2, 1), joint(scenes.REVOLUTE, 0, 3,
+                                                                                   local_anchor_a=(3.0, 0.0)),
+                joint(scenes.GEAR, 1, 3, joint1=1, joint2=2, ratio=r)], gravity=(0, 0))
+    sc.bodies["w"][1] = w0
+    inv = lambda rad: 1.0 / (0.5 * math.pi * rad ** 4)  # noqa: E731
+    iA, iC, iB = inv(ra), inv(rc), inv(rb)
+    P = -w0 / (iA + iC + r * r * iB)
+    o1 = po.OracleWorld(sc, allow_sleep=False)
+    o1.step(DT, 1, 3)  # one iteration: pins the effective mass
+    assert o1.bodies()["w"][2] == pytest.approx(-iC * P, rel=2e-5)
+    o, st = run(sc, 60)
+    for s_ in st:
+        assert s_["w"][1] == pytest.approx(w0 + iA * P, rel=2e-5)
+        assert s_["w"][2] == pytest.approx(-iC * P, rel=2e-5)
+        assert s_["w"][3] == pytest.approx(r * iB * P, rel=2e-5)
+        assert abs((s_["a"][1] - s_["a"][2]) + r * s_["a"][3]) < 1e-4
+    # the gear joint's impulse is its first read-back value: the first step's P,
+    # warm started and removed again on alternate steps
+    assert o.joints()[3][0] in (pytest.approx(0.0, abs=1e-6), pytest.approx(P, rel=1e-5))
+
+
+def test_gear_definitions_checked():
+    # the gear couples revolute or prismatic joints made before it
+    base = [(0, 0, 0, 0), (2, 0, 0, 0), (2, 3, 0, 0)]
+    for bad in ([joint(scenes.DISTANCE, 0, 1), joint(scenes.REVOLUTE, 0, 2),
+                 joint(scenes.GEAR, 1, 2, joint1=0, joint2=1, ratio=1.0)],
+                [joint(scenes.REVOLUTE, 0, 1), joint(scenes.GEAR, 1, 2, joint1=0, joint2=2, ratio=1.0),
+                 joint(scenes.REVOLUTE, 0, 2)]):
+        sc = world(base, [circle(1, 0.5), circle(2, 0.5)], bad, gravity=(0, 0))
+        with pytest.raises(Exception):
+            po.OracleWorld(sc)
