@@ -24,8 +24,8 @@ hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, i
 // exact order: the island search of the tier's worlds too big for one workgroup
 // (no joints), one island per wave
 hipError_t launch_islands_wide(const Dev &d, int cur, const WideBp *wide, int nwide, hipStream_t s);
-hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block, bool lead,
-                        hipStream_t s);
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block,
+                        int ldsBytes, bool lead, hipStream_t s);
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
                              bool lead, hipStream_t s);
 size_t broadphase_scratch_bytes(int proxies);

@@ -1594,7 +1594,7 @@ __device__ __forceinline__ void warm_start(const VelCon &v, float4 *bvel) {
     if (v.dynMask & 1) bvel[v.indexA] = va;
     if (v.dynMask & 2) bvel[v.indexB] = vb;
 }
-__device__ float solve_velocity(VelCon &v, float4 *bvel, bool wantResidual) {
+__device__ __forceinline__ float solve_velocity(VelCon &v, float4 *bvel, bool wantResidual) {
     float4 va = bvel[v.indexA], vb = bvel[v.indexB];
     float4 imp = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
     float r = wantResidual ? solve_velocity_core<true>(v, imp, va, vb) : solve_velocity_core<false>(v, imp, va, vb);
@@ -1606,326 +1606,12 @@ __device__ float solve_velocity(VelCon &v, float4 *bvel, bool wantResidual) {
     if (v.dynMask & 2) bvel[v.indexB] = vb;
     return r;
 }
-__device__ float solve_position(const PosCon &p, int indexA, int indexB, int dynMask, float4 *bpos) {
+__device__ __forceinline__ float solve_position(const PosCon &p, int indexA, int indexB, int dynMask, float4 *bpos) {
     float4 pa = bpos[indexA], pb = bpos[indexB];
     float ms = solve_position_core(p, pa, pb);
     if (dynMask & 1) bpos[indexA] = pa;
     if (dynMask & 2) bpos[indexB] = pb;
     return ms;
-}
-
-// Inlined into both solve kernels: as a called function it would take the address
-// of the kernel's Dev argument, which puts a copy of Dev in scratch memory and makes
-// the level loops of the LDS path reload device pointers from there.
-__device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead) {
-    __shared__ float s_res;
-    const WorldDesc wd = d.W[w];
-    const int nb = wd.bodyCount, nf = wd.fixCount;
-    const int nc = d.S[w].constraints;
-    const int nbatch = d.S[w].batchCount;
-    const int nisl = d.S[w].islandCount;
-    Contact *cont = d.cont[cur] + wd.contactBase;
-    const float *ctan = d.ctan[cur] + wd.contactBase;
-    uint32_t *bfl = d.bflags + wd.bodyBase;
-    float4 *bxf = d.bxf + wd.bodyBase;
-    float4 *bpos = d.bpos + wd.bodyBase;
-    float4 *bpos0 = d.bpos0 + wd.bodyBase;
-    float4 *bvel = d.bvel + wd.bodyBase;
-    float4 *bforce = d.bforce + wd.bodyBase;
-    const float4 *bmass = d.bmass + wd.bodyBase;
-    const float4 *bparam = d.bparam + wd.bodyBase;
-    const int *bisl = d.bisl + wd.bodyBase;
-    const int *fbody = d.fbody + wd.fixBase;
-    const float4 *fmat = d.fmat + wd.fixBase;
-    const Shape *shp = d.fshape + wd.fixBase;
-    float4 *fat = d.ffat + wd.fixBase;
-    int *moved = d.fmoved + wd.fixBase;
-    const int *ordC = d.ordC + wd.contactBase;
-    const int *cisl = d.cisl + wd.contactBase;
-    const int *sched = d.sched + wd.contactBase;
-    const int *boff = d.batchOff + wd.contactBase + 65 * w;
-    VelCon *vc = d.vc + wd.contactBase;
-    PosCon *pc = d.pc + wd.contactBase;
-    int *islMin = d.islMin + wd.islandBase;
-    int *islSolved = d.islSolved + wd.islandBase;
-    const int tid = threadIdx.x, BS = blockDim.x;
-    const float h = sp.dt;
-
-    // Integrate velocities, damping (b2island.d:82-116).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b];
-        bpos0[b] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-        uint32_t f = bfl[b];
-        if (body_type(f) != kDynamic) continue;
-        float4 vel = bvel[b], frc = bforce[b], ms = bmass[b], pr = bparam[b];
-        float2 v = v2(vel.x, vel.y);
-        float wv = vel.z;
-        float2 acc = add(scl(pr.z, v2(wd.gx, wd.gy)), scl(ms.x, v2(frc.x, frc.y)));
-        v = add(v, scl(h, acc));
-        wv += h * ms.y * frc.z;
-        float ld = 1.0f / (1.0f + h * pr.x);
-        float ad = 1.0f / (1.0f + h * pr.y);
-        v = v2(v.x * ld, v.y * ld);
-        wv *= ad;
-        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
-    }
-    __syncthreads();
-    ProfStamp prof;
-    if (tid == 0) prof.begin(d, lead);
-
-    // Constraint set-up.
-    for (int k = tid; k < nc; k += BS) {
-        int ci = ordC[k];
-        VelCon v;
-        PosCon p;
-        init_constraint(v, p, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp, d.useTan ? ctan[ci] : 0.0f);
-        vc[k] = v;
-        pc[k] = p;
-    }
-    __syncthreads();
-    const ImpRef gimp{reinterpret_cast<float4 *>(&vc[0].nImp0), (int)(sizeof(VelCon) / sizeof(float4))};
-    if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, gimp, bvel, bpos);
-
-    // A world with joints solves each island on a lane of its own, in the sequence of
-    // b2Island.Solve (b2island.d:137-224): contact warm start, joint init, then per
-    // velocity iteration the joints before the contacts, per position iteration the
-    // contacts before the joints.
-    const bool jointWorld = wd.jointCount > 0;
-    const int *icoff = d.islCoff + wd.bodyBase + w, *ijoff = d.islJoff + wd.bodyBase + w;
-    JointRec *J = d.joints + wd.jointBase;
-    const int *ordJ = d.ordJ + wd.jointBase;
-    const JointStep jst{sp.dt, sp.inv_dt, sp.dtRatio, (sp.flags & 1u) != 0};
-    if (jointWorld) {
-        for (int isl = tid; isl < nisl; isl += BS) {
-            if (sp.flags & 1u)
-                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) warm_start(vc[k], bvel);
-            for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_init(J[ordJ[q]], bpos, bvel, bmass, bparam, jst);
-        }
-    } else if (sp.flags & 1u) {
-        for (int bt = 0; bt < nbatch; ++bt) {
-            int e = boff[bt + 1];
-            for (int j = boff[bt] + tid; j < e; j += BS) warm_start(vc[sched[j]], bvel);
-            __syncthreads();
-        }
-    }
-    __syncthreads();
-    prof.at(kProfVel0);
-
-    // Velocity iterations (b2island.d:153-161); with the residual flag the
-    // loop runs until the velocity residual falls below the tolerance.
-    const bool residualMode = (sp.flags & (1u << 5)) != 0;
-    int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
-    int used = 0;
-    float lastRes = residualMode ? 0.0f : -1.0f;
-    for (int it = 0; it < iters; ++it) {
-        if (tid == 0) s_res = 0.0f;
-        __syncthreads();
-        float lres = 0.0f;
-        if (jointWorld) {
-            for (int isl = tid; isl < nisl; isl += BS) {
-                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_solve_velocity(J[ordJ[q]], bvel, jst);
-                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
-                    VelCon v = vc[k];
-                    float r = solve_velocity(v, bvel, residualMode);
-                    lres = fmaxr(lres, r);
-                    *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
-                }
-            }
-            __syncthreads();
-        }
-        for (int bt = 0; bt < nbatch; ++bt) {
-            int e = boff[bt + 1];
-            for (int j = boff[bt] + tid; j < e; j += BS) {
-                int k = sched[j];
-                VelCon v = vc[k];
-                float r = solve_velocity(v, bvel, residualMode);
-                lres = fmaxr(lres, r);
-                *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
-            }
-            __syncthreads();
-        }
-        ++used;
-        if (residualMode) {
-            atomicMax((int *)&s_res, __float_as_int(lres));
-            __syncthreads();
-            float r = s_res;
-            __syncthreads();
-            lastRes = r;
-            if (r <= sp.residualTol) break;
-        }
-    }
-    if (tid == 0) {
-        d.S[w].iterations = used;
-        d.S[w].residual = lastRes;
-    }
-    if (sp.flags & (1u << 6)) mlp_log_labels(d, w, gimp);
-
-    // StoreImpulses (b2contactsolver.d:774-787)
-    for (int k = tid; k < nc; k += BS) {
-        const VelCon &v = vc[k];
-        Contact *c = cont + v.contact;
-        c->pt0.z = v.nImp0;
-        c->pt0.w = v.tImp0;
-        if (v.pointCount > 1) {
-            c->pt1.z = v.nImp1;
-            c->pt1.w = v.tImp1;
-        } else if (man_count(c->meta) > 1) {
-            c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
-        }
-    }
-    __syncthreads();
-    prof.at(kProfVel1);
-    // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b], vel = bvel[b];
-        float2 v = v2(vel.x, vel.y);
-        float wv = vel.z;
-        float2 tr = scl(h, v);
-        if (dot(tr, tr) > kMaxTranslationSquared) {
-            float ratio = kMaxTranslation / sqrtf(tr.x * tr.x + tr.y * tr.y);
-            v = v2(v.x * ratio, v.y * ratio);
-        }
-        float rotation = h * wv;
-        if (rotation * rotation > kMaxRotationSquared) {
-            float ratio = kMaxRotation / fabsr(rotation);
-            wv *= ratio;
-        }
-        bpos[b] = make_float4(pos.x + h * v.x, pos.y + h * v.y, pos.z + h * wv, pos.w);
-        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
-    }
-    for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
-    __syncthreads();
-    prof.at(kProfPos0);
-
-    // Position iterations with the per-island early exit (b2island.d:206-224).
-    if (jointWorld) {
-        for (int isl = tid; isl < nisl; isl += BS) {
-            int solved = 0;
-            for (int it = 0; it < sp.positionIterations; ++it) {
-                float minSep = 0.0f;
-                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
-                    const PosCon p = pc[k];
-                    minSep = fminr(minSep, solve_position(p, p.indexA, p.indexB, p.dynMask, bpos));
-                }
-                const bool contactsOkay = minSep >= -3.0f * kLinearSlop;
-                bool jointsOkay = true;
-                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) {
-                    const bool jointOkay = joint_solve_position(J[ordJ[q]], bpos);
-                    jointsOkay = jointsOkay && jointOkay;
-                }
-                if (contactsOkay && jointsOkay) {
-                    solved = 1;
-                    break;
-                }
-            }
-            islSolved[isl] = solved;
-        }
-        __syncthreads();
-    }
-    for (int it = 0; it < (jointWorld ? 0 : sp.positionIterations); ++it) {
-        for (int i = tid; i < nisl; i += BS) islMin[i] = f2ord(0.0f);
-        __syncthreads();
-        for (int bt = 0; bt < nbatch; ++bt) {
-            int e = boff[bt + 1];
-            for (int j = boff[bt] + tid; j < e; j += BS) {
-                int k = sched[j];
-                const PosCon p = pc[k];
-                int isl = p.island;
-                if (islSolved[isl]) continue;
-                float ms = solve_position(p, p.indexA, p.indexB, p.dynMask, bpos);
-                atomicMin(&islMin[isl], f2ord(ms));
-            }
-            __syncthreads();
-        }
-        int open = 0;
-        for (int i = tid; i < nisl; i += BS) {
-            if (islSolved[i]) continue;
-            if (ord2f(islMin[i]) >= -3.0f * kLinearSlop) islSolved[i] = 1;
-            else open = 1;
-        }
-        if (!__syncthreads_or(open)) break;
-    }
-    __syncthreads();
-
-    // Copy back and SynchronizeTransform (b2island.d:227-235, b2body.d:1143-1147).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b];
-        float4 ms = bmass[b];
-        Xf x = xf_from(v2(pos.x, pos.y), pos.z, v2(ms.z, ms.w));
-        bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
-    }
-    __syncthreads();
-    prof.at(kProfPos1);
-    // Sleep (b2island.d:241-279).
-    if (sp.flags & 2u) {
-        for (int i = tid; i < nisl; i += BS) islMin[i] = __float_as_int(kMaxFloat);
-        __syncthreads();
-        const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
-        const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
-        for (int b = tid; b < nb; b += BS) {
-            int isl = bisl[b];
-            if (isl < 0) continue;
-            uint32_t f = bfl[b];
-            if (body_type(f) == kStatic) continue;
-            const float4 vel = bvel[b];
-            const float st = bpos[b].w;
-            const bool restless = ((f & kBodyAutoSleep) == 0) | (vel.z * vel.z > angTolSqr) |
-                                  ((vel.x * vel.x + vel.y * vel.y) > linTolSqr);
-            const float nst = restless ? 0.0f : st + h;
-            atomicMin(&islMin[isl], __float_as_int(nst));
-            bpos[b].w = nst;
-        }
-        __syncthreads();
-        for (int b = tid; b < nb; b += BS) {
-            int isl = bisl[b];
-            if (isl < 0) continue;
-            if (__int_as_float(islMin[isl]) >= kTimeToSleep && islSolved[isl]) {
-                // SetAwake(false) (b2body.d:836-845)
-                bfl[b] &= ~kBodyAwake;
-                bpos[b].w = 0.0f;
-                bvel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-        }
-    }
-    __syncthreads();
-
-    // SynchronizeFixtures + fat-AABB rule of b2DynamicTree.MoveProxy
-    // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
-    for (int f = tid; f < nf; f += BS) {
-        int b = fbody[f];
-        if (bisl[b] < 0 || body_type(bfl[b]) == kStatic) continue;
-        float4 p0 = bpos0[b];
-        float4 ms = bmass[b];
-        Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
-        Xf x2 = load_xf(bxf, b);
-        float4 a1 = shape_aabb(shp + f, x1);
-        float4 a2 = shape_aabb(shp + f, x2);
-        float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
-        float2 disp = sub(x2.p, x1.p);
-        float4 fa = fat[f];
-        bool contains = fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w;
-        if (contains) continue;
-        float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
-                                 aabb.w + kAabbExtension);
-        float2 dd = scl(kAabbMultiplier, disp);
-        if (dd.x < 0.0f) nbx.x += dd.x;
-        else nbx.z += dd.x;
-        if (dd.y < 0.0f) nbx.y += dd.y;
-        else nbx.w += dd.y;
-        fat[f] = nbx;
-        moved[f] = 1;
-    }
-    // b2World.ClearForces (b2world.d:443-450)
-    for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
-template <bool kLead>
-__global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, int worldOffset) {
-    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x], worldOffset + blockIdx.x == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -2095,6 +1781,529 @@ __device__ __forceinline__ void position_sweep(const int *Lbo, int nbatch, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// A world too big for one workgroup's LDS whose constrained bodies and islands
+// still fit (a 50,000-box pyramid has a few thousand bodies in contact): the
+// constraints' bodies and islands get compact numbers, and the exact-order level
+// sweeps of k_solve_wave run on wave 0 from LDS; integration, sleep and
+// synchronisation of every body stay in global memory (solve_world_global).  The
+// compact order is the order of first discovery, so only storage changes; every
+// constraint sees the bodies' values it would see in global memory.
+// ---------------------------------------------------------------------------
+struct CompactLds {
+    float4 *Lvel, *Lpos, *Limp;
+    int *Lbo, *Lmin, *Lsol, *blist, *ilist;
+    int cb, ci;
+};
+__host__ __device__ inline size_t compact_lds_bytes(int cb, int ci, int nc, int nbatch) {
+    return 2 * isl_align(16 * (size_t)cb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)(nbatch + 1)) +
+           2 * isl_align(4 * (size_t)ci) + isl_align(4 * (size_t)cb) + isl_align(4 * (size_t)ci);
+}
+
+// Constraint set-up, warm start, velocity iterations and StoreImpulses on the compact
+// state; velocities written back.  Returns false, having changed nothing but its
+// scratch maps, when the compact state does not fit in ldsBytes.
+__device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const StepParams &sp, int w, const WorldDesc &wd,
+                                                 int nc, int nbatch, char *lds, int ldsBytes, CompactLds &C) {
+    __shared__ int s_cb, s_ci;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    const float *ctan = d.ctan[cur] + wd.contactBase;
+    const uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    float4 *bvel = d.bvel + wd.bodyBase;
+    const float4 *bmass = d.bmass + wd.bodyBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    const int *ordC = d.ordC + wd.contactBase;
+    const int *cisl = d.cisl + wd.contactBase;
+    const int *sched = d.sched + wd.contactBase;
+    const int *boff = d.batchOff + wd.contactBase + 65 * w;
+    VelCon *vc = d.vc + wd.contactBase;
+    PosCon *pc = d.pc + wd.contactBase;
+    int *bmap = d.bscratch + wd.bodyBase;   // world body -> compact body (the search's stack is free now)
+    int *imap = d.islSolved + wd.islandBase; // island -> compact island (reset before the position phase)
+    for (int k = tid; k < nc; k += BS) {
+        const Contact &c = cont[ordC[k]];
+        bmap[fbody[c.fa]] = -1;
+        bmap[fbody[c.fb]] = -1;
+        imap[cisl[k]] = -1;
+    }
+    if (tid == 0) {
+        s_cb = 0;
+        s_ci = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < nc; k += BS) { // count each body and island once
+        const Contact &c = cont[ordC[k]];
+        if (atomicCAS(&bmap[fbody[c.fa]], -1, -2) == -1) atomicAdd(&s_cb, 1);
+        if (atomicCAS(&bmap[fbody[c.fb]], -1, -2) == -1) atomicAdd(&s_cb, 1);
+        if (atomicCAS(&imap[cisl[k]], -1, -2) == -1) atomicAdd(&s_ci, 1);
+    }
+    __syncthreads();
+    C.cb = s_cb;
+    C.ci = s_ci;
+    if (compact_lds_bytes(C.cb, C.ci, nc, nbatch) > (size_t)ldsBytes) return false;
+    char *p = lds;
+    C.Lvel = (float4 *)p; p += isl_align(16 * (size_t)C.cb);
+    C.Lpos = (float4 *)p; p += isl_align(16 * (size_t)C.cb);
+    C.Limp = (float4 *)p; p += isl_align(16 * (size_t)nc);
+    C.Lbo = (int *)p; p += isl_align(4 * (size_t)(nbatch + 1));
+    C.Lmin = (int *)p; p += isl_align(4 * (size_t)C.ci);
+    C.Lsol = (int *)p; p += isl_align(4 * (size_t)C.ci);
+    C.blist = (int *)p; p += isl_align(4 * (size_t)C.cb);
+    C.ilist = (int *)p;
+    __syncthreads(); // every thread has read the counts
+    if (tid == 0) {
+        s_cb = 0;
+        s_ci = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < nc; k += BS) { // number them
+        const Contact &c = cont[ordC[k]];
+        const int ba = fbody[c.fa], bb = fbody[c.fb], is = cisl[k];
+        if (atomicCAS(&bmap[ba], -2, -3) == -2) {
+            const int i = atomicAdd(&s_cb, 1);
+            C.blist[i] = ba;
+            bmap[ba] = i;
+        }
+        if (atomicCAS(&bmap[bb], -2, -3) == -2) {
+            const int i = atomicAdd(&s_cb, 1);
+            C.blist[i] = bb;
+            bmap[bb] = i;
+        }
+        if (atomicCAS(&imap[is], -2, -3) == -2) {
+            const int i = atomicAdd(&s_ci, 1);
+            C.ilist[i] = is;
+            imap[is] = i;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < C.cb; i += BS) {
+        const int b = C.blist[i];
+        C.Lvel[i] = bvel[b];
+        C.Lpos[i] = bpos[b];
+    }
+    for (int i = tid; i <= nbatch; i += BS) C.Lbo[i] = boff[i];
+    // constraints in schedule order (a level is a contiguous range), renumbered
+    for (int j = tid; j < nc; j += BS) {
+        const int k = sched[j], cix = ordC[k];
+        VelCon v;
+        PosCon q;
+        init_constraint(v, q, cont[cix], cix, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp,
+                        d.useTan ? ctan[cix] : 0.0f);
+        v.indexA = bmap[v.indexA];
+        v.indexB = bmap[v.indexB];
+        q.indexA = bmap[q.indexA];
+        q.indexB = bmap[q.indexB];
+        q.island = imap[q.island];
+        vc[j] = v;
+        pc[j] = q;
+        C.Limp[j] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+    }
+    __syncthreads();
+    const bool residualMode = (sp.flags & (1u << 5)) != 0;
+    if (tid < 64) {
+        const int lane = tid;
+        if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(C.Lbo, nbatch, vc, C.Lvel, C.Limp, lane);
+        const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
+        int used = 0;
+        float lastRes = residualMode ? 0.0f : -1.0f;
+        for (int it = 0; it < iters && nbatch > 0; ++it) {
+            ++used;
+            if (!residualMode) {
+                velocity_sweep<1>(C.Lbo, nbatch, vc, C.Lvel, C.Limp, lane);
+                continue;
+            }
+            float r = velocity_sweep<2>(C.Lbo, nbatch, vc, C.Lvel, C.Limp, lane);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r = fmaxr(r, __shfl_xor(r, o, 64));
+            lastRes = r;
+            if (r <= sp.residualTol) break;
+        }
+        if (lane == 0) {
+            d.S[w].iterations = used;
+            d.S[w].residual = lastRes;
+        }
+    }
+    __syncthreads();
+    // StoreImpulses (b2contactsolver.d:774-787) and the velocities back
+    for (int j = tid; j < nc; j += BS) {
+        const VelCon &v = vc[j];
+        const float4 im = C.Limp[j];
+        Contact *c = cont + v.contact;
+        c->pt0.z = im.x;
+        c->pt0.w = im.y;
+        if (v.pointCount > 1) {
+            c->pt1.z = im.z;
+            c->pt1.w = im.w;
+        } else if (man_count(c->meta) > 1) {
+            c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
+        }
+    }
+    for (int i = tid; i < C.cb; i += BS) bvel[C.blist[i]] = C.Lvel[i];
+    return true;
+}
+
+// Position iterations with the per-island early exit on the compact state, after
+// the positions were integrated in global memory; islSolved set for every island
+// (one without constraints is solved by its first iteration).
+__device__ __forceinline__ void compact_position(const Dev &d, const StepParams &sp, const WorldDesc &wd, int nbatch,
+                                                 int nisl, const CompactLds &C) {
+    const int tid = threadIdx.x, BS = blockDim.x;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    const PosCon *pc = d.pc + wd.contactBase;
+    int *islSolved = d.islSolved + wd.islandBase;
+    for (int i = tid; i < C.cb; i += BS) C.Lpos[i] = bpos[C.blist[i]];
+    for (int i = tid; i < C.ci; i += BS) C.Lsol[i] = 0;
+    for (int i = tid; i < nisl; i += BS) islSolved[i] = sp.positionIterations > 0 ? 1 : 0;
+    __syncthreads();
+    if (tid < 64) {
+        const int lane = tid;
+        for (int it = 0; it < sp.positionIterations; ++it) {
+            for (int i = lane; i < C.ci; i += 64)
+                if (!C.Lsol[i]) C.Lmin[i] = f2ord(0.0f);
+            if (nbatch > 0) position_sweep(C.Lbo, nbatch, pc, C.Lpos, C.Lmin, C.Lsol, lane);
+            bool open = false;
+            for (int i = lane; i < C.ci; i += 64) {
+                if (C.Lsol[i]) continue;
+                if (ord2f(C.Lmin[i]) >= -3.0f * kLinearSlop) C.Lsol[i] = 1;
+                else open = true;
+            }
+            if (!__ballot(open)) break;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < C.cb; i += BS) bpos[C.blist[i]] = C.Lpos[i];
+    for (int i = tid; i < C.ci; i += BS) islSolved[C.ilist[i]] = C.Lsol[i];
+    __syncthreads();
+}
+
+// Inlined into both solve kernels: as a called function it would take the address
+// of the kernel's Dev argument, which puts a copy of Dev in scratch memory and makes
+// the level loops of the LDS path reload device pointers from there.
+__device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead,
+                                                   char *lds, int ldsBytes) {
+    __shared__ float s_res;
+    const WorldDesc wd = d.W[w];
+    const int nb = wd.bodyCount, nf = wd.fixCount;
+    const int nc = d.S[w].constraints;
+    const int nbatch = d.S[w].batchCount;
+    const int nisl = d.S[w].islandCount;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    const float *ctan = d.ctan[cur] + wd.contactBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bxf = d.bxf + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    float4 *bpos0 = d.bpos0 + wd.bodyBase;
+    float4 *bvel = d.bvel + wd.bodyBase;
+    float4 *bforce = d.bforce + wd.bodyBase;
+    const float4 *bmass = d.bmass + wd.bodyBase;
+    const float4 *bparam = d.bparam + wd.bodyBase;
+    const int *bisl = d.bisl + wd.bodyBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    float4 *fat = d.ffat + wd.fixBase;
+    int *moved = d.fmoved + wd.fixBase;
+    const int *ordC = d.ordC + wd.contactBase;
+    const int *cisl = d.cisl + wd.contactBase;
+    const int *sched = d.sched + wd.contactBase;
+    const int *boff = d.batchOff + wd.contactBase + 65 * w;
+    VelCon *vc = d.vc + wd.contactBase;
+    PosCon *pc = d.pc + wd.contactBase;
+    int *islMin = d.islMin + wd.islandBase;
+    int *islSolved = d.islSolved + wd.islandBase;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const float h = sp.dt;
+
+    // Integrate velocities, damping (b2island.d:82-116).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b];
+        bpos0[b] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+        uint32_t f = bfl[b];
+        if (body_type(f) != kDynamic) continue;
+        float4 vel = bvel[b], frc = bforce[b], ms = bmass[b], pr = bparam[b];
+        float2 v = v2(vel.x, vel.y);
+        float wv = vel.z;
+        float2 acc = add(scl(pr.z, v2(wd.gx, wd.gy)), scl(ms.x, v2(frc.x, frc.y)));
+        v = add(v, scl(h, acc));
+        wv += h * ms.y * frc.z;
+        float ld = 1.0f / (1.0f + h * pr.x);
+        float ad = 1.0f / (1.0f + h * pr.y);
+        v = v2(v.x * ld, v.y * ld);
+        wv *= ad;
+        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+    }
+    __syncthreads();
+    ProfStamp prof;
+    if (tid == 0) prof.begin(d, lead);
+
+    // The compact LDS path when the constrained bodies fit (no joints, no learned
+    // warm start or logging, which address bodies by their world index).
+    const bool jointWorld = wd.jointCount > 0;
+    CompactLds C{};
+    const bool compact = !jointWorld && nc > 0 && ldsBytes > 0 && !(sp.flags & ((1u << 4) | (1u << 6))) &&
+                         compact_velocity(d, cur, sp, w, wd, nc, nbatch, lds, ldsBytes, C);
+    if (compact) prof.at(kProfVel0);
+    const int *icoff = d.islCoff + wd.bodyBase + w, *ijoff = d.islJoff + wd.bodyBase + w;
+    JointRec *J = d.joints + wd.jointBase;
+    const int *ordJ = d.ordJ + wd.jointBase;
+    const JointStep jst{sp.dt, sp.inv_dt, sp.dtRatio, (sp.flags & 1u) != 0};
+    if (!compact) {
+    // Constraint set-up.
+    for (int k = tid; k < nc; k += BS) {
+        int ci = ordC[k];
+        VelCon v;
+        PosCon p;
+        init_constraint(v, p, cont[ci], ci, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp, d.useTan ? ctan[ci] : 0.0f);
+        vc[k] = v;
+        pc[k] = p;
+    }
+    __syncthreads();
+    const ImpRef gimp{reinterpret_cast<float4 *>(&vc[0].nImp0), (int)(sizeof(VelCon) / sizeof(float4))};
+    if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, gimp, bvel, bpos);
+
+    // A world with joints solves each island on a lane of its own, in the sequence of
+    // b2Island.Solve (b2island.d:137-224): contact warm start, joint init, then per
+    // velocity iteration the joints before the contacts, per position iteration the
+    // contacts before the joints.
+    if (jointWorld) {
+        for (int isl = tid; isl < nisl; isl += BS) {
+            if (sp.flags & 1u)
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) warm_start(vc[k], bvel);
+            for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_init(J[ordJ[q]], bpos, bvel, bmass, bparam, jst);
+        }
+    } else if (sp.flags & 1u) {
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) warm_start(vc[sched[j]], bvel);
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    prof.at(kProfVel0);
+
+    // Velocity iterations (b2island.d:153-161); with the residual flag the
+    // loop runs until the velocity residual falls below the tolerance.
+    const bool residualMode = (sp.flags & (1u << 5)) != 0;
+    int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
+    int used = 0;
+    float lastRes = residualMode ? 0.0f : -1.0f;
+    for (int it = 0; it < iters; ++it) {
+        if (tid == 0) s_res = 0.0f;
+        __syncthreads();
+        float lres = 0.0f;
+        if (jointWorld) {
+            for (int isl = tid; isl < nisl; isl += BS) {
+                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) joint_solve_velocity(J[ordJ[q]], bvel, jst);
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
+                    VelCon v = vc[k];
+                    float r = solve_velocity(v, bvel, residualMode);
+                    lres = fmaxr(lres, r);
+                    *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+                }
+            }
+            __syncthreads();
+        }
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) {
+                int k = sched[j];
+                VelCon v = vc[k];
+                float r = solve_velocity(v, bvel, residualMode);
+                lres = fmaxr(lres, r);
+                *reinterpret_cast<float4 *>(&vc[k].nImp0) = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+            }
+            __syncthreads();
+        }
+        ++used;
+        if (residualMode) {
+            atomicMax((int *)&s_res, __float_as_int(lres));
+            __syncthreads();
+            float r = s_res;
+            __syncthreads();
+            lastRes = r;
+            if (r <= sp.residualTol) break;
+        }
+    }
+    if (tid == 0) {
+        d.S[w].iterations = used;
+        d.S[w].residual = lastRes;
+    }
+    if (sp.flags & (1u << 6)) mlp_log_labels(d, w, gimp);
+
+    // StoreImpulses (b2contactsolver.d:774-787)
+    for (int k = tid; k < nc; k += BS) {
+        const VelCon &v = vc[k];
+        Contact *c = cont + v.contact;
+        c->pt0.z = v.nImp0;
+        c->pt0.w = v.tImp0;
+        if (v.pointCount > 1) {
+            c->pt1.z = v.nImp1;
+            c->pt1.w = v.tImp1;
+        } else if (man_count(c->meta) > 1) {
+            c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
+        }
+    }
+    }
+    __syncthreads();
+    prof.at(kProfVel1);
+    // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b], vel = bvel[b];
+        float2 v = v2(vel.x, vel.y);
+        float wv = vel.z;
+        float2 tr = scl(h, v);
+        if (dot(tr, tr) > kMaxTranslationSquared) {
+            float ratio = kMaxTranslation / sqrtf(tr.x * tr.x + tr.y * tr.y);
+            v = v2(v.x * ratio, v.y * ratio);
+        }
+        float rotation = h * wv;
+        if (rotation * rotation > kMaxRotationSquared) {
+            float ratio = kMaxRotation / fabsr(rotation);
+            wv *= ratio;
+        }
+        bpos[b] = make_float4(pos.x + h * v.x, pos.y + h * v.y, pos.z + h * wv, pos.w);
+        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+    }
+    for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
+    __syncthreads();
+    prof.at(kProfPos0);
+
+    // Position iterations with the per-island early exit (b2island.d:206-224).
+    if (compact) compact_position(d, sp, wd, nbatch, nisl, C);
+    if (jointWorld) {
+        for (int isl = tid; isl < nisl; isl += BS) {
+            int solved = 0;
+            for (int it = 0; it < sp.positionIterations; ++it) {
+                float minSep = 0.0f;
+                for (int k = icoff[isl]; k < icoff[isl + 1]; ++k) {
+                    const PosCon p = pc[k];
+                    minSep = fminr(minSep, solve_position(p, p.indexA, p.indexB, p.dynMask, bpos));
+                }
+                const bool contactsOkay = minSep >= -3.0f * kLinearSlop;
+                bool jointsOkay = true;
+                for (int q = ijoff[isl]; q < ijoff[isl + 1]; ++q) {
+                    const bool jointOkay = joint_solve_position(J[ordJ[q]], bpos);
+                    jointsOkay = jointsOkay && jointOkay;
+                }
+                if (contactsOkay && jointsOkay) {
+                    solved = 1;
+                    break;
+                }
+            }
+            islSolved[isl] = solved;
+        }
+        __syncthreads();
+    }
+    for (int it = 0; it < (jointWorld || compact ? 0 : sp.positionIterations); ++it) {
+        for (int i = tid; i < nisl; i += BS) islMin[i] = f2ord(0.0f);
+        __syncthreads();
+        for (int bt = 0; bt < nbatch; ++bt) {
+            int e = boff[bt + 1];
+            for (int j = boff[bt] + tid; j < e; j += BS) {
+                int k = sched[j];
+                const PosCon p = pc[k];
+                int isl = p.island;
+                if (islSolved[isl]) continue;
+                float ms = solve_position(p, p.indexA, p.indexB, p.dynMask, bpos);
+                atomicMin(&islMin[isl], f2ord(ms));
+            }
+            __syncthreads();
+        }
+        int open = 0;
+        for (int i = tid; i < nisl; i += BS) {
+            if (islSolved[i]) continue;
+            if (ord2f(islMin[i]) >= -3.0f * kLinearSlop) islSolved[i] = 1;
+            else open = 1;
+        }
+        if (!__syncthreads_or(open)) break;
+    }
+    __syncthreads();
+
+    // Copy back and SynchronizeTransform (b2island.d:227-235, b2body.d:1143-1147).
+    for (int b = tid; b < nb; b += BS) {
+        if (bisl[b] < 0) continue;
+        float4 pos = bpos[b];
+        float4 ms = bmass[b];
+        Xf x = xf_from(v2(pos.x, pos.y), pos.z, v2(ms.z, ms.w));
+        bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+    }
+    __syncthreads();
+    prof.at(kProfPos1);
+    // Sleep (b2island.d:241-279).
+    if (sp.flags & 2u) {
+        for (int i = tid; i < nisl; i += BS) islMin[i] = __float_as_int(kMaxFloat);
+        __syncthreads();
+        const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
+        const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
+        for (int b = tid; b < nb; b += BS) {
+            int isl = bisl[b];
+            if (isl < 0) continue;
+            uint32_t f = bfl[b];
+            if (body_type(f) == kStatic) continue;
+            const float4 vel = bvel[b];
+            const float st = bpos[b].w;
+            const bool restless = ((f & kBodyAutoSleep) == 0) | (vel.z * vel.z > angTolSqr) |
+                                  ((vel.x * vel.x + vel.y * vel.y) > linTolSqr);
+            const float nst = restless ? 0.0f : st + h;
+            atomicMin(&islMin[isl], __float_as_int(nst));
+            bpos[b].w = nst;
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += BS) {
+            int isl = bisl[b];
+            if (isl < 0) continue;
+            if (__int_as_float(islMin[isl]) >= kTimeToSleep && islSolved[isl]) {
+                // SetAwake(false) (b2body.d:836-845)
+                bfl[b] &= ~kBodyAwake;
+                bpos[b].w = 0.0f;
+                bvel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+
+    // SynchronizeFixtures + fat-AABB rule of b2DynamicTree.MoveProxy
+    // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
+    for (int f = tid; f < nf; f += BS) {
+        int b = fbody[f];
+        if (bisl[b] < 0 || body_type(bfl[b]) == kStatic) continue;
+        float4 p0 = bpos0[b];
+        float4 ms = bmass[b];
+        Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
+        Xf x2 = load_xf(bxf, b);
+        float4 a1 = shape_aabb(shp + f, x1);
+        float4 a2 = shape_aabb(shp + f, x2);
+        float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+        float2 disp = sub(x2.p, x1.p);
+        float4 fa = fat[f];
+        bool contains = fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w;
+        if (contains) continue;
+        float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                 aabb.w + kAabbExtension);
+        float2 dd = scl(kAabbMultiplier, disp);
+        if (dd.x < 0.0f) nbx.x += dd.x;
+        else nbx.z += dd.x;
+        if (dd.y < 0.0f) nbx.y += dd.y;
+        else nbx.w += dd.y;
+        fat[f] = nbx;
+        moved[f] = 1;
+    }
+    // b2World.ClearForces (b2world.d:443-450)
+    for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <bool kLead>
+__global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
+    extern __shared__ __align__(16) char lds[];
+    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x], worldOffset + blockIdx.x == 0, lds, ldsBytes);
+}
+
 template <bool kLead>
 __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
@@ -2105,7 +2314,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     const int nbatch = d.S[w].batchCount;
     const int nisl = d.S[w].islandCount;
     if (wd.jointCount > 0 || solve_lds_bytes(nb, nc, nbatch, nisl) > (size_t)ldsBytes) {
-        solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0);
+        solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0, lds, ldsBytes);
         return;
     }
     char *p = lds;
@@ -3226,6 +3435,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     int n = d.S[w].contactCount;
     const int m = min(d.S[w].adjContacts, n);
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && lane == 0;
+    if (timed) {
+        d.ktime[32] = __builtin_amdgcn_s_memrealtime();
+        d.ktime[33] = __builtin_amdgcn_s_memtime();
+    }
     long long tWalk = 0, tEvent = 0, tMark = 0;
     if (timed) d.ktime[25] = __builtin_amdgcn_s_memtime();
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
@@ -3465,6 +3678,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         d.S[w].toiCallbacks = s_L.nlog;
         d.S[w].contactCount = n;
         if (error) d.S[w].error |= error;
+    }
+    if (timed) {
+        d.ktime[34] = __builtin_amdgcn_s_memrealtime();
+        d.ktime[35] = __builtin_amdgcn_s_memtime();
     }
 }
 
@@ -3929,7 +4146,7 @@ size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands
 
 template <bool kLead>
 static hipError_t configure_tier(int maxDynamicLds) {
-    // k_solve_wave and k_broadphase also hold a few bytes of static LDS
+    // the kernels other than k_islands also hold a few bytes of static LDS
     hipError_t e = hipFuncSetAttribute((const void *)k_islands<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        maxDynamicLds);
     if (e != hipSuccess) return e;
@@ -3937,6 +4154,9 @@ static hipError_t configure_tier(int maxDynamicLds) {
                             maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)k_collide<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            maxDynamicLds - 1024);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_solve<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_solve_wave<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3973,11 +4193,11 @@ hipError_t launch_islands_wide(const Dev &d, int cur, const WideBp *wide, int nw
     }
     return hipGetLastError();
 }
-hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block, bool lead,
-                        hipStream_t s) {
+hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int block,
+                        int ldsBytes, bool lead, hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
     if (block > 256) block = 256; // the kernel's launch bound
-    DBX_LAUNCH(k_solve, lead, dim3(nworlds), dim3(block), 0, s, d, cur, sp, worldOffset);
+    DBX_LAUNCH(k_solve, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, sp, worldOffset, ldsBytes);
     return hipGetLastError();
 }
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,

@@ -1208,7 +1208,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     }
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
-    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 32));
+    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 36));
     DBX_HIP(h, dalloc(h, &d.prof, (size_t)kProfStamps * (kProfSlots + 1)));
     DBX_HIP(h, hipStreamSynchronize(h->stream));
     h->pending.clear();
@@ -1256,7 +1256,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
         // (hundreds of constraints wide in a 20,000-body container) are swept from
         // global memory by a 256-thread workgroup, level batches separated by barriers.
         if (ordered && T.wideCount < T.count) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
-        else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, ordered ? 256 : T.block, lead, s));
+        else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, ordered ? 256 : T.block, ordered ? lds1 : 0, lead, s));
         if (timed) (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
         if (timed) (void)hipEventRecord(ev[4], s);
@@ -1648,7 +1648,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->tierSteps = 0;
     }
     if (h->d.ktime && getenv("DBX_KTIME")) {
-        long long t[32];
+        long long t[36];
         if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess) {
             fprintf(stderr, "dbx ktime (shader cycles) islands: stage %lld dfs %lld batch %lld [nc %lld nbatch %lld] | "
                             "solve: stage+init %lld warm %lld vel %lld pos %lld tail %lld\n",
@@ -1661,6 +1661,8 @@ dbx_status dbx_sync(dbx_handle *h) {
             fprintf(stderr, "dbx ktime toi: init+first search %lld, then %lld events: searches %lld events %lld "
                             "(total %lld, %lld contacts)\n",
                     t[26] - t[25], t[28], t[29], t[30], t[27] - t[25], t[31]);
+            fprintf(stderr, "dbx ktime toi kernel entry to exit: %.1f us wall clock, %lld shader cycles\n",
+                    (double)(t[34] - t[32]) * 1000.0 / h->wallKHz, t[35] - t[33]);
         }
     }
     return DBX_OK;
