@@ -76,6 +76,7 @@ struct WorldState {
     int adjContacts;  // contacts the last Collide kept, which its adjacency lists cover
     int toiCallbacks; // listener callbacks the last continuous pass made (may exceed its log's capacity)
     int joints;       // joints the last island search added to islands
+    int toiCand;      // contacts the continuous pass's first search of a big world gave a TOI (its candidate list)
 };
 
 // One listener callback of the continuous pass, laid out as dbx_contact_event:
@@ -214,6 +215,7 @@ struct Dev {
     int4 *toiC;             // per contact slot: (cached TOI as float bits, toiCount | flags, next edge of
                             // body A's and of body B's list of contacts made since the last Collide)
     int4 *toiB;             // per body: (sweep alpha0 as float bits, island flag, head edge of that list, 0)
+    int *toiCand;           // per world from contactBase: the slots that got a TOI in this pass, each once
     const int *bfixOff, *bfixCnt; // per body: its fixtures' range in bfixList
     const int *bfixList;    // per world from fixBase: fixture indices grouped by body
     // joints (b2Joint) of worlds that have them, from each world's jointBase

@@ -47,8 +47,16 @@ struct WideBp {
     float4 *part;           // bounds partials, one per bounds workgroup
     int *ctr;               // [0] pairs found, [1] candidates, [2] sweep axis
     unsigned long long *pkeys; // Pcap
+    // Collide's compaction and contact lists, and the continuous pass's lists of
+    // the contacts made after Collide, over the whole device
+    int nchunk;             // contact chunks of kWideCollideChunk (contact capacity)
+    int2 *chunk;            // nchunk + 1: kept / touching contacts per chunk, then their offsets
+    int *lcnt, *loff;       // bodyCount + 1: new contacts per body, then offsets
+    int *lent;              // 2 * contact capacity: (2 * slot + side) per body
 };
-size_t wide_scratch_bytes(int proxies, int pairCap);
+constexpr int kWideCollideChunk = 1024;
+// b.n, b.cap and b.desc set
+size_t wide_scratch_bytes(const WideBp &b);
 void wide_carve(WideBp &b, char *p);
 hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                              hipStream_t s, const WideBp *wide = nullptr, int nwide = 0);

@@ -253,6 +253,10 @@ __global__ void k_collide(Dev d, int cur, int worldOffset, int ldsBytes) {
         if (wake[b] != INT_MAX) wake_body(bfl, bpos, b);
     __syncthreads();
     if (timed) d.ktime[18] = __builtin_amdgcn_s_memtime();
+    if (pre) { // compaction and contact lists by the k_cw_* kernels over the whole device
+        if (timed) d.ktime[19] = d.ktime[20] = d.ktime[21] = d.ktime[18];
+        return;
+    }
 
     // Stable compaction of the surviving contacts into the other buffer.
     int m = 0, touching = 0;
@@ -2910,6 +2914,7 @@ constexpr int kToiPairCap = 512;        // pairs one event's broadphase query ma
 constexpr int kToiListCap = 128;        // woken bodies / moved proxies per event, sweep snapshots per batch
 constexpr int kToiDirtyCap = 512;       // contacts to look at again after an event
 constexpr int kTcCount = 0xffff, kTcToi = 1 << 16, kTcIsland = 1 << 17, kTcDirty = 1 << 18, kTcSnap = 1 << 19;
+constexpr int kTcCand = 1 << 20; // the slot is in the pass's candidate list (kept until the pass ends)
 
 struct ToiWorld {
     Contact *cont;
@@ -3130,7 +3135,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
     toi_update(W, L, mc, ci);
     cont[ci] = mc;
     const int t0 = W.tc[ci].y;
-    W.tc[ci].y = (((t0 & kTcCount) + 1) & kTcCount) | (t0 & (kTcIsland | kTcDirty | kTcSnap));
+    W.tc[ci].y = (((t0 & kTcCount) + 1) & kTcCount) | (t0 & (kTcIsland | kTcDirty | kTcSnap | kTcCand));
     if (!(mc.flags & kContactEnabled) || !(mc.flags & kContactTouching)) {
         cont[ci].flags = mc.flags & ~kContactEnabled;
         store_sweep(W, ia, backup1);
@@ -3392,26 +3397,85 @@ __device__ __forceinline__ ToiWorld toi_world(const Dev &d, int cur, int w, cons
 // The start of the continuous pass for a world too big for one workgroup (pad
 // bit 1), over the whole device: every sweep at alpha0 = 0 (k_toi_wide_bodies),
 // then every contact's cached TOI reset and, where the first search needs one,
-// computed one contact per lane (b2world.d:1131-1146,1176-1238); k_toi then
-// starts from these.
-__global__ __launch_bounds__(256) void k_toi_wide_bodies(Dev d, int w) {
+// computed one contact per lane (b2world.d:1131-1146,1176-1238), those contacts
+// entering the candidate list; then the contacts made since Collide (slots
+// adjContacts..n) linked into their bodies' lists, each list newest first as
+// the one-at-a-time prepend of b2Contact.Create builds it (counted per body,
+// scanned, scattered, sorted by slot; a static body's list is only searched, so
+// its order is left as scattered).  k_toi then starts from these.
+__global__ __launch_bounds__(256) void k_toi_wide_bodies(Dev d, int w, int *lcnt) {
     const WorldDesc wd = d.W[w];
     int4 *tb = d.toiB + wd.bodyBase;
-    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) tb[b] = make_int4(0, 0, -1, 0);
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) {
+        tb[b] = make_int4(0, 0, -1, 0);
+        lcnt[b] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.S[w].toiCand = 0;
 }
 __global__ __launch_bounds__(64) void k_toi_wide_first(Dev d, int cur, int w) {
     __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices];
     const WorldDesc wd = d.W[w];
     const ToiWorld W = toi_world(d, cur, w, wd);
     const int n = d.S[w].contactCount;
+    int *cand = d.toiCand + wd.contactBase;
     for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
         int4 t = make_int4(__float_as_int(1.0f), 0, -1, -1);
         if (toi_status(W, W.cont[i], 0) == 2) {
             t.x = __float_as_int(contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
                                              s_verts + 2 * kMaxPolygonVertices * threadIdx.x));
-            t.y = kTcToi;
+            t.y = kTcToi | kTcCand;
+            const int slot = atomicAdd(&d.S[w].toiCand, 1);
+            if (slot < wd.contactCap) cand[slot] = i;
         }
         W.tc[i] = t;
+    }
+}
+__global__ __launch_bounds__(256) void k_tl_count(Dev d, int cur, int w, int *lcnt) {
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount, m = min(d.S[w].adjContacts, n);
+    const Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    for (int j = m + blockIdx.x * 256 + (int)threadIdx.x; j < n; j += gridDim.x * 256) {
+        atomicAdd(&lcnt[fbody[cont[j].fa]], 1);
+        atomicAdd(&lcnt[fbody[cont[j].fb]], 1);
+    }
+}
+__global__ __launch_bounds__(256) void k_tl_scatter(Dev d, int cur, int w, int *lcnt, const int *loff, int *lent) {
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount, m = min(d.S[w].adjContacts, n);
+    const Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    for (int j = m + blockIdx.x * 256 + (int)threadIdx.x; j < n; j += gridDim.x * 256) {
+        const int ba = fbody[cont[j].fa], bb = fbody[cont[j].fb];
+        lent[loff[ba] + atomicAdd(&lcnt[ba], 1)] = 2 * j;
+        lent[loff[bb] + atomicAdd(&lcnt[bb], 1)] = 2 * j + 1;
+    }
+}
+__global__ __launch_bounds__(256) void k_tl_link(Dev d, int w, const int *lcnt, const int *loff, int *lent) {
+    const WorldDesc wd = d.W[w];
+    const uint32_t *bfl = d.bflags + wd.bodyBase;
+    int4 *tb = d.toiB + wd.bodyBase;
+    int4 *tc = d.toiC + wd.contactBase;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) {
+        const int cnt = lcnt[b];
+        if (cnt == 0) continue;
+        int *e = lent + loff[b];
+        if (body_type(bfl[b]) != kStatic)
+            for (int i = 1; i < cnt; ++i) { // descending: newest first
+                const int v = e[i];
+                int j = i - 1;
+                while (j >= 0 && e[j] < v) {
+                    e[j + 1] = e[j];
+                    --j;
+                }
+                e[j + 1] = v;
+            }
+        tb[b].z = e[0];
+        for (int i = 0; i < cnt; ++i) {
+            const int x = e[i], next = i + 1 < cnt ? e[i + 1] : -1;
+            if (x & 1) tc[x >> 1].w = next;
+            else tc[x >> 1].z = next;
+        }
     }
 }
 
@@ -3426,7 +3490,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
     __shared__ ToiLists s_L;
     __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices]; // per lane: the two proxies of one TOI
-    __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved;
+    __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved, s_ncand;
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
     const int lane = threadIdx.x;
@@ -3447,14 +3511,20 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
         for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
     }
+    // The slots that got a TOI in this pass, each once (kTcCand): the search for
+    // the first TOI looks at these only, since a slot without kTcToi is passed
+    // over and the minimum over (alpha, list position) does not depend on the
+    // order the slots are looked at in.
+    int *cand = d.toiCand + wd.contactBase;
     if (lane == 0) {
         s_L.woken = s_woken;
         s_L.moved = s_movedList;
         s_L.dirty = s_dirty;
         s_L.nwoken = s_L.nmoved = s_L.ndirty = s_L.overflow = s_L.nlog = 0;
+        s_ncand = pre ? d.S[w].toiCand : 0;
     }
     __syncthreads();
-    if (lane == 0) // contacts made since the Collide, oldest first, so that each list is newest first
+    if (lane == 0 && !pre) // contacts made since the Collide, oldest first, so that each list is newest first
         for (int j = m; j < n; ++j) link_contact(W, j);
     __syncthreads();
     // first search: every sweep is at alpha0 = 0, so no contact advances a body.
@@ -3481,7 +3551,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 const float a = contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
                                             s_verts + 2 * kMaxPolygonVertices * lane);
                 W.tc[i].x = __float_as_int(a);
-                W.tc[i].y = kTcToi;
+                W.tc[i].y = kTcToi | kTcCand;
+                cand[atomicAdd(&s_ncand, 1)] = i;
             }
             __syncthreads();
             if (lane == 0) s_nsnap = 0;
@@ -3539,8 +3610,10 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                     const int j = s_snap[k].contact;
                     const float a =
                         contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k], s_verts + 2 * kMaxPolygonVertices * lane);
+                    const int y = W.tc[j].y;
                     W.tc[j].x = __float_as_int(a);
-                    W.tc[j].y |= kTcToi;
+                    W.tc[j].y = y | kTcToi | kTcCand;
+                    if (!(y & kTcCand)) cand[atomicAdd(&s_ncand, 1)] = j;
                 }
                 __syncthreads();
             }
@@ -3550,21 +3623,28 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             }
             __syncthreads();
         }
-        // the first TOI (b2world.d:1148-1261): smallest alpha, first in list order
+        // the first TOI (b2world.d:1148-1261): smallest alpha, first in list order,
+        // over the candidate slots (every slot, should the list have overrun)
         float bestA = 1.0f;
         int bestPos = INT_MAX;
-        for (int i0 = lane; i0 < n; i0 += 256) { // four independent loads in flight per lane
-            int2 t[4];
+        const bool scanAll = s_ncand > wd.contactCap;
+        const int ns = scanAll ? n : s_ncand;
+        for (int i0 = lane; i0 < ns; i0 += 256) { // four independent loads in flight per lane
+            int slot[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int i = i0 + 64 * u;
-                t[u] = i < n ? make_int2(W.tc[i].x, W.tc[i].y) : make_int2(0, 0);
+                slot[u] = i < ns ? (scanAll ? i : cand[i]) : -1;
             }
+            int2 t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                t[u] = slot[u] >= 0 ? make_int2(W.tc[slot[u]].x, W.tc[slot[u]].y) : make_int2(0, 0);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (!(t[u].y & kTcToi)) continue;
                 const float a = __int_as_float(t[u].x);
-                const int pos = n - 1 - (i0 + 64 * u);
+                const int pos = n - 1 - slot[u];
                 if (a < bestA || (a == bestA && pos < bestPos)) {
                     bestA = a;
                     bestPos = pos;
@@ -3702,21 +3782,29 @@ constexpr int kWideBoundsBlocks = 512; // workgroups of the bounds pass
 constexpr int kWideEmitBlocks = 2048;  // workgroups of the candidate pass
 constexpr int kWideEmitRun = 16;       // consecutive candidates per thread
 
-size_t wide_scratch_bytes(int n, int cap) {
+size_t wide_scratch_bytes(const WideBp &b) {
+    const int n = b.n, cap = b.cap, nb = b.desc.bodyCount, nc = b.desc.contactCap;
     const int P = bp_pow2(n > 0 ? n : 1), Pc = bp_pow2(cap > 0 ? cap : 1), nblk = (n + 255) / 256 + 1;
+    const int nchunk = (nc + kWideCollideChunk - 1) / kWideCollideChunk;
     return isl_align(8 * (size_t)P) + isl_align(4 * (size_t)(n + 1)) + isl_align(4 * (size_t)nblk) +
-           isl_align(16 * (size_t)kWideBoundsBlocks) + isl_align(16) + isl_align(8 * (size_t)Pc);
+           isl_align(16 * (size_t)kWideBoundsBlocks) + isl_align(16) + isl_align(8 * (size_t)Pc) +
+           isl_align(8 * (size_t)(nchunk + 1)) + 2 * isl_align(4 * (size_t)(nb + 1)) + isl_align(8 * (size_t)nc);
 }
 void wide_carve(WideBp &b, char *p) {
     b.P = bp_pow2(b.n > 0 ? b.n : 1);
     b.Pcap = bp_pow2(b.cap > 0 ? b.cap : 1);
     b.nblk = (b.n + 255) / 256;
+    b.nchunk = (b.desc.contactCap + kWideCollideChunk - 1) / kWideCollideChunk;
     b.keys = (unsigned long long *)p; p += isl_align(8 * (size_t)b.P);
     b.cur = (int *)p; p += isl_align(4 * (size_t)(b.n + 1));
     b.bsum = (int *)p; p += isl_align(4 * (size_t)(b.nblk + 1));
     b.part = (float4 *)p; p += isl_align(16 * (size_t)kWideBoundsBlocks);
     b.ctr = (int *)p; p += isl_align(16);
-    b.pkeys = (unsigned long long *)p;
+    b.pkeys = (unsigned long long *)p; p += isl_align(8 * (size_t)b.Pcap);
+    b.chunk = (int2 *)p; p += isl_align(8 * (size_t)(b.nchunk + 1));
+    b.lcnt = (int *)p; p += isl_align(4 * (size_t)(b.desc.bodyCount + 1));
+    b.loff = (int *)p; p += isl_align(4 * (size_t)(b.desc.bodyCount + 1));
+    b.lent = (int *)p;
 }
 
 __device__ __forceinline__ float4 bounds_join(float4 a, float4 b) {
@@ -4126,6 +4214,146 @@ __global__ __launch_bounds__(256) void k_collide_wide(Dev d, int cur, int w) {
     }
 }
 
+// The end of Collide for such a world, after k_collide has updated and woken:
+// the stable compaction of the surviving contacts into the other buffer
+// (kWideCollideChunk contacts per workgroup, four consecutive per thread:
+// counted, the chunk counts scanned, then moved in order), and each body's
+// contact list in list order, newest first (counted, scanned, scattered, and
+// sorted per non-static body; a static body's list is only searched).
+__global__ __launch_bounds__(256) void k_cw_count(Dev d, int cur, int w, int2 *chunk) {
+    __shared__ int tmp[5];
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].contactCount;
+    const Contact *src = d.cont[cur] + wd.contactBase;
+    int *gCnt = d.adjCnt + wd.bodyBase;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) gCnt[b] = 0;
+    const int base = blockIdx.x * kWideCollideChunk + 4 * (int)threadIdx.x;
+    int keep = 0, touch = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int i = base + u;
+        if (i >= n) continue;
+        const uint32_t f = src[i].flags;
+        if (f & kContactDestroyed) continue;
+        ++keep;
+        touch += (f & kContactTouching) ? 1 : 0;
+    }
+    int tk, tt;
+    block_excl_scan(keep, tk, tmp);
+    block_excl_scan(touch, tt, tmp);
+    if (threadIdx.x == 0) chunk[blockIdx.x] = make_int2(tk, tt);
+}
+__global__ __launch_bounds__(1024) void k_cw_scan(Dev d, int w, int2 *chunk, int nchunk) {
+    __shared__ int tmp[17];
+    int run = 0, touch = 0;
+    for (int c0 = 0; c0 < nchunk; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const int2 v = i < nchunk ? chunk[i] : make_int2(0, 0);
+        int tk, tt;
+        const int ex = block_excl_scan(v.x, tk, tmp);
+        block_excl_scan(v.y, tt, tmp);
+        if (i < nchunk) chunk[i].x = run + ex;
+        run += tk;
+        touch += tt;
+    }
+    if (threadIdx.x == 0) {
+        d.S[w].collideCount = d.S[w].contactCount;
+        d.S[w].contactCount = run;
+        d.S[w].touching = touch;
+        d.S[w].adjContacts = run;
+    }
+}
+__global__ __launch_bounds__(256) void k_cw_scatter(Dev d, int cur, int w, const int2 *chunk) {
+    __shared__ int tmp[5];
+    const WorldDesc wd = d.W[w];
+    const int n = d.S[w].collideCount;
+    const Contact *src = d.cont[cur] + wd.contactBase;
+    Contact *dst = d.cont[cur ^ 1] + wd.contactBase;
+    const float *stan = d.ctan[cur] + wd.contactBase;
+    float *dtan = d.ctan[cur ^ 1] + wd.contactBase;
+    const int base = blockIdx.x * kWideCollideChunk + 4 * (int)threadIdx.x;
+    int keep = 0, cnt = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int i = base + u;
+        if (i < n && !(src[i].flags & kContactDestroyed)) {
+            keep |= 1 << u;
+            ++cnt;
+        }
+    }
+    int tot;
+    int off = chunk[blockIdx.x].x + block_excl_scan(cnt, tot, tmp);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (!((keep >> u) & 1)) continue;
+        Contact c = src[base + u];
+        c.flags &= ~kContactDormant;
+        dst[off] = c;
+        if (d.useTan) dtan[off] = stan[base + u];
+        ++off;
+    }
+}
+__global__ __launch_bounds__(256) void k_cw_adj_count(Dev d, int cur, int w) {
+    const WorldDesc wd = d.W[w];
+    const int m = d.S[w].contactCount;
+    const Contact *dst = d.cont[cur ^ 1] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    int *gCnt = d.adjCnt + wd.bodyBase;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < m; k += gridDim.x * 256) {
+        atomicAdd(&gCnt[fbody[dst[k].fa]], 1);
+        atomicAdd(&gCnt[fbody[dst[k].fb]], 1);
+    }
+}
+// Exclusive scan of cnt[0..n) into off, cnt cleared for use as a cursor.
+__global__ __launch_bounds__(1024) void k_count_scan(int *cnt, int *off, int n) {
+    __shared__ int tmp[17];
+    int run = 0;
+    for (int c0 = 0; c0 < n; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const int v = i < n ? cnt[i] : 0;
+        int tot;
+        const int ex = block_excl_scan(v, tot, tmp);
+        if (i < n) {
+            off[i] = run + ex;
+            cnt[i] = 0;
+        }
+        run += tot;
+    }
+}
+__global__ __launch_bounds__(256) void k_cw_adj_scatter(Dev d, int cur, int w) {
+    const WorldDesc wd = d.W[w];
+    const int m = d.S[w].contactCount;
+    const Contact *dst = d.cont[cur ^ 1] + wd.contactBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const int *gOff = d.adjOff + wd.bodyBase;
+    int *gCnt = d.adjCnt + wd.bodyBase;
+    int2 *adj = d.adjList + 2 * wd.contactBase;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < m; k += gridDim.x * 256) {
+        const int ba = fbody[dst[k].fa], bb = fbody[dst[k].fb];
+        adj[gOff[ba] + atomicAdd(&gCnt[ba], 1)] = make_int2(2 * k, bb);
+        adj[gOff[bb] + atomicAdd(&gCnt[bb], 1)] = make_int2(2 * k + 1, ba);
+    }
+}
+__global__ __launch_bounds__(256) void k_cw_adj_sort(Dev d, int w) {
+    const WorldDesc wd = d.W[w];
+    const int *gOff = d.adjOff + wd.bodyBase, *gCnt = d.adjCnt + wd.bodyBase;
+    const uint32_t *bfl = d.bflags + wd.bodyBase;
+    int2 *adj = d.adjList + 2 * wd.contactBase;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < wd.bodyCount; b += gridDim.x * 256) {
+        if (body_type(bfl[b]) == kStatic) continue;
+        const int o = gOff[b], cnt = gCnt[b];
+        for (int i = 1; i < cnt; ++i) {
+            const int2 v = adj[o + i];
+            int j = i - 1;
+            while (j >= 0 && adj[o + j].x < v.x) {
+                adj[o + j + 1] = adj[o + j];
+                --j;
+            }
+            adj[o + j + 1] = v;
+        }
+    }
+}
+
 hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                           hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
@@ -4137,6 +4365,21 @@ hipError_t launch_collide(const Dev &d, int cur, int worldOffset, int nworlds, i
         hipLaunchKernelGGL(k_collide_wide, dim3(gc > 0 ? gc : 1), dim3(256), 0, s, d, cur, wide[k].world);
     }
     DBX_LAUNCH(k_collide, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
+    for (int k = 0; k < nwide; ++k) {
+        const WideBp &b = wide[k];
+        const WorldDesc &D = b.desc;
+        const int w = b.world, cb = b.nchunk > 0 ? b.nchunk : 1;
+        const int nbg = (D.bodyCount + 255) / 256, ncg = (D.contactCap + 255) / 256;
+        const int gb = nbg < 1024 ? (nbg > 0 ? nbg : 1) : 1024, gc = ncg < 4096 ? (ncg > 0 ? ncg : 1) : 4096;
+        hipLaunchKernelGGL(k_cw_count, dim3(cb), dim3(256), 0, s, d, cur, w, b.chunk);
+        hipLaunchKernelGGL(k_cw_scan, dim3(1), dim3(1024), 0, s, d, w, b.chunk, b.nchunk);
+        hipLaunchKernelGGL(k_cw_scatter, dim3(cb), dim3(256), 0, s, d, cur, w, (const int2 *)b.chunk);
+        hipLaunchKernelGGL(k_cw_adj_count, dim3(gc), dim3(256), 0, s, d, cur, w);
+        hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(1024), 0, s, d.adjCnt + D.bodyBase, d.adjOff + D.bodyBase,
+                           D.bodyCount);
+        hipLaunchKernelGGL(k_cw_adj_scatter, dim3(gc), dim3(256), 0, s, d, cur, w);
+        hipLaunchKernelGGL(k_cw_adj_sort, dim3(gb), dim3(256), 0, s, d, w);
+    }
     return hipGetLastError();
 }
 size_t islands_scratch_bytes(int bodies, int contacts) { return isl_bytes(bodies, contacts); }
@@ -4211,11 +4454,18 @@ hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffs
                       hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
     for (int k = 0; k < nwide; ++k) {
-        const WorldDesc &D = wide[k].desc;
-        const int nbg = (D.bodyCount + 255) / 256, ncg = (D.contactCap + 63) / 64;
-        const int gb = nbg < 1024 ? nbg : 1024, gc = ncg < 8192 ? ncg : 8192;
-        hipLaunchKernelGGL(k_toi_wide_bodies, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, wide[k].world);
-        hipLaunchKernelGGL(k_toi_wide_first, dim3(gc > 0 ? gc : 1), dim3(64), 0, s, d, cur, wide[k].world);
+        const WideBp &b = wide[k];
+        const WorldDesc &D = b.desc;
+        const int nbg = (D.bodyCount + 255) / 256, ncg = (D.contactCap + 63) / 64, ncg4 = (D.contactCap + 255) / 256;
+        const int gb = nbg < 1024 ? nbg : 1024, gc = ncg < 8192 ? ncg : 8192, gc4 = ncg4 < 4096 ? ncg4 : 4096;
+        hipLaunchKernelGGL(k_toi_wide_bodies, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, b.world, b.lcnt);
+        hipLaunchKernelGGL(k_toi_wide_first, dim3(gc > 0 ? gc : 1), dim3(64), 0, s, d, cur, b.world);
+        hipLaunchKernelGGL(k_tl_count, dim3(gc4 > 0 ? gc4 : 1), dim3(256), 0, s, d, cur, b.world, b.lcnt);
+        hipLaunchKernelGGL(k_count_scan, dim3(1), dim3(1024), 0, s, b.lcnt, b.loff, D.bodyCount);
+        hipLaunchKernelGGL(k_tl_scatter, dim3(gc4 > 0 ? gc4 : 1), dim3(256), 0, s, d, cur, b.world, b.lcnt,
+                           (const int *)b.loff, b.lent);
+        hipLaunchKernelGGL(k_tl_link, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, b.world, (const int *)b.lcnt,
+                           (const int *)b.loff, b.lent);
     }
     DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(64), 0, s, d, cur, sp, worldOffset);
     return hipGetLastError();

@@ -1106,13 +1106,15 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.bpScratch, bpBytes));
     {
         size_t wideBytes = 0;
-        for (const WideBp &b : h->wide) wideBytes += wide_scratch_bytes(b.n, b.cap);
+        for (WideBp &b : h->wide) {
+            b.desc = h->W[b.world];
+            wideBytes += wide_scratch_bytes(b);
+        }
         char *wp = nullptr;
         if (wideBytes) DBX_HIP(h, dalloc(h, &wp, wideBytes));
         for (WideBp &b : h->wide) {
-            b.desc = h->W[b.world];
             wide_carve(b, wp);
-            wp += wide_scratch_bytes(b.n, b.cap);
+            wp += wide_scratch_bytes(b);
         }
     }
     DBX_HIP(h, dalloc(h, &d.vc, nc));
@@ -1120,6 +1122,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     DBX_HIP(h, dalloc(h, &d.islMin, nb));
     DBX_HIP(h, dalloc(h, &d.toiC, nc));
     DBX_HIP(h, dalloc(h, &d.toiB, nb));
+    DBX_HIP(h, dalloc(h, &d.toiCand, nc));
     // each body's fixtures (b2Body.m_fixtureList as a range), for the continuous pass;
     // the host arrays live until the stream synchronisation below
     std::vector<int> off(nb), cnt(nb), list(nf);
