@@ -3090,16 +3090,27 @@ struct SweepSnap {
 // The contact's TOI as a fraction of the step (b2world.d:1221-1238) from the
 // given sweep starts; the sweeps' ends are the bodies' current c and a.  vbuf:
 // this lane's LDS space for the two proxies' vertices.
-__device__ __noinline__ float contact_toi(const ToiWorld &W, int fa, int fb, const SweepSnap &sn, float2 *vbuf) {
-    Sweep sA = load_sweep(W, W.fbody[fa]), sB = load_sweep(W, W.fbody[fb]);
+// It takes the few arrays it reads, not the ToiWorld: a called function that
+// took the world's table by reference would keep a copy of it in scratch memory.
+__device__ __noinline__ float contact_toi(const Shape *shp, const int *fbody, const float4 *bmass, const float4 *bpos,
+                                         int fa, int fb, SweepSnap sn, float2 *vbuf) {
+    const int ba = fbody[fa], bb = fbody[fb];
+    const float4 mA = bmass[ba], mB = bmass[bb], pa = bpos[ba], pb = bpos[bb];
+    Sweep sA, sB;
+    sA.lc = v2(mA.z, mA.w);
     sA.c0 = v2(sn.ax, sn.ay);
     sA.a0 = sn.aa;
+    sA.c = v2(pa.x, pa.y);
+    sA.a = pa.z;
     sA.alpha0 = sn.aalpha;
+    sB.lc = v2(mB.z, mB.w);
     sB.c0 = v2(sn.bx, sn.by);
     sB.a0 = sn.ba;
+    sB.c = v2(pb.x, pb.y);
+    sB.a = pb.z;
     sB.alpha0 = sn.balpha;
     const float alpha0 = sA.alpha0;
-    const DProxy pA = make_proxy_in(W.shp + fa, vbuf), pB = make_proxy_in(W.shp + fb, vbuf + kMaxPolygonVertices);
+    const DProxy pA = make_proxy_in(shp + fa, vbuf), pB = make_proxy_in(shp + fb, vbuf + kMaxPolygonVertices);
     float beta;
     const int state = time_of_impact(beta, pA, pB, sA, sB, 1.0f);
     return state == kToiTouching ? fminr(alpha0 + (1.0f - alpha0) * beta, 1.0f) : 1.0f;
@@ -3124,8 +3135,9 @@ __device__ __forceinline__ SweepSnap snap_of(const ToiWorld &W, int j) {
 // One TOI event (b2world.d:1270-1405) on one lane: advance both bodies to the
 // TOI, update the contact, build the island and solve it (b2island.d:282-416).
 // Returns the island body count, 0 when the contact was not touching at the TOI.
-__device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, float minAlpha, const StepParams &sp,
-                                      int *ibod, VelCon *vcs, PosCon *pcs, float4 *imp) {
+// ibod, icon, vcs, pcs, imp: the island's bodies, contacts and solver state in LDS.
+__device__ __forceinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, float minAlpha, const StepParams &sp,
+                                         int *ibod, int *icon, VelCon *vcs, PosCon *pcs, float4 *imp) {
     Contact *cont = W.cont;
     const int ia = W.fbody[cont[ci].fa], ib = W.fbody[cont[ci].fb];
     const Sweep backup1 = load_sweep(W, ia), backup2 = load_sweep(W, ib);
@@ -3146,7 +3158,6 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
     }
     toi_wake(W, L, ia);
     toi_wake(W, L, ib);
-    int icon[kMaxToiContacts];
     int nbod = 0, ncon = 0;
     ibod[nbod++] = ia;
     ibod[nbod++] = ib;
@@ -3312,7 +3323,7 @@ __device__ __noinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci, fl
 // b2ContactManager.AddPair filters and b2Contact.Create (b2contactmanager.d:52-176)
 // for a pair that has no contact yet: appends the contact at slot n and puts it
 // at the head of both bodies' lists.  Returns whether a contact was made.
-__device__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int n, int cap, int &error) {
+__device__ __forceinline__ bool toi_add_pair(const ToiWorld &W, ToiLists &L, int fa, int fb, int n, int cap, int &error) {
     const int ba = W.fbody[fa], bb = W.fbody[fb];
     if (ba == bb) return false;
     bool exists = false;
@@ -3421,8 +3432,8 @@ __global__ __launch_bounds__(64) void k_toi_wide_first(Dev d, int cur, int w) {
     for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
         int4 t = make_int4(__float_as_int(1.0f), 0, -1, -1);
         if (toi_status(W, W.cont[i], 0) == 2) {
-            t.x = __float_as_int(contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
-                                             s_verts + 2 * kMaxPolygonVertices * threadIdx.x));
+            t.x = __float_as_int(contact_toi(W.shp, W.fbody, W.bmass, W.bpos, W.cont[i].fa, W.cont[i].fb,
+                                             snap_of(W, i), s_verts + 2 * kMaxPolygonVertices * threadIdx.x));
             t.y = kTcToi | kTcCand;
             const int slot = atomicAdd(&d.S[w].toiCand, 1);
             if (slot < wd.contactCap) cand[slot] = i;
@@ -3481,7 +3492,7 @@ __global__ __launch_bounds__(256) void k_tl_link(Dev d, int w, const int *lcnt, 
 
 template <bool kLead>
 __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
-    __shared__ int s_ibod[2 * kMaxToiContacts];
+    __shared__ int s_ibod[2 * kMaxToiContacts], s_icon[kMaxToiContacts];
     __shared__ VelCon s_vc[kMaxToiContacts];
     __shared__ PosCon s_pc[kMaxToiContacts];
     __shared__ float4 s_imp[kMaxToiContacts];
@@ -3548,8 +3559,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         if (ne > kToiDirtyCap - 256 || (base + 256 >= n && ne > 0)) {
             for (int k = lane; k < ne; k += 64) {
                 const int i = s_dirty[k];
-                const float a = contact_toi(W, W.cont[i].fa, W.cont[i].fb, snap_of(W, i),
-                                            s_verts + 2 * kMaxPolygonVertices * lane);
+                const float a = contact_toi(W.shp, W.fbody, W.bmass, W.bpos, W.cont[i].fa, W.cont[i].fb,
+                                            snap_of(W, i), s_verts + 2 * kMaxPolygonVertices * lane);
                 W.tc[i].x = __float_as_int(a);
                 W.tc[i].y = kTcToi | kTcCand;
                 cand[atomicAdd(&s_ncand, 1)] = i;
@@ -3609,7 +3620,8 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 for (int k = lane; k < ns; k += 64) {
                     const int j = s_snap[k].contact;
                     const float a =
-                        contact_toi(W, W.cont[j].fa, W.cont[j].fb, s_snap[k], s_verts + 2 * kMaxPolygonVertices * lane);
+                        contact_toi(W.shp, W.fbody, W.bmass, W.bpos, W.cont[j].fa, W.cont[j].fb, s_snap[k],
+                                    s_verts + 2 * kMaxPolygonVertices * lane);
                     const int y = W.tc[j].y;
                     W.tc[j].x = __float_as_int(a);
                     W.tc[j].y = y | kTcToi | kTcCand;
@@ -3667,7 +3679,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         if (lane == 0) {
             ToiLists &L = s_L;
             L.nwoken = L.nmoved = 0;
-            s_nb = toi_event(W, L, ci, bestA, sp, s_ibod, s_vc, s_pc, s_imp);
+            s_nb = toi_event(W, L, ci, bestA, sp, s_ibod, s_icon, s_vc, s_pc, s_imp);
             // contacts of bodies the event woke take part again
             for (int k = 0; k < L.nwoken; ++k)
                 body_contacts(W, L.woken[k], [&](int j, int) {
