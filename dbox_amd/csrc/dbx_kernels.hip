@@ -1671,7 +1671,8 @@ struct LevelWindow {
 
 // This lane's slot in level bt: constraints are stored in schedule order, so
 // the slot is the constraint; a lane without one reads slot 0 so that the loads
-// issued for it stay in bounds.
+// issued for it stay in bounds.  kPair: two lanes per constraint.
+template <int kPair = 0>
 __device__ __forceinline__ void level_slot(const LevelWindow &W, int bt, int nbatch, int lane, int &k, bool &have,
                                            int &lo, int &hi) {
     lo = 0;
@@ -1680,7 +1681,7 @@ __device__ __forceinline__ void level_slot(const LevelWindow &W, int bt, int nba
         lo = W.at(bt);
         hi = W.at(bt + 1);
     }
-    const int j = lo + lane;
+    const int j = lo + (lane >> kPair);
     have = j < hi;
     k = have ? j : 0;
 }
@@ -1725,11 +1726,49 @@ __device__ __forceinline__ float velocity_sweep(const int *Lbo, int nbatch, cons
     return lres;
 }
 
+// The position sweeps give each constraint two lanes (an even lane and the odd
+// lane after it), which compute the same constraint except that the even lane
+// evaluates b2Rot.Set of body A's angle and the odd lane of body B's, and the
+// two exchange the results (a DPP quad permutation [1,0,3,2]): each lane then
+// holds both rotations for the cost of one.  The two lanes of a pair always take
+// the same branches, so both are active at every exchange.
+__device__ __forceinline__ float pair_swap(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xb1, 0xf, 0xf, true));
+}
+__device__ __forceinline__ void pair_xfs(float2 cA, float aA, float2 lcA, float2 cB, float aB, float2 lcB, Xf &xfA,
+                                         Xf &xfB) {
+    const bool odd = (threadIdx.x & 1) != 0;
+    const Rot mine = rot(odd ? aB : aA);
+    Rot other;
+    other.s = pair_swap(mine.s);
+    other.c = pair_swap(mine.c);
+    xfA.q = odd ? other : mine;
+    xfB.q = odd ? mine : other;
+    xfA.p = sub(cA, mul(xfA.q, lcA)); // as xf_from
+    xfB.p = sub(cB, mul(xfB.q, lcB));
+}
+// solve_position_core with the transforms built by a lane pair.
+__device__ __forceinline__ float solve_position_pair(const PosCon &p, float4 &pa, float4 &pb) {
+    float2 cA = v2(pa.x, pa.y), cB = v2(pb.x, pb.y);
+    float aA = pa.z, aB = pb.z;
+    Xf xfA, xfB;
+    pair_xfs(cA, aA, p.lcA, cB, aB, p.lcB, xfA, xfB);
+    float minSep = fminr(0.0f, position_point(p, p.lp0, xfA, xfB, cA, aA, cB, aB));
+    if (p.pointCount > 1) {
+        pair_xfs(cA, aA, p.lcA, cB, aB, p.lcB, xfA, xfB);
+        minSep = fminr(minSep, position_point(p, p.lp1, xfA, xfB, cA, aA, cB, aB));
+    }
+    pa = make_float4(cA.x, cA.y, aA, pa.w);
+    pb = make_float4(cB.x, cB.y, aB, pb.w);
+    return minSep;
+}
+
 // As for velocities, both bodies are written back (b2contactsolver.d:141-144);
 // zero inverse mass and inertia leave a static or kinematic body where it is.
+// Both lanes of a pair write the same values.
 __device__ __forceinline__ void position_one(const PosCon &q, float4 *Lpos, int *Lmin) {
     float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
-    const float ms = solve_position_core(q, pa, pb);
+    const float ms = solve_position_pair(q, pa, pb);
     Lpos[q.indexA] = pa;
     Lpos[q.indexB] = pb;
     atomicMin(&Lmin[q.island], f2ord(ms));
@@ -1746,13 +1785,13 @@ __device__ __forceinline__ void position_level(const PosCon &q, bool have, int l
     const int solved = Lsol[q.island];
     asm volatile("" ::"v"(pa.x), "v"(pb.x), "v"(solved));
     if (have && !solved) {
-        const float ms = solve_position_core(q, pa, pb);
+        const float ms = solve_position_pair(q, pa, pb);
         Lpos[q.indexA] = pa;
         Lpos[q.indexB] = pb;
         atomicMin(&Lmin[q.island], f2ord(ms));
     }
-    for (int c = lo + 64; c < hi; c += 64) {
-        const int j2 = c + (int)(threadIdx.x & 63);
+    for (int c = lo + 32; c < hi; c += 32) { // levels wider than the 32 lane pairs
+        const int j2 = c + (int)((threadIdx.x & 63) >> 1);
         if (j2 < hi) {
             const PosCon q2 = pc[j2];
             if (!Lsol[q2.island]) position_one(q2, Lpos, Lmin);
@@ -1760,27 +1799,27 @@ __device__ __forceinline__ void position_level(const PosCon &q, bool have, int l
     }
 }
 
-// One position iteration over the levels, with the same two-set pipeline;
-// islands already solved are skipped (b2island.d:206-224 stops iterating an
-// island once it is solved).
+// One position iteration over the levels, with the same two-set pipeline and a
+// lane pair per constraint; islands already solved are skipped
+// (b2island.d:206-224 stops iterating an island once it is solved).
 __device__ __forceinline__ void position_sweep(const int *Lbo, int nbatch, const PosCon *__restrict__ pc,
                                                float4 *Lpos, int *Lmin, const int *Lsol, int lane) {
     LevelWindow W{Lbo, nbatch + 1, 0, 0};
     W.load(0);
     int k0, k1, lo0, hi0, lo1, hi1;
     bool h0, h1;
-    level_slot(W, 0, nbatch, lane, k0, h0, lo0, hi0);
+    level_slot<1>(W, 0, nbatch, lane, k0, h0, lo0, hi0);
     PosCon c0 = pc[k0];
-    level_slot(W, 1, nbatch, lane, k1, h1, lo1, hi1);
+    level_slot<1>(W, 1, nbatch, lane, k1, h1, lo1, hi1);
     PosCon c1 = pc[k1];
     for (int bt = 0; bt < nbatch; bt += 2) {
         W.cover(bt);
         position_level(c0, h0, lo0, hi0, pc, Lpos, Lmin, Lsol);
-        level_slot(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
+        level_slot<1>(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
         c0 = pc[k0];
         if (bt + 1 >= nbatch) break;
         position_level(c1, h1, lo1, hi1, pc, Lpos, Lmin, Lsol);
-        level_slot(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
+        level_slot<1>(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
         c1 = pc[k1];
     }
 }
@@ -1827,6 +1866,8 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
     PosCon *pc = d.pc + wd.contactBase;
     int *bmap = d.bscratch + wd.bodyBase;   // world body -> compact body (the search's stack is free now)
     int *imap = d.islSolved + wd.islandBase; // island -> compact island (reset before the position phase)
+    const bool timed = d.ktime && tid == 0 && w == d.order[0];
+    if (timed) d.ktime[36] = __builtin_amdgcn_s_memtime();
     for (int k = tid; k < nc; k += BS) {
         const Contact &c = cont[ordC[k]];
         bmap[fbody[c.fa]] = -1;
@@ -1847,6 +1888,7 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
     __syncthreads();
     C.cb = s_cb;
     C.ci = s_ci;
+    if (timed) d.ktime[37] = __builtin_amdgcn_s_memtime();
     if (compact_lds_bytes(C.cb, C.ci, nc, nbatch) > (size_t)ldsBytes) return false;
     char *p = lds;
     C.Lvel = (float4 *)p; p += isl_align(16 * (size_t)C.cb);
@@ -1883,6 +1925,7 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
         }
     }
     __syncthreads();
+    if (timed) d.ktime[38] = __builtin_amdgcn_s_memtime();
     for (int i = tid; i < C.cb; i += BS) {
         const int b = C.blist[i];
         C.Lvel[i] = bvel[b];
@@ -1906,10 +1949,12 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
         C.Limp[j] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
     }
     __syncthreads();
+    if (timed) d.ktime[39] = __builtin_amdgcn_s_memtime();
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
     if (tid < 64) {
         const int lane = tid;
         if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(C.Lbo, nbatch, vc, C.Lvel, C.Limp, lane);
+        if (timed) d.ktime[40] = __builtin_amdgcn_s_memtime();
         const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
         int used = 0;
         float lastRes = residualMode ? 0.0f : -1.0f;
@@ -1929,6 +1974,7 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
             d.S[w].iterations = used;
             d.S[w].residual = lastRes;
         }
+        if (timed) d.ktime[41] = __builtin_amdgcn_s_memtime();
     }
     __syncthreads();
     // StoreImpulses (b2contactsolver.d:774-787) and the velocities back
@@ -1946,6 +1992,12 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
         }
     }
     for (int i = tid; i < C.cb; i += BS) bvel[C.blist[i]] = C.Lvel[i];
+    if (timed) {
+        d.ktime[42] = __builtin_amdgcn_s_memtime();
+        d.ktime[45] = C.cb;
+        d.ktime[46] = nc;
+        d.ktime[47] = nbatch;
+    }
     return true;
 }
 
@@ -1958,6 +2010,8 @@ __device__ __forceinline__ void compact_position(const Dev &d, const StepParams 
     float4 *bpos = d.bpos + wd.bodyBase;
     const PosCon *pc = d.pc + wd.contactBase;
     int *islSolved = d.islSolved + wd.islandBase;
+    const bool timed = d.ktime && tid == 0 && wd.contactBase == d.W[d.order[0]].contactBase;
+    if (timed) d.ktime[43] = __builtin_amdgcn_s_memtime();
     for (int i = tid; i < C.cb; i += BS) C.Lpos[i] = bpos[C.blist[i]];
     for (int i = tid; i < C.ci; i += BS) C.Lsol[i] = 0;
     for (int i = tid; i < nisl; i += BS) islSolved[i] = sp.positionIterations > 0 ? 1 : 0;
@@ -1981,7 +2035,10 @@ __device__ __forceinline__ void compact_position(const Dev &d, const StepParams 
     for (int i = tid; i < C.cb; i += BS) bpos[C.blist[i]] = C.Lpos[i];
     for (int i = tid; i < C.ci; i += BS) islSolved[C.ilist[i]] = C.Lsol[i];
     __syncthreads();
+    if (timed) d.ktime[44] = __builtin_amdgcn_s_memtime();
 }
+
+constexpr int kBodyRun = 4; // bodies (fixtures) per thread and pass in solve_world_global
 
 // Inlined into both solve kernels: as a called function it would take the address
 // of the kernel's Dev argument, which puts a copy of Dev in scratch memory and makes
@@ -2021,24 +2078,44 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     const int tid = threadIdx.x, BS = blockDim.x;
     const float h = sp.dt;
 
-    // Integrate velocities, damping (b2island.d:82-116).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b];
-        bpos0[b] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-        uint32_t f = bfl[b];
-        if (body_type(f) != kDynamic) continue;
-        float4 vel = bvel[b], frc = bforce[b], ms = bmass[b], pr = bparam[b];
-        float2 v = v2(vel.x, vel.y);
-        float wv = vel.z;
-        float2 acc = add(scl(pr.z, v2(wd.gx, wd.gy)), scl(ms.x, v2(frc.x, frc.y)));
-        v = add(v, scl(h, acc));
-        wv += h * ms.y * frc.z;
-        float ld = 1.0f / (1.0f + h * pr.x);
-        float ad = 1.0f / (1.0f + h * pr.y);
-        v = v2(v.x * ld, v.y * ld);
-        wv *= ad;
-        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+    // Integrate velocities, damping (b2island.d:82-116).  The per-body passes of
+    // this path take kBodyRun bodies per thread with all their loads issued before
+    // the first store, so that a pass over a big world waits for memory once per
+    // run instead of once per body.
+    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
+        int isl[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        float4 pos[kBodyRun], vel[kBodyRun], frc[kBodyRun], ms[kBodyRun], pr[kBodyRun];
+        uint32_t fl[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            const int b = b0 + u * BS;
+            pos[u] = bpos[b];
+            fl[u] = bfl[b];
+            vel[u] = bvel[b];
+            frc[u] = bforce[b];
+            ms[u] = bmass[b];
+            pr[u] = bparam[b];
+        }
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            const int b = b0 + u * BS;
+            bpos0[b] = make_float4(pos[u].x, pos[u].y, pos[u].z, 0.0f);
+            if (body_type(fl[u]) != kDynamic) continue;
+            float2 v = v2(vel[u].x, vel[u].y);
+            float wv = vel[u].z;
+            float2 acc = add(scl(pr[u].z, v2(wd.gx, wd.gy)), scl(ms[u].x, v2(frc[u].x, frc[u].y)));
+            v = add(v, scl(h, acc));
+            wv += h * ms[u].y * frc[u].z;
+            float ld = 1.0f / (1.0f + h * pr[u].x);
+            float ad = 1.0f / (1.0f + h * pr[u].y);
+            v = v2(v.x * ld, v.y * ld);
+            wv *= ad;
+            bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
+        }
     }
     __syncthreads();
     ProfStamp prof;
@@ -2155,23 +2232,36 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     __syncthreads();
     prof.at(kProfVel1);
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b], vel = bvel[b];
-        float2 v = v2(vel.x, vel.y);
-        float wv = vel.z;
-        float2 tr = scl(h, v);
-        if (dot(tr, tr) > kMaxTranslationSquared) {
-            float ratio = kMaxTranslation / sqrtf(tr.x * tr.x + tr.y * tr.y);
-            v = v2(v.x * ratio, v.y * ratio);
+    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
+        int isl[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        float4 pos[kBodyRun], vel[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            pos[u] = bpos[b0 + u * BS];
+            vel[u] = bvel[b0 + u * BS];
         }
-        float rotation = h * wv;
-        if (rotation * rotation > kMaxRotationSquared) {
-            float ratio = kMaxRotation / fabsr(rotation);
-            wv *= ratio;
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            const int b = b0 + u * BS;
+            float2 v = v2(vel[u].x, vel[u].y);
+            float wv = vel[u].z;
+            float2 tr = scl(h, v);
+            if (dot(tr, tr) > kMaxTranslationSquared) {
+                float ratio = kMaxTranslation / sqrtf(tr.x * tr.x + tr.y * tr.y);
+                v = v2(v.x * ratio, v.y * ratio);
+            }
+            float rotation = h * wv;
+            if (rotation * rotation > kMaxRotationSquared) {
+                float ratio = kMaxRotation / fabsr(rotation);
+                wv *= ratio;
+            }
+            bpos[b] = make_float4(pos[u].x + h * v.x, pos[u].y + h * v.y, pos[u].z + h * wv, pos[u].w);
+            bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
         }
-        bpos[b] = make_float4(pos.x + h * v.x, pos.y + h * v.y, pos.z + h * wv, pos.w);
-        bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
     }
     for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
     __syncthreads();
@@ -2229,12 +2319,23 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     __syncthreads();
 
     // Copy back and SynchronizeTransform (b2island.d:227-235, b2body.d:1143-1147).
-    for (int b = tid; b < nb; b += BS) {
-        if (bisl[b] < 0) continue;
-        float4 pos = bpos[b];
-        float4 ms = bmass[b];
-        Xf x = xf_from(v2(pos.x, pos.y), pos.z, v2(ms.z, ms.w));
-        bxf[b] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
+        int isl[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        float4 pos[kBodyRun], ms[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            pos[u] = bpos[b0 + u * BS];
+            ms[u] = bmass[b0 + u * BS];
+        }
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (isl[u] < 0) continue;
+            Xf x = xf_from(v2(pos[u].x, pos[u].y), pos[u].z, v2(ms[u].z, ms[u].w));
+            bxf[b0 + u * BS] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+        }
     }
     __syncthreads();
     prof.at(kProfPos1);
@@ -2244,25 +2345,44 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
         __syncthreads();
         const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
         const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
-        for (int b = tid; b < nb; b += BS) {
-            int isl = bisl[b];
-            if (isl < 0) continue;
-            uint32_t f = bfl[b];
-            if (body_type(f) == kStatic) continue;
-            const float4 vel = bvel[b];
-            const float st = bpos[b].w;
-            const bool restless = ((f & kBodyAutoSleep) == 0) | (vel.z * vel.z > angTolSqr) |
-                                  ((vel.x * vel.x + vel.y * vel.y) > linTolSqr);
-            const float nst = restless ? 0.0f : st + h;
-            atomicMin(&islMin[isl], __float_as_int(nst));
-            bpos[b].w = nst;
+        for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
+            int isl[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+            uint32_t fl[kBodyRun];
+            float4 vel[kBodyRun];
+            float st[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) {
+                if (isl[u] < 0) continue;
+                fl[u] = bfl[b0 + u * BS];
+                vel[u] = bvel[b0 + u * BS];
+                st[u] = bpos[b0 + u * BS].w;
+            }
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) {
+                if (isl[u] < 0 || body_type(fl[u]) == kStatic) continue;
+                const bool restless = ((fl[u] & kBodyAutoSleep) == 0) | (vel[u].z * vel[u].z > angTolSqr) |
+                                      ((vel[u].x * vel[u].x + vel[u].y * vel[u].y) > linTolSqr);
+                const float nst = restless ? 0.0f : st[u] + h;
+                atomicMin(&islMin[isl[u]], __float_as_int(nst));
+                bpos[b0 + u * BS].w = nst;
+            }
         }
         __syncthreads();
-        for (int b = tid; b < nb; b += BS) {
-            int isl = bisl[b];
-            if (isl < 0) continue;
-            if (__int_as_float(islMin[isl]) >= kTimeToSleep && islSolved[isl]) {
+        for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
+            int isl[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+            bool sleep[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u)
+                sleep[u] = isl[u] >= 0 && __int_as_float(islMin[isl[u]]) >= kTimeToSleep && islSolved[isl[u]];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) {
+                if (!sleep[u]) continue;
                 // SetAwake(false) (b2body.d:836-845)
+                const int b = b0 + u * BS;
                 bfl[b] &= ~kBodyAwake;
                 bpos[b].w = 0.0f;
                 bvel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -2274,29 +2394,47 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
 
     // SynchronizeFixtures + fat-AABB rule of b2DynamicTree.MoveProxy
     // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
-    for (int f = tid; f < nf; f += BS) {
-        int b = fbody[f];
-        if (bisl[b] < 0 || body_type(bfl[b]) == kStatic) continue;
-        float4 p0 = bpos0[b];
-        float4 ms = bmass[b];
-        Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
-        Xf x2 = load_xf(bxf, b);
-        float4 a1 = shape_aabb(shp + f, x1);
-        float4 a2 = shape_aabb(shp + f, x2);
-        float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
-        float2 disp = sub(x2.p, x1.p);
-        float4 fa = fat[f];
-        bool contains = fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w;
-        if (contains) continue;
-        float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
-                                 aabb.w + kAabbExtension);
-        float2 dd = scl(kAabbMultiplier, disp);
-        if (dd.x < 0.0f) nbx.x += dd.x;
-        else nbx.z += dd.x;
-        if (dd.y < 0.0f) nbx.y += dd.y;
-        else nbx.w += dd.y;
-        fat[f] = nbx;
-        moved[f] = 1;
+    for (int f0 = tid; f0 < nf; f0 += kBodyRun * BS) {
+        int fb[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) fb[u] = f0 + u * BS < nf ? fbody[f0 + u * BS] : -1;
+        bool live[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) live[u] = fb[u] >= 0 && bisl[fb[u]] >= 0 && body_type(bfl[fb[u]]) != kStatic;
+        float4 p0[kBodyRun], ms[kBodyRun], xf2[kBodyRun], fa[kBodyRun];
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (!live[u]) continue;
+            p0[u] = bpos0[fb[u]];
+            ms[u] = bmass[fb[u]];
+            xf2[u] = bxf[fb[u]];
+            fa[u] = fat[f0 + u * BS];
+        }
+#pragma unroll
+        for (int u = 0; u < kBodyRun; ++u) {
+            if (!live[u]) continue;
+            const int f = f0 + u * BS;
+            Xf x1 = xf_from(v2(p0[u].x, p0[u].y), p0[u].z, v2(ms[u].z, ms[u].w));
+            Xf x2;
+            x2.p = v2(xf2[u].x, xf2[u].y);
+            x2.q.s = xf2[u].z;
+            x2.q.c = xf2[u].w;
+            float4 a1 = shape_aabb(shp + f, x1);
+            float4 a2 = shape_aabb(shp + f, x2);
+            float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+            float2 disp = sub(x2.p, x1.p);
+            bool contains = fa[u].x <= aabb.x && fa[u].y <= aabb.y && aabb.z <= fa[u].z && aabb.w <= fa[u].w;
+            if (contains) continue;
+            float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                                     aabb.w + kAabbExtension);
+            float2 dd = scl(kAabbMultiplier, disp);
+            if (dd.x < 0.0f) nbx.x += dd.x;
+            else nbx.z += dd.x;
+            if (dd.y < 0.0f) nbx.y += dd.y;
+            else nbx.w += dd.y;
+            fat[f] = nbx;
+            moved[f] = 1;
+        }
     }
     // b2World.ClearForces (b2world.d:443-450)
     for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

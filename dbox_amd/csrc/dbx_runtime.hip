@@ -1211,7 +1211,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
     }
     DBX_HIP(h, dalloc(h, &d.islSolved, nb));
     DBX_HIP(h, dalloc(h, &h->dPen, nw));
-    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 36));
+    if (getenv("DBX_KTIME")) DBX_HIP(h, dalloc(h, &d.ktime, 48));
     DBX_HIP(h, dalloc(h, &d.prof, (size_t)kProfStamps * (kProfSlots + 1)));
     DBX_HIP(h, hipStreamSynchronize(h->stream));
     h->pending.clear();
@@ -1651,7 +1651,7 @@ dbx_status dbx_sync(dbx_handle *h) {
         h->tierSteps = 0;
     }
     if (h->d.ktime && getenv("DBX_KTIME")) {
-        long long t[36];
+        long long t[48];
         if (hipMemcpy(t, h->d.ktime, sizeof t, hipMemcpyDeviceToHost) == hipSuccess) {
             fprintf(stderr, "dbx ktime (shader cycles) islands: stage %lld dfs %lld batch %lld [nc %lld nbatch %lld] | "
                             "solve: stage+init %lld warm %lld vel %lld pos %lld tail %lld\n",
@@ -1666,6 +1666,10 @@ dbx_status dbx_sync(dbx_handle *h) {
                     t[26] - t[25], t[28], t[29], t[30], t[27] - t[25], t[31]);
             fprintf(stderr, "dbx ktime toi kernel entry to exit: %.1f us wall clock, %lld shader cycles\n",
                     (double)(t[34] - t[32]) * 1000.0 / h->wallKHz, t[35] - t[33]);
+            fprintf(stderr, "dbx ktime big-world LDS solve: count %lld number %lld init %lld warm %lld velocity %lld "
+                            "store %lld position %lld [bodies %lld constraints %lld levels %lld]\n",
+                    t[37] - t[36], t[38] - t[37], t[39] - t[38], t[40] - t[39], t[41] - t[40], t[42] - t[41],
+                    t[44] - t[43], t[45], t[46], t[47]);
         }
     }
     return DBX_OK;
