@@ -10,7 +10,10 @@ against the oracle with continuous physics on.
   on either side, where without the pass most of them do;
 * the C1 pyramid with the pass on tracks the oracle like the discrete step;
 * batched worlds equal the same worlds stepped alone, and a replayed hipGraph
-  equals the eager step, bit for bit.
+  equals the eager step, bit for bit;
+* a TOI event whose moved proxy's query finds a few hundred proxies tracks the
+  oracle, and one whose query finds more than the pass's per-event pair list
+  holds (kToiPairCap) is reported as a capacity error naming that list.
 """
 from __future__ import annotations
 
@@ -132,3 +135,80 @@ def test_continuous_batched_equals_single_and_graph_equals_eager():
     for k in range(6):
         assert gr.bodies(k).tobytes() == batch.bodies(k).tobytes(), k
 
+
+
+def _box_bouncing_over_pins(pitch):
+    """A 6 m x 6 m box (no gravity, restitution 1) thrown at 120 m/s into a
+    vertical wall edge 0.6 m ahead; behind it tiny static sensor pins on a square
+    grid of the given pitch.  The TOI event bounces the box back beyond where the
+    step started, out of the fat AABB of its discrete sweep, so the event moves
+    its proxy and queries the pins its new fat AABB covers.  The pins' own fat
+    AABBs (0.2 m wide) never overlap one another, and sensors take no part in the
+    continuous pass."""
+    xs = np.arange(-12.0, -0.2, pitch)
+    ys = np.arange(-3.2, 3.2, pitch)
+    n_pins = len(xs) * len(ys)
+    b = scenes.body_defs(2 + n_pins)
+    f = scenes.fixture_defs(2 + n_pins)
+    b[0]["type"] = scenes.STATIC
+    f[0]["body"] = 0
+    f[0]["shape"] = scenes.SHAPE_EDGE
+    f[0]["p"][:4] = (0.0, -10.0, 0.0, 10.0)
+    b[1]["type"] = scenes.DYNAMIC
+    b[1]["px"] = -3.6
+    b[1]["vx"] = 120.0
+    b[1]["gravity_scale"] = 0.0
+    f[1]["body"] = 1
+    f[1]["shape"] = scenes.SHAPE_BOX
+    f[1]["p"][:2] = (3.0, 3.0)
+    f[1]["density"] = 1.0
+    f[1]["restitution"] = 1.0
+    gx, gy = np.meshgrid(xs, ys)
+    b[2:]["type"] = scenes.STATIC
+    b[2:]["px"] = gx.ravel()
+    b[2:]["py"] = gy.ravel()
+    f[2:]["body"] = np.arange(2, 2 + n_pins)
+    f[2:]["shape"] = scenes.SHAPE_CIRCLE
+    f[2:]["radius"] = 0.001
+    f[2:]["flags"] = scenes.SENSOR
+    return scenes.Scene(b, f, name=f"box_bouncing_over_pins_{pitch}")
+
+
+def test_event_query_below_the_list_cap_tracks_the_oracle():
+    # about 350 proxies in the event's query: under the pass's per-event pair
+    # list (512), so the GPU keeps the reference's AddPair order
+    print("case: box bouncing over sensor pins at 0.42 m, continuous", flush=True)
+    s = _box_bouncing_over_pins(0.42)
+    o = po.OracleWorld(s, continuous=True)
+    g = _gpu([s])
+    events = 0
+    for i in range(10):
+        o.step()
+        g.step(flags=FLAGS)
+        assert int(g.toi_events()[0]) == o.toi_events(), i
+        events += o.toi_events()
+        bo, bg = o.bodies(), g.bodies()
+        for fld in ("cx", "cy", "vx", "vy"):
+            np.testing.assert_allclose(bg[fld], bo[fld], rtol=0, atol=1e-5, err_msg=f"step {i + 1} {fld}")
+        if i == 0:
+            # the event's query made contacts: more than the discrete step alone has
+            d = po.OracleWorld(s, continuous=False)
+            d.step()
+            assert bg["vx"][1] < 0.0
+            assert len(g.contacts()) == len(o.contacts()) > len(d.contacts())
+    assert events >= 1
+    g.sync()
+    assert len(g.contacts()) == len(o.contacts())
+
+
+def test_event_query_over_the_list_cap_is_reported():
+    # over a thousand proxies in the event's query: more pairs than the per-event
+    # list holds; the pass drops them and the step reports a capacity error
+    # naming that list
+    print("case: box bouncing over sensor pins at 0.21 m, continuous, pair list cap", flush=True)
+    g = _gpu([_box_bouncing_over_pins(0.21)])
+    g.step(flags=FLAGS)
+    with pytest.raises(native.DbxError) as err:
+        g.sync()
+    assert err.value.status == native.DBX_E_CAPACITY
+    assert "continuous-pass" in str(err.value)
