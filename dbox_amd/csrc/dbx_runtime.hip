@@ -2225,7 +2225,7 @@ dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const floa
 // ---------------------------------------------------------------------------
 dbx_status dbx_set_residual(dbx_handle *h, float tolerance, int32_t max_iterations) {
     if (!h) return DBX_E_INVALID_ARG;
-    if (!(tolerance >= 0.0f) || !std::isfinite(tolerance) || max_iterations < 1 || max_iterations > 100000)
+    if (!std::isfinite(tolerance) || max_iterations < 1 || max_iterations > 100000)
         return fail(h, DBX_E_INVALID_ARG, "bad residual settings");
     h->residualTol = tolerance;
     h->maxIterations = max_iterations;

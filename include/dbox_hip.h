@@ -315,8 +315,9 @@ dbx_status dbx_totals(dbx_handle *h, int64_t out[4]);
 dbx_status dbx_read_solver_stats(dbx_handle *h, int32_t *iterations, float *max_penetration, int32_t nworlds);
 
 /* Per-world velocity residual after the last velocity iteration of the last
- * step in RESIDUAL mode (-1 otherwise).  With dbx_set_residual(h, 0, n) a step
- * runs exactly n iterations, so this is the residual after n iterations. */
+ * step in RESIDUAL mode (-1 otherwise).  With a negative tolerance
+ * (dbx_set_residual(h, -1, n)) a step runs exactly n iterations, so this is the
+ * residual after n iterations. */
 dbx_status dbx_read_residuals(dbx_handle *h, float *residual, int32_t nworlds);
 
 /* Per-world number of TOI events the last step's continuous pass solved (the
@@ -384,6 +385,8 @@ dbx_status dbx_collide_pairs(dbx_handle *h, const dbx_fixture_def *a, const floa
  * (the largest over points of the normal complementarity error |vn - bias| for a
  * positive impulse, else max(0, bias - vn), and of |vt| where friction is not
  * saturated) is at most tolerance, or after max_iterations.  Defaults 1e-3, 100.
+ * A negative tolerance never stops early (a residual is never negative): every
+ * step runs max_iterations iterations and reports the residual after them.
  * No counterpart in the reference, whose iteration count is fixed. */
 dbx_status dbx_set_residual(dbx_handle *h, float tolerance, int32_t max_iterations);
 

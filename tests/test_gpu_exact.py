@@ -136,3 +136,28 @@ def test_config3_all_4096_worlds_sampled_against_oracle_and_single():
     touching = sum(int(((ref[w].contacts()["flags"] & 2) != 0).sum()) for w in sample)
     print(f"config 3: {len(sample)} worlds bit-identical to the oracle and to single runs; {touching} touching contacts")
     assert touching > 1000
+
+
+@pytest.mark.parametrize("which,steps", [("config 1 pyramid", 60), ("config 2 container", 40), ("config 4 pyramid", 24)])
+def test_residual_mode_at_max_iterations_equals_the_plain_step(which, steps):
+    # RESIDUAL mode (no counterpart in dbox) sweeps until the velocity residual
+    # falls to the tolerance; with a negative tolerance and a cap of 8 it runs all
+    # eight sweeps, which make the same updates as the plain step's 8.  That pins
+    # its loops on the three solve paths: the one-wave LDS solve (config 1), the
+    # global-memory sweeps (config 2) and the compact LDS sweeps of a big world
+    # (config 4).
+    print(f"case: RESIDUAL at 8 iterations vs plain, {which}, {steps} steps", flush=True)
+    scene = {"config 1 pyramid": scenes.pyramid, "config 2 container": scenes.big_mixed_world,
+             "config 4 pyramid": scenes.big_pyramid}[which]()
+    plain, resid = _gpu([scene]), _gpu([scene])
+    resid.set_residual(-1.0, 8)
+    solved = 0
+    for i in range(steps):
+        plain.step(DT, 8, 3, DEFAULTS)
+        resid.step(DT, 8, 3, DEFAULTS | native.RESIDUAL)
+        it, _ = resid.solver_stats()
+        assert int(it[0]) in (0, 8), f"step {i + 1}: {int(it[0])} sweeps"
+        solved += int(it[0] == 8)
+        assert resid.bodies(0).tobytes() == plain.bodies(0).tobytes(), f"step {i + 1}: bodies"
+    assert solved > 0
+    assert resid.contacts(0).tobytes() == plain.contacts(0).tobytes()
