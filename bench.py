@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import threading
 import time
@@ -34,7 +35,8 @@ sys.path.insert(0, ROOT)
 from dbox_amd import native, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {"collide": "k_collide", "islands": "k_islands", "solve": "k_solve_wave", "broadphase": "k_broadphase"}
+KERNEL_NAMES = {"collide": "k_collide", "islands": "k_islands", "solve": "k_solve_wave", "broadphase": "k_broadphase",
+                "toi": "k_toi"}
 DT = 1.0 / 60.0
 VEL, POS = 8, 3
 
@@ -93,13 +95,15 @@ def _kernel_values(table: dict, kernel: str, last: int) -> list[float]:
 def summarize_counters(args) -> dict:
     """HBM traffic per launch of one kernel from four rocprofv3 --pmc runs (FETCH_SIZE and
     WRITE_SIZE of the profiled bench run, then of a plain device copy of --copy-mib MiB
-    each), written as the JSON file bench.py reads into roofline.traffic."""
+    each), written as the JSON file bench.py reads into roofline.traffic.  With --each,
+    --kernel is the regular expression the runs were filtered with, and every kernel it
+    matches gets its own entry under "kernels"."""
     fetch_dir, write_dir, cfetch_dir, cwrite_dir = args.summarize_pmc
     mib = args.copy_mib or 256
     out = {"kernel": args.kernel, "workload": args.workload, "dispatches": args.last}
+    tables, each = {}, {}
     for counter, d, cd in (("FETCH_SIZE", fetch_dir, cfetch_dir), ("WRITE_SIZE", write_dir, cwrite_dir)):
-        vals = _kernel_values(_per_dispatch(_rocprof_rows(d, "counter_collection.csv"), counter), args.kernel,
-                              args.last)
+        table = _per_dispatch(_rocprof_rows(d, "counter_collection.csv"), counter)
         # the copy run launches one kernel exactly 20 times: the copies
         copies = [v for v in _per_dispatch(_rocprof_rows(cd, "counter_collection.csv"), counter).values()
                   if len(v) == 20]
@@ -108,11 +112,27 @@ def summarize_counters(args) -> dict:
         ckib = float(np.median(list(copies[0].values())))
         scale = mib * 1024.0 / ckib
         key = counter.split("_")[0].lower()
-        out[f"{key}_kib_mean"] = float(np.mean(vals))
         out[f"{key}_copy_kib"] = ckib
         out[f"{key}_scale"] = scale
-        out[f"{key}_bytes"] = int(round(np.mean(vals) * 1024.0 * scale))
-    out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
+        tables[key] = (table, scale)
+    if args.each:
+        names = sorted({n.split("(")[0].replace("dbx::", "").replace("void ", "") for n in tables["fetch"][0]
+                        if re.search(args.kernel, n.split("(")[0])})
+        for name in names:
+            e = {}
+            for key, (table, scale) in tables.items():
+                vals = _kernel_values(table, name, args.last)
+                e[f"{key}_kib_mean"] = float(np.mean(vals))
+                e[f"{key}_bytes"] = int(round(np.mean(vals) * 1024.0 * scale))
+            e["traffic_bytes"] = e["fetch_bytes"] + e["write_bytes"]
+            each[name] = e
+        out["kernels"] = each
+    else:
+        for key, (table, scale) in tables.items():
+            vals = _kernel_values(table, args.kernel, args.last)
+            out[f"{key}_kib_mean"] = float(np.mean(vals))
+            out[f"{key}_bytes"] = int(round(np.mean(vals) * 1024.0 * scale))
+        out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
     pa = args.profiled_args
     out["commands"] = [
         f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '{args.kernel}' --output-format csv -d {fetch_dir} "
@@ -122,8 +142,8 @@ def summarize_counters(args) -> dict:
         f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {cfetch_dir} -o copy -- python3 bench.py --copy-mib {mib}",
         f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {cwrite_dir} -o copy -- python3 bench.py --copy-mib {mib}",
         "python3 bench.py --summarize-pmc " + " ".join(args.summarize_pmc) +
-        f" --kernel '{args.kernel}' --last {args.last} --copy-mib {mib} --workload '{args.workload}' "
-        f"--profiled-args '{pa}' --out {args.out}",
+        f" --kernel '{args.kernel}'" + (" --each" if args.each else "") +
+        f" --last {args.last} --copy-mib {mib} --workload '{args.workload}' --profiled-args '{pa}' --out {args.out}",
     ]
     out["note"] = ("scale = bytes the copy moves each way / what the counter reports for it; "
                    "bytes = mean counter value of the last dispatches (the timed window) x 1024 x scale")
@@ -165,6 +185,10 @@ def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int,
     if kernel == "broadphase":
         # fat AABBs and moved flags read, unique pairs written
         return nprox * 20 + npairs * 8
+    if kernel == "toi":
+        # the pass without events: every contact's TOI record written (16 B) and read by
+        # the first search (8 B), every body's sweep start reset (16 B)
+        return c * 24 + nbodies * 16
     # islands: contact keys and flags, body flags
     return c * 12 + nbodies * 8
 
@@ -407,6 +431,7 @@ def main():
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
                     help="print the lead tier's kernel durations from a rocprofv3 --kernel-trace folder and exit")
     ap.add_argument("--kernel", default="k_solve_wave<true>")
+    ap.add_argument("--each", action="store_true", help="--summarize-pmc: one entry per kernel --kernel matches")
     ap.add_argument("--last", type=int, default=600, help="dispatches (timed steps) the summaries average over")
     ap.add_argument("--workload", default="")
     ap.add_argument("--profiled-args", default="")
@@ -531,6 +556,23 @@ def main():
                                sum(shape_bytes(worlds[w].fixtures) for w in lead),
                                sum(h.pairs(w).shape[0] for w in lead))
     achieved = abytes / (avg_ms * 1e-3) / 1e9
+    # every lead-tier kernel against the same roofline, with its HBM bytes per launch
+    # where a FETCH_SIZE / WRITE_SIZE profile of this workload and window exists
+    each_path = os.path.join(ROOT, "profiles", f"lead_tier_4096worlds_steps{window}_traffic.json")
+    each_traffic = {}
+    if not args.coloured and not args.subset and len(worlds) == 4096 and os.path.exists(each_path):
+        with open(each_path) as f:
+            each_traffic = json.load(f)["kernels"]
+    lead_kernels = {}
+    for k in kern:
+        ms = kern[k] / max(ksteps, 1)
+        b = algorithmic_bytes(k, lead_t, sum(len(worlds[w].bodies) for w in lead),
+                              sum(worlds[w].dynamic_count for w in lead), sum(len(worlds[w].fixtures) for w in lead),
+                              sum(shape_bytes(worlds[w].fixtures) for w in lead), sum(h.pairs(w).shape[0] for w in lead))
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+        lead_kernels[k] = {"ms": ms, "algorithmic_bytes": b, "achieved_gbs": gbs,
+                           "frac": gbs / HBM_PEAK_GBS if gbs is not None else None,
+                           "traffic": each_traffic.get(KERNEL_NAMES[k] + "<true>", {}).get("traffic_bytes")}
     npairs = sum(h.pairs(w).shape[0] for w in range(len(worlds)))
     step_bytes = sum(algorithmic_bytes(k, tot, nbodies, ndyn, nprox, sbytes, npairs) for k in kern_ms)
     log("lead-tier ms/step: " + ", ".join(f"{k} {v / max(ksteps, 1):.3f}" for k, v in kern.items()) +
@@ -629,6 +671,8 @@ def main():
         },
         "latency_model": latency,
         "lead_tier_kernel_ms_per_step": {k: v / max(ksteps, 1) for k, v in kern.items()},
+        "lead_tier_kernels": lead_kernels,
+        "lead_tier_traffic_source": os.path.relpath(each_path, ROOT) if each_traffic else None,
         "profile_ms_per_step": {k: v / max(nsteps, 1) for k, v in prof.items()},
         "tiers": [{"worlds": int(r[1]), "lds": int(r[2]), "block": int(r[3])} for r in tier_rows],
         "graph_replay": graph,

@@ -53,6 +53,27 @@ def test_traffic_summary_scales_the_last_dispatches(tmp_path):
     assert any("--pmc FETCH_SIZE" in c for c in t["commands"])
 
 
+def test_traffic_summary_each_matching_kernel(tmp_path):
+    f, w, cf, cw = (str(tmp_path / n) for n in ("f", "w", "cf", "cw"))
+    # both lead-tier kernels counted in the same runs (filtered by "<true>"), another one not matched
+    _pmc(f, "FETCH_SIZE", [(KN, [9999] * 2 + [100, 300]), (OTHER, [9999] * 2 + [10, 30]),
+                           ("dbx::k_toi<false>(dbx::Dev)", [5] * 4)])
+    _pmc(w, "WRITE_SIZE", [(KN, [9999] * 2 + [40, 40]), (OTHER, [9999] * 2 + [8, 8]),
+                           ("dbx::k_toi<false>(dbx::Dev)", [5] * 4)])
+    _pmc(cf, "FETCH_SIZE", [("copy_kernel", [512] * 20)])
+    _pmc(cw, "WRITE_SIZE", [("copy_kernel", [1024] * 20)])
+    out = str(tmp_path / "t.json")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize-pmc", f, w, cf, cw,
+                        "--kernel", "<true>", "--each", "--last", "2", "--copy-mib", "1", "--out", out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    k = json.load(open(out))["kernels"]
+    assert sorted(k) == ["k_collide<true>", "k_solve_wave<true>"]
+    assert k["k_solve_wave<true>"]["fetch_bytes"] == 200 * 1024 * 2
+    assert k["k_solve_wave<true>"]["write_bytes"] == 40 * 1024
+    assert k["k_collide<true>"]["traffic_bytes"] == 20 * 1024 * 2 + 8 * 1024
+
+
 def test_trace_summary_averages_the_last_dispatches(tmp_path):
     d = str(tmp_path / "kt")
     rows, did = [], 0
