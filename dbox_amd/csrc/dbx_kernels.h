@@ -53,6 +53,11 @@ struct WideBp {
     int2 *chunk;            // nchunk + 1: kept / touching contacts per chunk, then their offsets
     int *lcnt, *loff;       // bodyCount + 1: new contacts per body, then offsets
     int *lent;              // 2 * contact capacity: (2 * slot + side) per body
+    // AddPair over the device: per pair its accept flag, per chunk of pairs the
+    // accepted count, then its first contact slot
+    int npchunk;            // pair chunks of kWideCollideChunk (pair capacity)
+    int *pacc;              // pair capacity
+    int2 *pchunk;           // npchunk + 1
 };
 constexpr int kWideCollideChunk = 1024;
 // b.n, b.cap and b.desc set

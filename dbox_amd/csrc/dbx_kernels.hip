@@ -2896,6 +2896,67 @@ __device__ __forceinline__ int bp_pairs(const float4 *gfat, const int *gmoved, i
     return total;
 }
 
+// b2ContactManager.AddPair's filters (b2contactmanager.d:52-176) for one pair of
+// the sorted unique pair list: no contact between the two fixtures yet (the
+// reference scans bodyB's contact list; a contact is on both lists, so the
+// shorter one is scanned), b2Body.ShouldCollide (at least one body dynamic, no
+// joint that forbids it), b2ContactFilter.ShouldCollide (b2worldcallbacks.d:52-65)
+// and a contact type for the two shapes.  The pairs of one list are unique and a
+// contact is only looked up among those made before the list, so each pair's
+// answer is independent of the others'.
+__device__ __forceinline__ bool add_pair_accept(int fa, int fb, const int *fbody, const uint32_t *bfl,
+                                                const int *adjOff, const int *adjCnt, const int2 *adj,
+                                                const Contact *cont, const JointAdj &jadj, const int2 *ffilter,
+                                                const Shape *shp) {
+    const int ba = fbody[fa], bb = fbody[fb];
+    if (ba == bb) return false;
+    const int eb = adjCnt[bb] <= adjCnt[ba] ? bb : ba;
+    const int o = adjOff[eb], n = adjCnt[eb];
+    for (int e = 0; e < n; ++e) {
+        const Contact &c = cont[adj[o + e].x >> 1];
+        if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) return false;
+    }
+    if (!(body_type(bfl[ba]) == kDynamic || body_type(bfl[bb]) == kDynamic) || joint_blocks(jadj, bb, ba))
+        return false;
+    const int2 A = ffilter[fa], B = ffilter[fb];
+    const int ga = A.y, gb = B.y;
+    bool accept;
+    if (ga == gb && ga != 0) accept = ga > 0;
+    else accept = ((A.x >> 16) & (B.x & 0xffff)) != 0 && ((A.x & 0xffff) & (B.x >> 16)) != 0;
+    return accept && pair_has_type(shp[fa].type, shp[fb].type);
+}
+// b2Contact.Create for an accepted pair at contact slot `slot`, with the
+// shape-type order swap, the mixed friction and restitution, and the wake of
+// both bodies unless a fixture is a sensor (b2contactmanager.d:169-173).
+__device__ __forceinline__ void add_pair_create(int fa, int fb, int slot, const int *fbody, const Shape *shp,
+                                                const float4 *fmat, uint32_t *bfl, float4 *bpos, Contact *cont,
+                                                float *ctan) {
+    if (!pair_is_primary(shp[fa].type, shp[fb].type)) {
+        const int t = fa;
+        fa = fb;
+        fb = t;
+    }
+    Contact c;
+    c.fa = fa;
+    c.fb = fb;
+    c.flags = kContactEnabled | ((shp[fa].flags | shp[fb].flags) & kShapeSensor ? kContactSensor : 0);
+    c.meta = 0;
+    c.friction = sqrtf(fmat[fa].x * fmat[fb].x); // b2MixFriction
+    const float ra = fmat[fa].y, rb = fmat[fb].y;
+    c.restitution = ra > rb ? ra : rb; // b2MixRestitution
+    c.id0 = 0;
+    c.id1 = 0;
+    c.man = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cont[slot] = c;
+    ctan[slot] = 0.0f;
+    if (!(c.flags & kContactSensor)) {
+        wake_body(bfl, bpos, fbody[fa]);
+        wake_body(bfl, bpos, fbody[fb]);
+    }
+}
+
 template <bool kLead>
 __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
@@ -2914,17 +2975,20 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
     uint32_t *bfl = d.bflags + wd.bodyBase;
     float4 *bpos = d.bpos + wd.bodyBase;
     Contact *cont = d.cont[cur] + wd.contactBase;
+    float *ctan = d.ctan[cur] + wd.contactBase;
     const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
     const int2 *adj = d.adjList + 2 * wd.contactBase;
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && tid == 0;
     if (timed) d.ktime[22] = __builtin_amdgcn_s_memtime();
+    if (wd.pad & 2) { // pairs found and added over the whole device (k_bpw_*, k_bw_*)
+        if (timed) d.ktime[23] = d.ktime[24] = d.ktime[22];
+        return;
+    }
 
     int total;
     const size_t need = bp_bytes(nf);
-    if (wd.pad & 2) // found by the device-wide pass, already in pairs[] in order
-        total = d.S[w].widePairs;
-    else if (need <= (size_t)ldsBytes)
+    if (need <= (size_t)ldsBytes)
         total = bp_pairs(fat, moved, nf, bp_carve(lds, nf), pairs, wd.pairCap, (int2 *)(lds + need),
                          (int)(((size_t)ldsBytes - need) / sizeof(int2)), s_tmp);
     else
@@ -2948,65 +3012,16 @@ __global__ void k_broadphase(Dev d, int cur, int worldOffset, int ldsBytes) {
         int accept = 0;
         int fa = 0, fb = 0;
         if (k < total) {
-            int2 pr = pairs[k];
+            const int2 pr = pairs[k];
             fa = pr.x;
             fb = pr.y;
-            int ba = fbody[fa], bb = fbody[fb];
-            accept = ba != bb;
-            if (accept) { // does a contact already exist?  The reference scans bodyB's
-                          // contact list; a contact is on both lists, so the shorter one is scanned
-                const int eb = adjCnt[bb] <= adjCnt[ba] ? bb : ba;
-                int o = adjOff[eb], n = adjCnt[eb];
-                for (int e = 0; e < n; ++e) {
-                    const Contact &c = cont[adj[o + e].x >> 1];
-                    if ((c.fa == fa && c.fb == fb) || (c.fa == fb && c.fb == fa)) {
-                        accept = 0;
-                        break;
-                    }
-                }
-            }
-            if (accept) { // b2Body.ShouldCollide: at least one body dynamic, no joint that forbids it
-                accept = (body_type(bfl[ba]) == kDynamic || body_type(bfl[bb]) == kDynamic) &&
-                         !joint_blocks(jadj, bb, ba);
-            }
-            if (accept) { // b2ContactFilter.ShouldCollide (b2worldcallbacks.d:52-65)
-                int2 A = ffilter[fa], B = ffilter[fb];
-                int ga = A.y, gb = B.y;
-                if (ga == gb && ga != 0) accept = ga > 0;
-                else accept = ((A.x >> 16) & (B.x & 0xffff)) != 0 && ((A.x & 0xffff) & (B.x >> 16)) != 0;
-            }
-            if (accept) accept = pair_has_type(shp[fa].type, shp[fb].type);
+            accept = add_pair_accept(fa, fb, fbody, bfl, adjOff, adjCnt, adj, cont, jadj, ffilter, shp);
         }
         int tot;
         int off = block_excl_scan(accept, tot, s_tmp);
         if (accept) {
-            int slot = base + created + off;
-            if (slot < wd.contactCap) {
-                if (!pair_is_primary(shp[fa].type, shp[fb].type)) {
-                    int t = fa;
-                    fa = fb;
-                    fb = t;
-                }
-                Contact c;
-                c.fa = fa;
-                c.fb = fb;
-                c.flags = kContactEnabled | ((shp[fa].flags | shp[fb].flags) & kShapeSensor ? kContactSensor : 0);
-                c.meta = 0;
-                c.friction = sqrtf(fmat[fa].x * fmat[fb].x);        // b2MixFriction
-                float ra = fmat[fa].y, rb = fmat[fb].y;
-                c.restitution = ra > rb ? ra : rb;                    // b2MixRestitution
-                c.id0 = 0;
-                c.id1 = 0;
-                c.man = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                c.pt0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                c.pt1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                cont[slot] = c;
-                d.ctan[cur][wd.contactBase + slot] = 0.0f;
-                if (!(c.flags & kContactSensor)) { // b2contactmanager.d:169-173
-                    wake_body(bfl, bpos, fbody[fa]);
-                    wake_body(bfl, bpos, fbody[fb]);
-                }
-            }
+            const int slot = base + created + off;
+            if (slot < wd.contactCap) add_pair_create(fa, fb, slot, fbody, shp, fmat, bfl, bpos, cont, ctan);
         }
         created += tot;
     }
@@ -3936,9 +3951,11 @@ size_t wide_scratch_bytes(const WideBp &b) {
     const int n = b.n, cap = b.cap, nb = b.desc.bodyCount, nc = b.desc.contactCap;
     const int P = bp_pow2(n > 0 ? n : 1), Pc = bp_pow2(cap > 0 ? cap : 1), nblk = (n + 255) / 256 + 1;
     const int nchunk = (nc + kWideCollideChunk - 1) / kWideCollideChunk;
+    const int npchunk = (cap + kWideCollideChunk - 1) / kWideCollideChunk;
     return isl_align(8 * (size_t)P) + isl_align(4 * (size_t)(n + 1)) + isl_align(4 * (size_t)nblk) +
            isl_align(16 * (size_t)kWideBoundsBlocks) + isl_align(16) + isl_align(8 * (size_t)Pc) +
-           isl_align(8 * (size_t)(nchunk + 1)) + 2 * isl_align(4 * (size_t)(nb + 1)) + isl_align(8 * (size_t)nc);
+           isl_align(8 * (size_t)(nchunk + 1)) + 2 * isl_align(4 * (size_t)(nb + 1)) + isl_align(8 * (size_t)nc) +
+           isl_align(4 * (size_t)(cap > 0 ? cap : 1)) + isl_align(8 * (size_t)(npchunk + 1));
 }
 void wide_carve(WideBp &b, char *p) {
     b.P = bp_pow2(b.n > 0 ? b.n : 1);
@@ -3954,7 +3971,10 @@ void wide_carve(WideBp &b, char *p) {
     b.chunk = (int2 *)p; p += isl_align(8 * (size_t)(b.nchunk + 1));
     b.lcnt = (int *)p; p += isl_align(4 * (size_t)(b.desc.bodyCount + 1));
     b.loff = (int *)p; p += isl_align(4 * (size_t)(b.desc.bodyCount + 1));
-    b.lent = (int *)p;
+    b.lent = (int *)p; p += isl_align(8 * (size_t)b.desc.contactCap);
+    b.npchunk = (b.cap + kWideCollideChunk - 1) / kWideCollideChunk;
+    b.pacc = (int *)p; p += isl_align(4 * (size_t)(b.cap > 0 ? b.cap : 1));
+    b.pchunk = (int2 *)p;
 }
 
 __device__ __forceinline__ float4 bounds_join(float4 a, float4 b) {
@@ -4620,12 +4640,116 @@ hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffs
     DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(64), 0, s, d, cur, sp, worldOffset);
     return hipGetLastError();
 }
+// AddPair for a world too big for one workgroup, over the whole device: each
+// pair's filters (add_pair_accept) on its own thread, kWideCollideChunk pairs per
+// workgroup and four consecutive per thread; the accepted counts of the chunks
+// scanned; then the contacts made in pair order from the chunks' first slots.
+// A pair list over the world's capacity adds nothing and is reported, as in
+// k_broadphase.
+__global__ __launch_bounds__(256) void k_bw_accept(Dev d, int cur, int w, int *pacc, int2 *pchunk) {
+    __shared__ int tmp[5];
+    const WorldDesc wd = d.W[w];
+    const int total = d.S[w].widePairs <= wd.pairCap ? d.S[w].widePairs : 0;
+    const int2 *pairs = d.pairs + wd.pairBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    const int2 *ffilter = d.ffilter + wd.fixBase;
+    const uint32_t *bfl = d.bflags + wd.bodyBase;
+    const Contact *cont = d.cont[cur] + wd.contactBase;
+    const int *adjOff = d.adjOff + wd.bodyBase, *adjCnt = d.adjCnt + wd.bodyBase;
+    const int2 *adj = d.adjList + 2 * wd.contactBase;
+    const JointAdj jadj = joint_adj(d, wd);
+    int *moved = d.fmoved + wd.fixBase; // the pair search has read them
+    for (int f = blockIdx.x * 256 + threadIdx.x; f < wd.fixCount; f += gridDim.x * 256) moved[f] = 0;
+    const int base = blockIdx.x * kWideCollideChunk + 4 * (int)threadIdx.x;
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int k = base + u;
+        if (k >= total) continue;
+        const int2 pr = pairs[k];
+        const int a = add_pair_accept(pr.x, pr.y, fbody, bfl, adjOff, adjCnt, adj, cont, jadj, ffilter, shp) ? 1 : 0;
+        pacc[k] = a;
+        cnt += a;
+    }
+    int tot;
+    block_excl_scan(cnt, tot, tmp);
+    if (threadIdx.x == 0) pchunk[blockIdx.x] = make_int2(tot, 0);
+}
+__global__ __launch_bounds__(1024) void k_bw_scan(Dev d, int w, int2 *pchunk, int npchunk) {
+    __shared__ int tmp[17];
+    const WorldDesc wd = d.W[w];
+    const int total = d.S[w].widePairs;
+    int run = d.S[w].contactCount;
+    for (int c0 = 0; c0 < npchunk; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const int v = i < npchunk ? pchunk[i].x : 0;
+        int tot;
+        const int ex = block_excl_scan(v, tot, tmp);
+        if (i < npchunk) pchunk[i].y = run + ex; // first contact slot of the chunk
+        run += tot;
+    }
+    if (threadIdx.x == 0) {
+        if (total > wd.pairCap) {
+            d.S[w].error |= 2;
+            d.S[w].pairCount = 0;
+        } else {
+            if (run > wd.contactCap) {
+                d.S[w].error |= 1;
+                run = wd.contactCap;
+            }
+            d.S[w].contactCount = run;
+            d.S[w].pairCount = total;
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_bw_create(Dev d, int cur, int w, const int *pacc, const int2 *pchunk) {
+    __shared__ int tmp[5];
+    const WorldDesc wd = d.W[w];
+    const int total = d.S[w].widePairs <= wd.pairCap ? d.S[w].widePairs : 0;
+    const int2 *pairs = d.pairs + wd.pairBase;
+    const int *fbody = d.fbody + wd.fixBase;
+    const Shape *shp = d.fshape + wd.fixBase;
+    const float4 *fmat = d.fmat + wd.fixBase;
+    uint32_t *bfl = d.bflags + wd.bodyBase;
+    float4 *bpos = d.bpos + wd.bodyBase;
+    Contact *cont = d.cont[cur] + wd.contactBase;
+    float *ctan = d.ctan[cur] + wd.contactBase;
+    const int base = blockIdx.x * kWideCollideChunk + 4 * (int)threadIdx.x;
+    int acc = 0, cnt = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int k = base + u;
+        if (k < total && pacc[k]) {
+            acc |= 1 << u;
+            ++cnt;
+        }
+    }
+    int tot;
+    int slot = pchunk[blockIdx.x].y + block_excl_scan(cnt, tot, tmp);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (!((acc >> u) & 1)) continue;
+        if (slot < wd.contactCap) {
+            const int2 pr = pairs[base + u];
+            add_pair_create(pr.x, pr.y, slot, fbody, shp, fmat, bfl, bpos, cont, ctan);
+        }
+        ++slot;
+    }
+}
+
 hipError_t launch_broadphase(const Dev &d, int cur, int worldOffset, int nworlds, int ldsBytes, int block, bool lead,
                              hipStream_t s, const WideBp *wide, int nwide) {
     if (nworlds <= 0) return hipSuccess;
     for (int i = 0; i < nwide; ++i) {
         const hipError_t e = launch_wide(d, wide[i], s);
         if (e != hipSuccess) return e;
+        const WideBp &b = wide[i];
+        const int cb = b.npchunk > 0 ? b.npchunk : 1;
+        hipLaunchKernelGGL(k_bw_accept, dim3(cb), dim3(256), 0, s, d, cur, b.world, b.pacc, b.pchunk);
+        hipLaunchKernelGGL(k_bw_scan, dim3(1), dim3(1024), 0, s, d, b.world, b.pchunk, b.npchunk);
+        hipLaunchKernelGGL(k_bw_create, dim3(cb), dim3(256), 0, s, d, cur, b.world, (const int *)b.pacc,
+                           (const int2 *)b.pchunk);
     }
     DBX_LAUNCH(k_broadphase, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, worldOffset, ldsBytes);
     return hipGetLastError();
