@@ -139,6 +139,10 @@ struct StepParams {
     float residualTol;
     int maxIterations;
 };
+// StepParams.flags bit: the islands of a big world searched by island (exact order,
+// no joints) are solved by several workgroups at once (k_solve_grp); not set in the
+// residual, learned and logging modes, which treat the world's constraints as one set.
+constexpr uint32_t kStepGrouped = 1u << 8;
 
 struct JointRec; // dbx_joints.h
 
@@ -176,6 +180,7 @@ struct Dev {
     int *iwParent;          // union-find parent, then the component's root
     int4 *iwAgg;            // per root: seed (largest awake member), constraints, non-static bodies, island
     int4 *iwIsl;            // per island: seed, first constraint, first stack slot, constraints
+    int *iwCur;             // per island: bodies placed in its list (its stack slots hold its bodies after the search)
     int *bscratch;          // DFS stack / schedule scratch
     int *wake;              // list position of the first wake in Collide
     unsigned long long *bmask; // colour masks

@@ -28,6 +28,10 @@ hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOf
                         int ldsBytes, bool lead, hipStream_t s);
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
                              bool lead, hipStream_t s);
+// exact order with kStepGrouped: the islands of the tier's big worlds (no joints)
+// solved by groups of workgroups; the solve launches above skip those worlds
+hipError_t launch_solve_groups(const Dev &d, int cur, const StepParams &sp, const WideBp *wide, int nwide,
+                               int ldsBytes, bool lead, hipStream_t s);
 size_t broadphase_scratch_bytes(int proxies);
 
 // A world whose broadphase scratch does not fit in one workgroup's LDS: its pairs
@@ -58,6 +62,7 @@ struct WideBp {
     int npchunk;            // pair chunks of kWideCollideChunk (pair capacity)
     int *pacc;              // pair capacity
     int2 *pchunk;           // npchunk + 1
+    int tierFirst;          // 1 when the world is the first of its tier's launch order
 };
 constexpr int kWideCollideChunk = 1024;
 // b.n, b.cap and b.desc set

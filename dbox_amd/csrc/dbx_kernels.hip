@@ -839,6 +839,7 @@ __global__ __launch_bounds__(256) void k_iw_init(Dev d, int cur, int w) {
         M.claim[i] = 64;
         d.iwParent[wd.bodyBase + i] = i;
         d.iwAgg[wd.bodyBase + i] = make_int4(-1, 0, 0, 0);
+        d.iwCur[wd.bodyBase + i] = 0;
     }
     if (i < 2 * m) {
         int2 ent = d.adjList[2 * wd.contactBase + i];
@@ -1038,7 +1039,9 @@ __global__ __launch_bounds__(64) void k_iw_dfs(Dev d, int cur, int w) {
     if (lane == 0 && mx >= 0) atomicMax(&d.S[w].batchCount, mx + 1);
 }
 
-// b2Body.SetAwake(true) for every body a search reached; level counts cleared.
+// b2Body.SetAwake(true) for every body a search reached; each island's bodies
+// listed in its stack slots (in no particular order: what the solve does per
+// body does not depend on it); level counts cleared.
 __global__ __launch_bounds__(256) void k_iw_finish(Dev d, int cur, int w) {
     const WorldDesc wd = d.W[w];
     const int nb = wd.bodyCount, m = d.S[w].contactCount;
@@ -1047,8 +1050,12 @@ __global__ __launch_bounds__(256) void k_iw_finish(Dev d, int cur, int w) {
     if (i < nb) {
         const int is = (M.st[i].y >> 4) - 1;
         d.bisl[wd.bodyBase + i] = is;
-        if (is >= 0) wake_body(d.bflags + wd.bodyBase, d.bpos + wd.bodyBase, i);
+        if (is >= 0) {
+            wake_body(d.bflags + wd.bodyBase, d.bpos + wd.bodyBase, i);
+            M.stack[d.iwIsl[wd.bodyBase + is].z + atomicAdd(&d.iwCur[wd.bodyBase + is], 1)] = i;
+        }
     }
+    if (i == 0) d.S[w].iterations = 0;
     int *boff = d.batchOff + wd.contactBase + 65 * w;
     if (i <= d.S[w].batchCount) boff[i] = 0;
 }
@@ -1835,20 +1842,30 @@ __device__ __forceinline__ void position_sweep(const int *Lbo, int nbatch, const
 // ---------------------------------------------------------------------------
 struct CompactLds {
     float4 *Lvel, *Lpos, *Limp;
-    int *Lbo, *Lmin, *Lsol, *blist, *ilist;
-    int cb, ci;
+    int *Lbo, *Lmin, *Lsol, *blist, *ilist, *Lsched, *Lcur, *Lstat;
+    int cb, ci, nl, c0; // compact bodies, islands, levels; first constraint (island order)
 };
-__host__ __device__ inline size_t compact_lds_bytes(int cb, int ci, int nc, int nbatch) {
-    return 2 * isl_align(16 * (size_t)cb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)(nbatch + 1)) +
-           2 * isl_align(4 * (size_t)ci) + isl_align(4 * (size_t)cb) + isl_align(4 * (size_t)ci);
+__host__ __device__ inline size_t compact_lds_bytes(int cb, int ci, int nc, int nl) {
+    return 2 * isl_align(16 * (size_t)cb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)(nl + 1)) +
+           2 * isl_align(4 * (size_t)ci) + isl_align(4 * (size_t)cb) + isl_align(4 * (size_t)ci) +
+           2 * isl_align(4 * (size_t)nc) + isl_align(4 * (size_t)nl);
 }
 
 // Constraint set-up, warm start, velocity iterations and StoreImpulses on the compact
-// state; velocities written back.  Returns false, having changed nothing but its
-// scratch maps, when the compact state does not fit in ldsBytes.
+// state of the constraints [c0, c1) (island order: whole islands); velocities written
+// back.  The constraints are put in level order by a counting sort in LDS (a level
+// is a set of constraints that share no dynamic body, so their order within it
+// changes nothing) and stored in that order from vc[c0] / pc[c0].  A non-static body
+// belongs to one island and gets one compact slot through bmap; a static body
+// (several islands, and so several workgroups, may share it) gets a slot of its own
+// in each constraint that has one (a contact has at most one static body), which
+// is never written back: zero inverse mass and inertia leave its state as loaded.
+// Returns false, having changed nothing but its scratch maps, when the compact state
+// does not fit in ldsBytes.  report: write the world's iteration count and residual.
 __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const StepParams &sp, int w, const WorldDesc &wd,
-                                                 int nc, int nbatch, char *lds, int ldsBytes, CompactLds &C) {
-    __shared__ int s_cb, s_ci;
+                                                 int c0, int c1, char *lds, int ldsBytes, CompactLds &C, bool report) {
+    __shared__ int s_cb, s_ci, s_nl;
+    __shared__ int s_scan[kMaxBlock / 64 + 1];
     const int tid = threadIdx.x, BS = blockDim.x;
     Contact *cont = d.cont[cur] + wd.contactBase;
     const float *ctan = d.ctan[cur] + wd.contactBase;
@@ -1860,89 +1877,114 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
     const float4 *fmat = d.fmat + wd.fixBase;
     const int *ordC = d.ordC + wd.contactBase;
     const int *cisl = d.cisl + wd.contactBase;
-    const int *sched = d.sched + wd.contactBase;
-    const int *boff = d.batchOff + wd.contactBase + 65 * w;
-    VelCon *vc = d.vc + wd.contactBase;
-    PosCon *pc = d.pc + wd.contactBase;
+    const int *cbatch = d.cbatch + wd.contactBase;
+    VelCon *vc = d.vc + wd.contactBase + c0;
+    PosCon *pc = d.pc + wd.contactBase + c0;
     int *bmap = d.bscratch + wd.bodyBase;   // world body -> compact body (the search's stack is free now)
     int *imap = d.islSolved + wd.islandBase; // island -> compact island (reset before the position phase)
-    const bool timed = d.ktime && tid == 0 && w == d.order[0];
+    const int nc = c1 - c0;
+    const bool timed = d.ktime && tid == 0 && c0 == 0 && w == d.order[0];
     if (timed) d.ktime[36] = __builtin_amdgcn_s_memtime();
-    for (int k = tid; k < nc; k += BS) {
+    auto is_static = [&](int b) { return body_type(bfl[b]) == kStatic; };
+    for (int k = c0 + tid; k < c1; k += BS) {
         const Contact &c = cont[ordC[k]];
-        bmap[fbody[c.fa]] = -1;
-        bmap[fbody[c.fb]] = -1;
+        const int ba = fbody[c.fa], bb = fbody[c.fb];
+        if (!is_static(ba)) bmap[ba] = -1;
+        if (!is_static(bb)) bmap[bb] = -1;
         imap[cisl[k]] = -1;
     }
     if (tid == 0) {
         s_cb = 0;
         s_ci = 0;
+        s_nl = 0;
     }
     __syncthreads();
-    for (int k = tid; k < nc; k += BS) { // count each body and island once
+    for (int k = c0 + tid; k < c1; k += BS) { // count each body and island once, and the levels
         const Contact &c = cont[ordC[k]];
-        if (atomicCAS(&bmap[fbody[c.fa]], -1, -2) == -1) atomicAdd(&s_cb, 1);
-        if (atomicCAS(&bmap[fbody[c.fb]], -1, -2) == -1) atomicAdd(&s_cb, 1);
+        const int ba = fbody[c.fa], bb = fbody[c.fb];
+        if (is_static(ba) || atomicCAS(&bmap[ba], -1, -2) == -1) atomicAdd(&s_cb, 1);
+        if (is_static(bb) || atomicCAS(&bmap[bb], -1, -2) == -1) atomicAdd(&s_cb, 1);
         if (atomicCAS(&imap[cisl[k]], -1, -2) == -1) atomicAdd(&s_ci, 1);
+        atomicMax(&s_nl, cbatch[k] + 1);
     }
     __syncthreads();
     C.cb = s_cb;
     C.ci = s_ci;
+    C.nl = s_nl;
+    C.c0 = c0;
     if (timed) d.ktime[37] = __builtin_amdgcn_s_memtime();
-    if (compact_lds_bytes(C.cb, C.ci, nc, nbatch) > (size_t)ldsBytes) return false;
+    if (compact_lds_bytes(C.cb, C.ci, nc, C.nl) > (size_t)ldsBytes) return false;
     char *p = lds;
     C.Lvel = (float4 *)p; p += isl_align(16 * (size_t)C.cb);
     C.Lpos = (float4 *)p; p += isl_align(16 * (size_t)C.cb);
     C.Limp = (float4 *)p; p += isl_align(16 * (size_t)nc);
-    C.Lbo = (int *)p; p += isl_align(4 * (size_t)(nbatch + 1));
+    C.Lbo = (int *)p; p += isl_align(4 * (size_t)(C.nl + 1));
     C.Lmin = (int *)p; p += isl_align(4 * (size_t)C.ci);
     C.Lsol = (int *)p; p += isl_align(4 * (size_t)C.ci);
     C.blist = (int *)p; p += isl_align(4 * (size_t)C.cb);
-    C.ilist = (int *)p;
+    C.ilist = (int *)p; p += isl_align(4 * (size_t)C.ci);
+    C.Lsched = (int *)p; p += isl_align(4 * (size_t)nc);
+    C.Lstat = (int *)p; p += isl_align(4 * (size_t)nc);
+    C.Lcur = (int *)p;
     __syncthreads(); // every thread has read the counts
     if (tid == 0) {
         s_cb = 0;
         s_ci = 0;
     }
+    for (int i = tid; i <= C.nl; i += BS) C.Lbo[i] = 0;
     __syncthreads();
-    for (int k = tid; k < nc; k += BS) { // number them
+    for (int k = c0 + tid; k < c1; k += BS) { // number them; constraints per level
         const Contact &c = cont[ordC[k]];
         const int ba = fbody[c.fa], bb = fbody[c.fb], is = cisl[k];
-        if (atomicCAS(&bmap[ba], -2, -3) == -2) {
+        int stat = -1;
+        if (is_static(ba)) {
+            stat = atomicAdd(&s_cb, 1);
+            C.blist[stat] = ~ba;
+        } else if (atomicCAS(&bmap[ba], -2, -3) == -2) {
             const int i = atomicAdd(&s_cb, 1);
             C.blist[i] = ba;
             bmap[ba] = i;
         }
-        if (atomicCAS(&bmap[bb], -2, -3) == -2) {
+        if (is_static(bb)) {
+            stat = atomicAdd(&s_cb, 1);
+            C.blist[stat] = ~bb;
+        } else if (atomicCAS(&bmap[bb], -2, -3) == -2) {
             const int i = atomicAdd(&s_cb, 1);
             C.blist[i] = bb;
             bmap[bb] = i;
         }
+        C.Lstat[k - c0] = stat;
         if (atomicCAS(&imap[is], -2, -3) == -2) {
             const int i = atomicAdd(&s_ci, 1);
             C.ilist[i] = is;
             imap[is] = i;
         }
+        atomicAdd(&C.Lbo[cbatch[k]], 1);
     }
     __syncthreads();
+    block_scan_array(C.Lbo, C.nl + 1, s_scan); // level offsets, C.Lbo[nl] = nc
+    for (int i = tid; i < C.nl; i += BS) C.Lcur[i] = C.Lbo[i];
+    __syncthreads();
+    for (int k = c0 + tid; k < c1; k += BS) C.Lsched[atomicAdd(&C.Lcur[cbatch[k]], 1)] = k;
     if (timed) d.ktime[38] = __builtin_amdgcn_s_memtime();
     for (int i = tid; i < C.cb; i += BS) {
-        const int b = C.blist[i];
+        const int b = C.blist[i] >= 0 ? C.blist[i] : ~C.blist[i];
         C.Lvel[i] = bvel[b];
         C.Lpos[i] = bpos[b];
     }
-    for (int i = tid; i <= nbatch; i += BS) C.Lbo[i] = boff[i];
-    // constraints in schedule order (a level is a contiguous range), renumbered
+    __syncthreads();
+    // constraints in level order (a level is a contiguous range), renumbered
     for (int j = tid; j < nc; j += BS) {
-        const int k = sched[j], cix = ordC[k];
+        const int k = C.Lsched[j], cix = ordC[k], stat = C.Lstat[k - c0];
         VelCon v;
         PosCon q;
         init_constraint(v, q, cont[cix], cix, cisl[k], fbody, fmat, bfl, bmass, bpos, bvel, sp,
                         d.useTan ? ctan[cix] : 0.0f);
-        v.indexA = bmap[v.indexA];
-        v.indexB = bmap[v.indexB];
-        q.indexA = bmap[q.indexA];
-        q.indexB = bmap[q.indexB];
+        auto slot = [&](int b) { return is_static(b) ? stat : bmap[b]; };
+        v.indexA = slot(v.indexA);
+        v.indexB = slot(v.indexB);
+        q.indexA = slot(q.indexA);
+        q.indexB = slot(q.indexB);
         q.island = imap[q.island];
         vc[j] = v;
         pc[j] = q;
@@ -1951,6 +1993,7 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
     __syncthreads();
     if (timed) d.ktime[39] = __builtin_amdgcn_s_memtime();
     const bool residualMode = (sp.flags & (1u << 5)) != 0;
+    const int nbatch = C.nl;
     if (tid < 64) {
         const int lane = tid;
         if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(C.Lbo, nbatch, vc, C.Lvel, C.Limp, lane);
@@ -1970,7 +2013,7 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
             lastRes = r;
             if (r <= sp.residualTol) break;
         }
-        if (lane == 0) {
+        if (lane == 0 && report) {
             d.S[w].iterations = used;
             d.S[w].residual = lastRes;
         }
@@ -1991,7 +2034,8 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
             c->flags |= kContactSolvedOne; // PostSolve reports vc.pointCount points (b2island.d:448-456)
         }
     }
-    for (int i = tid; i < C.cb; i += BS) bvel[C.blist[i]] = C.Lvel[i];
+    for (int i = tid; i < C.cb; i += BS)
+        if (C.blist[i] >= 0) bvel[C.blist[i]] = C.Lvel[i];
     if (timed) {
         d.ktime[42] = __builtin_amdgcn_s_memtime();
         d.ktime[45] = C.cb;
@@ -2002,26 +2046,26 @@ __device__ __forceinline__ bool compact_velocity(const Dev &d, int cur, const St
 }
 
 // Position iterations with the per-island early exit on the compact state, after
-// the positions were integrated in global memory; islSolved set for every island
-// (one without constraints is solved by its first iteration).
-__device__ __forceinline__ void compact_position(const Dev &d, const StepParams &sp, const WorldDesc &wd, int nbatch,
-                                                 int nisl, const CompactLds &C) {
+// the positions were integrated in global memory; islSolved set for the islands
+// [i0, i1) (one without constraints is solved by its first iteration).
+__device__ __forceinline__ void compact_position(const Dev &d, const StepParams &sp, const WorldDesc &wd, int i0,
+                                                 int i1, const CompactLds &C) {
     const int tid = threadIdx.x, BS = blockDim.x;
     float4 *bpos = d.bpos + wd.bodyBase;
-    const PosCon *pc = d.pc + wd.contactBase;
+    const PosCon *pc = d.pc + wd.contactBase + C.c0;
     int *islSolved = d.islSolved + wd.islandBase;
-    const bool timed = d.ktime && tid == 0 && wd.contactBase == d.W[d.order[0]].contactBase;
+    const bool timed = d.ktime && tid == 0 && C.c0 == 0 && wd.contactBase == d.W[d.order[0]].contactBase;
     if (timed) d.ktime[43] = __builtin_amdgcn_s_memtime();
-    for (int i = tid; i < C.cb; i += BS) C.Lpos[i] = bpos[C.blist[i]];
+    for (int i = tid; i < C.cb; i += BS) C.Lpos[i] = bpos[C.blist[i] >= 0 ? C.blist[i] : ~C.blist[i]];
     for (int i = tid; i < C.ci; i += BS) C.Lsol[i] = 0;
-    for (int i = tid; i < nisl; i += BS) islSolved[i] = sp.positionIterations > 0 ? 1 : 0;
+    for (int i = i0 + tid; i < i1; i += BS) islSolved[i] = sp.positionIterations > 0 ? 1 : 0;
     __syncthreads();
     if (tid < 64) {
         const int lane = tid;
         for (int it = 0; it < sp.positionIterations; ++it) {
             for (int i = lane; i < C.ci; i += 64)
                 if (!C.Lsol[i]) C.Lmin[i] = f2ord(0.0f);
-            if (nbatch > 0) position_sweep(C.Lbo, nbatch, pc, C.Lpos, C.Lmin, C.Lsol, lane);
+            if (C.nl > 0) position_sweep(C.Lbo, C.nl, pc, C.Lpos, C.Lmin, C.Lsol, lane);
             bool open = false;
             for (int i = lane; i < C.ci; i += 64) {
                 if (C.Lsol[i]) continue;
@@ -2032,7 +2076,8 @@ __device__ __forceinline__ void compact_position(const Dev &d, const StepParams 
         }
     }
     __syncthreads();
-    for (int i = tid; i < C.cb; i += BS) bpos[C.blist[i]] = C.Lpos[i];
+    for (int i = tid; i < C.cb; i += BS)
+        if (C.blist[i] >= 0) bpos[C.blist[i]] = C.Lpos[i];
     for (int i = tid; i < C.ci; i += BS) islSolved[C.ilist[i]] = C.Lsol[i];
     __syncthreads();
     if (timed) d.ktime[44] = __builtin_amdgcn_s_memtime();
@@ -2040,11 +2085,56 @@ __device__ __forceinline__ void compact_position(const Dev &d, const StepParams 
 
 constexpr int kBodyRun = 4; // bodies (fixtures) per thread and pass in solve_world_global
 
+// b2Fixture.Synchronize with the fat-AABB rule of b2DynamicTree.MoveProxy for
+// fixture f of a body whose sweep started at p0 (c0, a0) and ends at the
+// transform xf2: the swept AABB, and if the fat AABB fa does not hold it, a new fat
+// AABB extended by the displacement, and the fixture marked moved.
+__device__ __forceinline__ void fixture_move(int f, float4 p0, float4 ms, float4 xf2, float4 fa, const Shape *shp,
+                                             float4 *fat, int *moved) {
+    const Xf x1 = xf_from(v2(p0.x, p0.y), p0.z, v2(ms.z, ms.w));
+    Xf x2;
+    x2.p = v2(xf2.x, xf2.y);
+    x2.q.s = xf2.z;
+    x2.q.c = xf2.w;
+    const float4 a1 = shape_aabb(shp + f, x1);
+    const float4 a2 = shape_aabb(shp + f, x2);
+    const float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
+    const float2 disp = sub(x2.p, x1.p);
+    const bool contains = fa.x <= aabb.x && fa.y <= aabb.y && aabb.z <= fa.z && aabb.w <= fa.w;
+    if (contains) return;
+    float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
+                             aabb.w + kAabbExtension);
+    const float2 dd = scl(kAabbMultiplier, disp);
+    if (dd.x < 0.0f) nbx.x += dd.x;
+    else nbx.z += dd.x;
+    if (dd.y < 0.0f) nbx.y += dd.y;
+    else nbx.w += dd.y;
+    fat[f] = nbx;
+    moved[f] = 1;
+}
+
 // Inlined into both solve kernels: as a called function it would take the address
 // of the kernel's Dev argument, which puts a copy of Dev in scratch memory and makes
 // the level loops of the LDS path reload device pointers from there.
+// The first island of ids [0, n) whose first constraint (island order) is at or
+// after `at`; n when there is none.
+__device__ __forceinline__ int island_at(const int4 *isl, int n, int at) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (isl[mid].y < at) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// g, G: with G > 1 (a big world searched by island, k_solve_grp) this workgroup
+// solves the islands [i0, i1) that hold about a G-th of the constraints: islands
+// are independent in b2World.Solve (each its own b2Island.Solve over its own bodies
+// and contacts), their constraints [c0, c1) are contiguous in island order, and
+// their bodies are contiguous in the islands' body lists.  G == 1: every island.
 __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const StepParams &sp, int w, bool lead,
-                                                   char *lds, int ldsBytes) {
+                                                   char *lds, int ldsBytes, int g = 0, int G = 1) {
     __shared__ float s_res;
     const WorldDesc wd = d.W[w];
     const int nb = wd.bodyCount, nf = wd.fixCount;
@@ -2077,21 +2167,45 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     int *islSolved = d.islSolved + wd.islandBase;
     const int tid = threadIdx.x, BS = blockDim.x;
     const float h = sp.dt;
+    int i0 = 0, i1 = nisl, c0 = 0, c1 = nc;
+    const int *glist = nullptr; // the group's bodies (G > 1), else every body of the world in an island
+    int nbl = nb;
+    if (G > 1) {
+        const int4 *isl = d.iwIsl + wd.bodyBase;
+        i0 = island_at(isl, nisl, (int)((long long)nc * g / G));
+        i1 = g == G - 1 ? nisl : island_at(isl, nisl, (int)((long long)nc * (g + 1) / G));
+        c0 = i0 < nisl ? isl[i0].y : nc;
+        c1 = i1 < nisl ? isl[i1].y : nc;
+        const int *stack = isl_carve(d.islScratch + wd.islScratchBase, nb, d.S[w].contactCount).stack;
+        // the islands' body lists end at the last island's end; a range starting
+        // past the last island (i0 == nisl) is empty
+        const int end = nisl > 0 ? isl[nisl - 1].z + d.iwCur[wd.bodyBase + nisl - 1] : 0;
+        const int s0 = i0 < nisl ? isl[i0].z : end;
+        const int s1 = i1 < nisl ? isl[i1].z : end;
+        glist = stack + s0;
+        nbl = s1 > s0 ? s1 - s0 : 0;
+    }
+    // body j of this call's passes, -1 when it is not simulated
+    auto body_at = [&](int j) -> int {
+        if (j >= nbl) return -1;
+        if (glist) return glist[j];
+        return bisl[j] >= 0 ? j : -1;
+    };
 
     // Integrate velocities, damping (b2island.d:82-116).  The per-body passes of
     // this path take kBodyRun bodies per thread with all their loads issued before
     // the first store, so that a pass over a big world waits for memory once per
     // run instead of once per body.
-    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
-        int isl[kBodyRun];
+    for (int b0 = tid; b0 < nbl; b0 += kBodyRun * BS) {
+        int bb[kBodyRun];
 #pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        for (int u = 0; u < kBodyRun; ++u) bb[u] = body_at(b0 + u * BS);
         float4 pos[kBodyRun], vel[kBodyRun], frc[kBodyRun], ms[kBodyRun], pr[kBodyRun];
         uint32_t fl[kBodyRun];
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
-            const int b = b0 + u * BS;
+            if (bb[u] < 0) continue;
+            const int b = bb[u];
             pos[u] = bpos[b];
             fl[u] = bfl[b];
             vel[u] = bvel[b];
@@ -2101,8 +2215,8 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
         }
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
-            const int b = b0 + u * BS;
+            if (bb[u] < 0) continue;
+            const int b = bb[u];
             bpos0[b] = make_float4(pos[u].x, pos[u].y, pos[u].z, 0.0f);
             if (body_type(fl[u]) != kDynamic) continue;
             float2 v = v2(vel[u].x, vel[u].y);
@@ -2119,14 +2233,18 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     }
     __syncthreads();
     ProfStamp prof;
+    lead = lead && g == 0;
     if (tid == 0) prof.begin(d, lead);
 
     // The compact LDS path when the constrained bodies fit (no joints, no learned
-    // warm start or logging, which address bodies by their world index).
+    // warm start or logging, which address bodies by their world index).  With
+    // G > 1, a group without constraints reports nothing (the others report the
+    // same counts).
     const bool jointWorld = wd.jointCount > 0;
+    const bool report = G == 1 || c1 > c0;
     CompactLds C{};
-    const bool compact = !jointWorld && nc > 0 && ldsBytes > 0 && !(sp.flags & ((1u << 4) | (1u << 6))) &&
-                         compact_velocity(d, cur, sp, w, wd, nc, nbatch, lds, ldsBytes, C);
+    const bool compact = !jointWorld && c1 > c0 && ldsBytes > 0 && !(sp.flags & ((1u << 4) | (1u << 6))) &&
+                         compact_velocity(d, cur, sp, w, wd, c0, c1, lds, ldsBytes, C, report);
     if (compact) prof.at(kProfVel0);
     const int *icoff = d.islCoff + wd.bodyBase + w, *ijoff = d.islJoff + wd.bodyBase + w;
     JointRec *J = d.joints + wd.jointBase;
@@ -2134,7 +2252,7 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     const JointStep jst{sp.dt, sp.inv_dt, sp.dtRatio, (sp.flags & 1u) != 0};
     if (!compact) {
     // Constraint set-up.
-    for (int k = tid; k < nc; k += BS) {
+    for (int k = c0 + tid; k < c1; k += BS) {
         int ci = ordC[k];
         VelCon v;
         PosCon p;
@@ -2159,7 +2277,10 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     } else if (sp.flags & 1u) {
         for (int bt = 0; bt < nbatch; ++bt) {
             int e = boff[bt + 1];
-            for (int j = boff[bt] + tid; j < e; j += BS) warm_start(vc[sched[j]], bvel);
+            for (int j = boff[bt] + tid; j < e; j += BS) {
+                const int k = sched[j];
+                if (k >= c0 && k < c1) warm_start(vc[k], bvel);
+            }
             __syncthreads();
         }
     }
@@ -2192,6 +2313,7 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
             int e = boff[bt + 1];
             for (int j = boff[bt] + tid; j < e; j += BS) {
                 int k = sched[j];
+                if (k < c0 || k >= c1) continue;
                 VelCon v = vc[k];
                 float r = solve_velocity(v, bvel, residualMode);
                 lres = fmaxr(lres, r);
@@ -2209,14 +2331,14 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
             if (r <= sp.residualTol) break;
         }
     }
-    if (tid == 0) {
+    if (tid == 0 && report) {
         d.S[w].iterations = used;
         d.S[w].residual = lastRes;
     }
     if (sp.flags & (1u << 6)) mlp_log_labels(d, w, gimp);
 
     // StoreImpulses (b2contactsolver.d:774-787)
-    for (int k = tid; k < nc; k += BS) {
+    for (int k = c0 + tid; k < c1; k += BS) {
         const VelCon &v = vc[k];
         Contact *c = cont + v.contact;
         c->pt0.z = v.nImp0;
@@ -2232,21 +2354,21 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     __syncthreads();
     prof.at(kProfVel1);
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
-    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
-        int isl[kBodyRun];
+    for (int b0 = tid; b0 < nbl; b0 += kBodyRun * BS) {
+        int bb[kBodyRun];
 #pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        for (int u = 0; u < kBodyRun; ++u) bb[u] = body_at(b0 + u * BS);
         float4 pos[kBodyRun], vel[kBodyRun];
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
-            pos[u] = bpos[b0 + u * BS];
-            vel[u] = bvel[b0 + u * BS];
+            if (bb[u] < 0) continue;
+            pos[u] = bpos[bb[u]];
+            vel[u] = bvel[bb[u]];
         }
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
-            const int b = b0 + u * BS;
+            if (bb[u] < 0) continue;
+            const int b = bb[u];
             float2 v = v2(vel[u].x, vel[u].y);
             float wv = vel[u].z;
             float2 tr = scl(h, v);
@@ -2263,12 +2385,12 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
             bvel[b] = make_float4(v.x, v.y, wv, 0.0f);
         }
     }
-    for (int i = tid; i < nisl; i += BS) islSolved[i] = 0;
+    for (int i = i0 + tid; i < i1; i += BS) islSolved[i] = 0;
     __syncthreads();
     prof.at(kProfPos0);
 
     // Position iterations with the per-island early exit (b2island.d:206-224).
-    if (compact) compact_position(d, sp, wd, nbatch, nisl, C);
+    if (compact) compact_position(d, sp, wd, i0, i1, C);
     if (jointWorld) {
         for (int isl = tid; isl < nisl; isl += BS) {
             int solved = 0;
@@ -2294,12 +2416,13 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
         __syncthreads();
     }
     for (int it = 0; it < (jointWorld || compact ? 0 : sp.positionIterations); ++it) {
-        for (int i = tid; i < nisl; i += BS) islMin[i] = f2ord(0.0f);
+        for (int i = i0 + tid; i < i1; i += BS) islMin[i] = f2ord(0.0f);
         __syncthreads();
         for (int bt = 0; bt < nbatch; ++bt) {
             int e = boff[bt + 1];
             for (int j = boff[bt] + tid; j < e; j += BS) {
                 int k = sched[j];
+                if (k < c0 || k >= c1) continue;
                 const PosCon p = pc[k];
                 int isl = p.island;
                 if (islSolved[isl]) continue;
@@ -2309,7 +2432,7 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
             __syncthreads();
         }
         int open = 0;
-        for (int i = tid; i < nisl; i += BS) {
+        for (int i = i0 + tid; i < i1; i += BS) {
             if (islSolved[i]) continue;
             if (ord2f(islMin[i]) >= -3.0f * kLinearSlop) islSolved[i] = 1;
             else open = 1;
@@ -2319,70 +2442,72 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     __syncthreads();
 
     // Copy back and SynchronizeTransform (b2island.d:227-235, b2body.d:1143-1147).
-    for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
-        int isl[kBodyRun];
+    for (int b0 = tid; b0 < nbl; b0 += kBodyRun * BS) {
+        int bb[kBodyRun];
 #pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+        for (int u = 0; u < kBodyRun; ++u) bb[u] = body_at(b0 + u * BS);
         float4 pos[kBodyRun], ms[kBodyRun];
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
-            pos[u] = bpos[b0 + u * BS];
-            ms[u] = bmass[b0 + u * BS];
+            if (bb[u] < 0) continue;
+            pos[u] = bpos[bb[u]];
+            ms[u] = bmass[bb[u]];
         }
 #pragma unroll
         for (int u = 0; u < kBodyRun; ++u) {
-            if (isl[u] < 0) continue;
+            if (bb[u] < 0) continue;
             Xf x = xf_from(v2(pos[u].x, pos[u].y), pos[u].z, v2(ms[u].z, ms[u].w));
-            bxf[b0 + u * BS] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
+            bxf[bb[u]] = make_float4(x.p.x, x.p.y, x.q.s, x.q.c);
         }
     }
     __syncthreads();
     prof.at(kProfPos1);
     // Sleep (b2island.d:241-279).
     if (sp.flags & 2u) {
-        for (int i = tid; i < nisl; i += BS) islMin[i] = __float_as_int(kMaxFloat);
+        for (int i = i0 + tid; i < i1; i += BS) islMin[i] = __float_as_int(kMaxFloat);
         __syncthreads();
         const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
         const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
-        for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
-            int isl[kBodyRun];
+        for (int b0 = tid; b0 < nbl; b0 += kBodyRun * BS) {
+            int bb[kBodyRun];
 #pragma unroll
-            for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+            for (int u = 0; u < kBodyRun; ++u) bb[u] = body_at(b0 + u * BS);
             uint32_t fl[kBodyRun];
             float4 vel[kBodyRun];
             float st[kBodyRun];
 #pragma unroll
             for (int u = 0; u < kBodyRun; ++u) {
-                if (isl[u] < 0) continue;
-                fl[u] = bfl[b0 + u * BS];
-                vel[u] = bvel[b0 + u * BS];
-                st[u] = bpos[b0 + u * BS].w;
+                if (bb[u] < 0) continue;
+                fl[u] = bfl[bb[u]];
+                vel[u] = bvel[bb[u]];
+                st[u] = bpos[bb[u]].w;
             }
 #pragma unroll
             for (int u = 0; u < kBodyRun; ++u) {
-                if (isl[u] < 0 || body_type(fl[u]) == kStatic) continue;
+                if (bb[u] < 0 || body_type(fl[u]) == kStatic) continue;
                 const bool restless = ((fl[u] & kBodyAutoSleep) == 0) | (vel[u].z * vel[u].z > angTolSqr) |
                                       ((vel[u].x * vel[u].x + vel[u].y * vel[u].y) > linTolSqr);
                 const float nst = restless ? 0.0f : st[u] + h;
-                atomicMin(&islMin[isl[u]], __float_as_int(nst));
-                bpos[b0 + u * BS].w = nst;
+                atomicMin(&islMin[bisl[bb[u]]], __float_as_int(nst));
+                bpos[bb[u]].w = nst;
             }
         }
         __syncthreads();
-        for (int b0 = tid; b0 < nb; b0 += kBodyRun * BS) {
-            int isl[kBodyRun];
+        for (int b0 = tid; b0 < nbl; b0 += kBodyRun * BS) {
+            int bb[kBodyRun];
 #pragma unroll
-            for (int u = 0; u < kBodyRun; ++u) isl[u] = b0 + u * BS < nb ? bisl[b0 + u * BS] : -1;
+            for (int u = 0; u < kBodyRun; ++u) bb[u] = body_at(b0 + u * BS);
             bool sleep[kBodyRun];
 #pragma unroll
-            for (int u = 0; u < kBodyRun; ++u)
-                sleep[u] = isl[u] >= 0 && __int_as_float(islMin[isl[u]]) >= kTimeToSleep && islSolved[isl[u]];
+            for (int u = 0; u < kBodyRun; ++u) {
+                const int isl = bb[u] >= 0 ? bisl[bb[u]] : -1;
+                sleep[u] = isl >= 0 && __int_as_float(islMin[isl]) >= kTimeToSleep && islSolved[isl];
+            }
 #pragma unroll
             for (int u = 0; u < kBodyRun; ++u) {
                 if (!sleep[u]) continue;
                 // SetAwake(false) (b2body.d:836-845)
-                const int b = b0 + u * BS;
+                const int b = bb[u];
                 bfl[b] &= ~kBodyAwake;
                 bpos[b].w = 0.0f;
                 bvel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -2393,57 +2518,76 @@ __device__ __forceinline__ void solve_world_global(const Dev &d, int cur, const 
     __syncthreads();
 
     // SynchronizeFixtures + fat-AABB rule of b2DynamicTree.MoveProxy
-    // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
-    for (int f0 = tid; f0 < nf; f0 += kBodyRun * BS) {
-        int fb[kBodyRun];
-#pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) fb[u] = f0 + u * BS < nf ? fbody[f0 + u * BS] : -1;
-        bool live[kBodyRun];
-#pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) live[u] = fb[u] >= 0 && bisl[fb[u]] >= 0 && body_type(bfl[fb[u]]) != kStatic;
-        float4 p0[kBodyRun], ms[kBodyRun], xf2[kBodyRun], fa[kBodyRun];
-#pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) {
-            if (!live[u]) continue;
-            p0[u] = bpos0[fb[u]];
-            ms[u] = bmass[fb[u]];
-            xf2[u] = bxf[fb[u]];
-            fa[u] = fat[f0 + u * BS];
+    // (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184): over the
+    // fixtures of the world, or with G > 1 over the fixtures of the group's bodies.
+    if (glist) {
+        const int *bfixOff = d.bfixOff + wd.bodyBase, *bfixCnt = d.bfixCnt + wd.bodyBase;
+        const int *bfixList = d.bfixList + wd.fixBase;
+        for (int j = tid; j < nbl; j += BS) {
+            const int b = glist[j];
+            if (body_type(bfl[b]) == kStatic) continue;
+            const float4 p0 = bpos0[b], ms = bmass[b], xf2 = bxf[b];
+            const int q0 = bfixOff[b], q1 = q0 + bfixCnt[b];
+            for (int q = q0; q < q1; ++q) {
+                const int f = bfixList[q];
+                fixture_move(f, p0, ms, xf2, fat[f], shp, fat, moved);
+            }
         }
+    } else {
+        for (int f0 = tid; f0 < nf; f0 += kBodyRun * BS) {
+            int fb[kBodyRun];
 #pragma unroll
-        for (int u = 0; u < kBodyRun; ++u) {
-            if (!live[u]) continue;
-            const int f = f0 + u * BS;
-            Xf x1 = xf_from(v2(p0[u].x, p0[u].y), p0[u].z, v2(ms[u].z, ms[u].w));
-            Xf x2;
-            x2.p = v2(xf2[u].x, xf2[u].y);
-            x2.q.s = xf2[u].z;
-            x2.q.c = xf2[u].w;
-            float4 a1 = shape_aabb(shp + f, x1);
-            float4 a2 = shape_aabb(shp + f, x2);
-            float4 aabb = make_float4(fminr(a1.x, a2.x), fminr(a1.y, a2.y), fmaxr(a1.z, a2.z), fmaxr(a1.w, a2.w));
-            float2 disp = sub(x2.p, x1.p);
-            bool contains = fa[u].x <= aabb.x && fa[u].y <= aabb.y && aabb.z <= fa[u].z && aabb.w <= fa[u].w;
-            if (contains) continue;
-            float4 nbx = make_float4(aabb.x - kAabbExtension, aabb.y - kAabbExtension, aabb.z + kAabbExtension,
-                                     aabb.w + kAabbExtension);
-            float2 dd = scl(kAabbMultiplier, disp);
-            if (dd.x < 0.0f) nbx.x += dd.x;
-            else nbx.z += dd.x;
-            if (dd.y < 0.0f) nbx.y += dd.y;
-            else nbx.w += dd.y;
-            fat[f] = nbx;
-            moved[f] = 1;
+            for (int u = 0; u < kBodyRun; ++u) fb[u] = f0 + u * BS < nf ? fbody[f0 + u * BS] : -1;
+            bool live[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u)
+                live[u] = fb[u] >= 0 && bisl[fb[u]] >= 0 && body_type(bfl[fb[u]]) != kStatic;
+            float4 p0[kBodyRun], ms[kBodyRun], xf2[kBodyRun], fa[kBodyRun];
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u) {
+                if (!live[u]) continue;
+                p0[u] = bpos0[fb[u]];
+                ms[u] = bmass[fb[u]];
+                xf2[u] = bxf[fb[u]];
+                fa[u] = fat[f0 + u * BS];
+            }
+#pragma unroll
+            for (int u = 0; u < kBodyRun; ++u)
+                if (live[u]) fixture_move(f0 + u * BS, p0[u], ms[u], xf2[u], fa[u], shp, fat, moved);
         }
     }
-    // b2World.ClearForces (b2world.d:443-450)
-    for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // b2World.ClearForces (b2world.d:443-450): this call's bodies, and with G > 1 the
+    // bodies in no island dealt out over the groups (another group may still read
+    // the forces of its own bodies)
+    if (!glist)
+        for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    else {
+        for (int j = tid; j < nbl; j += BS) bforce[glist[j]] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int b = g * BS + tid; b < nb; b += G * BS)
+            if (bisl[b] < 0) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// A big world whose islands k_solve_grp solves (kStepGrouped; searched by island:
+// exact order, no joints): the per-world solve kernels skip it.
+__device__ __forceinline__ bool solve_grouped(const Dev &d, const StepParams &sp, const WorldDesc &wd) {
+    return (sp.flags & kStepGrouped) && (wd.pad & 2) && wd.jointCount == 0 && d.iwParent != nullptr;
 }
 
 template <bool kLead>
 __global__ __launch_bounds__(256) void k_solve(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
-    solve_world_global(d, cur, sp, d.order[worldOffset + blockIdx.x], worldOffset + blockIdx.x == 0, lds, ldsBytes);
+    const int w = d.order[worldOffset + blockIdx.x];
+    if (solve_grouped(d, sp, d.W[w])) return;
+    solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0, lds, ldsBytes);
+}
+
+// One big world's islands on G workgroups, each solving the islands that hold a
+// G-th of its constraints (solve_world_global with g = blockIdx.x).
+template <bool kLead>
+__global__ __launch_bounds__(256) void k_solve_grp(Dev d, int cur, StepParams sp, int w, int G, int ldsBytes, int lead) {
+    extern __shared__ __align__(16) char lds[];
+    solve_world_global(d, cur, sp, w, lead != 0, lds, ldsBytes, blockIdx.x, G);
 }
 
 template <bool kLead>
@@ -2451,6 +2595,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     extern __shared__ __align__(16) char lds[];
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
+    if (solve_grouped(d, sp, wd)) return;
     const int nb = wd.bodyCount, nf = wd.fixCount;
     const int nc = d.S[w].constraints;
     const int nbatch = d.S[w].batchCount;
@@ -4572,6 +4717,9 @@ static hipError_t configure_tier(int maxDynamicLds) {
     e = hipFuncSetAttribute((const void *)k_solve<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_solve_grp<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            maxDynamicLds - 1024);
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_solve_wave<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                maxDynamicLds - 1024);
 }
@@ -4611,6 +4759,19 @@ hipError_t launch_solve(const Dev &d, int cur, const StepParams &sp, int worldOf
     if (nworlds <= 0) return hipSuccess;
     if (block > 256) block = 256; // the kernel's launch bound
     DBX_LAUNCH(k_solve, lead, dim3(nworlds), dim3(block), ldsBytes, s, d, cur, sp, worldOffset, ldsBytes);
+    return hipGetLastError();
+}
+hipError_t launch_solve_groups(const Dev &d, int cur, const StepParams &sp, const WideBp *wide, int nwide,
+                               int ldsBytes, bool lead, hipStream_t s) {
+    for (int k = 0; k < nwide; ++k) {
+        const WorldDesc &D = wide[k].desc;
+        if (D.jointCount > 0) continue;
+        // about 1,024 contact slots per group: a few hundred constraints each
+        int G = D.contactCap / 1024;
+        G = G < 1 ? 1 : (G > 256 ? 256 : G);
+        const int first = lead && wide[k].tierFirst ? 1 : 0;
+        DBX_LAUNCH(k_solve_grp, lead, dim3(G), dim3(256), ldsBytes, s, d, cur, sp, wide[k].world, G, ldsBytes, first);
+    }
     return hipGetLastError();
 }
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,

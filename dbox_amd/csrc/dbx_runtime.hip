@@ -989,6 +989,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
                 D.pad |= 2;
                 WideBp b{};
                 b.world = order[i];
+                b.tierFirst = i == T.first ? 1 : 0;
                 b.n = D.fixCount;
                 b.cap = D.pairCap;
                 h->wide.push_back(b);
@@ -1090,6 +1091,7 @@ dbx_status dbx_finalize(dbx_handle *h) {
         DBX_HIP(h, dalloc(h, &d.iwParent, nb));
         DBX_HIP(h, dalloc(h, &d.iwAgg, nb));
         DBX_HIP(h, dalloc(h, &d.iwIsl, nb));
+        DBX_HIP(h, dalloc(h, &d.iwCur, nb));
     }
     DBX_HIP(h, dalloc(h, &d.bscratch, nb));
     DBX_HIP(h, dalloc(h, &d.wake, nb));
@@ -1260,6 +1262,8 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
         // global memory by a 256-thread workgroup, level batches separated by barriers.
         if (ordered && T.wideCount < T.count) DBX_HIP(h, launch_solve_wave(d, cout, sp, T.first, T.count, lds1, lead, s));
         else DBX_HIP(h, launch_solve(d, cout, sp, T.first, T.count, ordered ? 256 : T.block, ordered ? lds1 : 0, lead, s));
+        if (ordered && T.wideCount && d.iwParent && (sp.flags & kStepGrouped))
+            DBX_HIP(h, launch_solve_groups(d, cout, sp, wide, T.wideCount, lds1, lead, s));
         if (timed) (void)hipEventRecord(ev[3], s);
         DBX_HIP(h, launch_broadphase(d, cout, T.first, T.count, lds1, T.block, lead, s, wide, T.wideCount));
         if (timed) (void)hipEventRecord(ev[4], s);
@@ -1410,6 +1414,9 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     sp.flags = ((flags & DBX_STEP_WARM) ? 1u : 0u) | ((flags & DBX_STEP_SLEEP) ? 2u : 0u) |
                ((flags & DBX_STEP_RESIDUAL) ? (1u << 5) : 0u) | ((flags & DBX_STEP_LEARNED) ? (1u << 4) : 0u) |
                ((flags & DBX_STEP_LOG) ? (1u << 6) : 0u);
+    if ((flags & DBX_STEP_ORDERED) && !(flags & (DBX_STEP_RESIDUAL | DBX_STEP_LEARNED | DBX_STEP_LOG)) &&
+        !getenv("DBX_SOLVE_UNGROUPED"))
+        sp.flags |= kStepGrouped;
     if ((flags & (DBX_STEP_LEARNED | DBX_STEP_LOG)) && !h->staged) {
         dbx_status st = stage_mlp(h);
         if (st != DBX_OK) return st;

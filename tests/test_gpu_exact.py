@@ -161,3 +161,23 @@ def test_residual_mode_at_max_iterations_equals_the_plain_step(which, steps):
         assert resid.bodies(0).tobytes() == plain.bodies(0).tobytes(), f"step {i + 1}: bodies"
     assert solved > 0
     assert resid.contacts(0).tobytes() == plain.contacts(0).tobytes()
+
+
+@pytest.mark.parametrize("which,steps", [("config 2 container", 40), ("config 4 pyramid", 30)])
+def test_island_groups_equal_one_workgroup(which, steps, monkeypatch):
+    # A big world's islands are solved by many workgroups at once (each a range of
+    # islands with about 1,024 contact slots' worth of constraints); with
+    # DBX_SOLVE_UNGROUPED set a step solves them all on one workgroup.  Islands are
+    # independent in b2World.Solve, so every step must give the same bodies and
+    # contacts bit for bit, including steps where trailing groups hold no island.
+    print(f"case: island groups vs one workgroup, {which}, {steps} steps", flush=True)
+    scene = {"config 2 container": scenes.big_mixed_world, "config 4 pyramid": scenes.big_pyramid}[which]()
+    grouped, single = _gpu([scene]), _gpu([scene])
+    for i in range(steps):
+        monkeypatch.delenv("DBX_SOLVE_UNGROUPED", raising=False)
+        grouped.step(DT, 8, 3, DEFAULTS)
+        monkeypatch.setenv("DBX_SOLVE_UNGROUPED", "1")
+        single.step(DT, 8, 3, DEFAULTS)
+        assert int(grouped.toi_events()[0]) == int(single.toi_events()[0]), f"step {i + 1}: TOI events"
+        assert grouped.bodies(0).tobytes() == single.bodies(0).tobytes(), f"step {i + 1}: bodies"
+    assert grouped.contacts(0).tobytes() == single.contacts(0).tobytes()
