@@ -143,6 +143,13 @@ struct StepParams {
 // no joints) are solved by several workgroups at once (k_solve_grp); not set in the
 // residual, learned and logging modes, which treat the world's constraints as one set.
 constexpr uint32_t kStepGrouped = 1u << 8;
+// StepParams.flags bit: in the exact-order solve of a world on the lead tier's
+// two-wave workgroups, the warm start and the velocity iterations may overlap,
+// one sweep per wave (velocity_piped), where the world's schedule lets them; not
+// set in the residual mode, whose sweeps each end with a test of the whole sweep.
+constexpr uint32_t kStepPiped = 1u << 9;
+// With kStepPiped: overlap the sweeps even where the schedule says it gains little.
+constexpr uint32_t kStepPipedAlways = 1u << 10;
 
 struct JointRec; // dbx_joints.h
 

@@ -1636,7 +1636,7 @@ __device__ __forceinline__ float solve_position(const PosCon &p, int indexA, int
 // ---------------------------------------------------------------------------
 __host__ __device__ inline size_t solve_lds_bytes(int nb, int nc, int nbatch, int nisl) {
     return isl_align(32 * (size_t)nb) + isl_align(16 * (size_t)nc) + isl_align(4 * (size_t)(nbatch + 1)) +
-           isl_align(8 * (size_t)nisl);
+           2 * isl_align(4 * (size_t)nisl);
 }
 
 // One constraint of a level, on LDS body state.  kMode 0 = warm start,
@@ -1731,6 +1731,96 @@ __device__ __forceinline__ float velocity_sweep(const int *Lbo, int nbatch, cons
         c1 = vc[k1];
     }
     return lres;
+}
+
+// Overlapped sweeps on two waves.  Sweeps s = 0, 1, ... (the warm start when
+// warm starting, then the velocity iterations) run the same levels; wave s & 1
+// runs sweep s.  An update of body b in sweep s must follow every update of b in
+// sweep s - 1 and precede every one in sweep s + 1 (the other wave's sweeps); all
+// other orders are free.  So level l of sweep s waits only until the other wave
+// has finished level need(l) of sweep s - 1, where need(l) is the largest level
+// holding a constraint on a dynamic body of level l (the last level that body is
+// updated in).  Each wave publishes its progress in LDS after every level, as
+// s * nbatch + levels done in sweep s, and as (s + 2) * nbatch once sweep s is
+// done (INT_MAX after its last sweep); progress only grows.  A wave in sweep s
+// waits only on the other wave's sweep s - 1, never on its sweep s + 1, and that
+// wave is never waiting on sweep s: of two waves one can always go on.  A wave's
+// LDS operations are performed in issue order, so a publication follows the
+// level's writes, and the waiting wave issues its reads only after it has seen
+// the publication.  Each lane publishes its own copy (one conflict-free write),
+// read back by lane; the results are the sequential sweeps' bit for bit.
+__device__ __forceinline__ void pipe_wait(const int *Lnext, int other, int need, int &seen) {
+    while (seen < need)
+        seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(Lnext + 64 * other + (threadIdx.x & 63),
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    asm volatile("" ::: "memory");
+}
+__device__ __forceinline__ void pipe_publish(int *Lnext, int me, int progress) {
+    asm volatile("" ::: "memory");
+    __hip_atomic_store(Lnext + 64 * me + (threadIdx.x & 63), progress, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One sweep s of the overlapped velocity phase; Lneed holds need(l) per level.
+template <int kMode>
+__device__ __forceinline__ void velocity_sweep_piped(const int *Lbo, const int *Lneed, int nbatch,
+                                                     const VelCon *__restrict__ vc, float4 *Lvel, float4 *Limp,
+                                                     int lane, int s, int endProgress, int *Lnext, int me,
+                                                     int &seen) {
+    float lres = 0.0f;
+    LevelWindow W{Lbo, nbatch + 1, 0, 0}, N{Lneed, nbatch, 0, 0};
+    W.load(0);
+    N.load(0);
+    // the other wave's progress needed before level bt: past need(bt) in sweep s - 1
+    const int base = s > 0 ? (s - 1) * nbatch + 1 : INT_MIN / 2;
+    const int mine = s * nbatch;
+    int k0, k1, lo0, hi0, lo1, hi1;
+    bool h0, h1;
+    level_slot(W, 0, nbatch, lane, k0, h0, lo0, hi0);
+    VelCon c0 = vc[k0];
+    level_slot(W, 1, nbatch, lane, k1, h1, lo1, hi1);
+    VelCon c1 = vc[k1];
+    for (int bt = 0; bt < nbatch; bt += 2) {
+        W.cover(bt);
+        N.cover(bt);
+        pipe_wait(Lnext, me ^ 1, base + N.at(bt), seen);
+        velocity_level<kMode>(c0, k0, h0, lo0, hi0, vc, Lvel, Limp, lres);
+        pipe_publish(Lnext, me, bt + 1 < nbatch ? mine + bt + 1 : endProgress);
+        level_slot(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
+        c0 = vc[k0];
+        if (bt + 1 >= nbatch) break;
+        pipe_wait(Lnext, me ^ 1, base + N.at(bt + 1), seen);
+        velocity_level<kMode>(c1, k1, h1, lo1, hi1, vc, Lvel, Limp, lres);
+        pipe_publish(Lnext, me, bt + 2 < nbatch ? mine + bt + 2 : endProgress);
+        level_slot(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
+        c1 = vc[k1];
+    }
+}
+
+// The warm start and nvel velocity iterations, overlapped on waves 0 and 1 of the
+// workgroup.  Lnext: 128 ints of LDS that hold 0 on entry.  Stamps the warm
+// start's end (or the velocity phase's start) through prof.
+__device__ __forceinline__ void velocity_piped(const int *Lbo, const int *Lneed, int nbatch,
+                                               const VelCon *__restrict__ vc, float4 *Lvel, float4 *Limp,
+                                               int *Lnext, bool warm, int nvel, const ProfStamp &prof) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nsweeps = (warm ? 1 : 0) + nvel;
+    int seen = 0;
+    if (!warm) prof.at(kProfVel0);
+    if (wave >= 2) return;
+    if (wave >= nsweeps) {
+        pipe_publish(Lnext, wave, INT_MAX);
+        return;
+    }
+    for (int s = wave; s < nsweeps; s += 2) {
+        const int endProgress = s + 2 < nsweeps ? (s + 2) * nbatch : INT_MAX;
+        if (warm && s == 0) {
+            velocity_sweep_piped<0>(Lbo, Lneed, nbatch, vc, Lvel, Limp, lane, s, endProgress, Lnext, wave, seen);
+            prof.at(kProfVel0);
+        } else {
+            velocity_sweep_piped<1>(Lbo, Lneed, nbatch, vc, Lvel, Limp, lane, s, endProgress, Lnext, wave, seen);
+        }
+    }
 }
 
 // The position sweeps give each constraint two lanes (an even lane and the odd
@@ -2590,8 +2680,12 @@ __global__ __launch_bounds__(256) void k_solve_grp(Dev d, int cur, StepParams sp
     solve_world_global(d, cur, sp, w, lead != 0, lds, ldsBytes, blockIdx.x, G);
 }
 
+// The lead tier runs two waves per world (128 threads): the velocity phase
+// overlaps its sweeps on them (kStepPiped, velocity_piped) and the passes over
+// bodies and constraints are shared; the position sweeps stay on wave 0.  Other
+// tiers run one wave per world.
 template <bool kLead>
-__global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
+__global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams sp, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
@@ -2604,13 +2698,25 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         solve_world_global(d, cur, sp, w, worldOffset + blockIdx.x == 0, lds, ldsBytes);
         return;
     }
+    const int tid = threadIdx.x, BS = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    const bool residualMode = (sp.flags & (1u << 5)) != 0;
+    // overlapped sweeps: two waves, per body the level of its last constraint
+    // (Llast), per level need(l) (Lneed) and the published progress (Lnext)
+    const size_t pipeBytes = isl_align(4 * (size_t)nb) + isl_align(4 * (size_t)nbatch) + 16 + 4 * 128;
+    const bool piped = (sp.flags & kStepPiped) && BS >= 128 && !residualMode && nbatch > 0 &&
+                       (long long)(sp.velocityIterations + 3) * (nbatch + 1) < (1ll << 30) &&
+                       solve_lds_bytes(nb, nc, nbatch, nisl) + pipeBytes <= (size_t)ldsBytes;
     char *p = lds;
     float4 *Lvel = (float4 *)p; p += isl_align(16 * (size_t)nb);
     float4 *Lpos = (float4 *)p; p += isl_align(16 * (size_t)nb);
     float4 *Limp = (float4 *)p; p += isl_align(16 * (size_t)nc);
     int *Lbo = (int *)p; p += isl_align(4 * (size_t)(nbatch + 1));
     int *Lmin = (int *)p; p += isl_align(4 * (size_t)nisl);
-    int *Lsol = (int *)p;
+    int *Lsol = (int *)p; p += isl_align(4 * (size_t)nisl);
+    int *Llast = (int *)p; p += isl_align(4 * (size_t)nb);
+    int *Lneed = (int *)p; p += isl_align(4 * (size_t)nbatch);
+    int *Llag = (int *)p; p += 16;
+    int *Lnext = (int *)p;
 
     Contact *cont = d.cont[cur] + wd.contactBase;
     const float *ctan = d.ctan[cur] + wd.contactBase;
@@ -2634,14 +2740,22 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     const int *boff = d.batchOff + wd.contactBase + 65 * w;
     VelCon *vc = d.vc + wd.contactBase;
     PosCon *pc = d.pc + wd.contactBase;
-    const int lane = threadIdx.x;
     const float h = sp.dt;
 
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0;
-    if (timed && lane == 0) d.ktime[4] = __builtin_amdgcn_s_memtime();
-    for (int i = lane; i <= nbatch; i += 64) Lbo[i] = boff[i];
+    if (timed && tid == 0) {
+        d.ktime[4] = __builtin_amdgcn_s_memtime();
+        d.ktime[12] = __builtin_amdgcn_s_memrealtime();
+    }
+    for (int i = tid; i <= nbatch; i += BS) Lbo[i] = boff[i];
+    if (piped) {
+        for (int b = tid; b < nb; b += BS) Llast[b] = -1;
+        for (int i = tid; i < nbatch; i += BS) Lneed[i] = 0;
+        if (tid < 128) Lnext[tid] = 0;
+        if (tid == 0) Llag[0] = 0;
+    }
     // Integrate velocities with damping (b2island.d:82-116) while staging the bodies.
-    for (int b = lane; b < nb; b += 64) {
+    for (int b = tid; b < nb; b += BS) {
         const float4 pos = bpos[b], vel = bvel[b];
         const int isl = bisl[b];
         const bool dyn = body_type(bfl[b]) == kDynamic;
@@ -2665,10 +2779,11 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     }
     __syncthreads();
     ProfStamp prof;
-    if (lane == 0) prof.begin(d, worldOffset + blockIdx.x == 0);
+    if (tid == 0) prof.begin(d, worldOffset + blockIdx.x == 0);
     // Constraints are stored in schedule order (slot j holds constraint sched[j]),
     // so a level's constraints are a contiguous range of slots.
-    for (int j = lane; j < nc; j += 64) {
+    const int *cbatch = d.cbatch + wd.contactBase;
+    for (int j = tid; j < nc; j += BS) {
         const int k = sched[j];
         const int ci = ordC[k];
         VelCon v;
@@ -2678,16 +2793,45 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         vc[j] = v;
         pc[j] = q;
         Limp[j] = make_float4(v.nImp0, v.tImp0, v.nImp1, v.tImp1);
+        if (piped) { // the last level of each dynamic body
+            const int l = cbatch[k];
+            if (v.dynMask & 1) atomicMax(&Llast[v.indexA], l);
+            if (v.dynMask & 2) atomicMax(&Llast[v.indexB], l);
+        }
     }
     __syncthreads();
     if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, ImpRef{Limp, 1}, Lvel, Lpos);
 
-    if (timed && lane == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
-    const bool residualMode = (sp.flags & (1u << 5)) != 0;
-    if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lbo, nbatch, vc, Lvel, Limp, lane);
-    if (timed && lane == 0) d.ktime[6] = __builtin_amdgcn_s_memtime();
-    prof.at(kProfVel0);
-    {
+    if (timed && tid == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
+    bool overlap = false;
+    if (piped) {
+        // need(l): the last level of the dynamic bodies of level l's constraints;
+        // the lag: the most levels a sweep's level l must trail need(l) by
+        for (int j = tid; j < nc; j += BS) {
+            const int dm = vc[j].dynMask, ia = vc[j].indexA, ib = vc[j].indexB, l = cbatch[sched[j]];
+            const int e = max((dm & 1) ? Llast[ia] : 0, (dm & 2) ? Llast[ib] : 0);
+            atomicMax(&Lneed[l], e);
+            atomicMax(Llag, e - l);
+        }
+        __syncthreads();
+        // a sweep starts at least lag + 1 levels after the one before it, and two
+        // sweeps after it on the same wave: overlapped only when that gains a tenth
+        const int lag = __builtin_amdgcn_readfirstlane(Llag[0]) + 1;
+        const int ns = ((sp.flags & 1u) ? 1 : 0) + sp.velocityIterations;
+        const long long stride = max(lag, (nbatch + 1) / 2);
+        overlap = ns > 1 && ((sp.flags & kStepPipedAlways) || 10 * ((ns - 1) * stride + nbatch) <= 9ll * ns * nbatch);
+    }
+    if (overlap) {
+        velocity_piped(Lbo, Lneed, nbatch, vc, Lvel, Limp, Lnext, (sp.flags & 1u) != 0, sp.velocityIterations, prof);
+        if (tid == 0) {
+            d.S[w].iterations = sp.velocityIterations;
+            d.S[w].residual = -1.0f;
+        }
+        if (timed && lane == 0) d.ktime[6 + wave] = __builtin_amdgcn_s_memtime(); // each wave's last sweep done
+    } else if (wave == 0) {
+        if (nbatch > 0 && (sp.flags & 1u)) velocity_sweep<0>(Lbo, nbatch, vc, Lvel, Limp, lane);
+        if (timed && lane == 0) d.ktime[6] = __builtin_amdgcn_s_memtime();
+        prof.at(kProfVel0);
         const int iters = residualMode ? sp.maxIterations : sp.velocityIterations;
         int used = 0;
         float lastRes = residualMode ? 0.0f : -1.0f;
@@ -2707,11 +2851,12 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
             d.S[w].iterations = used;
             d.S[w].residual = lastRes;
         }
+        if (timed && lane == 0) d.ktime[7] = __builtin_amdgcn_s_memtime();
     }
-    if (timed && lane == 0) d.ktime[7] = __builtin_amdgcn_s_memtime();
+    __syncthreads();
     if (sp.flags & (1u << 6)) mlp_log_labels(d, w, ImpRef{Limp, 1});
     // StoreImpulses (b2contactsolver.d:774-787)
-    for (int k = lane; k < nc; k += 64) {
+    for (int k = tid; k < nc; k += BS) {
         const VelCon &v = vc[k];
         float4 im = Limp[k];
         Contact *c = cont + v.contact;
@@ -2726,7 +2871,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     }
     prof.at(kProfVel1);
     // Integrate positions with the translation / rotation clamps (b2island.d:168-200).
-    for (int b = lane; b < nb; b += 64) {
+    for (int b = tid; b < nb; b += BS) {
         if (bisl[b] < 0) continue;
         float4 pos = Lpos[b], vel = Lvel[b];
         float2 v = v2(vel.x, vel.y);
@@ -2744,10 +2889,11 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         Lpos[b] = make_float4(pos.x + h * v.x, pos.y + h * v.y, pos.z + h * wv, pos.w);
         Lvel[b] = make_float4(v.x, v.y, wv, 0.0f);
     }
-    for (int i = lane; i < nisl; i += 64) Lsol[i] = 0;
+    for (int i = tid; i < nisl; i += BS) Lsol[i] = 0;
+    __syncthreads();
     prof.at(kProfPos0);
-    // Position iterations with the per-island early exit (b2island.d:206-224).
-    for (int it = 0; it < sp.positionIterations; ++it) {
+    // Position iterations with the per-island early exit (b2island.d:206-224), on wave 0.
+    for (int it = 0; it < sp.positionIterations && wave == 0; ++it) {
         for (int i = lane; i < nisl; i += 64)
             if (!Lsol[i]) Lmin[i] = f2ord(0.0f);
         if (nbatch > 0) position_sweep(Lbo, nbatch, pc, Lpos, Lmin, Lsol, lane);
@@ -2759,13 +2905,15 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         }
         if (!__ballot(open)) break;
     }
-    if (timed && lane == 0) d.ktime[8] = __builtin_amdgcn_s_memtime();
+    if (timed && tid == 0) d.ktime[8] = __builtin_amdgcn_s_memtime();
+    __syncthreads();
     // Sleep (b2island.d:241-279) on the LDS state.
     if (sp.flags & 2u) {
-        for (int i = lane; i < nisl; i += 64) Lmin[i] = __float_as_int(kMaxFloat);
+        for (int i = tid; i < nisl; i += BS) Lmin[i] = __float_as_int(kMaxFloat);
+        __syncthreads();
         const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
         const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
-        for (int b = lane; b < nb; b += 64) {
+        for (int b = tid; b < nb; b += BS) {
             int isl = bisl[b];
             if (isl < 0) continue;
             uint32_t f = bfl[b];
@@ -2778,9 +2926,10 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
             atomicMin(&Lmin[isl], __float_as_int(nst));
             Lpos[b].w = nst;
         }
+        __syncthreads();
     }
     // Copy back, SynchronizeTransform, SetAwake(false) of islands that fell asleep.
-    for (int b = lane; b < nb; b += 64) {
+    for (int b = tid; b < nb; b += BS) {
         int isl = bisl[b];
         if (isl < 0) continue;
         float4 pos = Lpos[b], vel = Lvel[b];
@@ -2799,7 +2948,7 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
     __syncthreads();
     prof.at(kProfPos1);
     // SynchronizeFixtures + fat-AABB rule (b2body.d:1129-1141, b2fixture.d:480-502, b2dynamictree.d:140-184).
-    for (int f = lane; f < nf; f += 64) {
+    for (int f = tid; f < nf; f += BS) {
         int b = fbody[f];
         if (bisl[b] < 0 || body_type(bfl[b]) == kStatic) continue;
         float4 p0 = bpos0[b];
@@ -2823,8 +2972,11 @@ __global__ __launch_bounds__(64) void k_solve_wave(Dev d, int cur, StepParams sp
         fat[f] = nbx;
         moved[f] = 1;
     }
-    for (int b = lane; b < nb; b += 64) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (timed && lane == 0) d.ktime[9] = __builtin_amdgcn_s_memtime();
+    for (int b = tid; b < nb; b += BS) bforce[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (timed && tid == 0) {
+        d.ktime[9] = __builtin_amdgcn_s_memtime();
+        d.ktime[13] = __builtin_amdgcn_s_memrealtime();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -4777,7 +4929,8 @@ hipError_t launch_solve_groups(const Dev &d, int cur, const StepParams &sp, cons
 hipError_t launch_solve_wave(const Dev &d, int cur, const StepParams &sp, int worldOffset, int nworlds, int ldsBytes,
                              bool lead, hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    DBX_LAUNCH(k_solve_wave, lead, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, sp, worldOffset, ldsBytes);
+    DBX_LAUNCH(k_solve_wave, lead, dim3(nworlds), dim3(lead ? 128 : 64), ldsBytes, s, d, cur, sp, worldOffset,
+               ldsBytes);
     return hipGetLastError();
 }
 size_t broadphase_scratch_bytes(int proxies) { return bp_bytes(proxies); }

@@ -1417,6 +1417,8 @@ dbx_status dbx_step(dbx_handle *h, float dt, int32_t vel, int32_t pos, uint32_t 
     if ((flags & DBX_STEP_ORDERED) && !(flags & (DBX_STEP_RESIDUAL | DBX_STEP_LEARNED | DBX_STEP_LOG)) &&
         !getenv("DBX_SOLVE_UNGROUPED"))
         sp.flags |= kStepGrouped;
+    if ((flags & DBX_STEP_ORDERED) && !(flags & DBX_STEP_RESIDUAL) && !getenv("DBX_SWEEPS_SEQUENTIAL"))
+        sp.flags |= kStepPiped | (getenv("DBX_SWEEPS_OVERLAP_ALWAYS") ? kStepPipedAlways : 0u);
     if ((flags & (DBX_STEP_LEARNED | DBX_STEP_LOG)) && !h->staged) {
         dbx_status st = stage_mlp(h);
         if (st != DBX_OK) return st;
@@ -1673,6 +1675,10 @@ dbx_status dbx_sync(dbx_handle *h) {
                     t[26] - t[25], t[28], t[29], t[30], t[27] - t[25], t[31]);
             fprintf(stderr, "dbx ktime toi kernel entry to exit: %.1f us wall clock, %lld shader cycles\n",
                     (double)(t[34] - t[32]) * 1000.0 / h->wallKHz, t[35] - t[33]);
+            fprintf(stderr, "dbx ktime solve waves' HW_ID (overlapped sweeps): %llx %llx; solve %lld shader cycles in "
+                            "%.1f us wall clock\n",
+                    (unsigned long long)t[10], (unsigned long long)t[11], t[9] - t[4],
+                    (double)(t[13] - t[12]) * 1000.0 / h->wallKHz);
             fprintf(stderr, "dbx ktime big-world LDS solve: count %lld number %lld init %lld warm %lld velocity %lld "
                             "store %lld position %lld [bodies %lld constraints %lld levels %lld]\n",
                     t[37] - t[36], t[38] - t[37], t[39] - t[38], t[40] - t[39], t[41] - t[40], t[42] - t[41],

@@ -181,3 +181,41 @@ def test_island_groups_equal_one_workgroup(which, steps, monkeypatch):
         assert int(grouped.toi_events()[0]) == int(single.toi_events()[0]), f"step {i + 1}: TOI events"
         assert grouped.bodies(0).tobytes() == single.bodies(0).tobytes(), f"step {i + 1}: bodies"
     assert grouped.contacts(0).tobytes() == single.contacts(0).tobytes()
+
+
+@pytest.mark.parametrize("vel,warm", [(8, True), (3, False), (1, True)])
+def test_overlapped_sweeps_equal_sequential_sweeps(vel, warm, monkeypatch):
+    # The lead tier's workgroups may run the warm start and the velocity
+    # iterations overlapped on two waves (sweep s + 1 trailing sweep s only as far
+    # as each body's updates require), which they do where the schedule makes it
+    # pay; DBX_SWEEPS_OVERLAP_ALWAYS overlaps them at every step and
+    # DBX_SWEEPS_SEQUENTIAL runs them one after the other on one wave.  The
+    # overlapped order keeps every two updates of a body in the sequential order,
+    # so bodies and contacts must agree bit for bit across the three, with and
+    # without warm starting and for any number of iterations.
+    print(f"case: overlapped vs sequential sweeps, {vel} iterations, warm {warm}", flush=True)
+    worlds = scenes.batched_worlds(4096)
+    sizes = np.array([len(s.bodies) for s in worlds])
+    pick = list(np.argsort(-sizes, kind="stable")[:6]) + list(range(0, 4096, 128))
+    chosen = [worlds[int(i)] for i in pick]
+    flags = DEFAULTS if warm else DEFAULTS & ~native.WARM
+    modes = {"default": {}, "always": {"DBX_SWEEPS_OVERLAP_ALWAYS": "1"}, "sequential": {"DBX_SWEEPS_SEQUENTIAL": "1"}}
+    hs = {m: _gpu(chosen) for m in modes}
+    for i in range(160):
+        for m, env in modes.items():
+            for var in ("DBX_SWEEPS_OVERLAP_ALWAYS", "DBX_SWEEPS_SEQUENTIAL"):
+                monkeypatch.delenv(var, raising=False)
+            for var, val in env.items():
+                monkeypatch.setenv(var, val)
+            hs[m].step(DT, vel, 3, flags)
+        if i % 20 == 19:
+            for k in range(len(chosen)):
+                ref = hs["sequential"].bodies(k).tobytes()
+                for m in ("default", "always"):
+                    assert hs[m].bodies(k).tobytes() == ref, f"{m}, step {i + 1}, world {k}: bodies"
+    for k in range(len(chosen)):
+        ref = hs["sequential"].contacts(k).tobytes()
+        for m in ("default", "always"):
+            assert hs[m].contacts(k).tobytes() == ref, f"{m}, world {k}: contacts"
+    it, _ = hs["always"].solver_stats()
+    assert int(it[0]) == vel
