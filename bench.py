@@ -134,15 +134,17 @@ def summarize_counters(args) -> dict:
             out[f"{key}_bytes"] = int(round(np.mean(vals) * 1024.0 * scale))
         out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
     pa = args.profiled_args
+    rx = args.profiled_regex or args.kernel
     out["commands"] = [
-        f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '{args.kernel}' --output-format csv -d {fetch_dir} "
+        f"rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '{rx}' --output-format csv -d {fetch_dir} "
         f"-o pmc -- python3 bench.py {pa}",
-        f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex '{args.kernel}' --output-format csv -d {write_dir} "
+        f"rocprofv3 --pmc WRITE_SIZE --kernel-include-regex '{rx}' --output-format csv -d {write_dir} "
         f"-o pmc -- python3 bench.py {pa}",
         f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {cfetch_dir} -o copy -- python3 bench.py --copy-mib {mib}",
         f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {cwrite_dir} -o copy -- python3 bench.py --copy-mib {mib}",
         "python3 bench.py --summarize-pmc " + " ".join(args.summarize_pmc) +
         f" --kernel '{args.kernel}'" + (" --each" if args.each else "") +
+        (f" --profiled-regex '{args.profiled_regex}'" if args.profiled_regex else "") +
         f" --last {args.last} --copy-mib {mib} --workload '{args.workload}' --profiled-args '{pa}' --out {args.out}",
     ]
     out["note"] = ("scale = bytes the copy moves each way / what the counter reports for it; "
@@ -435,6 +437,8 @@ def main():
     ap.add_argument("--last", type=int, default=600, help="dispatches (timed steps) the summaries average over")
     ap.add_argument("--workload", default="")
     ap.add_argument("--profiled-args", default="")
+    ap.add_argument("--profiled-regex", default="",
+                    help="the --kernel-include-regex the counter runs used, when wider than --kernel")
     ap.add_argument("--out", default="")
     args = ap.parse_args()
     if args.summarize_pmc:
