@@ -592,8 +592,10 @@ def main():
         "world": lead[0], "constraints_last_step": int(len(sched)), "levels_per_sweep_last_step": levels,
         "phase_ms_per_step": ph,
         "us_per_velocity_level": 1e3 * ph["solveVelocity"] / max(levels * VEL, 1),
-        "note": "one wave walks the levels of the largest world in order; a level costs its instruction "
-                "chain (about 1,100 shader cycles per velocity level), not memory traffic",
+        "note": "a wave walks the levels of the largest world in order, one sweep at a time (two waves "
+                "overlap consecutive velocity sweeps where the bodies' level spread allows, so solveVelocity "
+                "counts from the warm start's end); a level costs its instruction chain (about 1,100 shader "
+                "cycles per velocity level), not memory traffic",
     } if not args.coloured else None
     # HBM bytes per launch of the named kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # summarised in profiles/ for this workload and window (counters cannot be read inside the timed run)
