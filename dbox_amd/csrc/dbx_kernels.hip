@@ -3658,25 +3658,35 @@ __device__ __forceinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci,
         init_constraint(vcs[k], pcs[k], cont[icon[k]], icon[k], 0, W.fbody, W.fmat, W.bfl, W.bmass, W.bpos, W.bvel, ss,
                         W.ctan[icon[k]]);
     // SolveTOIPositionConstraints (b2contactsolver.d:152-242): only the two TOI
-    // bodies move
-    for (int it = 0; it < 20; ++it) {
-        float minSep = 0.0f;
-        for (int k = 0; k < ncon; ++k) {
-            PosCon p = pcs[k];
-            if (p.indexA != ia && p.indexA != ib) {
-                p.invMassA = 0.0f;
-                p.invIA = 0.0f;
+    // bodies move.  An island of the TOI contact alone (the usual event) keeps its
+    // two bodies in registers over the iterations.
+    if (ncon == 1) {
+        const PosCon p = pcs[0]; // its bodies are the two TOI bodies
+        float4 pa = W.bpos[p.indexA], pb = W.bpos[p.indexB];
+        for (int it = 0; it < 20; ++it)
+            if (fminr(0.0f, solve_position_core(p, pa, pb, kToiBaumgarte)) >= -1.5f * kLinearSlop) break;
+        W.bpos[p.indexA] = pa;
+        W.bpos[p.indexB] = pb;
+    } else {
+        for (int it = 0; it < 20; ++it) {
+            float minSep = 0.0f;
+            for (int k = 0; k < ncon; ++k) {
+                PosCon p = pcs[k];
+                if (p.indexA != ia && p.indexA != ib) {
+                    p.invMassA = 0.0f;
+                    p.invIA = 0.0f;
+                }
+                if (p.indexB != ia && p.indexB != ib) {
+                    p.invMassB = 0.0f;
+                    p.invIB = 0.0f;
+                }
+                float4 pa = W.bpos[p.indexA], pb = W.bpos[p.indexB];
+                minSep = fminr(minSep, solve_position_core(p, pa, pb, kToiBaumgarte));
+                W.bpos[p.indexA] = pa;
+                W.bpos[p.indexB] = pb;
             }
-            if (p.indexB != ia && p.indexB != ib) {
-                p.invMassB = 0.0f;
-                p.invIB = 0.0f;
-            }
-            float4 pa = W.bpos[p.indexA], pb = W.bpos[p.indexB];
-            minSep = fminr(minSep, solve_position_core(p, pa, pb, kToiBaumgarte));
-            W.bpos[p.indexA] = pa;
-            W.bpos[p.indexB] = pb;
+            if (minSep >= -1.5f * kLinearSlop) break;
         }
-        if (minSep >= -1.5f * kLinearSlop) break;
     }
     // leap of faith to the new safe state
     {
@@ -3689,16 +3699,25 @@ __device__ __forceinline__ int toi_event(const ToiWorld &W, ToiLists &L, int ci,
                         W.ctan[icon[k]]);
         imp[k] = make_float4(vcs[k].nImp0, vcs[k].tImp0, vcs[k].nImp1, vcs[k].tImp1);
     }
-    for (int it = 0; it < sp.velocityIterations; ++it)
-        for (int k = 0; k < ncon; ++k) {
-            const VelCon &v = vcs[k];
-            float4 va = W.bvel[v.indexA], vb = W.bvel[v.indexB];
-            float4 im = imp[k];
-            solve_velocity_core<false>(v, im, va, vb);
-            imp[k] = im;
-            W.bvel[v.indexA] = va;
-            W.bvel[v.indexB] = vb;
-        }
+    if (ncon == 1) { // the two bodies and the impulses stay in registers
+        const VelCon v = vcs[0];
+        float4 va = W.bvel[v.indexA], vb = W.bvel[v.indexB], im = imp[0];
+        for (int it = 0; it < sp.velocityIterations; ++it) solve_velocity_core<false>(v, im, va, vb);
+        imp[0] = im;
+        W.bvel[v.indexA] = va;
+        W.bvel[v.indexB] = vb;
+    } else {
+        for (int it = 0; it < sp.velocityIterations; ++it)
+            for (int k = 0; k < ncon; ++k) {
+                const VelCon &v = vcs[k];
+                float4 va = W.bvel[v.indexA], vb = W.bvel[v.indexB];
+                float4 im = imp[k];
+                solve_velocity_core<false>(v, im, va, vb);
+                imp[k] = im;
+                W.bvel[v.indexA] = va;
+                W.bvel[v.indexB] = vb;
+            }
+    }
     // b2Island.Report of the sub-step island (b2island.d:415,438-461): the
     // solver's impulses, which are not stored into the manifolds
     for (int k = 0; k < ncon; ++k) toi_log(W, L, 3, cont[icon[k]], vcs[k].pointCount, imp[k]);
