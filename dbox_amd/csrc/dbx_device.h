@@ -241,7 +241,7 @@ struct Dev {
     const int *toiLogOff;   // nworlds + 1 offsets into toiLog
 };
 
-constexpr int kProfSlots = 256;
+constexpr int kProfSlots = 4096; // steps between two syncs whose phases are kept (256 KiB)
 constexpr int kProfStamps = 8;
 // Stamp indices of one slot: solveInit from kProfInit0 to kProfVel0, solveVelocity to
 // kProfVel1, solvePosition from kProfPos0 to kProfPos1.

@@ -192,7 +192,9 @@ typedef struct {
  *   broadphase SynchronizeFixtures + FindNewContacts (b2world.d:1102-1123)
  *   solveTOI   b2World.SolveTOI
  * A step replayed from a hipGraph records step and the three island-solve phases only;
- * split (DBX_STEP_PRESOLVE) steps are not timed. */
+ * split (DBX_STEP_PRESOLVE) steps are not timed.  The island-solve phases of the last
+ * 4,096 steps before a dbx_sync are kept; an earlier step of a longer run without a
+ * sync reports them as 0. */
 typedef struct {
     float step, collide, solve, solveInit, solveVelocity, solvePosition, broadphase, solveTOI;
 } dbx_profile;
