@@ -418,8 +418,10 @@ __device__ __forceinline__ void wave_order() {
 
 __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered, int worldOffset, int w,
                                               const WorldDesc &wd, int nb, int m, IslMem M) {
-    const int lane = threadIdx.x;
+    // the staging and output passes use the whole workgroup; the search itself runs on wave 0
+    const int tid = threadIdx.x, BS = blockDim.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long ltmask = (1ull << lane) - 1ull;
+    __shared__ int s_res[3];
     Contact *cont = d.cont[cur] + wd.contactBase;
     uint32_t *gfl = d.bflags + wd.bodyBase;
     float4 *bpos = d.bpos + wd.bodyBase;
@@ -433,8 +435,8 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
     const int2 *gadj = d.adjList + 2 * wd.contactBase;
 
     const bool timed = d.ktime && worldOffset + blockIdx.x == 0;
-    if (timed && lane == 0) d.ktime[0] = __builtin_amdgcn_s_memtime();
-    for (int b = lane; b < nb; b += 64) {
+    if (timed && tid == 0) d.ktime[0] = __builtin_amdgcn_s_memtime();
+    for (int b = tid; b < nb; b += BS) {
         M.ao[b] = make_int2(d.adjOff[wd.bodyBase + b], d.adjCnt[wd.bodyBase + b]);
         uint32_t f = gfl[b];
         int t = body_type(f);
@@ -443,7 +445,7 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
         M.claim[b] = 64;
         M.mask[b] = 0ull;
     }
-    for (int e = lane; e < 2 * m; e += 64) {
+    for (int e = tid; e < 2 * m; e += BS) {
         int2 ent = gadj[e];
         const uint32_t cf = cont[ent.x >> 1].flags;
         if (!((cf & kContactTouching) && (cf & kContactEnabled)) || (cf & kContactSensor)) ent.x = -1 - ent.x;
@@ -451,12 +453,12 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
     }
     __syncthreads();
 
-    if (timed && lane == 0) d.ktime[1] = __builtin_amdgcn_s_memtime();
+    if (timed && tid == 0) d.ktime[1] = __builtin_amdgcn_s_memtime();
     int nc = 0, nisl = 0, maxBatch = -1, overflow = 0;
     int laneMax = -1; // ordered: this lane's largest level (reduced over the wave after the search)
     const bool dupPossible = (wd.pad & 1) != 0;
     // Seeds in body-list order (newest body first), 64 candidates at a time.
-    for (int top0 = nb - 1; top0 >= 0; top0 -= 64) {
+    for (int top0 = wave == 0 ? nb - 1 : -1; top0 >= 0; top0 -= 64) {
         int lo = top0 - 63 < 0 ? 0 : top0 - 63;
         for (;;) {
             int sb = top0 - lane;
@@ -618,18 +620,26 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
         const int mx = wave_last(wave_incl_max(laneMax));
         if (mx > maxBatch) maxBatch = mx;
     }
-    if (timed && lane == 0) {
+    if (timed && tid == 0) {
         d.ktime[2] = __builtin_amdgcn_s_memtime();
         d.ktime[15] = nc;
     }
+    if (tid == 0) {
+        s_res[0] = nc;
+        s_res[1] = nisl;
+        s_res[2] = maxBatch;
+    }
     // b2Body.SetAwake(true) for every body the search reached, then the outputs.
     __syncthreads();
-    for (int b = lane; b < nb; b += 64) {
+    nc = s_res[0];
+    nisl = s_res[1];
+    maxBatch = s_res[2];
+    for (int b = tid; b < nb; b += BS) {
         int isl = (M.st[b].y >> 4) - 1;
         gbisl[b] = isl;
         if (isl >= 0) wake_body(gfl, bpos, b);
     }
-    for (int i = lane; i < nc; i += 64) {
+    for (int i = tid; i < nc; i += BS) {
         int4 o = M.out[i];
         ordC[i] = o.x;
         cisl[i] = o.y;
@@ -639,12 +649,12 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
 
     // Group the constraints by batch (order inside a batch does not matter).
     const int nbatch = maxBatch + 1;
-    for (int i = lane; i <= nbatch; i += 64) boff[i] = 0;
+    for (int i = tid; i <= nbatch; i += BS) boff[i] = 0;
     __syncthreads();
-    for (int k = lane; k < nc; k += 64) atomicAdd(&boff[M.out[k].z + 1], 1);
+    for (int k = tid; k < nc; k += BS) atomicAdd(&boff[M.out[k].z + 1], 1);
     __syncthreads();
     int run = 0;
-    for (int c0 = 0; c0 <= nbatch; c0 += 64) {
+    for (int c0 = 0; c0 <= nbatch && wave == 0; c0 += 64) {
         int i = c0 + lane;
         int v = i <= nbatch ? boff[i] : 0;
         int inc = wave_incl_scan(v);
@@ -655,12 +665,12 @@ __device__ __forceinline__ void island_search(const Dev &d, int cur, int ordered
         run += __shfl(inc, 63, 64);
     }
     __syncthreads();
-    for (int k = lane; k < nc; k += 64) sched[atomicAdd(&bcur[M.out[k].z], 1)] = k;
-    if (timed && lane == 0) {
+    for (int k = tid; k < nc; k += BS) sched[atomicAdd(&bcur[M.out[k].z], 1)] = k;
+    if (timed && tid == 0) {
         d.ktime[3] = __builtin_amdgcn_s_memtime();
         d.ktime[14] = nbatch;
     }
-    if (lane == 0) {
+    if (tid == 0) {
         d.S[w].islandCount = nisl;
         d.S[w].batchCount = nbatch;
         d.S[w].constraints = nc;
@@ -1092,8 +1102,10 @@ __global__ __launch_bounds__(256) void k_iw_scatter(Dev d, int w) {
     d.sched[wd.contactBase + atomicAdd(&d.batchCur[wd.contactBase + 65 * w + b], 1)] = k;
 }
 
+// The lead tier runs four waves per world: they share the staging and output
+// passes, the search itself runs on wave 0.  Other tiers run one wave per world.
 template <bool kLead>
-__global__ __launch_bounds__(64) void k_islands(Dev d, int cur, int ordered, int worldOffset, int ldsBytes) {
+__global__ __launch_bounds__(256) void k_islands(Dev d, int cur, int ordered, int worldOffset, int ldsBytes) {
     extern __shared__ __align__(16) char lds[];
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
@@ -4875,9 +4887,9 @@ size_t solve_wave_lds_bytes(int bodies, int constraints, int levels, int islands
 
 template <bool kLead>
 static hipError_t configure_tier(int maxDynamicLds) {
-    // the kernels other than k_islands also hold a few bytes of static LDS
+    // every kernel here also holds a few bytes of static LDS
     hipError_t e = hipFuncSetAttribute((const void *)k_islands<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       maxDynamicLds);
+                                       maxDynamicLds - 1024);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)k_broadphase<kLead>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             maxDynamicLds - 1024);
@@ -4903,7 +4915,8 @@ hipError_t configure_kernels(int maxDynamicLds) {
 hipError_t launch_islands(const Dev &d, int cur, int ordered, int worldOffset, int nworlds, int ldsBytes, bool lead,
                           hipStream_t s) {
     if (nworlds <= 0) return hipSuccess;
-    DBX_LAUNCH(k_islands, lead, dim3(nworlds), dim3(64), ldsBytes, s, d, cur, ordered, worldOffset, ldsBytes);
+    DBX_LAUNCH(k_islands, lead, dim3(nworlds), dim3(lead ? 256 : 64), ldsBytes, s, d, cur, ordered, worldOffset,
+               ldsBytes);
     return hipGetLastError();
 }
 hipError_t launch_islands_wide(const Dev &d, int cur, const WideBp *wide, int nwide, hipStream_t s) {

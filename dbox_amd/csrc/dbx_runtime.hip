@@ -1253,7 +1253,7 @@ static dbx_status step_tier(dbx_handle *h, int t, const StepParams &sp, uint32_t
     if (timed) (void)hipEventRecord(ev[1], s);
     if (sp.dt > 0.0f) {
         const int ordered = (flags & DBX_STEP_ORDERED) ? 1 : 0;
-        DBX_HIP(h, launch_islands(d, cout, ordered, T.first, T.count, T.lds, lead, s));
+        DBX_HIP(h, launch_islands(d, cout, ordered, T.first, T.count, lds1, lead, s));
         if (ordered && T.wideCount && d.iwParent) DBX_HIP(h, launch_islands_wide(d, cout, wide, T.wideCount, s));
         if (timed) (void)hipEventRecord(ev[2], s);
         // Exact order: one wave per world with the state in LDS, except for a tier of
