@@ -3972,7 +3972,7 @@ __global__ __launch_bounds__(256) void k_tl_link(Dev d, int w, const int *lcnt, 
 }
 
 template <bool kLead>
-__global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
+__global__ __launch_bounds__(256) void k_toi(Dev d, int cur, StepParams sp, int worldOffset) {
     __shared__ int s_ibod[2 * kMaxToiContacts], s_icon[kMaxToiContacts];
     __shared__ VelCon s_vc[kMaxToiContacts];
     __shared__ PosCon s_pc[kMaxToiContacts];
@@ -3981,16 +3981,21 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     __shared__ SweepSnap s_snap[kToiListCap];
     __shared__ int s_woken[kToiListCap], s_movedList[kToiListCap], s_dirty[kToiDirtyCap];
     __shared__ ToiLists s_L;
-    __shared__ float2 s_verts[64 * 2 * kMaxPolygonVertices]; // per lane: the two proxies of one TOI
+    __shared__ float2 s_verts[(kLead ? 256 : 64) * 2 * kMaxPolygonVertices]; // per thread: the two proxies of one TOI
+    __shared__ float s_bestA[4];
+    __shared__ int s_bestPos[4], s_err;
     __shared__ int s_nb, s_np, s_n, s_nsnap, s_nmoved, s_ncand;
     const int w = d.order[worldOffset + blockIdx.x];
     const WorldDesc wd = d.W[w];
-    const int lane = threadIdx.x;
+    // the lead tier runs four waves per world, the other tiers one; the events
+    // themselves run on thread 0
+    const int tid = threadIdx.x, BS = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    float2 *verts = s_verts + 2 * kMaxPolygonVertices * tid;
     const ToiWorld W = toi_world(d, cur, w, wd);
     const int nb = wd.bodyCount, nf = wd.fixCount;
     int n = d.S[w].contactCount;
     const int m = min(d.S[w].adjContacts, n);
-    const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && lane == 0;
+    const bool timed = d.ktime && worldOffset + blockIdx.x == 0 && tid == 0;
     if (timed) {
         d.ktime[32] = __builtin_amdgcn_s_memrealtime();
         d.ktime[33] = __builtin_amdgcn_s_memtime();
@@ -4000,15 +4005,15 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
     // every sweep at alpha0 = 0 and every cached TOI invalid (b2world.d:1131-1146)
     const bool pre = (wd.pad & 2) != 0; // k_toi_wide_* did the resets and the first search
     if (!pre) {
-        for (int b = lane; b < nb; b += 64) W.tb[b] = make_int4(0, 0, -1, 0);
-        for (int i = lane; i < n; i += 64) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
+        for (int b = tid; b < nb; b += BS) W.tb[b] = make_int4(0, 0, -1, 0);
+        for (int i = tid; i < n; i += BS) W.tc[i] = make_int4(__float_as_int(1.0f), 0, -1, -1);
     }
     // The slots that got a TOI in this pass, each once (kTcCand): the search for
     // the first TOI looks at these only, since a slot without kTcToi is passed
     // over and the minimum over (alpha, list position) does not depend on the
     // order the slots are looked at in.
     int *cand = d.toiCand + wd.contactBase;
-    if (lane == 0) {
+    if (tid == 0) {
         s_L.woken = s_woken;
         s_L.moved = s_movedList;
         s_L.dirty = s_dirty;
@@ -4016,38 +4021,40 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         s_ncand = pre ? d.S[w].toiCand : 0;
     }
     __syncthreads();
-    if (lane == 0 && !pre) // contacts made since the Collide, oldest first, so that each list is newest first
+    if (tid == 0 && !pre) // contacts made since the Collide, oldest first, so that each list is newest first
         for (int j = m; j < n; ++j) link_contact(W, j);
     __syncthreads();
     // first search: every sweep is at alpha0 = 0, so no contact advances a body.
     // The contacts that need a TOI are gathered 256 at a time (four independent
-    // load chains per lane) and their TOIs dealt out evenly over the lanes.
-    if (lane == 0) s_nsnap = 0;
+    // load chains per thread of one wave, one per thread of four) and their TOIs
+    // dealt out evenly over the threads.
+    const int per = 256 / BS;
+    if (tid == 0) s_nsnap = 0;
     __syncthreads();
     for (int base = 0; base < (pre ? 0 : n); base += 256) {
         int need[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = base + lane + 64 * u;
+            const int i = base + tid + BS * u;
             need[u] = 0;
-            if (i < n) need[u] = toi_status(W, W.cont[i], 0) == 2;
+            if (u < per && i < n) need[u] = toi_status(W, W.cont[i], 0) == 2;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (need[u]) s_dirty[atomicAdd(&s_nsnap, 1)] = base + lane + 64 * u;
+            if (need[u]) s_dirty[atomicAdd(&s_nsnap, 1)] = base + tid + BS * u;
         __syncthreads();
         const int ne = s_nsnap;
         if (ne > kToiDirtyCap - 256 || (base + 256 >= n && ne > 0)) {
-            for (int k = lane; k < ne; k += 64) {
+            for (int k = tid; k < ne; k += BS) {
                 const int i = s_dirty[k];
                 const float a = contact_toi(W.shp, W.fbody, W.bmass, W.bpos, W.cont[i].fa, W.cont[i].fb,
-                                            snap_of(W, i), s_verts + 2 * kMaxPolygonVertices * lane);
+                                            snap_of(W, i), verts);
                 W.tc[i].x = __float_as_int(a);
                 W.tc[i].y = kTcToi | kTcCand;
                 cand[atomicAdd(&s_ncand, 1)] = i;
             }
             __syncthreads();
-            if (lane == 0) s_nsnap = 0;
+            if (tid == 0) s_nsnap = 0;
             __syncthreads();
         }
     }
@@ -4064,7 +4071,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             // other's time, then the TOI is computed from the sweeps as they stand
             const bool all = (s_L.overflow & 1) != 0;
             const int nd = all ? n : s_L.ndirty;
-            if (lane == 0 && !all) // list order is descending slot order
+            if (tid == 0 && !all) // list order is descending slot order
                 for (int i = 1; i < nd; ++i) {
                     const int v = s_dirty[i];
                     int j = i - 1;
@@ -4075,7 +4082,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                     s_dirty[j + 1] = v;
                 }
             for (int c0 = 0; c0 < nd; c0 += 64) {
-                if (lane == 0) {
+                if (tid == 0) {
                     int k = 0;
                     const int c1 = c0 + 64 < nd ? c0 + 64 : nd;
                     for (int q = c0; q < c1; ++q) {
@@ -4098,11 +4105,11 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 }
                 __syncthreads();
                 const int ns = s_nsnap;
-                for (int k = lane; k < ns; k += 64) {
+                for (int k = tid; k < ns; k += BS) {
                     const int j = s_snap[k].contact;
                     const float a =
                         contact_toi(W.shp, W.fbody, W.bmass, W.bpos, W.cont[j].fa, W.cont[j].fb, s_snap[k],
-                                    s_verts + 2 * kMaxPolygonVertices * lane);
+                                    verts);
                     const int y = W.tc[j].y;
                     W.tc[j].x = __float_as_int(a);
                     W.tc[j].y = y | kTcToi | kTcCand;
@@ -4110,7 +4117,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 }
                 __syncthreads();
             }
-            if (lane == 0) {
+            if (tid == 0) {
                 s_L.ndirty = 0;
                 s_L.overflow &= ~1;
             }
@@ -4122,11 +4129,11 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         int bestPos = INT_MAX;
         const bool scanAll = s_ncand > wd.contactCap;
         const int ns = scanAll ? n : s_ncand;
-        for (int i0 = lane; i0 < ns; i0 += 256) { // four independent loads in flight per lane
+        for (int i0 = tid; i0 < ns; i0 += 4 * BS) { // four independent loads in flight per thread
             int slot[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int i = i0 + 64 * u;
+                const int i = i0 + BS * u;
                 slot[u] = i < ns ? (scanAll ? i : cand[i]) : -1;
             }
             int2 t[4];
@@ -4153,11 +4160,27 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 bestPos = op;
             }
         }
+        if (BS > 64) { // the waves' minima (the minimum over (alpha, position) does not depend on grouping)
+            if (lane == 0) {
+                s_bestA[wave] = bestA;
+                s_bestPos[wave] = bestPos;
+            }
+            __syncthreads();
+            for (int v = 0; v < BS / 64; ++v) {
+                const float oa = s_bestA[v];
+                const int op = s_bestPos[v];
+                if (oa < bestA || (oa == bestA && op < bestPos)) {
+                    bestA = oa;
+                    bestPos = op;
+                }
+            }
+            __syncthreads();
+        }
         if (timed) tWalk += __builtin_amdgcn_s_memtime() - tMark;
         if (bestPos == INT_MAX || 1.0f - 10.0f * kEps < bestA) break;
         const int ci = n - 1 - bestPos;
         if (timed) tMark = __builtin_amdgcn_s_memtime();
-        if (lane == 0) {
+        if (tid == 0) {
             ToiLists &L = s_L;
             L.nwoken = L.nmoved = 0;
             s_nb = toi_event(W, L, ci, bestA, sp, s_ibod, s_icon, s_vc, s_pc, s_imp);
@@ -4184,16 +4207,16 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         for (int k = 0; k < nmoved; ++k) {
             const int mv = s_movedList[k];
             const float4 fm = W.fat[mv];
-            for (int p0 = lane; p0 < nf; p0 += 256) { // four independent loads in flight per lane
+            for (int p0 = tid; p0 < nf; p0 += 4 * BS) { // four independent loads in flight per thread
                 float4 fp[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int p = p0 + 64 * u;
+                    const int p = p0 + BS * u;
                     fp[u] = p < nf ? W.fat[p] : make_float4(kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int p = p0 + 64 * u;
+                    const int p = p0 + BS * u;
                     if (p >= nf || p == mv || !overlap(fm, fp[u])) continue;
                     const int slot = atomicAdd(&s_np, 1);
                     const unsigned int lo = p < mv ? p : mv, hi = p < mv ? mv : p;
@@ -4202,7 +4225,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
             }
         }
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
             int np = s_np;
             if (np > kToiPairCap) {
                 error |= 4;
@@ -4233,9 +4256,11 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
                 });
             for (int k = 0; k < nmoved; ++k) W.moved[s_movedList[k]] = 0;
             s_n = nn;
+            s_err = error;
         }
         __syncthreads();
-        for (int f = lane; f < nf && (error & 4); f += 64) W.moved[f] = 0; // untracked moved proxies
+        error = s_err; // thread 0's, for every thread
+        for (int f = tid; f < nf && (error & 4); f += BS) W.moved[f] = 0; // untracked moved proxies
         n = s_n;
         if (timed) tEvent += __builtin_amdgcn_s_memtime() - tMark;
     }
@@ -4246,7 +4271,7 @@ __global__ __launch_bounds__(64) void k_toi(Dev d, int cur, StepParams sp, int w
         d.ktime[30] = tEvent;
         d.ktime[31] = n;
     }
-    if (lane == 0) {
+    if (tid == 0) {
         d.S[w].toiEvents = events;
         d.S[w].toiCallbacks = s_L.nlog;
         d.S[w].contactCount = n;
@@ -4983,7 +5008,7 @@ hipError_t launch_toi(const Dev &d, int cur, const StepParams &sp, int worldOffs
         hipLaunchKernelGGL(k_tl_link, dim3(gb > 0 ? gb : 1), dim3(256), 0, s, d, b.world, (const int *)b.lcnt,
                            (const int *)b.loff, b.lent);
     }
-    DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(64), 0, s, d, cur, sp, worldOffset);
+    DBX_LAUNCH(k_toi, lead, dim3(nworlds), dim3(lead ? 256 : 64), 0, s, d, cur, sp, worldOffset);
     return hipGetLastError();
 }
 // AddPair for a world too big for one workgroup, over the whole device: each
