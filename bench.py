@@ -174,6 +174,31 @@ def summarize_trace(args) -> str:
     return "\n".join(lines)
 
 
+def summarize_steps(args) -> str:
+    """Markdown table of every kernel of a --single-world rocprofv3 --kernel-trace run: the
+    time each takes per timed step, summed over its dispatches from the one that starts
+    with the (--skip + 1)-th k_collide dispatch on (the first timed step), divided by --last."""
+    rows = _rocprof_rows(args.summarize_steps, "kernel_trace.csv")
+
+    def name(r):
+        return r["Kernel_Name"].split("(")[0].replace("dbx::", "").replace("void ", "")
+
+    starts = sorted(int(r["Start_Timestamp"]) for r in rows if name(r) == "k_collide<true>")
+    t0 = starts[args.skip]
+    tot, cnt = {}, {}
+    for r in rows:
+        if int(r["Start_Timestamp"]) >= t0:
+            k = name(r)
+            tot[k] = tot.get(k, 0) + int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            cnt[k] = cnt.get(k, 0) + 1
+    lines = ["| kernel | µs per step (timed steps) | dispatches |", "|---|---|---|"]
+    for k in sorted(tot, key=lambda k: -tot[k]):
+        lines.append(f"| `{k}` | {tot[k] / args.last / 1e3:.1f} | {cnt[k]} |")
+    lines += ["", f"sum of kernel time: {sum(tot.values()) / args.last / 1e3:.1f} µs per step over "
+              f"{len(starts) - args.skip} collide dispatches in the window"]
+    return "\n".join(lines)
+
+
 def algorithmic_bytes(kernel: str, t: dict, nbodies: int, ndyn: int, nprox: int, sbytes: int, npairs: int) -> float:
     """Compulsory HBM bytes of one launch of each step kernel (per step)."""
     c, touch = t["contacts"], t["touching"]
@@ -435,6 +460,9 @@ def main():
     ap.add_argument("--kernel", default="k_solve_wave<true>")
     ap.add_argument("--each", action="store_true", help="--summarize-pmc: one entry per kernel --kernel matches")
     ap.add_argument("--last", type=int, default=600, help="dispatches (timed steps) the summaries average over")
+    ap.add_argument("--summarize-steps", metavar="TRACE_DIR",
+                    help="print each kernel's time per timed step of a --single-world kernel-trace run and exit")
+    ap.add_argument("--skip", type=int, default=0, help="untimed steps before the window (--summarize-steps)")
     ap.add_argument("--workload", default="")
     ap.add_argument("--profiled-args", default="")
     ap.add_argument("--profiled-regex", default="",
@@ -448,6 +476,9 @@ def main():
             with open(args.out, "w") as f:
                 f.write(text + "\n")
         print(text)
+        return
+    if args.summarize_steps:
+        print(summarize_steps(args))
         return
     if args.summarize_trace:
         print(summarize_trace(args))
