@@ -503,8 +503,8 @@ def main():
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "copies": 20}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (worlds shard across GPUs with no "
-              "exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the worlds would be dealt out "
+              "over the GPUs, one process and handle each, with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     t_gen = time.perf_counter()

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
-Tiers:
+Accuracy tiers:
   * broadphase pairs bit-exact with b2BroadPhase.UpdatePairs on the same fat AABBs;
   * manifold points, normals, separations within 1e-5 m;
   * after 60 steps bodies within 1e-3 m / 1e-3 rad of the sequential-impulse run,
