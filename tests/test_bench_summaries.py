@@ -92,3 +92,27 @@ def test_trace_summary_averages_the_last_dispatches(tmp_path):
     assert r.returncode == 0, r.stderr
     row = [ln for ln in r.stdout.splitlines() if ln.startswith("| `k_solve_wave<true>` | 2 ")]
     assert row and "| 3.0 | 2.0 | 4.0 |" in row[0], r.stdout
+
+
+def test_step_summary_sums_each_kernel_over_the_timed_steps(tmp_path):
+    # --summarize-steps: every kernel's dispatches from the one starting with the
+    # (--skip + 1)-th k_collide<true> dispatch on, summed and divided by --last
+    d = str(tmp_path / "kt")
+    wide = "dbx::k_iw_dfs(dbx::Dev, int, int)"
+    rows, t = [], 0
+    for step in range(5):  # 2 untimed steps, then 3 timed ones
+        for name, ns in ((OTHER, 1_000 + step), (wide, 2_000), (wide, 3_000), (KN, 10_000 * (step + 1))):
+            t += 100_000
+            rows.append({"Dispatch_Id": len(rows) + 1, "Kernel_Name": name, "Start_Timestamp": t,
+                         "End_Timestamp": t + ns})
+    _write(os.path.join(d, "host", "1_run_kernel_trace.csv"),
+           ["Dispatch_Id", "Kernel_Name", "Start_Timestamp", "End_Timestamp"], rows)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize-steps", d, "--skip", "2",
+                        "--last", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = {ln.split("`")[1]: ln for ln in r.stdout.splitlines() if ln.startswith("| `")}
+    # k_solve_wave<true>: (30 + 40 + 50) us / 3 steps; k_iw_dfs: 2 dispatches per step of 2 + 3 us
+    assert "| 40.0 | 3 |" in lines["k_solve_wave<true>"], r.stdout
+    assert "| 5.0 | 6 |" in lines["k_iw_dfs"], r.stdout
+    assert "| 1.0 | 3 |" in lines["k_collide<true>"], r.stdout
+    assert "over 3 collide dispatches" in r.stdout
