@@ -1933,6 +1933,92 @@ __device__ __forceinline__ void position_sweep(const int *Lbo, int nbatch, const
     }
 }
 
+// The three position iterations overlapped on two waves (sweeps 0 and 2 on wave
+// 0, sweep 1 on wave 1), with the dependency waits of velocity_piped.  A sweep
+// depends on the per-island exit test of the sweep before it, which it cannot
+// wait for, so it runs speculatively: an island already known to be solved is
+// skipped, the others are solved, and a body's value before its first update of
+// sweep s (its first level, Lflag bit per slot) is kept in Lbak[s - 1].  Once
+// every sweep is done, the bodies of an island that the sweep-0 test solved get
+// their values from before sweep 1 back, those of an island first solved by the
+// sweep-1 test their values from before sweep 2: the positions of the sequential
+// iterations with their early exit, bit for bit (islands share no dynamic body).
+// Each sweep's test is taken by its wave after its last level, before that wave
+// publishes the sweep as finished; LminS: that sweep's per-island minimum
+// separation; Lver: per island, bit s = first solved by the sweep-s test.
+// The even lane's value of x on both lanes of a pair.
+__device__ __forceinline__ int pair_even(int x) {
+    const int other = __builtin_amdgcn_mov_dpp(x, 0xb1, 0xf, 0xf, true);
+    return (threadIdx.x & 1) ? other : x;
+}
+__device__ __forceinline__ void position_level_piped(const PosCon &q, int slot, bool have, int lo, int hi,
+                                                     const PosCon *__restrict__ pc, float4 *Lpos, int *LminS,
+                                                     const int *Lver, int skip, const int *Lflag, float4 *Lbak) {
+    float4 pa = Lpos[q.indexA], pb = Lpos[q.indexB];
+    int ver = Lver[q.island];
+    const int fl = Lflag[slot];
+    asm volatile("" ::"v"(pa.x), "v"(pb.x), "v"(ver), "v"(fl));
+    ver = pair_even(ver); // the other wave may set a verdict bit meanwhile: the pair takes one reading
+    if (have && Lbak) {
+        if (fl & 1) Lbak[q.indexA] = pa;
+        if (fl & 2) Lbak[q.indexB] = pb;
+    }
+    if (have && !(ver & skip)) {
+        const float ms = solve_position_pair(q, pa, pb);
+        Lpos[q.indexA] = pa;
+        Lpos[q.indexB] = pb;
+        atomicMin(&LminS[q.island], f2ord(ms));
+    }
+    for (int c = lo + 32; c < hi; c += 32) { // levels wider than the 32 lane pairs
+        const int j2 = c + (int)((threadIdx.x & 63) >> 1);
+        if (j2 < hi) {
+            const PosCon q2 = pc[j2];
+            const int f2 = Lflag[j2];
+            if (Lbak) {
+                if (f2 & 1) Lbak[q2.indexA] = Lpos[q2.indexA];
+                if (f2 & 2) Lbak[q2.indexB] = Lpos[q2.indexB];
+            }
+            if (!(pair_even(Lver[q2.island]) & skip)) position_one(q2, Lpos, LminS);
+        }
+    }
+}
+__device__ __forceinline__ void position_sweep_piped(const int *Lbo, const int *Lneed, int nbatch,
+                                                     const PosCon *__restrict__ pc, float4 *Lpos, int *LminS,
+                                                     const int *Lver, int skip, const int *Lflag, float4 *Lbak,
+                                                     int lane, int s, int *Lnext, int me, int &seen) {
+    LevelWindow W{Lbo, nbatch + 1, 0, 0}, N{Lneed, nbatch, 0, 0};
+    W.load(0);
+    N.load(0);
+    const int base = s > 0 ? (s - 1) * nbatch + 1 : INT_MIN / 2;
+    const int mine = s * nbatch;
+    int k0, k1, lo0, hi0, lo1, hi1;
+    bool h0, h1;
+    level_slot<1>(W, 0, nbatch, lane, k0, h0, lo0, hi0);
+    PosCon c0 = pc[k0];
+    level_slot<1>(W, 1, nbatch, lane, k1, h1, lo1, hi1);
+    PosCon c1 = pc[k1];
+    for (int bt = 0; bt < nbatch; bt += 2) {
+        W.cover(bt);
+        N.cover(bt);
+        pipe_wait(Lnext, me ^ 1, base + N.at(bt), seen);
+        position_level_piped(c0, k0, h0, lo0, hi0, pc, Lpos, LminS, Lver, skip, Lflag, Lbak);
+        if (bt + 1 < nbatch) pipe_publish(Lnext, me, mine + bt + 1); // the last level: after the test
+        level_slot<1>(W, bt + 2, nbatch, lane, k0, h0, lo0, hi0);
+        c0 = pc[k0];
+        if (bt + 1 >= nbatch) break;
+        pipe_wait(Lnext, me ^ 1, base + N.at(bt + 1), seen);
+        position_level_piped(c1, k1, h1, lo1, hi1, pc, Lpos, LminS, Lver, skip, Lflag, Lbak);
+        if (bt + 2 < nbatch) pipe_publish(Lnext, me, mine + bt + 2);
+        level_slot<1>(W, bt + 3, nbatch, lane, k1, h1, lo1, hi1);
+        c1 = pc[k1];
+    }
+}
+// The sweep-s exit test of the islands no earlier test solved: bit s of Lver.
+__device__ __forceinline__ void position_test(int *Lver, const int *LminS, int nisl, int s, int lane) {
+    for (int i = lane; i < nisl; i += 64)
+        if (!(Lver[i] & ((1 << s) - 1)) && ord2f(LminS[i]) >= -3.0f * kLinearSlop) Lver[i] |= 1 << s;
+}
+
 // ---------------------------------------------------------------------------
 // A world too big for one workgroup's LDS whose constrained bodies and islands
 // still fit (a 50,000-box pyramid has a few thousand bodies in contact): the
@@ -2728,7 +2814,22 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
     int *Llast = (int *)p; p += isl_align(4 * (size_t)nb);
     int *Lneed = (int *)p; p += isl_align(4 * (size_t)nbatch);
     int *Llag = (int *)p; p += 16;
-    int *Lnext = (int *)p;
+    int *Lnext = (int *)p; p += 4 * 128;
+    // overlapped position iterations (three of them): per body the first level of
+    // its constraints, per slot its first-update bits, the two sweeps' backups, the
+    // later sweeps' minimum separations, the tests' verdicts and a second progress pair
+    const size_t posBytes = isl_align(4 * (size_t)nb) + isl_align(4 * (size_t)nc) + 2 * isl_align(16 * (size_t)nb) +
+                            3 * isl_align(4 * (size_t)nisl) + 4 * 128;
+    const bool pipedPos = piped && sp.positionIterations == 3 &&
+                          solve_lds_bytes(nb, nc, nbatch, nisl) + pipeBytes + posBytes <= (size_t)ldsBytes;
+    int *Lfirst = (int *)p; p += isl_align(4 * (size_t)nb);
+    int *Lflag = (int *)p; p += isl_align(4 * (size_t)nc);
+    float4 *Lbak1 = (float4 *)p; p += isl_align(16 * (size_t)nb);
+    float4 *Lbak2 = (float4 *)p; p += isl_align(16 * (size_t)nb);
+    int *Lmin1 = (int *)p; p += isl_align(4 * (size_t)nisl);
+    int *Lmin2 = (int *)p; p += isl_align(4 * (size_t)nisl);
+    int *Lver = (int *)p; p += isl_align(4 * (size_t)nisl);
+    int *Lnext2 = (int *)p;
 
     Contact *cont = d.cont[cur] + wd.contactBase;
     const float *ctan = d.ctan[cur] + wd.contactBase;
@@ -2765,6 +2866,10 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
         for (int i = tid; i < nbatch; i += BS) Lneed[i] = 0;
         if (tid < 128) Lnext[tid] = 0;
         if (tid == 0) Llag[0] = 0;
+    }
+    if (pipedPos) {
+        for (int b = tid; b < nb; b += BS) Lfirst[b] = INT_MAX;
+        if (tid < 128) Lnext2[tid] = 0;
     }
     // Integrate velocities with damping (b2island.d:82-116) while staging the bodies.
     for (int b = tid; b < nb; b += BS) {
@@ -2809,13 +2914,17 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
             const int l = cbatch[k];
             if (v.dynMask & 1) atomicMax(&Llast[v.indexA], l);
             if (v.dynMask & 2) atomicMax(&Llast[v.indexB], l);
+            if (pipedPos) {
+                if (v.dynMask & 1) atomicMin(&Lfirst[v.indexA], l);
+                if (v.dynMask & 2) atomicMin(&Lfirst[v.indexB], l);
+            }
         }
     }
     __syncthreads();
     if (sp.flags & ((1u << 4) | (1u << 6))) mlp_new_points(d, w, nc, sp.flags, vc, pc, cont, ImpRef{Limp, 1}, Lvel, Lpos);
 
     if (timed && tid == 0) d.ktime[5] = __builtin_amdgcn_s_memtime();
-    bool overlap = false;
+    bool overlap = false, overlapPos = false;
     if (piped) {
         // need(l): the last level of the dynamic bodies of level l's constraints;
         // the lag: the most levels a sweep's level l must trail need(l) by
@@ -2824,6 +2933,10 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
             const int e = max((dm & 1) ? Llast[ia] : 0, (dm & 2) ? Llast[ib] : 0);
             atomicMax(&Lneed[l], e);
             atomicMax(Llag, e - l);
+            if (pipedPos) { // the position constraint's bodies, in its own order
+                const int pm = pc[j].dynMask, pa = pc[j].indexA, pb = pc[j].indexB;
+                Lflag[j] = ((pm & 1) && Lfirst[pa] == l ? 1 : 0) | ((pm & 2) && Lfirst[pb] == l ? 2 : 0);
+            }
         }
         __syncthreads();
         // a sweep starts at least lag + 1 levels after the one before it, and two
@@ -2832,6 +2945,7 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
         const int ns = ((sp.flags & 1u) ? 1 : 0) + sp.velocityIterations;
         const long long stride = max(lag, (nbatch + 1) / 2);
         overlap = ns > 1 && ((sp.flags & kStepPipedAlways) || 10 * ((ns - 1) * stride + nbatch) <= 9ll * ns * nbatch);
+        overlapPos = pipedPos && ((sp.flags & kStepPipedAlways) || 10 * (2 * stride + nbatch) <= 27ll * nbatch);
     }
     if (overlap) {
         velocity_piped(Lbo, Lneed, nbatch, vc, Lvel, Limp, Lnext, (sp.flags & 1u) != 0, sp.velocityIterations, prof);
@@ -2902,10 +3016,43 @@ __global__ __launch_bounds__(128) void k_solve_wave(Dev d, int cur, StepParams s
         Lvel[b] = make_float4(v.x, v.y, wv, 0.0f);
     }
     for (int i = tid; i < nisl; i += BS) Lsol[i] = 0;
+    if (overlapPos)
+        for (int i = tid; i < nisl; i += BS) {
+            Lmin[i] = f2ord(0.0f);
+            Lmin1[i] = f2ord(0.0f);
+            Lmin2[i] = f2ord(0.0f);
+            Lver[i] = 0;
+        }
     __syncthreads();
     prof.at(kProfPos0);
+    if (overlapPos) {
+        // the three position iterations overlapped (position_sweep_piped)
+        int seen = 0;
+        if (wave == 0) {
+            position_sweep_piped(Lbo, Lneed, nbatch, pc, Lpos, Lmin, Lver, 0, Lflag, nullptr, lane, 0, Lnext2, 0, seen);
+            position_test(Lver, Lmin, nisl, 0, lane);
+            pipe_publish(Lnext2, 0, 2 * nbatch);
+            position_sweep_piped(Lbo, Lneed, nbatch, pc, Lpos, Lmin2, Lver, 3, Lflag, Lbak2, lane, 2, Lnext2, 0, seen);
+            position_test(Lver, Lmin2, nisl, 2, lane);
+        } else if (wave == 1) {
+            position_sweep_piped(Lbo, Lneed, nbatch, pc, Lpos, Lmin1, Lver, 1, Lflag, Lbak1, lane, 1, Lnext2, 1, seen);
+            position_test(Lver, Lmin1, nisl, 1, lane);
+            pipe_publish(Lnext2, 1, INT_MAX);
+        }
+        __syncthreads();
+        // islands solved by the sweep-0 test get their bodies from before sweep 1,
+        // those first solved by the sweep-1 test from before sweep 2
+        for (int b = tid; b < nb; b += BS) {
+            const int isl = bisl[b];
+            if (isl < 0 || Lfirst[b] == INT_MAX) continue;
+            const int v = Lver[isl];
+            if (v & 1) Lpos[b] = Lbak1[b];
+            else if (v & 2) Lpos[b] = Lbak2[b];
+        }
+        for (int i = tid; i < nisl; i += BS) Lsol[i] = Lver[i] != 0;
+    }
     // Position iterations with the per-island early exit (b2island.d:206-224), on wave 0.
-    for (int it = 0; it < sp.positionIterations && wave == 0; ++it) {
+    for (int it = 0; it < sp.positionIterations && wave == 0 && !overlapPos; ++it) {
         for (int i = lane; i < nisl; i += 64)
             if (!Lsol[i]) Lmin[i] = f2ord(0.0f);
         if (nbatch > 0) position_sweep(Lbo, nbatch, pc, Lpos, Lmin, Lsol, lane);
